@@ -1,0 +1,344 @@
+"""ctypes bindings of the native libraries (see package docstring)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "lib")
+
+DIM_DTYPE = np.dtype([("query", "<i4"), ("order", "<i4"), ("start_pos", "<i4"),
+                      ("end_pos", "<i4"), ("weight", "<f4")])
+RANGE_DTYPE = np.dtype([("query", "<i4"), ("dim", "<i4"), ("low", "<i4"), ("up", "<i4"),
+                        ("weight", "<f4"), ("order", "<i4")])
+
+GENIE_E = {1: "runtime", 2: "out of memory", 3: "invalid argument", 4: "not built"}
+
+
+class GenieError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{GENIE_E.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _load(name):
+    path = os.path.join(_LIB, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build the native libraries first "
+                          "(python genie_amd/build.py)")
+    return C.CDLL(path)
+
+
+_hip = _load("libgenie_hip.so")
+_host = _load("libgenie.so")
+_synth = _load("libgenie_synth.so")
+
+
+def lib_paths():
+    return [os.path.join(_LIB, n) for n in ("libgenie_hip.so", "libgenie.so", "libgenie_synth.so")]
+
+
+_p = C.c_void_p
+_u32, _u64, _i32, _i64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_int64
+
+
+def _sig(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+class _Desc(C.Structure):
+    _fields_ = [("n_objects", _u32), ("n_dims", _u32), ("n_keys", _u32), ("n_postings", _u64),
+                ("postings", _p), ("key_start", _p), ("key_dim", _p), ("n_segments", _u32),
+                ("seg_upos", _p), ("codec", _i32), ("n_words", _u64), ("words", _p),
+                ("seg_cpos", _p), ("single_value", _i32)]
+
+
+_sig(_hip, "genie_hip_init", C.c_int, C.c_int)
+_sig(_hip, "genie_hip_index_create", C.c_int, C.POINTER(_Desc), C.POINTER(_p))
+_sig(_hip, "genie_hip_index_destroy", C.c_int, _p)
+_sig(_hip, "genie_hip_index_bytes", _u64, _p)
+_sig(_hip, "genie_hip_count_bounds", C.c_int, _p, _p, _u64, _u32, _p)
+_sig(_hip, "genie_hip_match_topk", C.c_int, _p, _p, _u64, _u32, _p, _u32, _p, _p)
+_sig(_hip, "genie_hip_plan_create", C.c_int, _p, _u32, _u64, _u32, _u32, C.c_int, C.POINTER(_p))
+_sig(_hip, "genie_hip_plan_destroy", C.c_int, _p)
+_sig(_hip, "genie_hip_plan_upload", C.c_int, _p, _p, _u64, _u32, _p)
+_sig(_hip, "genie_hip_plan_run", C.c_int, _p, _p)
+_sig(_hip, "genie_hip_plan_download", C.c_int, _p, _p, _p, _p)
+_sig(_hip, "genie_hip_plan_set_timing", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_kernel_ms", C.c_int, _p, C.c_char_p, C.POINTER(C.c_double))
+_sig(_hip, "genie_hip_plan_bytes", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64))
+_sig(_hip, "genie_hip_decode", C.c_int, C.c_int, _p, _p, _u32, _u32, _p, _p)
+_sig(_hip, "genie_hip_last_error", C.c_char_p)
+
+_sig(_host, "genie_table_build_rows", C.c_int, _p, _u32, _u32, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_p))
+_sig(_host, "genie_table_build_sets", C.c_int, _p, _p, _u32, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_p))
+_sig(_host, "genie_table_destroy", None, _p)
+_sig(_host, "genie_table_sizes", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64),
+     C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u64))
+_sig(_host, "genie_table_export", C.c_int, _p, _p, _p, _p, _p, _p, _p, _p)
+_sig(_host, "genie_table_compile", C.c_int, _p, _p, _u64, _u32, C.c_int, _i32, _p, _u64, C.POINTER(_u64))
+_sig(_host, "genie_table_device", C.c_int, _p, C.POINTER(_p))
+_sig(_host, "genie_table_knn", C.c_int, _p, _p, _u64, _u32, _u32, C.c_int, _i32, _p, _p)
+_sig(_host, "genie_table_last_error", C.c_char_p)
+
+_sig(_synth, "synth_rows", C.c_int, _u64, _u32, _u32, C.c_int, C.c_double, C.c_double, _i32, _i32, _u32, _p)
+_sig(_synth, "synth_queries", C.c_int, _u64, _p, _u32, _u32, _u32, _i32, _i32, _i32, C.c_double, _u32, _p, _p)
+_sig(_synth, "synth_sets", C.c_int, _u64, _u32, _u32, _u32, _u32, C.c_double, _p, _p)
+_sig(_synth, "synth_uniform_int", C.c_int, _u64, _u64, _u64, _i32, _i32, _p)
+_sig(_synth, "synth_exponential", C.c_int, _u64, _u64, C.c_double, _p)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+def _chk_hip(rc):
+    if rc != 0:
+        raise GenieError(rc, _hip.genie_hip_last_error().decode())
+
+
+def _chk_host(rc):
+    if rc != 0:
+        raise GenieError(rc, _host.genie_table_last_error().decode())
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Index:
+    """An engine index handle resident in HBM."""
+
+    def __init__(self, handle, owned, keep=None):
+        self.handle = handle
+        self._owned = owned
+        self._keep = keep
+
+    @classmethod
+    def create(cls, n_objects, n_dims, postings, key_start, key_dim, seg_upos, single_value=True,
+               codec=0, words=None, seg_cpos=None):
+        arrs = dict(postings=_c(postings, np.uint32), key_start=_c(key_start, np.uint64),
+                    key_dim=_c(key_dim, np.uint32), seg_upos=_c(seg_upos, np.uint32))
+        if codec:
+            arrs["words"] = _c(words, np.uint32)
+            arrs["seg_cpos"] = _c(seg_cpos, np.uint32)
+        d = _Desc(n_objects=n_objects, n_dims=n_dims, n_keys=len(arrs["key_dim"]),
+                  n_postings=len(arrs["postings"]), postings=_ptr(arrs["postings"]),
+                  key_start=_ptr(arrs["key_start"]), key_dim=_ptr(arrs["key_dim"]),
+                  n_segments=len(arrs["seg_upos"]) - 1, seg_upos=_ptr(arrs["seg_upos"]),
+                  codec=codec, n_words=len(arrs["words"]) if codec else 0,
+                  words=_ptr(arrs.get("words")), seg_cpos=_ptr(arrs.get("seg_cpos")),
+                  single_value=1 if single_value else 0)
+        h = _p()
+        _chk_hip(_hip.genie_hip_index_create(C.byref(d), C.byref(h)))
+        return cls(h, True)
+
+    def bytes(self):
+        return int(_hip.genie_hip_index_bytes(self.handle))
+
+    def count_bounds(self, items, n_queries):
+        items = _c(items, DIM_DTYPE)
+        out = np.zeros(n_queries, np.uint32)
+        _chk_hip(_hip.genie_hip_count_bounds(self.handle, _ptr(items), len(items), n_queries, _ptr(out)))
+        return out
+
+    def close(self):
+        if self._owned and self.handle:
+            _hip.genie_hip_index_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def match_topk(index, items, n_queries, k, count_bound=0):
+    """Plain per-batch search through the C ABI; k is an int or per-query array.
+    Returns (ids, counts) as flat int32 arrays, query-major."""
+    items = _c(items, DIM_DTYPE)
+    kq = _c(np.broadcast_to(np.asarray(k, dtype=np.uint32), (n_queries,)), np.uint32)
+    tot = int(kq.sum())
+    ids = np.zeros(tot, np.int32)
+    cnt = np.zeros(tot, np.int32)
+    _chk_hip(_hip.genie_hip_match_topk(index.handle, _ptr(items), len(items), n_queries, _ptr(kq),
+                                       count_bound, _ptr(ids), _ptr(cnt)))
+    return ids, cnt
+
+
+class Plan:
+    """Device-resident batch (genie_hip_plan_*): upload once, run many times."""
+
+    def __init__(self, index, max_queries, max_items, max_k, count_bound=0, mode=0):
+        self.index = index
+        h = _p()
+        _chk_hip(_hip.genie_hip_plan_create(index.handle, max_queries, max_items, max_k, count_bound,
+                                            mode, C.byref(h)))
+        self.handle = h
+        self.total_k = 0
+
+    def upload(self, items, n_queries, k):
+        items = _c(items, DIM_DTYPE)
+        kq = _c(np.broadcast_to(np.asarray(k, dtype=np.uint32), (n_queries,)), np.uint32)
+        _chk_hip(_hip.genie_hip_plan_upload(self.handle, _ptr(items), len(items), n_queries, _ptr(kq)))
+        self.total_k = int(kq.sum())
+
+    def run(self, stream=None):
+        _chk_hip(_hip.genie_hip_plan_run(self.handle, stream))
+
+    def download(self, stream=None):
+        ids = np.zeros(self.total_k, np.int32)
+        cnt = np.zeros(self.total_k, np.int32)
+        _chk_hip(_hip.genie_hip_plan_download(self.handle, stream, _ptr(ids), _ptr(cnt)))
+        return ids, cnt
+
+    def set_timing(self, on=True):
+        _chk_hip(_hip.genie_hip_plan_set_timing(self.handle, 1 if on else 0))
+
+    def kernel_ms(self, name):
+        v = C.c_double()
+        _chk_hip(_hip.genie_hip_plan_kernel_ms(self.handle, name.encode(), C.byref(v)))
+        return v.value
+
+    def bytes(self):
+        a, b = _u64(), _u64()
+        _chk_hip(_hip.genie_hip_plan_bytes(self.handle, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.handle:
+            _hip.genie_hip_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Table:
+    """A GENIE table built by the C++ host library (inv_table / inv_compr_table)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def from_rows(cls, rows, max_length=0, load_balance=False, codec=0, via_lists=False):
+        rows = _c(rows, np.int32)
+        n, d = rows.shape
+        h = _p()
+        _chk_host(_host.genie_table_build_rows(_ptr(rows), n, d, max_length, int(load_balance), codec,
+                                               int(via_lists), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_sets(cls, row_ptr, vals, max_length=0, load_balance=False, codec=0, via_lists=False):
+        row_ptr = _c(row_ptr, np.int64)
+        vals = _c(vals, np.int32)
+        h = _p()
+        _chk_host(_host.genie_table_build_sets(_ptr(row_ptr), _ptr(vals), len(row_ptr) - 1, max_length,
+                                               int(load_balance), codec, int(via_lists), C.byref(h)))
+        return cls(h)
+
+    def sizes(self):
+        a, b, c, e, f, g = _u64(), _u64(), _u64(), _u32(), _u32(), _u64()
+        _chk_host(_host.genie_table_sizes(self.handle, C.byref(a), C.byref(b), C.byref(c), C.byref(e),
+                                          C.byref(f), C.byref(g)))
+        return dict(n_post=a.value, n_seg=b.value, n_words=c.value, n_dims=e.value, n_objects=f.value,
+                    n_keyseg=g.value)
+
+    def export(self):
+        s = self.sizes()
+        inv = np.zeros(s["n_post"], np.int32)
+        pos = np.zeros(s["n_seg"] + 1, np.int32)
+        words = np.zeros(s["n_words"], np.uint32)
+        cpos = np.zeros(s["n_seg"] + 1 if s["n_words"] else 0, np.int32)
+        lo = np.zeros(s["n_dims"], np.int32)
+        hi = np.zeros(s["n_dims"], np.int32)
+        ks = np.zeros(s["n_keyseg"], np.int32)
+        _chk_host(_host.genie_table_export(self.handle, _ptr(inv), _ptr(pos), _ptr(words) if len(words) else None,
+                                           _ptr(cpos) if len(cpos) else None, _ptr(lo), _ptr(hi), _ptr(ks)))
+        return dict(inv=inv, inv_pos=pos, words=words, cpos=cpos, lo=lo, hi=hi, keyseg=ks, **s)
+
+    def compile(self, ranges, n_queries, load_balance=False, max_load=0, cap=None):
+        ranges = _c(ranges, RANGE_DTYPE)
+        s = self.sizes()
+        if cap is None:
+            cap = max(1024, 4 * len(ranges) + s["n_seg"] * 2 + (s["n_post"] // max(1, max_load) + 1 if load_balance else 0))
+        out = np.zeros(cap, DIM_DTYPE)
+        n = _u64()
+        _chk_host(_host.genie_table_compile(self.handle, _ptr(ranges), len(ranges), n_queries, int(load_balance),
+                                            max_load, _ptr(out), cap, C.byref(n)))
+        return out[: n.value].copy()
+
+    def device(self):
+        h = _p()
+        _chk_host(_host.genie_table_device(self.handle, C.byref(h)))
+        return Index(h, False, keep=self)
+
+    def knn(self, ranges, n_queries, k, load_balance=False, max_load=0):
+        ranges = _c(ranges, RANGE_DTYPE)
+        ids = np.zeros(n_queries * k, np.int32)
+        cnt = np.zeros(n_queries * k, np.int32)
+        _chk_host(_host.genie_table_knn(self.handle, _ptr(ranges), len(ranges), n_queries, k, int(load_balance),
+                                        max_load, _ptr(ids), _ptr(cnt)))
+        return ids, cnt
+
+    def close(self):
+        if self.handle:
+            _host.genie_table_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class synth:
+    """Deterministic synthetic workloads (genie_amd/csrc/synth)."""
+    NORMAL, ZIPF_PERM, UNIFORM = 0, 1, 2
+
+    @staticmethod
+    def rows(seed, n, d, model, p0=0.0, p1=0.0, lo=0, hi=255, n_values=256):
+        out = np.empty((n, d), np.int32)
+        if _synth.synth_rows(seed, n, d, model, p0, p1, lo, hi, n_values, _ptr(out)):
+            raise ValueError("synth_rows")
+        return out
+
+    @staticmethod
+    def queries(seed, rows, q, noise=0, lo=0, hi=255, redraw_p=-1.0, n_values=0):
+        rows = _c(rows, np.int32)
+        n, d = rows.shape
+        out = np.empty((q, d), np.int32)
+        src = np.empty(q, np.uint32)
+        if _synth.synth_queries(seed, _ptr(rows), n, d, q, noise, lo, hi, redraw_p, n_values, _ptr(out), _ptr(src)):
+            raise ValueError("synth_queries")
+        return out, src
+
+    @staticmethod
+    def sets(seed, n, len_lo, len_hi, vocab, s):
+        row_ptr = np.empty(n + 1, np.int64)
+        if _synth.synth_sets(seed, n, len_lo, len_hi, vocab, s, _ptr(row_ptr), None):
+            raise ValueError("synth_sets")
+        vals = np.empty(int(row_ptr[-1]), np.int32)
+        _synth.synth_sets(seed, n, len_lo, len_hi, vocab, s, _ptr(row_ptr), _ptr(vals))
+        return row_ptr, vals
+
+    @staticmethod
+    def uniform_int(seed, stream, n, lo, hi):
+        out = np.empty(n, np.int32)
+        _synth.synth_uniform_int(seed, stream, n, lo, hi, _ptr(out))
+        return out
+
+    @staticmethod
+    def exponential(seed, n, rate):
+        out = np.empty(n, np.float64)
+        _synth.synth_exponential(seed, n, rate, _ptr(out))
+        return out

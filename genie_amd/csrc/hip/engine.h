@@ -1,0 +1,65 @@
+// Internal structures shared by the engine translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "genie_hip.h"
+
+namespace gh {
+
+// Objects per directory tile.  Every workgroup tile is a multiple of it.
+constexpr uint32_t kDirTile = 8192;
+// Keys whose list is at most this long have no directory entry: a workgroup
+// reads the whole list and keeps the ids inside its tile.
+constexpr uint32_t kShortList = 256;
+// Work items longer than this many postings are split into chunks on the
+// device so that the waves of a workgroup share long lists evenly.
+constexpr uint32_t kChunk = 32768;
+constexpr uint32_t kNoDir = 0xFFFFFFFFu;
+
+// One compiled chunk of a work item: postings [sp, ep) covering keys k0..k1.
+struct CChunk {
+    uint32_t k0, k1, sp, ep, w, q;
+};
+
+// Segment view of a compressed index.
+struct DevIndex {
+    uint32_t n_objects = 0, n_dims = 0, n_keys = 0;
+    uint64_t n_postings = 0;
+    int32_t codec = 0;
+    int32_t single_value = 1;
+    uint32_t n_segments = 0;
+    uint64_t n_words = 0;
+    uint32_t n_dtiles = 0;
+    // device
+    uint32_t* post = nullptr;       // [n_postings]   (uncompressed)
+    uint32_t* key_start = nullptr;  // [n_keys + 1]   positions (u32)
+    uint32_t* dir_off = nullptr;    // [n_keys]
+    uint32_t* dir = nullptr;        // [n_dir_keys * (n_dtiles + 1)]
+    uint32_t* words = nullptr;      // [n_words]      (compressed)
+    uint32_t* seg_cpos = nullptr;   // [n_segments + 1]
+    uint32_t* seg_upos = nullptr;   // [n_segments + 1]
+    uint32_t* seg_key = nullptr;    // [n_segments]
+    uint32_t* seg_first = nullptr;  // [n_segments] first id of each segment
+    uint32_t* seg_last = nullptr;   // [n_segments] last id
+    uint32_t* key_seg = nullptr;    // [n_keys + 1] first segment of each key
+    uint32_t* sdir = nullptr;       // [n_dir_keys * (n_dtiles + 1)] segment directory (compressed)
+    uint64_t bytes = 0;
+    // host copies used for validation and count bounds
+    std::vector<uint64_t> h_key_start;
+    std::vector<uint32_t> h_key_dim;
+    std::vector<uint32_t> h_key_mult;  // max multiplicity of an id within each key's list
+    std::vector<uint32_t> h_seg_cpos;
+    std::vector<uint32_t> h_seg_key;
+};
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+}  // namespace gh
+
+struct genie_index {
+    gh::DevIndex d;
+};

@@ -1,0 +1,1360 @@
+// MI355X (gfx950) match-count top-k engine: index upload, tile directory,
+// device-side item compilation, the LDS-tiled scan/count/select kernel and
+// the per-query merge.  C ABI in include/genie_hip.h.
+//
+// Data flow for one batch (plain form, one query per workgroup):
+//   items (Query::dim, 20 B) --k_chunk_count/scan/k_chunk_write--> chunks
+//   chunks --k_scan_tile<LB>--> per (tile, query): counts in LDS, exact
+//        per-tile top-k under (count desc, id asc), pruned by the query's
+//        adaptive threshold, appended to the query's candidate list
+//   candidates --k_merge--> exact top-k per query, sorted, padded (0,0).
+//
+// The reference runs one 256-thread block per work item with counts in a
+// global bitmap and a global c-PQ hash table (match.cu:43-158,
+// match_device_utils.h:161-473) and then heap_count_topk
+// (heap_count.cu:161-282).  Here the count table lives in LDS per object
+// tile and the threshold / top-k are exact.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "engine.h"
+#include "wave.h"
+
+namespace gh {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(e_ == hipErrorOutOfMemory ? GENIE_E_OOM : GENIE_E_RUNTIME,      \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+template <class T>
+static int dev_alloc(T** p, size_t n, uint64_t* acct = nullptr)
+{
+    *p = nullptr;
+    if (n == 0)
+        n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? GENIE_E_OOM : GENIE_E_RUNTIME,
+                    std::string("hipMalloc(") + std::to_string(n * sizeof(T)) +
+                        " B): " + hipGetErrorString(e));
+    }
+    if (acct)
+        *acct += n * sizeof(T);
+    return GENIE_OK;
+}
+
+template <class T>
+static void dev_free(T*& p)
+{
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Small utility kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+    k_fill_u32<<<blocks, 256, 0, s>>>(p, v, n);
+}
+
+// Exclusive scan of n u32 values, 1024 per block (256 threads x 4).
+__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                     uint32_t* block_sums)
+{
+    __shared__ uint32_t s_w[4];
+    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t v[4], t = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0u;
+        t += v[j];
+    }
+    uint32_t incl = wave_incl_scan(t);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 63)
+        s_w[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int i = 0; i < w; ++i)
+        woff += s_w[i];
+    uint32_t run = woff + incl - t;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (base + j < n)
+            out[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 255)
+        block_sums[blockIdx.x] = woff + incl;
+}
+
+__global__ void __launch_bounds__(256) k_scan_sums(uint32_t* sums, uint32_t nb, uint32_t* total)
+{
+    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0)
+        s_carry = 0;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+        uint32_t i = b0 + threadIdx.x;
+        uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t incl = wave_incl_scan(v);
+        const int w = threadIdx.x >> 6;
+        if (lane_id() == 63)
+            s_w[w] = incl;
+        __syncthreads();
+        uint32_t woff = s_carry;
+        for (int j = 0; j < w; ++j)
+            woff += s_w[j];
+        if (i < nb)
+            sums[i] = woff + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 255)
+            s_carry = woff + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *total = s_carry;
+}
+
+__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums)
+{
+    uint32_t i = blockIdx.x * 1024u + threadIdx.x;
+    uint32_t add = sums[blockIdx.x];
+    for (int j = 0; j < 4; ++j, i += 256)
+        if (i < n)
+            out[i] += add;
+}
+
+// ---------------------------------------------------------------------------
+// Index directory: for every long key and every directory-tile boundary b,
+// the position inside the key's list of the first id >= b * kDirTile.
+// ---------------------------------------------------------------------------
+
+__global__ void k_build_dir(const uint32_t* __restrict__ post, const uint32_t* __restrict__ key_start,
+                            const uint32_t* __restrict__ dir_keys, uint32_t n_dir_keys,
+                            uint32_t n_dtiles, uint32_t* __restrict__ dir)
+{
+    const uint64_t per = (uint64_t)n_dtiles + 1;
+    const uint64_t total = (uint64_t)n_dir_keys * per;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t dk = (uint32_t)(i / per), b = (uint32_t)(i % per);
+        const uint32_t k = dir_keys[dk];
+        const uint32_t s = key_start[k], e = key_start[k + 1];
+        const uint64_t target = (uint64_t)b * kDirTile;
+        uint32_t lo = s, hi = e;  // first position with post[p] >= target
+        while (lo < hi) {
+            uint32_t mid = lo + ((hi - lo) >> 1);
+            if ((uint64_t)post[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        dir[i] = lo - s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-side item compilation
+// ---------------------------------------------------------------------------
+
+// Last key whose first position is <= pos (key_start is non-decreasing).
+__device__ __forceinline__ uint32_t key_of(const uint32_t* key_start, uint32_t n_keys, uint32_t pos)
+{
+    uint32_t lo = 0, hi = n_keys;  // answer in [0, n_keys)
+    while (hi - lo > 1) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (key_start[mid] <= pos)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_chunk_count(const genie_dim_t* __restrict__ items, uint32_t n_items,
+                              uint32_t* __restrict__ nchunks)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items)
+        return;
+    const genie_dim_t it = items[i];
+    uint32_t len = it.end_pos > it.start_pos ? (uint32_t)(it.end_pos - it.start_pos) : 0u;
+    nchunks[i] = (len + kChunk - 1) / kChunk;
+}
+
+__global__ void k_chunk_write(const genie_dim_t* __restrict__ items, uint32_t n_items,
+                              const uint32_t* __restrict__ chunk_ptr,
+                              const uint32_t* __restrict__ key_start, uint32_t n_keys,
+                              CChunk* __restrict__ chunks)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items)
+        return;
+    const genie_dim_t it = items[i];
+    if (it.end_pos <= it.start_pos)
+        return;
+    const uint32_t w = (uint32_t)it.weight;
+    uint32_t c = chunk_ptr[i];
+    for (uint32_t sp = (uint32_t)it.start_pos; sp < (uint32_t)it.end_pos; sp += kChunk, ++c) {
+        uint32_t ep = min((uint32_t)it.end_pos, sp + kChunk);
+        CChunk ch;
+        ch.k0 = key_of(key_start, n_keys, sp);
+        ch.k1 = key_of(key_start, n_keys, ep - 1);
+        ch.sp = sp;
+        ch.ep = ep;
+        ch.w = w;
+        ch.q = (uint32_t)it.query;
+        chunks[c] = ch;
+    }
+}
+
+// qc_ptr[q] = first chunk of query q (items sorted by query, item_qptr CSR).
+__global__ void k_query_chunk_ptr(const uint32_t* __restrict__ item_qptr, uint32_t n_queries,
+                                  const uint32_t* __restrict__ chunk_ptr, uint32_t n_items,
+                                  const uint32_t* __restrict__ total, uint32_t* __restrict__ qc_ptr)
+{
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > n_queries)
+        return;
+    uint32_t ip = item_qptr[q];
+    qc_ptr[q] = ip >= n_items ? *total : chunk_ptr[ip];
+}
+
+// ---------------------------------------------------------------------------
+// Scan + count + per-tile select (plain form)
+// ---------------------------------------------------------------------------
+
+constexpr int kScanThreads = 512;
+constexpr int kScanWaves = kScanThreads / kWave;
+constexpr int kCntWords = 16384;  // 64 KiB of counters per workgroup
+constexpr uint32_t kEmitFail = 0xFFFFFFFFu;
+
+struct ScanArgs {
+    const uint32_t* post;
+    const uint32_t* key_start;
+    const uint32_t* dir_off;
+    const uint32_t* dir;
+    const CChunk* chunks;
+    const uint32_t* qc_ptr;
+    const uint32_t* kq;
+    uint32_t n_objects, n_queries, n_tiles, tile_objs, dt_per_tile, n_dtiles;
+    uint32_t* gthr;       // per query adaptive threshold (max of finished tiles' k-th counts)
+    uint32_t* cand_cnt;   // per query candidates appended
+    const uint32_t* cand_cap;
+    const uint64_t* cand_base;
+    uint64_t* cand;       // keys: count << 32 | ~id
+    uint32_t* overflow;
+};
+
+template <int LB>
+__device__ __forceinline__ uint32_t lane_count(uint32_t v, int j)
+{
+    if constexpr (LB == 32)
+        return v;
+    else
+        return (v >> (LB * j)) & ((1u << LB) - 1u);
+}
+
+__device__ __forceinline__ uint64_t make_key(uint32_t count, uint32_t id)
+{
+    return ((uint64_t)count << 32) | (uint32_t)(~id);
+}
+
+// Count postings of one key sub-span [a, b) into the tile counters.
+template <int LB>
+__device__ __forceinline__ void count_span(const uint32_t* __restrict__ post, uint32_t a, uint32_t b,
+                                           uint32_t obj0, uint32_t tile_objs, uint32_t w,
+                                           bool filter, uint32_t* cnt)
+{
+    constexpr int PER = 32 / LB;
+    constexpr int SH = PER == 4 ? 2 : (PER == 2 ? 1 : 0);
+    const int lane = lane_id();
+    uint32_t p = a + lane;
+    // four loads in flight per lane
+    for (; p + 3 * kWave < b; p += 4 * kWave) {
+        uint32_t id[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            id[u] = __builtin_nontemporal_load(&post[p + u * kWave]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            uint32_t loc = id[u] - obj0;
+            if (!filter || loc < tile_objs)
+                atomicAdd(&cnt[loc >> SH], w << ((loc & (PER - 1)) * LB));
+        }
+    }
+    for (; p < b; p += kWave) {
+        uint32_t loc = post[p] - obj0;
+        if (!filter || loc < tile_objs)
+            atomicAdd(&cnt[loc >> SH], w << ((loc & (PER - 1)) * LB));
+    }
+}
+
+template <int LB>
+__global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
+{
+    constexpr int PER = 32 / LB;
+    __shared__ uint4 s_cnt4[kCntWords / 4];
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(s_cnt4);
+    __shared__ uint32_t s_next;
+    __shared__ uint32_t s_hist[64];
+    __shared__ uint32_t s_wv[kScanWaves];
+    __shared__ uint32_t s_res[8];
+
+    const uint32_t t = blockIdx.x / a.n_queries;
+    const uint32_t q = blockIdx.x % a.n_queries;
+    const uint32_t obj0 = t * a.tile_objs;
+    const uint32_t dt0 = t * a.dt_per_tile;
+    const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+
+    for (int i = tid; i < kCntWords / 4; i += kScanThreads)
+        s_cnt4[i] = make_uint4(0, 0, 0, 0);
+    if (tid == 0)
+        s_next = 0;
+    __syncthreads();
+
+    // ---- count: waves take chunks dynamically --------------------------
+    const uint32_t c0 = a.qc_ptr[q], nch = a.qc_ptr[q + 1] - c0;
+    for (;;) {
+        uint32_t ci = 0;
+        if (lane == 0)
+            ci = atomicAdd(&s_next, 1u);
+        ci = bcast_first(__shfl((int)ci, 0, kWave));
+        if (ci >= nch)
+            break;
+        const CChunk ch = a.chunks[c0 + ci];
+        for (uint32_t k = ch.k0; k <= ch.k1; ++k) {
+            const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
+            uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
+            if (lo >= hi)
+                continue;
+            const uint32_t doff = a.dir_off[k];
+            bool filter = true;
+            if (doff != kNoDir) {
+                const uint32_t* d = a.dir + doff;
+                lo = max(lo, ks + d[dt0]);
+                hi = min(hi, ks + d[dt1]);
+                filter = false;
+            }
+            if (lo < hi)
+                count_span<LB>(a.post, lo, hi, obj0, a.tile_objs, ch.w, filter, cnt);
+        }
+    }
+    __syncthreads();
+
+    // ---- select: exact tile top-k under (count desc, id asc) ------------
+    const uint32_t k = a.kq[q];
+    const uint32_t wpw = kCntWords / kScanWaves;  // words per wave (contiguous)
+    const uint32_t wbeg = wv * wpw;
+
+    uint32_t m = 0;
+    for (uint32_t i = lane; i < wpw; i += kWave) {
+        uint32_t v = cnt[wbeg + i];
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            m = max(m, lane_count<LB>(v, j));
+    }
+    m = wave_max(m);
+    if (lane == 0)
+        s_wv[wv] = m;
+    if (tid == 0)  // one read, shared, so every thread takes the same branches
+        s_res[6] = __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    uint32_t M = 0;
+    for (int i = 0; i < kScanWaves; ++i)
+        M = max(M, s_wv[i]);
+    const uint32_t lower = max(1u, s_res[6]);
+    if (M < lower)
+        return;  // nothing in this tile can reach the query's top-k
+
+    // Threshold search in 64-wide count windows from the top.
+    uint32_t top = M, cum = 0, ct = lower, ngt = 0, neq = 0, found = 0;
+    for (;;) {
+        const uint32_t wlo = top >= lower + 63 ? top - 63 : lower;
+        __syncthreads();
+        if (tid < 64)
+            s_hist[tid] = 0;
+        __syncthreads();
+        for (uint32_t i = lane; i < wpw; i += kWave) {
+            uint32_t v = cnt[wbeg + i];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                uint32_t c = lane_count<LB>(v, j);
+                hist_add_aggregated(s_hist, c - wlo, c >= wlo && c <= top);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t f = 0, cc = wlo, g = cum, e = 0;
+            for (uint32_t c = top + 1; c-- > wlo;) {
+                uint32_t h = s_hist[c - wlo];
+                if (g + h >= k) {
+                    f = 1;
+                    cc = c;
+                    e = h;
+                    break;
+                }
+                g += h;
+            }
+            s_res[0] = f;
+            s_res[1] = cc;
+            s_res[2] = g;
+            s_res[3] = e;
+        }
+        __syncthreads();
+        found = s_res[0];
+        cum = s_res[2];
+        if (found) {
+            ct = s_res[1];
+            ngt = cum;
+            neq = s_res[3];
+            break;
+        }
+        if (wlo == lower) {
+            ct = lower;  // fewer than k objects reach `lower`: emit them all
+            ngt = cum;
+            neq = 0;
+            break;
+        }
+        top = wlo - 1;
+    }
+
+    const uint32_t need = found ? k - ngt : 0;
+    const bool trim = found && neq > need;
+    const uint32_t nemit = found ? ngt + min(neq, need) : ngt;
+    if (tid == 0) {
+        uint32_t base = atomicAdd(&a.cand_cnt[q], nemit);
+        if ((uint64_t)base + nemit > a.cand_cap[q]) {
+            atomicOr(a.overflow, 1u);
+            base = kEmitFail;
+        }
+        if (found)
+            atomicMax(&a.gthr[q], ct);
+        s_res[4] = base;
+        s_res[5] = 0;  // unordered slot counter
+    }
+    __syncthreads();
+    const uint32_t base = s_res[4];
+    if (base == kEmitFail || nemit == 0)
+        return;
+    uint64_t* out = a.cand + a.cand_base[q] + base;
+
+    // Unordered emission: objects strictly above ct (and, when nothing has to
+    // be trimmed, those equal to ct as well).
+    const uint32_t cut = trim ? ct + 1 : ct;
+    for (uint32_t i = lane; i < wpw; i += kWave) {
+        uint32_t v = cnt[wbeg + i];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            uint32_t c = lane_count<LB>(v, j);
+            bool take = c >= cut;
+            uint64_t bm = ballot(take);
+            if (bm) {
+                uint32_t wbase = 0;
+                if (lane == __builtin_ctzll(bm))
+                    wbase = atomicAdd(&s_res[5], (uint32_t)__popcll(bm));
+                wbase = (uint32_t)__shfl((int)wbase, __builtin_ctzll(bm), kWave);
+                if (take) {
+                    uint32_t id = obj0 + (wbeg + i) * PER + j;
+                    out[wbase + lane_prefix(bm)] = make_key(c, id);
+                }
+            }
+        }
+    }
+    if (!trim)
+        return;
+
+    // Ordered emission of ties at ct: the `need` smallest ids.  Pass A counts
+    // ties per wave range, pass B ranks them in id order.
+    uint32_t e = 0;
+    for (uint32_t i = lane; i < wpw; i += kWave) {
+        uint32_t v = cnt[wbeg + i];
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            e += lane_count<LB>(v, j) == ct;
+    }
+    e = wave_sum(e);
+    __syncthreads();
+    if (lane == 0)
+        s_wv[wv] = e;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int i = 0; i < wv; ++i)
+        run += s_wv[i];
+    for (uint32_t i0 = 0; i0 < wpw && run < need; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        uint32_t v = i < wpw ? cnt[wbeg + i] : 0u;
+        uint32_t mask = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            mask |= (uint32_t)(i < wpw && lane_count<LB>(v, j) == ct) << j;
+        const uint32_t pc = (uint32_t)__popc(mask);
+        const uint32_t incl = wave_incl_scan(pc);
+        uint32_t r = run + incl - pc;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (mask & (1u << j)) {
+                if (r < need) {
+                    uint32_t id = obj0 + (wbeg + i) * PER + j;
+                    out[ngt + r] = make_key(ct, id);
+                }
+                ++r;
+            }
+        }
+        run += (uint32_t)__shfl((int)incl, 63, kWave);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge: exact top-k of each query's candidate keys, sorted descending.
+// ---------------------------------------------------------------------------
+
+constexpr int kMergeThreads = 256;
+constexpr int kMergeMaxK = GENIE_MAX_K;
+
+struct MergeArgs {
+    const uint64_t* cand;
+    const uint64_t* cand_base;
+    const uint32_t* cand_cnt;
+    const uint32_t* cand_cap;
+    const uint32_t* kq;
+    const uint64_t* out_off;
+    int32_t* out_ids;
+    int32_t* out_counts;
+    uint32_t key_bits;  // significant bits of a key (32 + bits of the count bound)
+};
+
+__global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
+{
+    __shared__ uint64_t s_keys[kMergeMaxK];
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_res[4];
+    const uint32_t q = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t k = a.kq[q];
+    const uint32_t M = min(a.cand_cnt[q], a.cand_cap[q]);
+    const uint64_t* keys = a.cand + a.cand_base[q];
+
+    uint64_t kth = 0;  // every key >= kth is in the result
+    if (M > k) {
+        uint64_t prefix = 0, mask = 0;
+        uint32_t remaining = k;
+        int shift = ((int)((a.key_bits + 7) / 8) - 1) * 8;
+        for (; shift >= 0; shift -= 8) {
+            if (tid < 256)
+                s_hist[tid] = 0;
+            __syncthreads();
+            for (uint32_t i0 = 0; i0 < M; i0 += kMergeThreads) {
+                const uint32_t i = i0 + tid;
+                uint64_t key = i < M ? keys[i] : 0;
+                bool take = i < M && (key & mask) == prefix;
+                hist_add_aggregated(s_hist, (uint32_t)(key >> shift) & 255u, take);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t cum = 0, b = 255;
+                for (int d = 255; d >= 0; --d) {
+                    if (cum + s_hist[d] >= remaining) {
+                        b = (uint32_t)d;
+                        break;
+                    }
+                    cum += s_hist[d];
+                }
+                s_res[0] = b;
+                s_res[1] = remaining - cum;
+            }
+            __syncthreads();
+            prefix |= (uint64_t)s_res[0] << shift;
+            mask |= (uint64_t)255 << shift;
+            remaining = s_res[1];
+            __syncthreads();
+        }
+        kth = prefix;
+    }
+    // collect the selected keys
+    if (tid == 0)
+        s_res[2] = 0;
+    uint32_t n2 = 1;
+    const uint32_t nsel = min(M, k);
+    while (n2 < nsel)
+        n2 <<= 1;
+    for (uint32_t i = tid; i < n2; i += kMergeThreads)
+        s_keys[i] = 0;
+    __syncthreads();
+    for (uint32_t i0 = 0; i0 < M; i0 += kMergeThreads) {
+        const uint32_t i = i0 + tid;
+        uint64_t key = i < M ? keys[i] : 0;
+        bool take = i < M && key >= kth;
+        uint64_t bm = ballot(take);
+        if (bm) {
+            uint32_t wb = 0;
+            if (lane_id() == __builtin_ctzll(bm))
+                wb = atomicAdd(&s_res[2], (uint32_t)__popcll(bm));
+            wb = (uint32_t)__shfl((int)wb, __builtin_ctzll(bm), kWave);
+            if (take) {
+                uint32_t slot = wb + lane_prefix(bm);
+                if (slot < n2)
+                    s_keys[slot] = key;
+            }
+        }
+    }
+    __syncthreads();
+    // bitonic sort, descending
+    for (uint32_t size = 2; size <= n2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = tid; i < n2; i += kMergeThreads) {
+                uint32_t j = i ^ stride;
+                if (j > i) {
+                    bool desc = (i & size) == 0;
+                    uint64_t x = s_keys[i], y = s_keys[j];
+                    if (desc ? (x < y) : (x > y)) {
+                        s_keys[i] = y;
+                        s_keys[j] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint64_t o = a.out_off[q];
+    for (uint32_t i = tid; i < k; i += kMergeThreads) {
+        uint64_t key = i < nsel ? s_keys[i] : 0;
+        if (key == 0) {
+            a.out_ids[o + i] = 0;
+            a.out_counts[o + i] = 0;
+        } else {
+            a.out_ids[o + i] = (int32_t)(~(uint32_t)key);
+            a.out_counts[o + i] = (int32_t)(key >> 32);
+        }
+    }
+}
+
+}  // namespace gh
+
+using namespace gh;
+
+// ===========================================================================
+// Plan
+// ===========================================================================
+
+struct genie_plan {
+    genie_index* ix = nullptr;
+    int mode = 0;
+    uint32_t max_queries = 0, max_k = 0, count_bound = 0;
+    uint64_t max_items = 0;
+    uint32_t lane_bits = 8;
+    // batch shape of the last upload
+    uint32_t n_queries = 0;
+    uint64_t n_items = 0, n_chunks = 0, total_k = 0, total_cap = 0;
+    uint32_t cap_factor = 8;
+    uint64_t logical_bytes = 0, compulsory_bytes = 0;
+    // device buffers
+    genie_dim_t* d_items = nullptr;
+    uint32_t* d_item_qptr = nullptr;
+    uint32_t* d_nchunks = nullptr;
+    uint32_t* d_chunk_ptr = nullptr;
+    uint32_t* d_block_sums = nullptr;
+    uint32_t* d_total = nullptr;
+    CChunk* d_chunks = nullptr;
+    uint64_t chunks_cap = 0;
+    uint32_t* d_qc_ptr = nullptr;
+    uint32_t* d_kq = nullptr;
+    uint32_t* d_gthr = nullptr;
+    uint32_t* d_cand_cnt = nullptr;
+    uint32_t* d_cand_cap = nullptr;
+    uint64_t* d_cand_base = nullptr;
+    uint64_t* d_cand = nullptr;
+    uint64_t cand_alloc = 0;
+    uint32_t* d_overflow = nullptr;
+    uint64_t* d_out_off = nullptr;
+    int32_t* d_out_ids = nullptr;
+    int32_t* d_out_counts = nullptr;
+    uint64_t out_alloc = 0;
+    std::vector<uint32_t> h_kq;
+    std::vector<uint64_t> h_out_off;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    double ms_compile = 0, ms_scan = 0, ms_merge = 0;
+    bool ran = false;
+};
+
+static uint32_t bits_for(uint32_t bound)
+{
+    return bound <= 255 ? 8u : (bound <= 65535 ? 16u : 32u);
+}
+
+static uint32_t tile_objs_for(uint32_t lane_bits) { return (uint32_t)kCntWords * (32u / lane_bits); }
+
+static void plan_free_buffers(genie_plan* p)
+{
+    dev_free(p->d_items);
+    dev_free(p->d_item_qptr);
+    dev_free(p->d_nchunks);
+    dev_free(p->d_chunk_ptr);
+    dev_free(p->d_block_sums);
+    dev_free(p->d_total);
+    dev_free(p->d_chunks);
+    dev_free(p->d_qc_ptr);
+    dev_free(p->d_kq);
+    dev_free(p->d_gthr);
+    dev_free(p->d_cand_cnt);
+    dev_free(p->d_cand_cap);
+    dev_free(p->d_cand_base);
+    dev_free(p->d_cand);
+    dev_free(p->d_overflow);
+    dev_free(p->d_out_off);
+    dev_free(p->d_out_ids);
+    dev_free(p->d_out_counts);
+}
+
+// Count bound of every query: per (query, order) group the largest weight
+// times how often one object can occur in the group's spans (1 per distinct
+// dim for single-value tables; the sum of per-key multiplicities for
+// bijectMap tables), summed over groups.
+static int compute_bounds(const DevIndex& d, const genie_dim_t* items, uint64_t n_items,
+                          uint32_t n_queries, uint32_t* out)
+{
+    struct G {
+        uint32_t q;
+        int32_t order;
+        uint32_t w;
+        uint32_t key;
+    };
+    std::vector<G> v;
+    v.reserve(n_items * 2);
+    const uint64_t* ks = d.h_key_start.data();
+    const uint32_t nk = d.n_keys;
+    auto key_of_pos = [&](uint64_t pos) -> uint32_t {
+        return (uint32_t)(std::upper_bound(ks, ks + nk + 1, pos) - ks - 1);
+    };
+    for (uint64_t i = 0; i < n_items; ++i) {
+        const genie_dim_t& it = items[i];
+        if (it.end_pos <= it.start_pos)
+            continue;
+        uint32_t k0, k1;
+        if (d.codec == 0) {
+            k0 = key_of_pos((uint64_t)it.start_pos);
+            k1 = key_of_pos((uint64_t)it.end_pos - 1);
+        } else {
+            const uint32_t* cp = d.h_seg_cpos.data();
+            uint32_t s0 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) - cp - 1);
+            uint32_t s1 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos - 1) - cp - 1);
+            k0 = d.h_seg_key[s0];
+            k1 = d.h_seg_key[s1];
+        }
+        for (uint32_t k = k0; k <= k1; ++k)
+            v.push_back({(uint32_t)it.query, it.order, (uint32_t)it.weight, k});
+    }
+    std::sort(v.begin(), v.end(), [](const G& a, const G& b) {
+        if (a.q != b.q) return a.q < b.q;
+        if (a.order != b.order) return a.order < b.order;
+        return a.key < b.key;
+    });
+    std::vector<uint64_t> acc(n_queries, 0);
+    for (size_t i = 0; i < v.size();) {
+        size_t j = i;
+        uint32_t wmax = 0;
+        uint64_t mult = 0;
+        std::unordered_set<uint32_t> dims;
+        uint32_t last_key = 0xFFFFFFFFu;
+        for (; j < v.size() && v[j].q == v[i].q && v[j].order == v[i].order; ++j) {
+            wmax = std::max(wmax, v[j].w);
+            if (v[j].key == last_key)
+                continue;
+            last_key = v[j].key;
+            if (d.single_value)
+                dims.insert(d.h_key_dim[v[j].key]);
+            else
+                mult += d.h_key_mult[v[j].key];
+        }
+        if (d.single_value)
+            mult = dims.size();
+        acc[v[i].q] += (uint64_t)wmax * mult;
+        i = j;
+    }
+    for (uint32_t q = 0; q < n_queries; ++q)
+        out[q] = (uint32_t)std::min<uint64_t>(acc[q], 0xFFFFFFFFull);
+    return GENIE_OK;
+}
+
+static int validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries)
+{
+    const uint64_t limit = d.codec == 0 ? d.n_postings : d.n_words;
+    for (uint64_t i = 0; i < n_items; ++i) {
+        const genie_dim_t& it = items[i];
+        if (it.query < 0 || (uint32_t)it.query >= n_queries)
+            return fail(GENIE_E_INVALID, "item " + std::to_string(i) + ": query id out of range");
+        if (it.start_pos < 0 || it.end_pos < 0 || (uint64_t)it.end_pos > limit ||
+            (it.end_pos > it.start_pos && (uint64_t)it.start_pos >= limit))
+            return fail(GENIE_E_INVALID, "item " + std::to_string(i) + ": span outside the index");
+        if (!(it.weight >= 0.0f) || it.weight != (float)(uint32_t)it.weight || it.weight > 65535.0f)
+            return fail(GENIE_E_INVALID, "item " + std::to_string(i) +
+                                             ": weight must be a non-negative integer <= 65535");
+        if (d.codec != 0 && it.end_pos > it.start_pos) {
+            // compressed items cover whole segments
+            const uint32_t* cp = d.h_seg_cpos.data();
+            if (!std::binary_search(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) ||
+                !std::binary_search(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos))
+                return fail(GENIE_E_INVALID, "item " + std::to_string(i) +
+                                                 ": compressed span does not align with segments");
+        }
+    }
+    return GENIE_OK;
+}
+
+extern "C" {
+
+const char* genie_hip_last_error(void) { return g_err.c_str(); }
+
+int genie_hip_init(int device)
+{
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (n <= 0)
+        return fail(GENIE_E_RUNTIME, "no HIP device");
+    if (device < 0 || device >= n)
+        device = 0;  // Init falls back to device 0 (init.cu:21-28)
+    HIP_TRY(hipSetDevice(device));
+    return GENIE_OK;
+}
+
+int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
+{
+    if (!desc || !out)
+        return fail(GENIE_E_INVALID, "null argument");
+    *out = nullptr;
+    if (desc->n_keys == 0 || desc->n_objects == 0 || !desc->key_start || !desc->key_dim)
+        return fail(GENIE_E_INVALID, "empty index");
+    if (desc->codec != 0)
+        return fail(GENIE_E_INVALID, "compressed indexes are not supported by this build");
+    if (!desc->postings && desc->n_postings)
+        return fail(GENIE_E_INVALID, "missing postings");
+    if (desc->n_postings >= 0xFFFFFFFFull)
+        return fail(GENIE_E_INVALID, "index too large: more than 2^32-1 postings");
+    if (desc->n_objects >= 0xFFFFFFFFu)
+        return fail(GENIE_E_INVALID, "too many objects");
+    if (desc->key_start[0] != 0 || desc->key_start[desc->n_keys] != desc->n_postings)
+        return fail(GENIE_E_INVALID, "key_start must run from 0 to n_postings");
+    for (uint32_t k = 0; k < desc->n_keys; ++k)
+        if (desc->key_start[k] > desc->key_start[k + 1])
+            return fail(GENIE_E_INVALID, "key_start must be non-decreasing");
+    for (uint64_t p = 0; p < desc->n_postings; ++p)
+        if (desc->postings[p] >= desc->n_objects)
+            return fail(GENIE_E_INVALID, "posting id out of range at position " + std::to_string(p));
+
+    genie_index* ix = new genie_index();
+    DevIndex& d = ix->d;
+    d.n_objects = desc->n_objects;
+    d.n_dims = desc->n_dims;
+    d.n_keys = desc->n_keys;
+    d.n_postings = desc->n_postings;
+    d.codec = desc->codec;
+    d.single_value = desc->single_value;
+    d.n_dtiles = (desc->n_objects + kDirTile - 1) / kDirTile;
+    d.h_key_start.assign(desc->key_start, desc->key_start + desc->n_keys + 1);
+    d.h_key_dim.assign(desc->key_dim, desc->key_dim + desc->n_keys);
+    d.h_key_mult.assign(desc->n_keys, 1);
+    std::vector<uint32_t> ks32(desc->n_keys + 1), dir_off(desc->n_keys, kNoDir), dir_keys;
+    for (uint32_t k = 0; k <= desc->n_keys; ++k)
+        ks32[k] = (uint32_t)desc->key_start[k];
+    for (uint32_t k = 0; k < desc->n_keys; ++k) {
+        const uint64_t s = desc->key_start[k], e = desc->key_start[k + 1];
+        uint32_t best = e > s ? 1 : 0, run = 1;
+        for (uint64_t p = s + 1; p < e; ++p) {
+            if (desc->postings[p] < desc->postings[p - 1]) {
+                delete ix;
+                return fail(GENIE_E_INVALID, "posting list of key " + std::to_string(k) + " is not sorted");
+            }
+            run = desc->postings[p] == desc->postings[p - 1] ? run + 1 : 1;
+            best = std::max(best, run);
+        }
+        d.h_key_mult[k] = best;
+        if (e - s > kShortList) {
+            dir_off[k] = (uint32_t)(dir_keys.size() * (uint64_t)(d.n_dtiles + 1));
+            dir_keys.push_back(k);
+        }
+    }
+    if ((uint64_t)dir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull) {
+        delete ix;
+        return fail(GENIE_E_INVALID, "tile directory too large");
+    }
+    int rc = GENIE_OK;
+    uint32_t* d_dir_keys = nullptr;
+    do {
+        if ((rc = dev_alloc(&d.post, d.n_postings, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.key_start, d.n_keys + 1, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.dir_off, d.n_keys, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.dir, dir_keys.size() * (uint64_t)(d.n_dtiles + 1), &d.bytes))) break;
+        if ((rc = dev_alloc(&d_dir_keys, dir_keys.size()))) break;
+        hipError_t e;
+        if ((e = hipMemcpy(d.post, desc->postings, d.n_postings * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.key_start, ks32.data(), ks32.size() * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.dir_off, dir_off.data(), dir_off.size() * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_dir_keys, dir_keys.data(), dir_keys.size() * 4, hipMemcpyHostToDevice))) {
+            rc = fail(GENIE_E_RUNTIME, std::string("index upload: ") + hipGetErrorString(e));
+            break;
+        }
+        if (!dir_keys.empty()) {
+            uint64_t total = dir_keys.size() * (uint64_t)(d.n_dtiles + 1);
+            uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+            k_build_dir<<<blocks, 256>>>(d.post, d.key_start, d_dir_keys, (uint32_t)dir_keys.size(),
+                                         d.n_dtiles, d.dir);
+            if ((e = hipGetLastError()) || (e = hipDeviceSynchronize())) {
+                rc = fail(GENIE_E_RUNTIME, std::string("directory build: ") + hipGetErrorString(e));
+                break;
+            }
+        }
+    } while (0);
+    dev_free(d_dir_keys);
+    if (rc) {
+        genie_hip_index_destroy(ix);
+        return rc;
+    }
+    *out = ix;
+    return GENIE_OK;
+}
+
+int genie_hip_index_destroy(genie_index* ix)
+{
+    if (!ix)
+        return GENIE_OK;
+    DevIndex& d = ix->d;
+    dev_free(d.post);
+    dev_free(d.key_start);
+    dev_free(d.dir_off);
+    dev_free(d.dir);
+    dev_free(d.words);
+    dev_free(d.seg_cpos);
+    dev_free(d.seg_upos);
+    dev_free(d.seg_key);
+    dev_free(d.seg_first);
+    dev_free(d.seg_last);
+    dev_free(d.key_seg);
+    dev_free(d.sdir);
+    delete ix;
+    return GENIE_OK;
+}
+
+uint64_t genie_hip_index_bytes(const genie_index* ix) { return ix ? ix->d.bytes : 0; }
+
+int genie_hip_count_bounds(const genie_index* ix, const genie_dim_t* items, uint64_t n_items,
+                           uint32_t n_queries, uint32_t* out_bounds)
+{
+    if (!ix || (!items && n_items) || !out_bounds)
+        return fail(GENIE_E_INVALID, "null argument");
+    int rc = validate_items(ix->d, items, n_items, n_queries);
+    if (rc)
+        return rc;
+    return compute_bounds(ix->d, items, n_items, n_queries, out_bounds);
+}
+
+int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_items, uint32_t max_k,
+                          uint32_t count_bound, int mode, genie_plan** out)
+{
+    if (!ix || !out)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (max_queries == 0 || max_k == 0 || max_k > GENIE_MAX_K)
+        return fail(GENIE_E_INVALID, "max_queries must be > 0 and max_k in [1, GENIE_MAX_K]");
+    if (mode != 0)
+        return fail(GENIE_E_INVALID, "unknown plan mode");
+    genie_plan* p = new genie_plan();
+    p->ix = ix;
+    p->mode = mode;
+    p->max_queries = max_queries;
+    p->max_items = max_items;
+    p->max_k = max_k;
+    p->count_bound = count_bound;
+    p->lane_bits = bits_for(count_bound);
+    int rc = GENIE_OK;
+    uint64_t nb = (max_items + 1023) / 1024 + 1;
+    do {
+        if ((rc = dev_alloc(&p->d_items, max_items))) break;
+        if ((rc = dev_alloc(&p->d_item_qptr, max_queries + 1))) break;
+        if ((rc = dev_alloc(&p->d_nchunks, max_items))) break;
+        if ((rc = dev_alloc(&p->d_chunk_ptr, max_items))) break;
+        if ((rc = dev_alloc(&p->d_block_sums, nb))) break;
+        if ((rc = dev_alloc(&p->d_total, 1))) break;
+        if ((rc = dev_alloc(&p->d_qc_ptr, max_queries + 1))) break;
+        if ((rc = dev_alloc(&p->d_kq, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_gthr, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_cand_cnt, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_cand_cap, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_cand_base, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_overflow, 1))) break;
+        if ((rc = dev_alloc(&p->d_out_off, max_queries + 1))) break;
+        for (auto& e : p->ev)
+            if (hipEventCreate(&e) != hipSuccess) {
+                rc = fail(GENIE_E_RUNTIME, "hipEventCreate");
+                break;
+            }
+    } while (0);
+    if (rc) {
+        genie_hip_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_destroy(genie_plan* p)
+{
+    if (!p)
+        return GENIE_OK;
+    plan_free_buffers(p);
+    for (auto& e : p->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    delete p;
+    return GENIE_OK;
+}
+
+// Candidate capacity per query: cap_factor tiles' worth of k, at most all tiles.
+static int plan_size_candidates(genie_plan* p)
+{
+    const DevIndex& d = p->ix->d;
+    const uint32_t n_tiles = (d.n_objects + tile_objs_for(p->lane_bits) - 1) / tile_objs_for(p->lane_bits);
+    const uint32_t f = std::min<uint32_t>(p->cap_factor, n_tiles);
+    std::vector<uint32_t> cap(p->n_queries);
+    std::vector<uint64_t> base(p->n_queries);
+    uint64_t tot = 0;
+    for (uint32_t q = 0; q < p->n_queries; ++q) {
+        cap[q] = f * p->h_kq[q];
+        base[q] = tot;
+        tot += cap[q];
+    }
+    if (tot > p->cand_alloc) {
+        dev_free(p->d_cand);
+        p->cand_alloc = 0;
+        if (int rc = dev_alloc(&p->d_cand, tot))
+            return rc;
+        p->cand_alloc = tot;
+    }
+    p->total_cap = tot;
+    HIP_TRY(hipMemcpy(p->d_cand_cap, cap.data(), cap.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_cand_base, base.data(), base.size() * 8, hipMemcpyHostToDevice));
+    return GENIE_OK;
+}
+
+int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                          const uint32_t* k_per_query)
+{
+    if (!p || (!items && n_items) || !k_per_query)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (n_queries == 0 || n_queries > p->max_queries || n_items > p->max_items)
+        return fail(GENIE_E_INVALID, "batch larger than the plan");
+    const DevIndex& d = p->ix->d;
+    if (int rc = validate_items(d, items, n_items, n_queries))
+        return rc;
+    for (uint32_t q = 0; q < n_queries; ++q)
+        if (k_per_query[q] == 0 || k_per_query[q] > p->max_k)
+            return fail(GENIE_E_INVALID, "k of query " + std::to_string(q) + " outside [1, max_k]");
+    // items grouped by query (stable), CSR pointer
+    std::vector<uint32_t> qptr(n_queries + 1, 0);
+    for (uint64_t i = 0; i < n_items; ++i)
+        qptr[items[i].query + 1]++;
+    for (uint32_t q = 0; q < n_queries; ++q)
+        qptr[q + 1] += qptr[q];
+    std::vector<genie_dim_t> sorted(n_items);
+    {
+        std::vector<uint32_t> fillp(qptr.begin(), qptr.end() - 1);
+        for (uint64_t i = 0; i < n_items; ++i)
+            sorted[fillp[items[i].query]++] = items[i];
+    }
+    // count bound of this batch
+    if (p->count_bound == 0) {
+        std::vector<uint32_t> b(n_queries);
+        compute_bounds(d, sorted.data(), n_items, n_queries, b.data());
+        uint32_t mx = 0;
+        for (uint32_t v : b)
+            mx = std::max(mx, v);
+        p->lane_bits = bits_for(mx);
+    }
+    // chunks and algorithmic bytes
+    uint64_t nch = 0, logical = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> spans;
+    spans.reserve(n_items);
+    for (uint64_t i = 0; i < n_items; ++i) {
+        const genie_dim_t& it = sorted[i];
+        if (it.end_pos > it.start_pos) {
+            uint64_t len = (uint64_t)(it.end_pos - it.start_pos);
+            nch += (len + kChunk - 1) / kChunk;
+            logical += 4 * len;
+            spans.push_back({(uint32_t)it.start_pos, (uint32_t)it.end_pos});
+        }
+    }
+    std::sort(spans.begin(), spans.end());
+    uint64_t compulsory = 0, cur_e = 0;
+    for (auto& s : spans) {  // union of position ranges
+        uint64_t a = std::max<uint64_t>(s.first, cur_e);
+        if (s.second > a)
+            compulsory += 4 * (s.second - a);
+        cur_e = std::max<uint64_t>(cur_e, s.second);
+    }
+    p->logical_bytes = logical;
+    p->compulsory_bytes = compulsory;
+    if (nch > p->chunks_cap) {
+        dev_free(p->d_chunks);
+        p->chunks_cap = 0;
+        if (int rc = dev_alloc(&p->d_chunks, nch))
+            return rc;
+        p->chunks_cap = nch;
+    }
+    p->n_queries = n_queries;
+    p->n_items = n_items;
+    p->n_chunks = nch;
+    p->h_kq.assign(k_per_query, k_per_query + n_queries);
+    p->h_out_off.assign(n_queries + 1, 0);
+    for (uint32_t q = 0; q < n_queries; ++q)
+        p->h_out_off[q + 1] = p->h_out_off[q] + k_per_query[q];
+    p->total_k = p->h_out_off[n_queries];
+    if (p->total_k > p->out_alloc) {
+        dev_free(p->d_out_ids);
+        dev_free(p->d_out_counts);
+        p->out_alloc = 0;
+        if (int rc = dev_alloc(&p->d_out_ids, p->total_k))
+            return rc;
+        if (int rc = dev_alloc(&p->d_out_counts, p->total_k))
+            return rc;
+        p->out_alloc = p->total_k;
+    }
+    if (n_items)
+        HIP_TRY(hipMemcpy(p->d_items, sorted.data(), n_items * sizeof(genie_dim_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_item_qptr, qptr.data(), qptr.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_kq, k_per_query, n_queries * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_out_off, p->h_out_off.data(), p->h_out_off.size() * 8, hipMemcpyHostToDevice));
+    p->cap_factor = 8;
+    return plan_size_candidates(p);
+}
+
+static int launch_scan(genie_plan* p, hipStream_t s)
+{
+    const DevIndex& d = p->ix->d;
+    ScanArgs a;
+    a.post = d.post;
+    a.key_start = d.key_start;
+    a.dir_off = d.dir_off;
+    a.dir = d.dir;
+    a.chunks = p->d_chunks;
+    a.qc_ptr = p->d_qc_ptr;
+    a.kq = p->d_kq;
+    a.n_objects = d.n_objects;
+    a.n_queries = p->n_queries;
+    a.tile_objs = tile_objs_for(p->lane_bits);
+    a.n_tiles = (d.n_objects + a.tile_objs - 1) / a.tile_objs;
+    a.dt_per_tile = a.tile_objs / kDirTile;
+    a.n_dtiles = d.n_dtiles;
+    a.gthr = p->d_gthr;
+    a.cand_cnt = p->d_cand_cnt;
+    a.cand_cap = p->d_cand_cap;
+    a.cand_base = p->d_cand_base;
+    a.cand = p->d_cand;
+    a.overflow = p->d_overflow;
+    const uint64_t blocks = (uint64_t)a.n_tiles * p->n_queries;
+    if (blocks >= 0x7FFFFFFFull)
+        return fail(GENIE_E_INVALID, "batch too large for one launch");
+    switch (p->lane_bits) {
+    case 8: k_scan_tile<8><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+    case 16: k_scan_tile<16><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+    default: k_scan_tile<32><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+    }
+    return GENIE_OK;
+}
+
+static int plan_run_once(genie_plan* p, hipStream_t s)
+{
+    const DevIndex& d = p->ix->d;
+    const uint32_t nq = p->n_queries;
+    const uint32_t ni = (uint32_t)p->n_items;
+    if (p->timing)
+        HIP_TRY(hipEventRecord(p->ev[0], s));
+    fill_u32(p->d_gthr, 0, nq, s);
+    fill_u32(p->d_cand_cnt, 0, nq, s);
+    fill_u32(p->d_overflow, 0, 1, s);
+    // compile items into chunks
+    if (ni) {
+        k_chunk_count<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, p->d_nchunks);
+        const uint32_t nb = (ni + 1023) / 1024;
+        k_scan_blocks<<<nb, 256, 0, s>>>(p->d_nchunks, p->d_chunk_ptr, ni, p->d_block_sums);
+        k_scan_sums<<<1, 256, 0, s>>>(p->d_block_sums, nb, p->d_total);
+        k_scan_add<<<nb, 256, 0, s>>>(p->d_chunk_ptr, ni, p->d_block_sums);
+        k_chunk_write<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, p->d_chunk_ptr, d.key_start,
+                                                      d.n_keys, p->d_chunks);
+    } else {
+        fill_u32(p->d_total, 0, 1, s);
+    }
+    k_query_chunk_ptr<<<(nq + 1 + 255) / 256, 256, 0, s>>>(p->d_item_qptr, nq, p->d_chunk_ptr, ni,
+                                                          p->d_total, p->d_qc_ptr);
+    if (p->timing)
+        HIP_TRY(hipEventRecord(p->ev[1], s));
+    if (int rc = launch_scan(p, s))
+        return rc;
+    if (p->timing)
+        HIP_TRY(hipEventRecord(p->ev[2], s));
+    MergeArgs m;
+    m.cand = p->d_cand;
+    m.cand_base = p->d_cand_base;
+    m.cand_cnt = p->d_cand_cnt;
+    m.cand_cap = p->d_cand_cap;
+    m.kq = p->d_kq;
+    m.out_off = p->d_out_off;
+    m.out_ids = p->d_out_ids;
+    m.out_counts = p->d_out_counts;
+    m.key_bits = 32 + p->lane_bits;
+    k_merge<<<nq, kMergeThreads, 0, s>>>(m);
+    if (p->timing)
+        HIP_TRY(hipEventRecord(p->ev[3], s));
+    HIP_TRY(hipGetLastError());
+    p->ran = true;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_run(genie_plan* p, void* stream)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (p->n_queries == 0)
+        return fail(GENIE_E_NOT_BUILT, "no batch uploaded");
+    return plan_run_once(p, (hipStream_t)stream);
+}
+
+int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts)
+{
+    if (!p || !out_ids || !out_counts)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (!p->ran)
+        return fail(GENIE_E_NOT_BUILT, "plan has not run");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(s));
+    // candidate overflow: grow the capacity and run again (the reference
+    // retries on hash-table overflow in the same way, match.cu:356-421)
+    for (int attempt = 0;; ++attempt) {
+        uint32_t ov = 0;
+        HIP_TRY(hipMemcpy(&ov, p->d_overflow, 4, hipMemcpyDeviceToHost));
+        if (!ov)
+            break;
+        const DevIndex& d = p->ix->d;
+        const uint32_t n_tiles = (d.n_objects + tile_objs_for(p->lane_bits) - 1) / tile_objs_for(p->lane_bits);
+        if (p->cap_factor >= n_tiles || attempt > 8)
+            return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
+        p->cap_factor = std::min<uint32_t>(p->cap_factor * 4, n_tiles);
+        if (int rc = plan_size_candidates(p))
+            return rc;
+        if (int rc = plan_run_once(p, s))
+            return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    if (p->timing) {
+        float a = 0, b = 0, c = 0;
+        HIP_TRY(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
+        HIP_TRY(hipEventElapsedTime(&c, p->ev[2], p->ev[3]));
+        p->ms_compile = a;
+        p->ms_scan = b;
+        p->ms_merge = c;
+    }
+    HIP_TRY(hipMemcpy(out_ids, p->d_out_ids, p->total_k * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(out_counts, p->d_out_counts, p->total_k * 4, hipMemcpyDeviceToHost));
+    return GENIE_OK;
+}
+
+int genie_hip_plan_set_timing(genie_plan* p, int on)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    p->timing = on != 0;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms)
+{
+    if (!p || !name || !ms)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (p->timing && p->ran) {
+        // refresh from the events of the last run
+        float v = 0;
+        if (!strcmp(name, "compile") && hipEventElapsedTime(&v, p->ev[0], p->ev[1]) == hipSuccess) { *ms = v; return GENIE_OK; }
+        if (!strcmp(name, "scan") && hipEventElapsedTime(&v, p->ev[1], p->ev[2]) == hipSuccess) { *ms = v; return GENIE_OK; }
+        if (!strcmp(name, "merge") && hipEventElapsedTime(&v, p->ev[2], p->ev[3]) == hipSuccess) { *ms = v; return GENIE_OK; }
+    }
+    return fail(GENIE_E_INVALID, std::string("no timing for kernel ") + name);
+}
+
+int genie_hip_plan_bytes(genie_plan* p, uint64_t* logical, uint64_t* compulsory)
+{
+    if (!p || !logical || !compulsory)
+        return fail(GENIE_E_INVALID, "null argument");
+    *logical = p->logical_bytes;
+    *compulsory = p->compulsory_bytes;
+    return GENIE_OK;
+}
+
+int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                         const uint32_t* k_per_query, uint32_t count_bound, int32_t* out_ids,
+                         int32_t* out_counts)
+{
+    if (!ix || (!items && n_items) || !k_per_query || !out_ids || !out_counts)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (n_queries == 0)
+        return GENIE_OK;
+    uint32_t max_k = 0;
+    for (uint32_t q = 0; q < n_queries; ++q)
+        max_k = std::max(max_k, k_per_query[q]);
+    if (max_k == 0 || max_k > GENIE_MAX_K)
+        return fail(GENIE_E_INVALID, "k outside [1, GENIE_MAX_K]");
+    genie_plan* p = nullptr;
+    int rc = genie_hip_plan_create(ix, n_queries, n_items, max_k, count_bound, 0, &p);
+    if (rc)
+        return rc;
+    if (!(rc = genie_hip_plan_upload(p, items, n_items, n_queries, k_per_query)) &&
+        !(rc = genie_hip_plan_run(p, nullptr)))
+        rc = genie_hip_plan_download(p, nullptr, out_ids, out_counts);
+    genie_hip_plan_destroy(p);
+    return rc;
+}
+
+int genie_hip_decode(int codec, const uint32_t* words, const uint32_t* cpos, uint32_t n_arrays,
+                     uint32_t out_stride, uint32_t* out, uint32_t* out_len)
+{
+    (void)codec; (void)words; (void)cpos; (void)n_arrays; (void)out_stride; (void)out; (void)out_len;
+    return fail(GENIE_E_INVALID, "device decoders are not part of this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+/*
+ * genie_hip.h -- C ABI of the MI355X match-count top-k engine.
+ *
+ * Everything below GENIE's matching::knn (src/genie/matching/knn.cu:106-183)
+ * is replaced by these entry points: the host library (genie_amd/csrc/host,
+ * which keeps GENIE's table-build / query / top-k C++ API) calls only this
+ * header.  Plain pointers and sizes, int status codes, no C++ or torch types.
+ *
+ * Status codes: 0 ok; GENIE_E_RUNTIME (HIP error), GENIE_E_OOM (device
+ * allocation failed), GENIE_E_INVALID (bad argument), GENIE_E_NOT_BUILT
+ * (index handle empty / wrong kind).  genie_hip_last_error() returns a
+ * thread-local message for the last failure on the calling thread.
+ * The host library maps them onto the reference's exception classes
+ * (src/genie/exception/exception.h:10-46).
+ */
+#ifndef GENIE_HIP_H_
+#define GENIE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GENIE_OK 0
+#define GENIE_E_RUNTIME 1
+#define GENIE_E_OOM 2
+#define GENIE_E_INVALID 3
+#define GENIE_E_NOT_BUILT 4
+
+/* Largest k per query any entry point accepts. */
+#define GENIE_MAX_K 4096
+
+/* Work item, byte-for-byte the reference's Query::dim (src/genie/query/
+ * query.h:59-68): 20 bytes, [start_pos, end_pos) into the index's flat
+ * posting array (uncompressed) or compressed word array (compressed). */
+typedef struct genie_dim_t {
+    int32_t query;
+    int32_t order;
+    int32_t start_pos;
+    int32_t end_pos;
+    float weight;
+} genie_dim_t;
+
+/* Description of a built index.  Mirrors what inv_table / inv_compr_table
+ * hold after build() (src/genie/table/inv_table.h:110-174,
+ * inv_compr_table.cu:31-119), re-expressed with a dense key numbering:
+ * key j of dim d is key_base[d] + (value - lo[d]); key_start[j] is the
+ * position in the uncompressed posting array where that key's ids begin
+ * (n_keys + 1 entries, last = n_postings).  Segments are the _inv_pos splits
+ * (seg_upos, n_segments + 1 entries); with codec != 0 the words array holds
+ * each segment compressed independently and seg_cpos (n_segments + 1) gives
+ * their word offsets -- the positions compressed work items refer to. */
+typedef struct genie_index_desc {
+    uint32_t n_objects;          /* inv_table::i_size() */
+    uint32_t n_dims;             /* inv_table::m_size() */
+    uint32_t n_keys;
+    uint64_t n_postings;
+    const uint32_t* postings;    /* [n_postings] ids, key-major, ascending per key (may be NULL when codec != 0) */
+    const uint64_t* key_start;   /* [n_keys + 1] */
+    const uint32_t* key_dim;     /* [n_keys] dim of each key */
+    uint32_t n_segments;
+    const uint32_t* seg_upos;    /* [n_segments + 1] uncompressed positions (_inv_pos) */
+    int32_t codec;               /* COMPRESSION_TYPE (DeviceCodecFactory.h:38-53); 0 = uncompressed */
+    uint64_t n_words;
+    const uint32_t* words;       /* [n_words] compressed segments (codec != 0) */
+    const uint32_t* seg_cpos;    /* [n_segments + 1] word offsets (codec != 0) */
+    int32_t single_value;        /* 1: every object holds exactly one value per dim */
+} genie_index_desc;
+
+typedef struct genie_index genie_index;
+
+/* Select the device (Init, src/genie/utility/init.cu:7-34; an out-of-range id
+ * falls back to device 0 as the reference does). */
+int genie_hip_init(int device);
+
+/* Upload an index and keep it resident in HBM (inv_table::cpy_data_to_gpu,
+ * inv_table.cu:30-45; inv_compr_table::cpy_data_to_gpu, inv_compr_table.cu:
+ * 211-223), then build the per-key tile directory on the device. */
+int genie_hip_index_create(const genie_index_desc* desc, genie_index** out);
+int genie_hip_index_destroy(genie_index* ix);
+/* Bytes of HBM the handle holds. */
+uint64_t genie_hip_index_bytes(const genie_index* ix);
+
+/* Upper bound on any query's count, from the items (weights must be
+ * non-negative integers).  out_bounds[q] for q < n_queries. */
+int genie_hip_count_bounds(const genie_index* ix, const genie_dim_t* items, uint64_t n_items,
+                           uint32_t n_queries, uint32_t* out_bounds);
+
+/* Plain per-batch search: replaces knn() -> match()/match_integrated() ->
+ * heap_count_topk() -> extract_index_and_count() (knn.cu:106-183,
+ * match.cu:224-464, heap_count.cu:161-282, knn.cu:48-55).  items in HOST
+ * memory (any order; item.query in [0, n_queries)); k_per_query[q] in
+ * [1, GENIE_MAX_K]; count_bound 0 = derive from the items.  Output is
+ * query-major with out_offsets[q] = sum of k over earlier queries, k_q slots
+ * per query, ordered count descending then id ascending, padded with
+ * (id 0, count 0) (heap_count.cu:254-255).  Synchronous. */
+int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_items,
+                         uint32_t n_queries, const uint32_t* k_per_query, uint32_t count_bound,
+                         int32_t* out_ids, int32_t* out_counts);
+
+/* ---- device-resident form (inputs already in HBM) ---------------------- */
+/* A plan holds device scratch for batches up to the given shape, so the hot
+ * call allocates nothing.  mode: 0 plain (one query per workgroup),
+ * 1 batched (queries packed into count lanes, persistent work queue). */
+typedef struct genie_plan genie_plan;
+int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_items,
+                          uint32_t max_k, uint32_t count_bound, int mode, genie_plan** out);
+int genie_hip_plan_destroy(genie_plan* p);
+/* Copy a batch (host arrays) into the plan's device buffers. */
+int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items,
+                          uint32_t n_queries, const uint32_t* k_per_query);
+/* Run the search on the uploaded batch on `stream` (a hipStream_t, NULL =
+ * default stream); asynchronous. */
+int genie_hip_plan_run(genie_plan* p, void* stream);
+/* Copy the results of the last run to host buffers (synchronises stream). */
+int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts);
+/* Per-kernel timing of the last run when timing is on (HIP events on the
+ * plan's stream): names are "scan", "select", "merge"; ms is the total
+ * duration of that kernel's launches in the run. */
+int genie_hip_plan_set_timing(genie_plan* p, int on);
+int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms);
+/* Algorithmic bytes of the last uploaded batch: logical = 4 B x sum of
+ * item span lengths (uncompressed) or words (compressed); compulsory =
+ * the same over distinct keys/segments touched, counted once. */
+int genie_hip_plan_bytes(genie_plan* p, uint64_t* logical, uint64_t* compulsory);
+
+/* ---- codec decode (tests and measurement) ------------------------------ */
+/* Decode n_arrays compressed arrays ([cpos[i], cpos[i+1]) words of `words`)
+ * on the device with the engine's wave decoders; output i goes to
+ * out[i * out_stride ...], its length to out_len[i].  Host buffers. */
+int genie_hip_decode(int codec, const uint32_t* words, const uint32_t* cpos, uint32_t n_arrays,
+                     uint32_t out_stride, uint32_t* out, uint32_t* out_len);
+
+const char* genie_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

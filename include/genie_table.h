@@ -1,0 +1,71 @@
+/*
+ * genie_table.h -- C entry points over the C++ host library (table build,
+ * query compilation, legacy knn_search), for callers without C++ (the Python
+ * tests and benchmark use it through ctypes).  They wrap exactly the C++ API
+ * of include/genie/ and report its exceptions as status codes:
+ * 0 ok, GENIE_E_* of genie_hip.h otherwise, message in genie_table_last_error().
+ */
+#ifndef GENIE_TABLE_C_H_
+#define GENIE_TABLE_C_H_
+
+#include <stdint.h>
+
+#include "genie_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct genie_table genie_table;
+
+/* Query range (Query::attr arguments, src/genie/query/query.h:43-54). */
+typedef struct genie_range_t {
+    int32_t query;
+    int32_t dim;
+    int32_t low;
+    int32_t up;
+    float weight;
+    int32_t order;
+} genie_range_t;
+
+/* Single-value table from row-major rows [n][d].  via_lists = 1 builds it
+ * through inv_list::invert + inv_table::append + build (the GENIE path);
+ * 0 uses the direct row builder (same layout).  codec 0 = uncompressed,
+ * otherwise an inv_compr_table with that COMPRESSION_TYPE. */
+int genie_table_build_rows(const int32_t* rows, uint32_t n, uint32_t d, uint32_t max_length, int use_load_balance,
+                           int codec, int via_lists, genie_table** out);
+/* bijectMap table from CSR set rows. */
+int genie_table_build_sets(const int64_t* row_ptr, const int32_t* vals, uint32_t n, uint32_t max_length,
+                           int use_load_balance, int codec, int via_lists, genie_table** out);
+void genie_table_destroy(genie_table* t);
+
+int genie_table_sizes(genie_table* t, uint64_t* n_post, uint64_t* n_seg, uint64_t* n_words, uint32_t* n_dims,
+                      uint32_t* n_objects, uint64_t* n_keyseg);
+/* Export the layout: inv [n_post], inv_pos [n_seg+1] (uncompressed), words
+ * [n_words] and cpos [n_seg+1] (compressed tables), per-dim bounds, and the
+ * _inv_index_map entries of every dim for values lo..hi+1 concatenated
+ * (n_keyseg entries).  Any pointer may be NULL. */
+int genie_table_export(genie_table* t, int32_t* inv, int32_t* inv_pos, uint32_t* words, int32_t* cpos, int32_t* lo,
+                       int32_t* hi, int32_t* keyseg);
+
+/* Compile ranges of n_queries queries (range.query in [0, n_queries)) with
+ * Query::attr then build / build_and_apply_load_balance / build_compressed
+ * and dump, query by query (match.cu:161-210). */
+int genie_table_compile(genie_table* t, const genie_range_t* ranges, uint64_t n_ranges, uint32_t n_queries,
+                        int use_load_balance, int32_t max_load, genie_dim_t* out, uint64_t cap, uint64_t* n_out);
+
+/* Engine handle of the table (cpy_data_to_gpu); owned by the table. */
+int genie_table_device(genie_table* t, genie_index** out);
+
+/* Legacy search: queries from ranges, genie::original::knn_search(table,
+ * queries, ids, counts, config); k slots per query. */
+int genie_table_knn(genie_table* t, const genie_range_t* ranges, uint64_t n_ranges, uint32_t n_queries, uint32_t k,
+                    int use_load_balance, int32_t max_load, int32_t* out_ids, int32_t* out_counts);
+
+const char* genie_table_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

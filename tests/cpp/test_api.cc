@@ -1,0 +1,294 @@
+// The reference's own API tests, restated against this build's C++ API
+// (test/test_csv_input_sift_data.cc, test_refactoring_new_interface.cc,
+// test_csv_input_tweets.cc, test_multiple_search.cc, test_hashtable_overflow.cc,
+// test_csv_input_multiload.cc, test_csv_multiload_tweets.cc).  Counts are
+// the exact ones (the reference's 16s on tweets_20 are wrapped 4-bit
+// counters).  Usage: test_api <golden dir> [test name]
+#include <genie/genie.h>
+#include <genie/original/interface.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace genie::original;
+using genie::table::inv_table;
+using std::string;
+using std::vector;
+
+static string G;
+static int failures = 0;
+
+#define CHECK(c)                                                                 \
+    do {                                                                         \
+        if (!(c)) {                                                              \
+            std::printf("  CHECK FAILED %s:%d: %s\n", __FILE__, __LINE__, #c);   \
+            ++failures;                                                          \
+        }                                                                        \
+    } while (0)
+
+static GPUGenie_Config base_config(vector<vector<int> >& data, vector<vector<int> >& queries, int dim, int k)
+{
+    GPUGenie_Config config;
+    config.dim = dim;
+    config.count_threshold = 14;
+    config.num_of_topk = k;
+    config.hashtable_size = 14 * k * 1.5f;
+    config.query_radius = 0;
+    config.use_device = 0;
+    config.use_adaptive_range = false;
+    config.selectivity = 0.0f;
+    config.query_points = &queries;
+    config.data_points = &data;
+    config.use_load_balance = false;
+    config.posting_list_max_length = 6400;
+    config.multiplier = 1.5f;
+    config.use_multirange = false;
+    config.data_type = 0;
+    config.search_type = 0;
+    config.max_data_size = 0;
+    config.num_of_queries = 3;
+    return config;
+}
+
+static void test_sift_legacy()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 5, 5);
+    genie::utility::read_file(data, (G + "/sift_20.csv").c_str(), -1);
+    genie::utility::read_file(queries, (G + "/sift_20.csv").c_str(), config.num_of_queries);
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    vector<int>& inv = *table[0].inv();
+    const int want[6] = {8, 9, 7, 0, 2, 4};
+    for (int i = 0; i < 6; ++i)
+        CHECK(inv[i] == want[i]);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    CHECK(result[0] == 0 && count[0] == 5);
+    CHECK(result[1] == 4 && count[1] == 2);
+    CHECK(result[5] == 1 && count[5] == 5);
+    CHECK(result[10] == 2 && count[10] == 5);
+    delete[] table;
+}
+
+static void test_sift_new_interface()
+{
+    genie::Config config = genie::Config().SetK(5).SetNumOfQueries(5);
+    std::shared_ptr<genie::ExecutionPolicy> policy = genie::MakePolicy(config);
+    genie::SearchResult r = genie::Search(policy, G + "/sift_20.csv", G + "/sift_20.csv");
+    CHECK(r.first.size() == 25);
+    CHECK(r.first[0] == 0 && r.second[0] == 5);
+    CHECK(r.first[1] == 4 && r.second[1] == 2);
+    CHECK(r.first[5] == 1 && r.second[5] == 5);
+    CHECK(r.first[10] == 2 && r.second[10] == 5);
+    // 2nd-level interface with a range policy
+    genie::Config c2 = genie::Config().SetK(5).SetNumOfQueries(3).SetQueryRange(1);
+    std::shared_ptr<genie::ExecutionPolicy> p2 = genie::MakePolicy(c2);
+    genie::TableData td = genie::LoadTableDataFromCsv(G + "/sift_20.csv");
+    genie::QueryData qd = genie::LoadQueryDataFromCsv(G + "/sift_20.csv", p2);
+    auto table = genie::BuildTable(p2, td);
+    auto qs = genie::BuildQuery(p2, table, qd);
+    genie::SearchResult r2 = genie::Match(p2, table, qs);
+    CHECK(r2.first.size() == 15);
+    CHECK(r2.second[0] == 5 && r2.first[0] == 0);  // a row always matches itself in every dim
+}
+
+static void test_invalid_config()
+{
+    bool threw = false;
+    try {
+        genie::MakePolicy(genie::Config().SetNumOfQueries(3));
+    } catch (genie::exception::InvalidConfigurationException&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        genie::MakePolicy(genie::Config().SetK(0).SetNumOfQueries(3));
+    } catch (genie::exception::InvalidConfigurationException&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        genie::Config().LoadFromFile("x");
+    } catch (genie::exception::NotImplementedException&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+static void test_tweets()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 14, 5);
+    config.hashtable_size = 100 * 5 * 1.5f;
+    config.search_type = 1;
+    genie::utility::read_file(data, (G + "/tweets_20.csv").c_str(), -1);
+    genie::utility::read_file(queries, (G + "/tweets_20.csv").c_str(), config.num_of_queries);
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    vector<int>& inv = *table[0].inv();
+    const int want[6] = {0, 11, 0, 11, 0, 11};
+    for (int i = 0; i < 6; ++i)
+        CHECK(inv[i] == want[i]);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    CHECK((result[0] == 0 && result[1] == 11) || (result[0] == 11 && result[1] == 0));
+    CHECK(count[0] == 17 && count[1] == 17);
+    CHECK(result[5] == 1 && count[5] == 14);
+    CHECK(result[10] == 2 && count[10] == 18);
+    delete[] table;
+}
+
+static void test_multiple_search()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 5, 5);
+    genie::utility::read_file(data, (G + "/sift_20.csv").c_str(), -1);
+    genie::utility::read_file(queries, (G + "/sift_20.csv").c_str(), config.num_of_queries);
+    config.save_to_gpu = true;
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    CHECK(table[0].is_stored_in_gpu);
+    vector<int> r1, c1, r2, c2;
+    knn_search_after_preprocess(config, table, r1, c1);
+    knn_search_after_preprocess(config, table, r2, c2);
+    CHECK(r1 == r2 && c1 == c2);
+    CHECK(r1[0] == 0 && c1[0] == 5);
+    delete[] table;
+}
+
+static void test_hashtable_overflow()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 14, 10);
+    config.hashtable_size = 3;  // the reference's overflow-retry trigger
+    config.search_type = 1;
+    config.num_of_queries = 20;
+    genie::utility::read_file(data, (G + "/tweets_20.csv").c_str(), -1);
+    genie::utility::read_file(queries, (G + "/tweets_20.csv").c_str(), config.num_of_queries);
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    CHECK(result.size() == 200);
+    // every row matches itself with its full length
+    for (int q = 0; q < 20; ++q)
+        CHECK(count[(size_t)q * 10] >= (int)data[(size_t)q].size());
+    delete[] table;
+}
+
+static void test_multiload_tweets()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 14, 5);
+    config.hashtable_size = 100 * 5 * 1.5f;
+    config.search_type = 1;
+    config.max_data_size = 10;
+    genie::utility::read_file(data, (G + "/tweets_20.csv").c_str(), -1);
+    genie::utility::read_file(queries, (G + "/tweets_20.csv").c_str(), config.num_of_queries);
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    CHECK(table[0].get_total_num_of_table() == 2);
+    vector<int>& inv = *table[0].inv();
+    const int want[6] = {0, 0, 0, 0, 0, 7};
+    for (int i = 0; i < 6; ++i)
+        CHECK(inv[i] == want[i]);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    CHECK((result[0] == 0 && result[1] == 11) || (result[0] == 11 && result[1] == 0));
+    CHECK(count[0] == 17 && count[1] == 17);
+    CHECK(result[5] == 1 && count[5] == 14);
+    CHECK(result[10] == 2 && count[10] == 18);
+    delete[] table;
+}
+
+static void test_multiload_sift()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 5, 5);
+    config.max_data_size = 7;  // 20 rows -> tables of 7, 6, 7
+    genie::utility::read_file(data, (G + "/sift_20.csv").c_str(), -1);
+    genie::utility::read_file(queries, (G + "/sift_20.csv").c_str(), config.num_of_queries);
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    CHECK(table[0].get_total_num_of_table() == 3);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    CHECK(result[0] == 0 && count[0] == 5);
+    CHECK(result[1] == 4 && count[1] == 2);
+    CHECK(result[5] == 1 && count[5] == 5);
+    CHECK(result[10] == 2 && count[10] == 5);
+    delete[] table;
+}
+
+static void test_compressed_sift()
+{
+    // compressed tables need load-balanced segments of <= 1024 ids
+    for (int codec = 1; codec <= 13; ++codec) {
+        vector<vector<int> > data, queries;
+        GPUGenie_Config config = base_config(data, queries, 5, 5);
+        config.use_load_balance = true;
+        config.posting_list_max_length = 1024;
+        config.compression = (genie::compression::COMPRESSION_TYPE)codec;
+        genie::utility::read_file(data, (G + "/sift_20.csv").c_str(), -1);
+        genie::utility::read_file(queries, (G + "/sift_20.csv").c_str(), config.num_of_queries);
+        init_genie(config);
+        inv_table* table = nullptr;
+        preprocess_for_knn_csv(config, table);
+        vector<int> result, count;
+        knn_search_after_preprocess(config, table, result, count);
+        CHECK(result[0] == 0 && count[0] == 5);
+        CHECK(result[1] == 4 && count[1] == 2);
+        CHECK(result[5] == 1 && count[5] == 5);
+        CHECK(result[10] == 2 && count[10] == 5);
+        delete[] table;
+    }
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 2) {
+        std::printf("usage: %s <golden dir> [test]\n", argv[0]);
+        return 2;
+    }
+    G = argv[1];
+    std::map<string, std::function<void()> > tests = {
+        {"sift_legacy", test_sift_legacy},
+        {"sift_new_interface", test_sift_new_interface},
+        {"invalid_config", test_invalid_config},
+        {"tweets", test_tweets},
+        {"multiple_search", test_multiple_search},
+        {"hashtable_overflow", test_hashtable_overflow},
+        {"multiload_tweets", test_multiload_tweets},
+        {"multiload_sift", test_multiload_sift},
+        {"compressed_sift", test_compressed_sift},
+    };
+    for (auto& kv : tests) {
+        if (argc > 2 && kv.first != argv[2])
+            continue;
+        std::printf("[ RUN ] %s\n", kv.first.c_str());
+        std::fflush(stdout);
+        const int before = failures;
+        try {
+            kv.second();
+        } catch (std::exception& e) {
+            std::printf("  EXCEPTION: %s\n", e.what());
+            ++failures;
+        }
+        std::printf("[ %s ] %s\n", failures == before ? " OK " : "FAIL", kv.first.c_str());
+        std::fflush(stdout);
+    }
+    return failures ? 1 : 0;
+}

@@ -1,0 +1,186 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle, on the GPU.
+Counts are bit-exact; ids identical (both sides order count desc, id asc)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, load_csv
+from helpers import assert_topk_equal, split_rows
+from oracle import oracle as O
+
+import genie_amd
+from genie_amd import DIM_DTYPE, RANGE_DTYPE, GenieError, Plan, Table, match_topk, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_single(rows, ranges, n_queries, k):
+    res_i, res_c = [], []
+    for q in range(n_queries):
+        kq = int(k[q]) if np.ndim(k) else int(k)
+        i, c = O.topk(O.count_exhaustive_single(rows, ranges[ranges["query"] == q]), kq)
+        res_i.append(i)
+        res_c.append(c)
+    return np.concatenate(res_i), np.concatenate(res_c)
+
+
+def oracle_sets(row_ptr, vals, ranges, n_queries, k):
+    res_i, res_c = [], []
+    for q in range(n_queries):
+        kq = int(k[q]) if np.ndim(k) else int(k)
+        i, c = O.topk(O.count_exhaustive_biject(row_ptr, vals, ranges[ranges["query"] == q]), kq)
+        res_i.append(i)
+        res_c.append(c)
+    return np.concatenate(res_i), np.concatenate(res_c)
+
+
+def run_engine(table, ranges, n_queries, k, lb=False, max_load=0, shuffle=False):
+    items = table.compile(ranges, n_queries, lb, max_load)
+    if shuffle:
+        items = items[np.random.default_rng(0).permutation(len(items))]
+    return match_topk(table.device(), items, n_queries, k)
+
+
+def test_cfg0_plain():
+    w = W.cfg0()
+    t = Table.from_rows(w["rows"])
+    ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"])
+    ri, rc = oracle_single(w["rows"], w["ranges"], w["n_queries"], w["k"])
+    assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], "cfg0")
+    # every query is a noisy copy of a row: that row is found with a high count
+    assert np.mean(cnt.reshape(w["n_queries"], -1)[:, 0] >= 15) > 0.9
+
+
+def test_sift20_goldens_through_legacy_knn():
+    rows = np.array([r[:5] for r in load_csv("sift_20.csv")], np.int32)
+    t = Table.from_rows(rows)
+    ranges = W.single_ranges(rows[:3], 0)
+    ids, cnt = t.knn(ranges, 3, 5)
+    assert (ids[0], cnt[0]) == (0, 5) and (ids[1], cnt[1]) == (4, 2)
+    assert (ids[5], cnt[5]) == (1, 5) and (ids[10], cnt[10]) == (2, 5)
+    ri, rc = oracle_single(rows, ranges, 3, 5)
+    assert_topk_equal(ids, cnt, ri, rc, 5, 3, "sift20")
+
+
+def test_tweets20_exact_counts():
+    rows = load_csv("tweets_20.csv")
+    rp, vals = split_rows(rows)
+    t = Table.from_sets(rp, vals)
+    qp, qv = split_rows(rows)
+    ranges = W.set_ranges(qp, qv, np.ones(len(qv), np.float32))
+    ids, cnt = t.knn(ranges, 20, 5)
+    assert sorted(ids[:2]) == [0, 11] and list(cnt[:2]) == [17, 17]
+    assert (ids[5], cnt[5]) == (1, 14) and (ids[10], cnt[10]) == (2, 18)
+    ri, rc = oracle_sets(rp, vals, ranges, 20, 5)
+    assert_topk_equal(ids, cnt, ri, rc, 5, 20, "tweets20")
+
+
+@pytest.mark.parametrize("n", [70_000, 200_003])
+def test_many_tiles_zipf_ranges(n):
+    w = W.cfg1(n=n, d=32, q=24, k=100)
+    t = Table.from_rows(w["rows"])
+    ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"])
+    ri, rc = oracle_single(w["rows"], w["ranges"], w["n_queries"], w["k"])
+    assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], f"zipf n={n}")
+
+
+def test_weights_widen_count_lanes():
+    # bound 32*200 = 6400 > 255 -> 16-bit lanes; 32 * 3000 > 65535 -> 32-bit lanes
+    w = W.cfg1(n=90_000, d=32, q=8, k=50)
+    t = Table.from_rows(w["rows"])
+    for wmax in (200, 3000):
+        r = w["ranges"].copy()
+        r["weight"] = np.random.default_rng(wmax).integers(1, wmax + 1, len(r))
+        ids, cnt = run_engine(t, r, w["n_queries"], w["k"])
+        ri, rc = oracle_single(w["rows"], r, w["n_queries"], w["k"])
+        assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], f"weights<={wmax}")
+
+
+def test_edge_cases_padding_empty_and_k():
+    w = W.cfg0(n=3000, d=16, q=6)
+    t = Table.from_rows(w["rows"])
+    r = w["ranges"]
+    keep = (r["query"] != 2)                      # query 2 has no ranges at all
+    r = r[keep].copy()
+    r = r[~((r["query"] == 3) & (r["dim"] > 0))]  # query 3: one range -> fewer matches than k
+    r[r["query"] == 4] ["low"]
+    m4 = r["query"] == 4
+    r["low"][m4] = 1000                           # query 4: disjoint ranges only
+    r["up"][m4] = 2000
+    for k in (1, 7, 300, 4096):
+        ids, cnt = run_engine(t, r, 6, k)
+        ri, rc = oracle_single(w["rows"], r, 6, k)
+        assert_topk_equal(ids, cnt, ri, rc, k, 6, f"edge k={k}")
+        c = cnt.reshape(6, k)
+        assert np.all(c[2] == 0) and np.all(c[4] == 0)
+    # per-query k
+    kq = np.array([1, 10, 100, 1000, 5, 4096], np.uint32)
+    ids, cnt = run_engine(t, r, 6, kq)
+    ri, rc = oracle_single(w["rows"], r, 6, kq)
+    assert_topk_equal(ids, cnt, ri, rc, kq, 6, "per-query k")
+
+
+def test_load_balanced_and_unsorted_items():
+    w = W.cfg1(n=120_000, d=16, q=10, k=64)
+    t = Table.from_rows(w["rows"])
+    ri, rc = oracle_single(w["rows"], w["ranges"], w["n_queries"], w["k"])
+    ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"], lb=True, max_load=777, shuffle=True)
+    assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], "load balanced")
+
+
+def test_bijectmap_duplicates_and_weights():
+    rp, vals = genie_amd.synth.sets(31, 50_000, 5, 40, 3000, 1.05)
+    qp, qv = genie_amd.synth.sets(32, 40, 1, 64, 3000, 1.05)
+    wts = genie_amd.synth.uniform_int(33, 0, len(qv), 1, 4)
+    ranges = W.set_ranges(qp, qv, wts)
+    t = Table.from_sets(rp, vals)
+    kq = np.array([10, 100, 1000, 7] * 10, np.uint32)
+    ids, cnt = run_engine(t, ranges, 40, kq)
+    ri, rc = oracle_sets(rp, vals, ranges, 40, kq)
+    assert_topk_equal(ids, cnt, ri, rc, kq, 40, "biject")
+
+
+def test_plan_reuse_matches_one_shot():
+    w = W.cfg1(n=80_000, d=24, q=16, k=32)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], w["n_queries"])
+    one = match_topk(t.device(), items, w["n_queries"], w["k"])
+    p = Plan(t.device(), 64, len(items) + 100, 128)
+    p.upload(items, w["n_queries"], w["k"])
+    for _ in range(3):
+        p.run()
+        got = p.download()
+        assert np.array_equal(got[0], one[0]) and np.array_equal(got[1], one[1])
+    # a smaller batch on the same plan
+    sub = items[items["query"] < 5]
+    p.upload(sub, 5, 32)
+    p.run()
+    got = p.download()
+    assert np.array_equal(got[0], one[0][:5 * 32]) and np.array_equal(got[1], one[1][:5 * 32])
+
+
+def test_invalid_inputs_are_rejected():
+    w = W.cfg0(n=500, d=8, q=2)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 2)
+    bad = items.copy()
+    bad["weight"][0] = 1.5
+    with pytest.raises(GenieError):
+        match_topk(t.device(), bad, 2, 5)
+    bad = items.copy()
+    bad["end_pos"][0] = 10**9
+    with pytest.raises(GenieError):
+        match_topk(t.device(), bad, 2, 5)
+    with pytest.raises(GenieError):
+        match_topk(t.device(), items, 2, 0)
+    with pytest.raises(GenieError):
+        match_topk(t.device(), items, 1, 5)  # item refers to query 1 of 1
+
+
+def test_cpp_api_reference_tests():
+    exe = os.path.join(ROOT, "genie_amd", "lib", "test_api")
+    r = subprocess.run([exe, GOLDEN], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
