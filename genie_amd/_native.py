@@ -72,6 +72,17 @@ _sig(_hip, "genie_hip_plan_kernel_ms", C.c_int, _p, C.c_char_p, C.POINTER(C.c_do
 _sig(_hip, "genie_hip_plan_bytes", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64))
 _sig(_hip, "genie_hip_decode", C.c_int, C.c_int, _p, _p, _u32, _u32, _p, _p)
 _sig(_hip, "genie_hip_last_error", C.c_char_p)
+_sig(_hip, "genie_hip_stream_synchronize", C.c_int, _p)
+_sig(_hip, "genie_hip_device_synchronize", C.c_int)
+_sig(_hip, "genie_hip_copy_probe", C.c_int, _u64)
+
+
+def copy_probe(nbytes):
+    _chk_hip(_hip.genie_hip_copy_probe(nbytes))
+
+
+def device_synchronize():
+    _chk_hip(_hip.genie_hip_device_synchronize())
 
 _sig(_host, "genie_table_build_rows", C.c_int, _p, _u32, _u32, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_p))
 _sig(_host, "genie_table_build_sets", C.c_int, _p, _p, _u32, _u32, C.c_int, C.c_int, C.c_int, C.POINTER(_p))

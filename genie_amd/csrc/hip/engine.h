@@ -20,8 +20,10 @@ constexpr uint32_t kChunk = 32768;
 constexpr uint32_t kNoDir = 0xFFFFFFFFu;
 
 // One compiled chunk of a work item: postings [sp, ep) covering keys k0..k1.
+// kp is the position of the chunk's first (chunk, key) pair among all pairs
+// of the batch (exclusive prefix of k1 - k0 + 1 over chunks).
 struct CChunk {
-    uint32_t k0, k1, sp, ep, w, q;
+    uint32_t k0, k1, sp, ep, w, q, kp, pad;
 };
 
 // Segment view of a compressed index.

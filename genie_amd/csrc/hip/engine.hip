@@ -240,8 +240,39 @@ __global__ void k_chunk_write(const genie_dim_t* __restrict__ items, uint32_t n_
         ch.ep = ep;
         ch.w = w;
         ch.q = (uint32_t)it.query;
+        ch.kp = 0;
+        ch.pad = 0;
         chunks[c] = ch;
     }
+}
+
+// Number of keys each chunk covers (for the pair prefix).
+__global__ void k_chunk_nkeys(const CChunk* __restrict__ chunks, const uint32_t* __restrict__ total,
+                              uint32_t* __restrict__ nk)
+{
+    const uint32_t n = *total;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        nk[c] = chunks[c].k1 - chunks[c].k0 + 1;
+}
+
+__global__ void k_chunk_set_kp(CChunk* __restrict__ chunks, const uint32_t* __restrict__ total,
+                               const uint32_t* __restrict__ kptr)
+{
+    const uint32_t n = *total;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        chunks[c].kp = kptr[c];
+}
+
+// qk_ptr[q] = first pair of query q.
+__global__ void k_query_pair_ptr(const uint32_t* __restrict__ qc_ptr, uint32_t n_queries,
+                                 const uint32_t* __restrict__ kptr, const uint32_t* __restrict__ total_chunks,
+                                 const uint32_t* __restrict__ total_pairs, uint32_t* __restrict__ qk_ptr)
+{
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > n_queries)
+        return;
+    uint32_t c = qc_ptr[q];
+    qk_ptr[q] = c >= *total_chunks ? *total_pairs : kptr[c];
 }
 
 // qc_ptr[q] = first chunk of query q (items sorted by query, item_qptr CSR).
@@ -272,6 +303,7 @@ struct ScanArgs {
     const uint32_t* dir;
     const CChunk* chunks;
     const uint32_t* qc_ptr;
+    const uint32_t* qk_ptr;
     const uint32_t* kq;
     uint32_t n_objects, n_queries, n_tiles, tile_objs, dt_per_tile, n_dtiles;
     uint32_t* gthr;       // per query adaptive threshold (max of finished tiles' k-th counts)
@@ -296,42 +328,77 @@ __device__ __forceinline__ uint64_t make_key(uint32_t count, uint32_t id)
     return ((uint64_t)count << 32) | (uint32_t)(~id);
 }
 
-// Count postings of one key sub-span [a, b) into the tile counters.
-template <int LB>
-__device__ __forceinline__ void count_span(const uint32_t* __restrict__ post, uint32_t a, uint32_t b,
-                                           uint32_t obj0, uint32_t tile_objs, uint32_t w,
-                                           bool filter, uint32_t* cnt)
+// Count postings of one key sub-span [a, b) into the tile counters.  Lanes
+// read 16-byte groups of ids, four groups per lane issued back to back (the
+// loads carry no branches, so all four are in flight together: indexes past
+// the end are clamped to the last group and their ids are not counted);
+// the unaligned head and tail go through single-id loads.  FILTER: the span
+// may hold ids outside the tile (short lists without a directory entry).
+template <int LB, bool FILTER>
+__device__ __forceinline__ void count_one(uint32_t id, uint32_t obj0, uint32_t tile_objs, uint32_t w,
+                                          uint32_t* cnt)
 {
     constexpr int PER = 32 / LB;
     constexpr int SH = PER == 4 ? 2 : (PER == 2 ? 1 : 0);
+    const uint32_t loc = id - obj0;
+    // Ids outside the tile (and lanes with nothing to add, w == 0) go to a
+    // per-lane scratch word past the counters: no branch, no shared address.
+    const bool ok = !FILTER || loc < tile_objs;
+    const uint32_t word = ok ? (loc >> SH) : (uint32_t)(kCntWords + lane_id());
+    atomicAdd(&cnt[word], ok ? w << ((loc & (PER - 1)) * LB) : 0u);
+}
+
+template <int LB, bool FILTER>
+__device__ __forceinline__ void count_span(const uint32_t* __restrict__ post, uint32_t a, uint32_t b,
+                                           uint32_t obj0, uint32_t tile_objs, uint32_t w, uint32_t* cnt)
+{
     const int lane = lane_id();
-    uint32_t p = a + lane;
-    // four loads in flight per lane
-    for (; p + 3 * kWave < b; p += 4 * kWave) {
-        uint32_t id[4];
+    const uint32_t a4 = (a + 3u) & ~3u;        // first 16-byte aligned position
+    const uint32_t b4 = b & ~3u;               // end of the aligned body
+    if (a4 >= b4) {                            // short span: one id per lane
+        for (uint32_t p = a + lane; p < b; p += kWave)
+            count_one<LB, FILTER>(post[p], obj0, tile_objs, w, cnt);
+        return;
+    }
+    if (lane < (int)(a4 - a))
+        count_one<LB, FILTER>(post[a + lane], obj0, tile_objs, w, cnt);
+    if (lane < (int)(b - b4))
+        count_one<LB, FILTER>(post[b4 + lane], obj0, tile_objs, w, cnt);
+    const uint4* body = reinterpret_cast<const uint4*>(post + a4);
+    const uint32_t ng = (b4 - a4) >> 2;        // 16-byte groups, >= 1
+    for (uint32_t g0 = 0; g0 < ng; g0 += 4 * kWave) {
+        uint4 v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            id[u] = __builtin_nontemporal_load(&post[p + u * kWave]);
+            v[u] = body[min(g0 + u * kWave + lane, ng - 1)];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            uint32_t loc = id[u] - obj0;
-            if (!filter || loc < tile_objs)
-                atomicAdd(&cnt[loc >> SH], w << ((loc & (PER - 1)) * LB));
+            // lanes past the end count into the scratch word instead of
+            // branching, which would let the compiler sink their load
+            const bool in = g0 + u * kWave + lane < ng;
+            const uint32_t past = obj0 + tile_objs;  // an id just outside the tile
+            count_one<LB, true>(in ? v[u].x : past, obj0, tile_objs, w, cnt);
+            count_one<LB, true>(in ? v[u].y : past, obj0, tile_objs, w, cnt);
+            count_one<LB, true>(in ? v[u].z : past, obj0, tile_objs, w, cnt);
+            count_one<LB, true>(in ? v[u].w : past, obj0, tile_objs, w, cnt);
         }
     }
-    for (; p < b; p += kWave) {
-        uint32_t loc = post[p] - obj0;
-        if (!filter || loc < tile_objs)
-            atomicAdd(&cnt[loc >> SH], w << ((loc & (PER - 1)) * LB));
-    }
 }
+
+// Sub-span of one (chunk, key) pair inside the workgroup's tile.
+struct Span {
+    uint32_t lo, hi;   // posting positions
+    uint32_t wf;       // weight << 1 | filter
+};
+constexpr int kSpanCap = 1024;
 
 template <int LB>
 __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
 {
     constexpr int PER = 32 / LB;
-    __shared__ uint4 s_cnt4[kCntWords / 4];
+    __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];  // counters + per-lane scratch
     uint32_t* cnt = reinterpret_cast<uint32_t*>(s_cnt4);
+    __shared__ Span s_span[kSpanCap];
     __shared__ uint32_t s_next;
     __shared__ uint32_t s_hist[64];
     __shared__ uint32_t s_wv[kScanWaves];
@@ -346,38 +413,65 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
 
     for (int i = tid; i < kCntWords / 4; i += kScanThreads)
         s_cnt4[i] = make_uint4(0, 0, 0, 0);
-    if (tid == 0)
-        s_next = 0;
-    __syncthreads();
 
-    // ---- count: waves take chunks dynamically --------------------------
-    const uint32_t c0 = a.qc_ptr[q], nch = a.qc_ptr[q + 1] - c0;
-    for (;;) {
-        uint32_t ci = 0;
-        if (lane == 0)
-            ci = atomicAdd(&s_next, 1u);
-        ci = bcast_first(__shfl((int)ci, 0, kWave));
-        if (ci >= nch)
-            break;
-        const CChunk ch = a.chunks[c0 + ci];
-        for (uint32_t k = ch.k0; k <= ch.k1; ++k) {
-            const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
-            uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
-            if (lo >= hi)
+    // ---- count ------------------------------------------------------------
+    // Rounds of up to kSpanCap (chunk, key) pairs: every thread resolves the
+    // pairs of its chunks inside the round's window (independent loads, so
+    // the metadata latency is paid once per round), then the waves take the
+    // resolved spans dynamically.
+    const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
+    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
+    for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
+        const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
+        if (tid == 0)
+            s_next = 0;
+        for (uint32_t c = c0 + tid; c < c1; c += kScanThreads) {
+            const CChunk ch = a.chunks[c];
+            const uint32_t cp = ch.kp - pbase;                  // chunk's first pair, query-relative
+            const uint32_t nk = ch.k1 - ch.k0 + 1;
+            if (cp + nk <= w0 || cp >= w0 + wn)
                 continue;
-            const uint32_t doff = a.dir_off[k];
-            bool filter = true;
-            if (doff != kNoDir) {
-                const uint32_t* d = a.dir + doff;
-                lo = max(lo, ks + d[dt0]);
-                hi = min(hi, ks + d[dt1]);
-                filter = false;
+            const uint32_t j0 = cp < w0 ? w0 - cp : 0;
+            const uint32_t j1 = min(nk, w0 + wn - cp);
+            for (uint32_t j = j0; j < j1; ++j) {
+                const uint32_t k = ch.k0 + j;
+                const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
+                uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
+                const uint32_t doff = a.dir_off[k];
+                uint32_t filter = 1;
+                if (doff != kNoDir) {
+                    const uint32_t* d = a.dir + doff;
+                    lo = max(lo, ks + d[dt0]);
+                    hi = min(hi, ks + d[dt1]);
+                    filter = 0;
+                }
+                Span sp;
+                sp.lo = lo;
+                sp.hi = max(lo, hi);
+                sp.wf = ch.w << 1 | filter;
+                s_span[cp + j - w0] = sp;
             }
-            if (lo < hi)
-                count_span<LB>(a.post, lo, hi, obj0, a.tile_objs, ch.w, filter, cnt);
         }
+        __syncthreads();
+        for (;;) {
+            uint32_t si = 0;
+            if (lane == 0)
+                si = atomicAdd(&s_next, 1u);
+            si = bcast_first(__shfl((int)si, 0, kWave));
+            if (si >= wn)
+                break;
+            const Span sp = s_span[si];
+            if (sp.lo < sp.hi) {
+                if (sp.wf & 1)
+                    count_span<LB, true>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
+                else
+                    count_span<LB, false>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
+    if (npairs == 0)
+        __syncthreads();  // the zeroing above must finish before the select passes
 
     // ---- select: exact tile top-k under (count desc, id asc) ------------
     const uint32_t k = a.kq[q];
@@ -695,6 +789,11 @@ struct genie_plan {
     CChunk* d_chunks = nullptr;
     uint64_t chunks_cap = 0;
     uint32_t* d_qc_ptr = nullptr;
+    uint32_t* d_qk_ptr = nullptr;
+    uint32_t* d_nk = nullptr;        // per chunk keys, then pair prefix
+    uint32_t* d_kptr = nullptr;
+    uint32_t* d_chunk_sums = nullptr;
+    uint32_t* d_total_pairs = nullptr;
     uint32_t* d_kq = nullptr;
     uint32_t* d_gthr = nullptr;
     uint32_t* d_cand_cnt = nullptr;
@@ -733,6 +832,11 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_total);
     dev_free(p->d_chunks);
     dev_free(p->d_qc_ptr);
+    dev_free(p->d_qk_ptr);
+    dev_free(p->d_nk);
+    dev_free(p->d_kptr);
+    dev_free(p->d_chunk_sums);
+    dev_free(p->d_total_pairs);
     dev_free(p->d_kq);
     dev_free(p->d_gthr);
     dev_free(p->d_cand_cnt);
@@ -843,6 +947,39 @@ static int validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t 
 extern "C" {
 
 const char* genie_hip_last_error(void) { return g_err.c_str(); }
+
+int genie_hip_stream_synchronize(void* stream)
+{
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return GENIE_OK;
+}
+
+int genie_hip_copy_probe(uint64_t bytes)
+{
+    uint8_t *a = nullptr, *b = nullptr;
+    if (int rc = dev_alloc(&a, bytes))
+        return rc;
+    if (int rc = dev_alloc(&b, bytes)) {
+        dev_free(a);
+        return rc;
+    }
+    hipError_t e = hipMemset(a, 1, bytes);
+    if (e == hipSuccess)
+        e = hipMemcpy(b, a, bytes, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    dev_free(a);
+    dev_free(b);
+    if (e != hipSuccess)
+        return fail(GENIE_E_RUNTIME, std::string("copy probe: ") + hipGetErrorString(e));
+    return GENIE_OK;
+}
+
+int genie_hip_device_synchronize(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    return GENIE_OK;
+}
 
 int genie_hip_init(int device)
 {
@@ -1013,6 +1150,8 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         if ((rc = dev_alloc(&p->d_block_sums, nb))) break;
         if ((rc = dev_alloc(&p->d_total, 1))) break;
         if ((rc = dev_alloc(&p->d_qc_ptr, max_queries + 1))) break;
+        if ((rc = dev_alloc(&p->d_qk_ptr, max_queries + 1))) break;
+        if ((rc = dev_alloc(&p->d_total_pairs, 1))) break;
         if ((rc = dev_alloc(&p->d_kq, max_queries))) break;
         if ((rc = dev_alloc(&p->d_gthr, max_queries))) break;
         if ((rc = dev_alloc(&p->d_cand_cnt, max_queries))) break;
@@ -1132,8 +1271,17 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     p->compulsory_bytes = compulsory;
     if (nch > p->chunks_cap) {
         dev_free(p->d_chunks);
+        dev_free(p->d_nk);
+        dev_free(p->d_kptr);
+        dev_free(p->d_chunk_sums);
         p->chunks_cap = 0;
         if (int rc = dev_alloc(&p->d_chunks, nch))
+            return rc;
+        if (int rc = dev_alloc(&p->d_nk, nch))
+            return rc;
+        if (int rc = dev_alloc(&p->d_kptr, nch))
+            return rc;
+        if (int rc = dev_alloc(&p->d_chunk_sums, (nch + 1023) / 1024 + 1))
             return rc;
         p->chunks_cap = nch;
     }
@@ -1174,6 +1322,7 @@ static int launch_scan(genie_plan* p, hipStream_t s)
     a.dir = d.dir;
     a.chunks = p->d_chunks;
     a.qc_ptr = p->d_qc_ptr;
+    a.qk_ptr = p->d_qk_ptr;
     a.kq = p->d_kq;
     a.n_objects = d.n_objects;
     a.n_queries = p->n_queries;
@@ -1222,6 +1371,21 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     }
     k_query_chunk_ptr<<<(nq + 1 + 255) / 256, 256, 0, s>>>(p->d_item_qptr, nq, p->d_chunk_ptr, ni,
                                                           p->d_total, p->d_qc_ptr);
+    // pair prefix: every (chunk, key) pair gets a batch-wide position
+    if (p->n_chunks) {
+        const uint32_t nc = (uint32_t)p->n_chunks;
+        const uint32_t g = std::min<uint32_t>((nc + 255) / 256, 4096);
+        k_chunk_nkeys<<<g, 256, 0, s>>>(p->d_chunks, p->d_total, p->d_nk);
+        const uint32_t nb = (nc + 1023) / 1024;
+        k_scan_blocks<<<nb, 256, 0, s>>>(p->d_nk, p->d_kptr, nc, p->d_chunk_sums);
+        k_scan_sums<<<1, 256, 0, s>>>(p->d_chunk_sums, nb, p->d_total_pairs);
+        k_scan_add<<<nb, 256, 0, s>>>(p->d_kptr, nc, p->d_chunk_sums);
+        k_chunk_set_kp<<<g, 256, 0, s>>>(p->d_chunks, p->d_total, p->d_kptr);
+    } else {
+        fill_u32(p->d_total_pairs, 0, 1, s);
+    }
+    k_query_pair_ptr<<<(nq + 1 + 255) / 256, 256, 0, s>>>(p->d_qc_ptr, nq, p->d_kptr, p->d_total,
+                                                         p->d_total_pairs, p->d_qk_ptr);
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[1], s));
     if (int rc = launch_scan(p, s))

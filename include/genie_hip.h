@@ -116,9 +116,9 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
 int genie_hip_plan_run(genie_plan* p, void* stream);
 /* Copy the results of the last run to host buffers (synchronises stream). */
 int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts);
-/* Per-kernel timing of the last run when timing is on (HIP events on the
- * plan's stream): names are "scan", "select", "merge"; ms is the total
- * duration of that kernel's launches in the run. */
+/* Per-stage timing of the last run when timing is on (HIP events recorded
+ * on the run's stream): "compile" (item -> chunk kernels), "scan" (the
+ * count + per-tile select kernel), "merge"; ms is the stage's duration. */
 int genie_hip_plan_set_timing(genie_plan* p, int on);
 int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms);
 /* Algorithmic bytes of the last uploaded batch: logical = 4 B x sum of
@@ -132,6 +132,15 @@ int genie_hip_plan_bytes(genie_plan* p, uint64_t* logical, uint64_t* compulsory)
  * out[i * out_stride ...], its length to out_len[i].  Host buffers. */
 int genie_hip_decode(int codec, const uint32_t* words, const uint32_t* cpos, uint32_t n_arrays,
                      uint32_t out_stride, uint32_t* out, uint32_t* out_len);
+
+/* Measurement aid: one device-to-device copy of `bytes` between two fresh
+ * buffers (a plain copy kernel of known size, to scale HBM traffic counters). */
+int genie_hip_copy_probe(uint64_t bytes);
+
+/* Wait for all work on `stream` (NULL = default stream). */
+int genie_hip_stream_synchronize(void* stream);
+/* Wait for all work on the current device. */
+int genie_hip_device_synchronize(void);
 
 const char* genie_hip_last_error(void);
 

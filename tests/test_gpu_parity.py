@@ -49,8 +49,9 @@ def test_cfg0_plain():
     ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"])
     ri, rc = oracle_single(w["rows"], w["ranges"], w["n_queries"], w["k"])
     assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], "cfg0")
-    # every query is a noisy copy of a row: that row is found with a high count
-    assert np.mean(cnt.reshape(w["n_queries"], -1)[:, 0] >= 15) > 0.9
+    # every query is a copy of a row with U{-3..3} noise per dim, so the row
+    # matches about 128/7 dims exactly and is ranked first almost always
+    assert np.mean(ids.reshape(w["n_queries"], -1)[:, 0] == w["src"]) > 0.9
 
 
 def test_sift20_goldens_through_legacy_knn():
@@ -105,7 +106,6 @@ def test_edge_cases_padding_empty_and_k():
     keep = (r["query"] != 2)                      # query 2 has no ranges at all
     r = r[keep].copy()
     r = r[~((r["query"] == 3) & (r["dim"] > 0))]  # query 3: one range -> fewer matches than k
-    r[r["query"] == 4] ["low"]
     m4 = r["query"] == 4
     r["low"][m4] = 1000                           # query 4: disjoint ranges only
     r["up"][m4] = 2000
