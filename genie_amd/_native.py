@@ -77,6 +77,19 @@ _sig(_hip, "genie_hip_device_synchronize", C.c_int)
 _sig(_hip, "genie_hip_copy_probe", C.c_int, _u64)
 
 
+def decode(codec, words, cpos, stride=1024):
+    """Decode compressed arrays on the device (genie_hip_decode); returns a
+    list of uint32 arrays (None for a malformed array)."""
+    words = _c(words, np.uint32)
+    cpos = _c(cpos, np.uint32)
+    n = len(cpos) - 1
+    out = np.zeros(n * stride, np.uint32)
+    ln = np.zeros(n, np.uint32)
+    _chk_hip(_hip.genie_hip_decode(codec, _ptr(words), _ptr(cpos), n, stride, _ptr(out), _ptr(ln)))
+    return [None if l == 0xFFFFFFFF else out[i * stride:i * stride + min(int(l), stride)].copy()
+            for i, l in enumerate(ln)]
+
+
 def copy_probe(nbytes):
     _chk_hip(_hip.genie_hip_copy_probe(nbytes))
 
@@ -283,8 +296,14 @@ class Table:
             cap = max(1024, 4 * len(ranges) + s["n_seg"] * 2 + (s["n_post"] // max(1, max_load) + 1 if load_balance else 0))
         out = np.zeros(cap, DIM_DTYPE)
         n = _u64()
-        _chk_host(_host.genie_table_compile(self.handle, _ptr(ranges), len(ranges), n_queries, int(load_balance),
-                                            max_load, _ptr(out), cap, C.byref(n)))
+        rc = _host.genie_table_compile(self.handle, _ptr(ranges), len(ranges), n_queries, int(load_balance),
+                                       max_load, _ptr(out), cap, C.byref(n))
+        if rc != 0 and n.value > cap:  # capacity too small: the call reported the size needed
+            cap = n.value
+            out = np.zeros(cap, DIM_DTYPE)
+            rc = _host.genie_table_compile(self.handle, _ptr(ranges), len(ranges), n_queries, int(load_balance),
+                                           max_load, _ptr(out), cap, C.byref(n))
+        _chk_host(rc)
         return out[: n.value].copy()
 
     def device(self):

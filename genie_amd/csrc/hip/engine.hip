@@ -25,6 +25,7 @@
 #include <unordered_set>
 #include <vector>
 
+#include "codecs_dev.h"
 #include "engine.h"
 #include "wave.h"
 
@@ -190,6 +191,119 @@ __global__ void k_build_dir(const uint32_t* __restrict__ post, const uint32_t* _
 }
 
 // ---------------------------------------------------------------------------
+// Compressed index: segment bounds and segment directory
+// ---------------------------------------------------------------------------
+
+constexpr int kDecodeWaves = 4;
+
+// Decode every segment once: first and last id, decoded length check
+// (against the uncompressed segment positions) and ascending order.
+__global__ void __launch_bounds__(kDecodeWaves * kWave)
+    k_seg_bounds(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
+                 const uint32_t* __restrict__ upos, uint32_t n_seg, uint32_t n_objects,
+                 uint32_t* __restrict__ first, uint32_t* __restrict__ last, uint32_t* __restrict__ err)
+{
+    __shared__ DecodeScratch S[kDecodeWaves];
+    const int wv = threadIdx.x / kWave, lane = lane_id();
+    for (uint32_t sgi = blockIdx.x * kDecodeWaves + wv; sgi < n_seg; sgi += gridDim.x * kDecodeWaves) {
+        const uint32_t w0 = cpos[sgi], nw = cpos[sgi + 1] - w0;
+        const uint32_t want = upos[sgi + 1] - upos[sgi];
+        if (nw > kSegMax || want > kSegMax) {
+            if (lane == 0)
+                atomicOr(err, 1u);
+            continue;
+        }
+        stage_words(S[wv], words, w0, nw);
+        uint32_t v[kPerLane];
+        const uint32_t n = wave_decode(codec, S[wv], nw, v);
+        bool bad = n != want || n == 0;
+        uint32_t prev = lane ? 0u : 0u;
+        // ascending within the segment and inside the object range
+        uint32_t lastv = (uint32_t)__shfl_up((int)v[kPerLane - 1], 1, kWave);
+        if (lane == 0)
+            lastv = 0;
+        for (int j = 0; j < kPerLane; ++j) {
+            const uint32_t i = lane * kPerLane + j;
+            if (i < n) {
+                const uint32_t before = j ? v[j - 1] : lastv;
+                if ((i > 0 && v[j] < before) || v[j] >= n_objects)
+                    bad = true;
+            }
+        }
+        (void)prev;
+        if (ballot(bad) != 0) {
+            if (lane == 0)
+                atomicOr(err, 1u);
+            wave_sync();
+            continue;
+        }
+        if (!bad && n) {
+            if (lane == 0)
+                first[sgi] = v[0];
+            if (lane == (int)((n - 1) / kPerLane))
+                last[sgi] = v[(n - 1) % kPerLane];
+        }
+        wave_sync();
+    }
+}
+
+// For every long key and directory-tile boundary b: the first segment of the
+// key (relative) whose last id is >= b * kDirTile.
+__global__ void k_build_sdir(const uint32_t* __restrict__ seg_last, const uint32_t* __restrict__ key_seg,
+                             const uint32_t* __restrict__ dir_keys, uint32_t n_dir_keys, uint32_t n_dtiles,
+                             uint32_t* __restrict__ sdir)
+{
+    const uint64_t per = (uint64_t)n_dtiles + 1;
+    const uint64_t total = (uint64_t)n_dir_keys * per;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = dir_keys[i / per], b = (uint32_t)(i % per);
+        const uint32_t s = key_seg[k], e = key_seg[k + 1];
+        const uint64_t target = (uint64_t)b * kDirTile;
+        uint32_t lo = s, hi = e;
+        while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if ((uint64_t)seg_last[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        sdir[i] = lo - s;
+    }
+}
+
+// Stand-alone decode of n arrays (tests, measurement).
+__global__ void __launch_bounds__(kDecodeWaves * kWave)
+    k_decode_arrays(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
+                    uint32_t n_arrays, uint32_t stride, uint32_t* __restrict__ out, uint32_t* __restrict__ out_len)
+{
+    __shared__ DecodeScratch S[kDecodeWaves];
+    const int wv = threadIdx.x / kWave, lane = lane_id();
+    for (uint32_t a = blockIdx.x * kDecodeWaves + wv; a < n_arrays; a += gridDim.x * kDecodeWaves) {
+        const uint32_t w0 = cpos[a], nw = cpos[a + 1] - w0;
+        if (nw > kSegMax) {
+            if (lane == 0)
+                out_len[a] = 0xFFFFFFFFu;
+            continue;
+        }
+        stage_words(S[wv], words, w0, nw);
+        uint32_t v[kPerLane];
+        const uint32_t n = wave_decode(codec, S[wv], nw, v);
+        if (lane == 0)
+            out_len[a] = n;
+        if (n != 0xFFFFFFFFu) {
+            const uint32_t lim = min(n, stride);
+            for (int j = 0; j < kPerLane; ++j) {
+                const uint32_t i = lane * kPerLane + j;
+                if (i < lim)
+                    out[(uint64_t)a * stride + i] = v[j];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Device-side item compilation
 // ---------------------------------------------------------------------------
 
@@ -275,6 +389,80 @@ __global__ void k_query_pair_ptr(const uint32_t* __restrict__ qc_ptr, uint32_t n
     qk_ptr[q] = c >= *total_chunks ? *total_pairs : kptr[c];
 }
 
+// Compressed items cover whole segments; consecutive items of one query with
+// the same weight and adjacent segments merge into one run (a chunk whose
+// sp/ep are segment indices), so a workgroup jumps over a long list with the
+// segment directory instead of visiting one item per segment.
+__device__ __forceinline__ uint32_t seg_at(const uint32_t* cpos, uint32_t n_seg, uint32_t pos)
+{
+    uint32_t lo = 0, hi = n_seg;  // last s with cpos[s] <= pos
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (cpos[mid] <= pos)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool citem_segs(const genie_dim_t& it, const uint32_t* cpos, uint32_t n_seg,
+                                           uint32_t* s0, uint32_t* s1)
+{
+    if (it.end_pos <= it.start_pos)
+        return false;
+    *s0 = seg_at(cpos, n_seg, (uint32_t)it.start_pos);
+    *s1 = seg_at(cpos, n_seg, (uint32_t)it.end_pos - 1) + 1;
+    return true;
+}
+
+__device__ __forceinline__ bool crun_starts(const genie_dim_t* items, uint32_t i, const uint32_t* cpos,
+                                            uint32_t n_seg)
+{
+    uint32_t a0, a1, b0, b1;
+    if (!citem_segs(items[i], cpos, n_seg, &a0, &a1))
+        return false;
+    if (i == 0 || !citem_segs(items[i - 1], cpos, n_seg, &b0, &b1))
+        return true;
+    return items[i].query != items[i - 1].query || items[i].weight != items[i - 1].weight || a0 != b1;
+}
+
+__global__ void k_crun_flags(const genie_dim_t* __restrict__ items, uint32_t n_items,
+                             const uint32_t* __restrict__ cpos, uint32_t n_seg, uint32_t* __restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_items)
+        flag[i] = crun_starts(items, i, cpos, n_seg) ? 1u : 0u;
+}
+
+__global__ void k_crun_write(const genie_dim_t* __restrict__ items, uint32_t n_items,
+                             const uint32_t* __restrict__ cpos, uint32_t n_seg,
+                             const uint32_t* __restrict__ seg_key, const uint32_t* __restrict__ rptr,
+                             CChunk* __restrict__ runs)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items)
+        return;
+    uint32_t s0, s1;
+    if (!citem_segs(items[i], cpos, n_seg, &s0, &s1))
+        return;
+    const bool start = crun_starts(items, i, cpos, n_seg);
+    const uint32_t r = start ? rptr[i] : rptr[i] - 1;
+    if (start) {
+        runs[r].sp = s0;
+        runs[r].k0 = seg_key[s0];
+        runs[r].w = (uint32_t)items[i].weight;
+        runs[r].q = (uint32_t)items[i].query;
+        runs[r].kp = 0;
+        runs[r].pad = 0;
+    }
+    if (i + 1 == n_items || crun_starts(items, i + 1, cpos, n_seg) ||
+        items[i + 1].end_pos <= items[i + 1].start_pos) {
+        runs[r].ep = s1;
+        runs[r].k1 = seg_key[s1 - 1];
+    }
+}
+
 // qc_ptr[q] = first chunk of query q (items sorted by query, item_qptr CSR).
 __global__ void k_query_chunk_ptr(const uint32_t* __restrict__ item_qptr, uint32_t n_queries,
                                   const uint32_t* __restrict__ chunk_ptr, uint32_t n_items,
@@ -297,6 +485,16 @@ constexpr int kCntWords = 16384;  // 64 KiB of counters per workgroup
 constexpr uint32_t kEmitFail = 0xFFFFFFFFu;
 
 struct ScanArgs {
+    // compressed index (codec != 0)
+    int codec;
+    const uint32_t* words;
+    const uint32_t* seg_cpos;
+    const uint32_t* seg_first;
+    const uint32_t* seg_last;
+    const uint32_t* key_seg;
+    const uint32_t* sdir_off;
+    const uint32_t* sdir;
+    // uncompressed index
     const uint32_t* post;
     const uint32_t* key_start;
     const uint32_t* dir_off;
@@ -385,94 +583,25 @@ __device__ __forceinline__ void count_span(const uint32_t* __restrict__ post, ui
     }
 }
 
-// Sub-span of one (chunk, key) pair inside the workgroup's tile.
-struct Span {
-    uint32_t lo, hi;   // posting positions
-    uint32_t wf;       // weight << 1 | filter
+// Shared scratch of the select passes.
+struct SelShared {
+    uint32_t hist[64];
+    uint32_t wv[kScanWaves];
+    uint32_t res[8];
 };
-constexpr int kSpanCap = 1024;
 
+// Exact per-tile top-k under (count desc, id asc), pruned by the query's
+// adaptive threshold, appended to the query's candidate list.  Called by all
+// threads of the workgroup after the counters are complete.
 template <int LB>
-__global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
+__device__ __forceinline__ void tile_select(const ScanArgs& a, const uint32_t* cnt, uint32_t q, uint32_t obj0,
+                                            SelShared& S)
 {
     constexpr int PER = 32 / LB;
-    __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];  // counters + per-lane scratch
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(s_cnt4);
-    __shared__ Span s_span[kSpanCap];
-    __shared__ uint32_t s_next;
-    __shared__ uint32_t s_hist[64];
-    __shared__ uint32_t s_wv[kScanWaves];
-    __shared__ uint32_t s_res[8];
-
-    const uint32_t t = blockIdx.x / a.n_queries;
-    const uint32_t q = blockIdx.x % a.n_queries;
-    const uint32_t obj0 = t * a.tile_objs;
-    const uint32_t dt0 = t * a.dt_per_tile;
-    const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-
-    for (int i = tid; i < kCntWords / 4; i += kScanThreads)
-        s_cnt4[i] = make_uint4(0, 0, 0, 0);
-
-    // ---- count ------------------------------------------------------------
-    // Rounds of up to kSpanCap (chunk, key) pairs: every thread resolves the
-    // pairs of its chunks inside the round's window (independent loads, so
-    // the metadata latency is paid once per round), then the waves take the
-    // resolved spans dynamically.
-    const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
-    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
-    for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
-        const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
-        if (tid == 0)
-            s_next = 0;
-        for (uint32_t c = c0 + tid; c < c1; c += kScanThreads) {
-            const CChunk ch = a.chunks[c];
-            const uint32_t cp = ch.kp - pbase;                  // chunk's first pair, query-relative
-            const uint32_t nk = ch.k1 - ch.k0 + 1;
-            if (cp + nk <= w0 || cp >= w0 + wn)
-                continue;
-            const uint32_t j0 = cp < w0 ? w0 - cp : 0;
-            const uint32_t j1 = min(nk, w0 + wn - cp);
-            for (uint32_t j = j0; j < j1; ++j) {
-                const uint32_t k = ch.k0 + j;
-                const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
-                uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
-                const uint32_t doff = a.dir_off[k];
-                uint32_t filter = 1;
-                if (doff != kNoDir) {
-                    const uint32_t* d = a.dir + doff;
-                    lo = max(lo, ks + d[dt0]);
-                    hi = min(hi, ks + d[dt1]);
-                    filter = 0;
-                }
-                Span sp;
-                sp.lo = lo;
-                sp.hi = max(lo, hi);
-                sp.wf = ch.w << 1 | filter;
-                s_span[cp + j - w0] = sp;
-            }
-        }
-        __syncthreads();
-        for (;;) {
-            uint32_t si = 0;
-            if (lane == 0)
-                si = atomicAdd(&s_next, 1u);
-            si = bcast_first(__shfl((int)si, 0, kWave));
-            if (si >= wn)
-                break;
-            const Span sp = s_span[si];
-            if (sp.lo < sp.hi) {
-                if (sp.wf & 1)
-                    count_span<LB, true>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
-                else
-                    count_span<LB, false>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
-            }
-        }
-        __syncthreads();
-    }
-    if (npairs == 0)
-        __syncthreads();  // the zeroing above must finish before the select passes
-
+    uint32_t* s_hist = S.hist;
+    uint32_t* s_wv = S.wv;
+    uint32_t* s_res = S.res;
     // ---- select: exact tile top-k under (count desc, id asc) ------------
     const uint32_t k = a.kq[q];
     const uint32_t wpw = kCntWords / kScanWaves;  // words per wave (contiguous)
@@ -634,6 +763,216 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
         }
         run += (uint32_t)__shfl((int)incl, 63, kWave);
     }
+}
+
+// Sub-span of one (chunk, key) pair inside the workgroup's tile.
+struct Span {
+    uint32_t lo, hi;   // posting positions
+    uint32_t wf;       // weight << 1 | filter
+};
+constexpr int kSpanCap = 1024;
+
+template <int LB>
+__global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
+{
+    __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];  // counters + per-lane scratch
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(s_cnt4);
+    __shared__ Span s_span[kSpanCap];
+    __shared__ uint32_t s_next;
+    __shared__ SelShared s_sel;
+
+    const uint32_t t = blockIdx.x / a.n_queries;
+    const uint32_t q = blockIdx.x % a.n_queries;
+    const uint32_t obj0 = t * a.tile_objs;
+    const uint32_t dt0 = t * a.dt_per_tile;
+    const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
+    const int tid = threadIdx.x, lane = lane_id();
+
+    for (int i = tid; i < kCntWords / 4; i += kScanThreads)
+        s_cnt4[i] = make_uint4(0, 0, 0, 0);
+
+    // ---- count ------------------------------------------------------------
+    // Rounds of up to kSpanCap (chunk, key) pairs: every thread resolves the
+    // pairs of its chunks inside the round's window (independent loads, so
+    // the metadata latency is paid once per round), then the waves take the
+    // resolved spans dynamically.
+    const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
+    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
+    for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
+        const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
+        if (tid == 0)
+            s_next = 0;
+        for (uint32_t c = c0 + tid; c < c1; c += kScanThreads) {
+            const CChunk ch = a.chunks[c];
+            const uint32_t cp = ch.kp - pbase;                  // chunk's first pair, query-relative
+            const uint32_t nk = ch.k1 - ch.k0 + 1;
+            if (cp + nk <= w0 || cp >= w0 + wn)
+                continue;
+            const uint32_t j0 = cp < w0 ? w0 - cp : 0;
+            const uint32_t j1 = min(nk, w0 + wn - cp);
+            for (uint32_t j = j0; j < j1; ++j) {
+                const uint32_t k = ch.k0 + j;
+                const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
+                uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
+                const uint32_t doff = a.dir_off[k];
+                uint32_t filter = 1;
+                if (doff != kNoDir) {
+                    const uint32_t* d = a.dir + doff;
+                    lo = max(lo, ks + d[dt0]);
+                    hi = min(hi, ks + d[dt1]);
+                    filter = 0;
+                }
+                Span sp;
+                sp.lo = lo;
+                sp.hi = max(lo, hi);
+                sp.wf = ch.w << 1 | filter;
+                s_span[cp + j - w0] = sp;
+            }
+        }
+        __syncthreads();
+        for (;;) {
+            uint32_t si = 0;
+            if (lane == 0)
+                si = atomicAdd(&s_next, 1u);
+            si = bcast_first(__shfl((int)si, 0, kWave));
+            if (si >= wn)
+                break;
+            const Span sp = s_span[si];
+            if (sp.lo < sp.hi) {
+                if (sp.wf & 1)
+                    count_span<LB, true>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
+                else
+                    count_span<LB, false>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
+            }
+        }
+        __syncthreads();
+    }
+    if (npairs == 0)
+        __syncthreads();  // the zeroing above must finish before the select passes
+
+    tile_select<LB>(a, cnt, q, obj0, s_sel);
+}
+
+// Compressed form of k_scan_tile: pairs resolve to ranges of compressed
+// segments overlapping the tile (segment directory for long keys), the waves
+// take single segments, stage and decode them (codecs_dev.h) and count the
+// decoded ids that fall inside the tile.
+template <int LB>
+__global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
+{
+    __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(s_cnt4);
+    __shared__ Span s_span[kSpanCap];
+    __shared__ uint32_t s_pre[kSpanCap + 1];   // prefix of segments per span
+    __shared__ DecodeScratch s_dec[kScanWaves];
+    __shared__ uint32_t s_next;
+    __shared__ uint32_t s_wsum[kScanWaves];
+    __shared__ SelShared s_sel;
+
+    const uint32_t t = blockIdx.x / a.n_queries;
+    const uint32_t q = blockIdx.x % a.n_queries;
+    const uint32_t obj0 = t * a.tile_objs;
+    const uint32_t obj1 = obj0 + a.tile_objs;
+    const uint32_t dt0 = t * a.dt_per_tile;
+    const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+
+    for (int i = tid; i < kCntWords / 4; i += kScanThreads)
+        s_cnt4[i] = make_uint4(0, 0, 0, 0);
+
+    const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
+    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
+    for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
+        const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
+        if (tid == 0)
+            s_next = 0;
+        for (uint32_t c = c0 + tid; c < c1; c += kScanThreads) {
+            const CChunk ch = a.chunks[c];
+            const uint32_t cp = ch.kp - pbase;
+            const uint32_t nk = ch.k1 - ch.k0 + 1;
+            if (cp + nk <= w0 || cp >= w0 + wn)
+                continue;
+            const uint32_t j0 = cp < w0 ? w0 - cp : 0;
+            const uint32_t j1 = min(nk, w0 + wn - cp);
+            for (uint32_t j = j0; j < j1; ++j) {
+                const uint32_t k = ch.k0 + j;
+                const uint32_t ks = a.key_seg[k], ke = a.key_seg[k + 1];
+                uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
+                const uint32_t doff = a.sdir_off[k];
+                if (doff != kNoDir) {
+                    const uint32_t* d = a.sdir + doff;
+                    const uint32_t mid = ks + d[dt1];
+                    const uint32_t end = mid + (mid < ke && a.seg_first[mid] < obj1 ? 1u : 0u);
+                    lo = max(lo, ks + d[dt0]);
+                    hi = min(hi, end);
+                }
+                Span sp;
+                sp.lo = lo;
+                sp.hi = max(lo, hi);
+                sp.wf = ch.w << 1 | 1u;
+                s_span[cp + j - w0] = sp;
+            }
+        }
+        __syncthreads();
+        // prefix of segment counts over the window's spans (2 per thread)
+        {
+            const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+            const uint32_t n0 = i0 < wn ? s_span[i0].hi - s_span[i0].lo : 0u;
+            const uint32_t n1 = i1 < wn ? s_span[i1].hi - s_span[i1].lo : 0u;
+            const uint32_t incl = wave_incl_scan(n0 + n1);
+            if (lane == kWave - 1)
+                s_wsum[wv] = incl;
+            __syncthreads();
+            uint32_t off = 0;
+            for (int i = 0; i < wv; ++i)
+                off += s_wsum[i];
+            const uint32_t ex = off + incl - (n0 + n1);
+            if (i0 <= wn)
+                s_pre[i0] = ex;
+            if (i1 <= wn)
+                s_pre[i1] = ex + n0;
+            if (i1 + 1 == wn)
+                s_pre[wn] = ex + n0 + n1;  // the total, also when wn is the full window
+        }
+        __syncthreads();
+        const uint32_t nsegs = s_pre[wn];
+        DecodeScratch& D = s_dec[wv];
+        for (;;) {
+            uint32_t g = 0;
+            if (lane == 0)
+                g = atomicAdd(&s_next, 1u);
+            g = bcast_first(__shfl((int)g, 0, kWave));
+            if (g >= nsegs)
+                break;
+            uint32_t lo = 0, hi = wn;  // span with s_pre[span] <= g < s_pre[span + 1]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= g)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const Span sp = s_span[lo];
+            const uint32_t sg = sp.lo + (g - s_pre[lo]);
+            if (a.seg_last[sg] < obj0 || a.seg_first[sg] >= obj1)
+                continue;  // segment entirely outside the tile
+            const uint32_t cw0 = a.seg_cpos[sg], nw = a.seg_cpos[sg + 1] - cw0;
+            stage_words(D, a.words, cw0, nw);
+            uint32_t v[kPerLane];
+            const uint32_t n = wave_decode(a.codec, D, nw, v);
+            const uint32_t w = sp.wf >> 1;
+#pragma unroll
+            for (int j = 0; j < kPerLane; ++j) {
+                const uint32_t i = lane * kPerLane + j;
+                count_one<LB, true>(i < n ? v[j] : obj1, obj0, a.tile_objs, w, cnt);
+            }
+            wave_sync();  // the staging buffer is reused by the next segment
+        }
+        __syncthreads();
+    }
+    if (npairs == 0)
+        __syncthreads();
+    tile_select<LB>(a, cnt, q, obj0, s_sel);
 }
 
 // ---------------------------------------------------------------------------
@@ -944,6 +1283,142 @@ static int validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t 
     return GENIE_OK;
 }
 
+// Compressed index: words, segment offsets, key -> segment map, per-segment
+// first/last ids (one device decode pass that also validates every segment)
+// and the segment directory of long keys.
+static int create_compressed(const genie_index_desc* desc, genie_index** out)
+{
+    const uint32_t S = desc->n_segments, K = desc->n_keys;
+    const uint32_t* up = desc->seg_upos;
+    const uint32_t* cp = desc->seg_cpos;
+    if (up[0] != 0 || up[S] != desc->n_postings || cp[0] != 0 || cp[S] != desc->n_words)
+        return fail(GENIE_E_INVALID, "segment offsets must run from 0 to the array sizes");
+    for (uint32_t i = 0; i < S; ++i)
+        if (up[i + 1] <= up[i] || up[i + 1] - up[i] > kSegMax || cp[i + 1] <= cp[i] || cp[i + 1] - cp[i] > kSegMax)
+            return fail(GENIE_E_INVALID, "segment " + std::to_string(i) + " is empty or longer than " +
+                                             std::to_string(kSegMax) + " ids or words");
+    // every key starts on a segment boundary
+    std::vector<uint32_t> key_seg(K + 1), seg_key(S);
+    for (uint32_t k = 0; k <= K; ++k) {
+        const uint64_t ks = desc->key_start[k];
+        const uint32_t sidx = (uint32_t)(std::lower_bound(up, up + S + 1, (uint32_t)ks) - up);
+        if (sidx > S || (sidx < S && up[sidx] != ks) || (sidx == S && ks != desc->n_postings))
+            return fail(GENIE_E_INVALID, "key " + std::to_string(k) + " does not start on a segment boundary");
+        key_seg[k] = sidx;
+    }
+    for (uint32_t k = 0; k < K; ++k)
+        for (uint32_t sg = key_seg[k]; sg < key_seg[k + 1]; ++sg)
+            seg_key[sg] = k;
+
+    genie_index* ix = new genie_index();
+    DevIndex& d = ix->d;
+    d.n_objects = desc->n_objects;
+    d.n_dims = desc->n_dims;
+    d.n_keys = K;
+    d.n_postings = desc->n_postings;
+    d.codec = desc->codec;
+    d.single_value = desc->single_value;
+    d.n_segments = S;
+    d.n_words = desc->n_words;
+    d.n_dtiles = (desc->n_objects + kDirTile - 1) / kDirTile;
+    d.h_key_start.assign(desc->key_start, desc->key_start + K + 1);
+    d.h_key_dim.assign(desc->key_dim, desc->key_dim + K);
+    d.h_key_mult.assign(K, 1);
+    d.h_seg_cpos.assign(cp, cp + S + 1);
+    d.h_seg_key = seg_key;
+    if (desc->postings) {
+        for (uint32_t k = 0; k < K; ++k) {
+            uint32_t best = 1, run = 1;
+            for (uint64_t p = desc->key_start[k] + 1; p < desc->key_start[k + 1]; ++p) {
+                run = desc->postings[p] == desc->postings[p - 1] ? run + 1 : 1;
+                best = std::max(best, run);
+            }
+            d.h_key_mult[k] = best;
+        }
+    }
+    std::vector<uint32_t> sdir_off(K, kNoDir), dir_keys;
+    for (uint32_t k = 0; k < K; ++k)
+        if (key_seg[k + 1] - key_seg[k] > 4) {
+            sdir_off[k] = (uint32_t)(dir_keys.size() * (uint64_t)(d.n_dtiles + 1));
+            dir_keys.push_back(k);
+        }
+    if ((uint64_t)dir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull) {
+        delete ix;
+        return fail(GENIE_E_INVALID, "segment directory too large");
+    }
+    int rc = GENIE_OK;
+    uint32_t *d_upos = nullptr, *d_dir_keys = nullptr, *d_err = nullptr;
+    do {
+        if ((rc = dev_alloc(&d.words, d.n_words, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.seg_cpos, S + 1, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.seg_key, S, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.seg_first, S, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.seg_last, S, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.key_seg, K + 1, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.dir_off, K, &d.bytes))) break;  // segment directory offsets
+        if ((rc = dev_alloc(&d.sdir, dir_keys.size() * (uint64_t)(d.n_dtiles + 1), &d.bytes))) break;
+        if ((rc = dev_alloc(&d_upos, S + 1))) break;
+        if ((rc = dev_alloc(&d_dir_keys, dir_keys.size()))) break;
+        if ((rc = dev_alloc(&d_err, 1))) break;
+        hipError_t e;
+        const uint32_t zero = 0;
+        if ((e = hipMemcpy(d.words, desc->words, d.n_words * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.seg_cpos, cp, (S + 1) * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_upos, up, (S + 1) * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.seg_key, seg_key.data(), S * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.key_seg, key_seg.data(), (K + 1) * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.dir_off, sdir_off.data(), K * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_dir_keys, dir_keys.data(), dir_keys.size() * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_err, &zero, 4, hipMemcpyHostToDevice))) {
+            rc = fail(GENIE_E_RUNTIME, std::string("index upload: ") + hipGetErrorString(e));
+            break;
+        }
+        const uint32_t blocks = std::min<uint32_t>((S + kDecodeWaves - 1) / kDecodeWaves, 65535);
+        k_seg_bounds<<<blocks, kDecodeWaves * kWave>>>(d.codec, d.words, d.seg_cpos, d_upos, S, d.n_objects,
+                                                        d.seg_first, d.seg_last, d_err);
+        uint32_t err = 0;
+        if ((e = hipGetLastError()) || (e = hipDeviceSynchronize()) ||
+            (e = hipMemcpy(&err, d_err, 4, hipMemcpyDeviceToHost))) {
+            rc = fail(GENIE_E_RUNTIME, std::string("segment decode: ") + hipGetErrorString(e));
+            break;
+        }
+        if (err) {
+            rc = fail(GENIE_E_INVALID, "a compressed segment does not decode to its ascending ids");
+            break;
+        }
+        std::vector<uint32_t> first(S), last(S);
+        if ((e = hipMemcpy(first.data(), d.seg_first, S * 4, hipMemcpyDeviceToHost)) ||
+            (e = hipMemcpy(last.data(), d.seg_last, S * 4, hipMemcpyDeviceToHost))) {
+            rc = fail(GENIE_E_RUNTIME, std::string("segment bounds: ") + hipGetErrorString(e));
+            break;
+        }
+        for (uint32_t sg = 0; sg + 1 < S && !rc; ++sg)
+            if (seg_key[sg] == seg_key[sg + 1] && last[sg] > first[sg + 1])
+                rc = fail(GENIE_E_INVALID, "segments of key " + std::to_string(seg_key[sg]) + " are not ascending");
+        if (rc)
+            break;
+        if (!dir_keys.empty()) {
+            const uint64_t total = dir_keys.size() * (uint64_t)(d.n_dtiles + 1);
+            const uint32_t g = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+            k_build_sdir<<<g, 256>>>(d.seg_last, d.key_seg, d_dir_keys, (uint32_t)dir_keys.size(), d.n_dtiles,
+                                      d.sdir);
+            if ((e = hipGetLastError()) || (e = hipDeviceSynchronize())) {
+                rc = fail(GENIE_E_RUNTIME, std::string("segment directory: ") + hipGetErrorString(e));
+                break;
+            }
+        }
+    } while (0);
+    dev_free(d_upos);
+    dev_free(d_dir_keys);
+    dev_free(d_err);
+    if (rc) {
+        genie_hip_index_destroy(ix);
+        return rc;
+    }
+    *out = ix;
+    return GENIE_OK;
+}
+
 extern "C" {
 
 const char* genie_hip_last_error(void) { return g_err.c_str(); }
@@ -1000,10 +1475,17 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
     *out = nullptr;
     if (desc->n_keys == 0 || desc->n_objects == 0 || !desc->key_start || !desc->key_dim)
         return fail(GENIE_E_INVALID, "empty index");
-    if (desc->codec != 0)
-        return fail(GENIE_E_INVALID, "compressed indexes are not supported by this build");
-    if (!desc->postings && desc->n_postings)
+    if (desc->codec != 0) {
+        CodecShape cs;
+        if (!codec_shape(desc->codec, &cs))
+            return fail(GENIE_E_INVALID, "unknown codec " + std::to_string(desc->codec));
+        if (!desc->words || !desc->seg_cpos || !desc->seg_upos || desc->n_segments == 0)
+            return fail(GENIE_E_INVALID, "compressed index needs words, seg_cpos and seg_upos");
+        if (!desc->postings && !desc->single_value)
+            return fail(GENIE_E_INVALID, "compressed set-valued index needs the uncompressed postings too");
+    } else if (!desc->postings && desc->n_postings) {
         return fail(GENIE_E_INVALID, "missing postings");
+    }
     if (desc->n_postings >= 0xFFFFFFFFull)
         return fail(GENIE_E_INVALID, "index too large: more than 2^32-1 postings");
     if (desc->n_objects >= 0xFFFFFFFFu)
@@ -1013,9 +1495,12 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
     for (uint32_t k = 0; k < desc->n_keys; ++k)
         if (desc->key_start[k] > desc->key_start[k + 1])
             return fail(GENIE_E_INVALID, "key_start must be non-decreasing");
-    for (uint64_t p = 0; p < desc->n_postings; ++p)
-        if (desc->postings[p] >= desc->n_objects)
-            return fail(GENIE_E_INVALID, "posting id out of range at position " + std::to_string(p));
+    if (desc->postings)
+        for (uint64_t p = 0; p < desc->n_postings; ++p)
+            if (desc->postings[p] >= desc->n_objects)
+                return fail(GENIE_E_INVALID, "posting id out of range at position " + std::to_string(p));
+    if (desc->codec != 0)
+        return create_compressed(desc, out);
 
     genie_index* ix = new genie_index();
     DevIndex& d = ix->d;
@@ -1250,14 +1735,24 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     uint64_t nch = 0, logical = 0;
     std::vector<std::pair<uint32_t, uint32_t>> spans;
     spans.reserve(n_items);
+    bool prev_ok = false;
     for (uint64_t i = 0; i < n_items; ++i) {
         const genie_dim_t& it = sorted[i];
         if (it.end_pos > it.start_pos) {
             uint64_t len = (uint64_t)(it.end_pos - it.start_pos);
-            nch += (len + kChunk - 1) / kChunk;
+            if (d.codec == 0) {
+                nch += (len + kChunk - 1) / kChunk;
+            } else {
+                // runs of contiguous segments (k_crun_flags uses the same rule)
+                const genie_dim_t& pv = sorted[i - (i ? 1 : 0)];
+                const bool starts = i == 0 || !prev_ok || pv.query != it.query || pv.weight != it.weight ||
+                                    pv.end_pos != it.start_pos;
+                nch += starts ? 1 : 0;
+            }
             logical += 4 * len;
             spans.push_back({(uint32_t)it.start_pos, (uint32_t)it.end_pos});
         }
+        prev_ok = it.end_pos > it.start_pos;
     }
     std::sort(spans.begin(), spans.end());
     uint64_t compulsory = 0, cur_e = 0;
@@ -1316,6 +1811,14 @@ static int launch_scan(genie_plan* p, hipStream_t s)
 {
     const DevIndex& d = p->ix->d;
     ScanArgs a;
+    a.codec = d.codec;
+    a.words = d.words;
+    a.seg_cpos = d.seg_cpos;
+    a.seg_first = d.seg_first;
+    a.seg_last = d.seg_last;
+    a.key_seg = d.key_seg;
+    a.sdir_off = d.dir_off;
+    a.sdir = d.sdir;
     a.post = d.post;
     a.key_start = d.key_start;
     a.dir_off = d.dir_off;
@@ -1339,10 +1842,18 @@ static int launch_scan(genie_plan* p, hipStream_t s)
     const uint64_t blocks = (uint64_t)a.n_tiles * p->n_queries;
     if (blocks >= 0x7FFFFFFFull)
         return fail(GENIE_E_INVALID, "batch too large for one launch");
-    switch (p->lane_bits) {
-    case 8: k_scan_tile<8><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-    case 16: k_scan_tile<16><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-    default: k_scan_tile<32><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+    if (d.codec == 0) {
+        switch (p->lane_bits) {
+        case 8: k_scan_tile<8><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+        case 16: k_scan_tile<16><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+        default: k_scan_tile<32><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+        }
+    } else {
+        switch (p->lane_bits) {
+        case 8: k_scan_tile_c<8><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+        case 16: k_scan_tile_c<16><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+        default: k_scan_tile_c<32><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
+        }
     }
     return GENIE_OK;
 }
@@ -1357,8 +1868,8 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     fill_u32(p->d_gthr, 0, nq, s);
     fill_u32(p->d_cand_cnt, 0, nq, s);
     fill_u32(p->d_overflow, 0, 1, s);
-    // compile items into chunks
-    if (ni) {
+    // compile items into chunks (uncompressed) or segment runs (compressed)
+    if (ni && d.codec == 0) {
         k_chunk_count<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, p->d_nchunks);
         const uint32_t nb = (ni + 1023) / 1024;
         k_scan_blocks<<<nb, 256, 0, s>>>(p->d_nchunks, p->d_chunk_ptr, ni, p->d_block_sums);
@@ -1366,6 +1877,14 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
         k_scan_add<<<nb, 256, 0, s>>>(p->d_chunk_ptr, ni, p->d_block_sums);
         k_chunk_write<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, p->d_chunk_ptr, d.key_start,
                                                       d.n_keys, p->d_chunks);
+    } else if (ni) {
+        k_crun_flags<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, d.seg_cpos, d.n_segments, p->d_nchunks);
+        const uint32_t nb = (ni + 1023) / 1024;
+        k_scan_blocks<<<nb, 256, 0, s>>>(p->d_nchunks, p->d_chunk_ptr, ni, p->d_block_sums);
+        k_scan_sums<<<1, 256, 0, s>>>(p->d_block_sums, nb, p->d_total);
+        k_scan_add<<<nb, 256, 0, s>>>(p->d_chunk_ptr, ni, p->d_block_sums);
+        k_crun_write<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, d.seg_cpos, d.n_segments, d.seg_key,
+                                                     p->d_chunk_ptr, p->d_chunks);
     } else {
         fill_u32(p->d_total, 0, 1, s);
     }
@@ -1517,8 +2036,44 @@ int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_i
 int genie_hip_decode(int codec, const uint32_t* words, const uint32_t* cpos, uint32_t n_arrays,
                      uint32_t out_stride, uint32_t* out, uint32_t* out_len)
 {
-    (void)codec; (void)words; (void)cpos; (void)n_arrays; (void)out_stride; (void)out; (void)out_len;
-    return fail(GENIE_E_INVALID, "device decoders are not part of this build");
+    CodecShape cs;
+    if (!codec_shape(codec, &cs))
+        return fail(GENIE_E_INVALID, "unknown codec " + std::to_string(codec));
+    if (n_arrays == 0)
+        return GENIE_OK;
+    if (!words || !cpos || !out || !out_len)
+        return fail(GENIE_E_INVALID, "null argument");
+    const uint64_t nw = cpos[n_arrays];
+    for (uint32_t i = 0; i < n_arrays; ++i)
+        if (cpos[i + 1] < cpos[i])
+            return fail(GENIE_E_INVALID, "array offsets must be non-decreasing");
+    uint32_t *d_w = nullptr, *d_c = nullptr, *d_o = nullptr, *d_l = nullptr;
+    int rc = GENIE_OK;
+    do {
+        if ((rc = dev_alloc(&d_w, nw + 4))) break;
+        if ((rc = dev_alloc(&d_c, n_arrays + 1))) break;
+        if ((rc = dev_alloc(&d_o, (uint64_t)n_arrays * out_stride))) break;
+        if ((rc = dev_alloc(&d_l, n_arrays))) break;
+        hipError_t e;
+        if ((e = hipMemcpy(d_w, words, nw * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_c, cpos, (n_arrays + 1) * 4, hipMemcpyHostToDevice))) {
+            rc = fail(GENIE_E_RUNTIME, hipGetErrorString(e));
+            break;
+        }
+        const uint32_t blocks = std::min<uint32_t>((n_arrays + kDecodeWaves - 1) / kDecodeWaves, 65535);
+        k_decode_arrays<<<blocks, kDecodeWaves * kWave>>>(codec, d_w, d_c, n_arrays, out_stride, d_o, d_l);
+        if ((e = hipGetLastError()) || (e = hipDeviceSynchronize()) ||
+            (e = hipMemcpy(out, d_o, (uint64_t)n_arrays * out_stride * 4, hipMemcpyDeviceToHost)) ||
+            (e = hipMemcpy(out_len, d_l, n_arrays * 4, hipMemcpyDeviceToHost))) {
+            rc = fail(GENIE_E_RUNTIME, hipGetErrorString(e));
+            break;
+        }
+    } while (0);
+    dev_free(d_w);
+    dev_free(d_c);
+    dev_free(d_o);
+    dev_free(d_l);
+    return rc;
 }
 
 }  // extern "C"

@@ -175,6 +175,7 @@ int genie_table_compile(genie_table* tt, const genie_range_t* ranges, uint64_t n
         std::vector<genie::query::Query> qs = make_queries(*tt->t, ranges, n_ranges, n_queries, use_load_balance, 1);
         std::vector<genie::query::Query::dim> dims;
         genie::matching::build_queries(qs, *tt->t, dims, max_load);
+        *n_out = dims.size();
         if (dims.size() > cap)
             throw genie::exception::cpu_runtime_error("output capacity too small");
         static_assert(sizeof(genie::query::Query::dim) == sizeof(genie_dim_t), "layout");

@@ -1,0 +1,107 @@
+"""Compressed posting lists on the GPU: the wave decoders against the oracle's
+codecs, and compressed search against uncompressed search and the oracle
+(bit-exact counts; compressed and uncompressed layouts compared as in
+test/test_compression_kernels.cu:236-258, for every query and slot)."""
+import numpy as np
+import pytest
+
+from helpers import assert_topk_equal
+from oracle import oracle as O
+from test_oracle import mixed_values
+
+import genie_amd
+from genie_amd import Table, match_topk, workloads as W
+from genie_amd._native import decode as gpu_decode
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("codec", range(1, 14))
+def test_device_decoders_match_oracle(codec):
+    vals = mixed_values(1024, seed=666 + codec)
+    lengths = list(range(1, 130)) + list(range(130, 1025, 29)) + [1023, 1024]
+    words, cpos = [], [0]
+    inputs = []
+    for i, n in enumerate(lengths):
+        x = vals[:n] if i % 2 else np.sort(vals[:n])
+        w = O.encode(codec, x)
+        if len(w) > 1024:  # the engine decodes segments of at most 1,024 words
+            continue
+        inputs.append(x)
+        words.append(w)
+        cpos.append(cpos[-1] + len(w))
+    out = gpu_decode(codec, np.concatenate(words), np.array(cpos, np.uint32))
+    for x, y in zip(inputs, out):
+        assert y is not None and np.array_equal(x, y), (codec, len(x))
+
+
+def _compare(rows, ranges, nq, k, codec, max_length=1024, label=""):
+    plain = Table.from_rows(rows)
+    comp = Table.from_rows(rows, max_length, True, codec=codec)
+    ids_u, cnt_u = match_topk(plain.device(), plain.compile(ranges, nq), nq, k)
+    ids_c, cnt_c = match_topk(comp.device(), comp.compile(ranges, nq), nq, k)
+    assert np.array_equal(cnt_u, cnt_c) and np.array_equal(ids_u, ids_c), f"{label} codec {codec}"
+    return ids_c, cnt_c
+
+
+@pytest.mark.parametrize("codec", range(1, 14))
+def test_compressed_search_every_codec(codec):
+    w = W.cfg1(n=60_000, d=16, q=6, k=40)
+    ids, cnt = _compare(w["rows"], w["ranges"], w["n_queries"], w["k"], codec, 512, "zipf")
+    res_i, res_c = [], []
+    for q in range(w["n_queries"]):
+        i, c = O.topk(O.count_exhaustive_single(w["rows"], w["ranges"][w["ranges"]["query"] == q]), w["k"])
+        res_i.append(i)
+        res_c.append(c)
+    assert_topk_equal(ids, cnt, np.concatenate(res_i), np.concatenate(res_c), w["k"], w["n_queries"], "oracle")
+
+
+def test_lsh_heavyweight_many_tiles():
+    # LSH-like: uniform buckets, exact match, SERIAL_DELTA_COMP_BP32_VARINT
+    w = W.cfg2(n=300_000, d=32, q=24, k=100, buckets=2048)
+    _compare(w["rows"], w["ranges"], w["n_queries"], w["k"], 13, 1024, "lsh")
+
+
+def test_zipf_middleweight_ranges_and_mixed_k():
+    w = W.cfg4(n=150_000, d=24, q=20)
+    _compare(w["rows"], w["ranges"], w["n_queries"], w["k"], 11, 1024, "cfg4-like")
+
+
+def test_compressed_bijectmap_weights():
+    rp, vals = genie_amd.synth.sets(41, 40_000, 5, 40, 2000, 1.05)
+    qp, qv = genie_amd.synth.sets(42, 16, 1, 64, 2000, 1.05)
+    ranges = W.set_ranges(qp, qv, genie_amd.synth.uniform_int(43, 0, len(qv), 1, 4))
+    plain = Table.from_sets(rp, vals)
+    comp = Table.from_sets(rp, vals, 1024, True, codec=6)
+    a = match_topk(plain.device(), plain.compile(ranges, 16), 16, 100)
+    b = match_topk(comp.device(), comp.compile(ranges, 16), 16, 100)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_corrupt_segment_is_rejected():
+    w = W.cfg0(n=600, d=4, q=1)
+    t = Table.from_rows(w["rows"], 64, True, codec=3)
+    e = t.export()
+    words = e["words"].copy()
+    words[1] ^= 0x1F000000  # first BP32 header: an impossible bit width
+    with pytest.raises(genie_amd.GenieError):
+        genie_amd.Index.create(600, 4, e["inv"], _key_start(e), _key_dim(e), e["inv_pos"], True, 3, words, e["cpos"])
+    # the untouched words build fine
+    genie_amd.Index.create(600, 4, e["inv"], _key_start(e), _key_dim(e), e["inv_pos"], True, 3, e["words"], e["cpos"])
+
+
+def _key_start(e):
+    ks = [0]
+    for d in range(len(e["lo"])):
+        seg = e["keyseg"][_ks_off(e, d):_ks_off(e, d + 1)]
+        for v in range(e["hi"][d] - e["lo"][d] + 1):
+            ks.append(e["inv_pos"][seg[v + 1]])
+    return np.array(ks, np.uint64)
+
+
+def _key_dim(e):
+    return np.concatenate([np.full(e["hi"][d] - e["lo"][d] + 1, d, np.uint32) for d in range(len(e["lo"]))])
+
+
+def _ks_off(e, d):
+    return int(sum(e["hi"][j] - e["lo"][j] + 2 for j in range(d)))
