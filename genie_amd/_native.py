@@ -70,6 +70,12 @@ _sig(_hip, "genie_hip_plan_download", C.c_int, _p, _p, _p, _p)
 _sig(_hip, "genie_hip_plan_set_timing", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_kernel_ms", C.c_int, _p, C.c_char_p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_plan_bytes", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64))
+_sig(_hip, "genie_hip_plan_set_graph", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_set_config", C.c_int, _p, _u32, _u32, _u32)
+_sig(_hip, "genie_hip_plan_get_config", C.c_int, _p, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32),
+     C.POINTER(_u32))
+_sig(_hip, "genie_hip_plan_autotune", C.c_int, _p, C.c_int, _p, C.POINTER(_u32))
+_sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_decode", C.c_int, C.c_int, _p, _p, _u32, _u32, _p, _p)
 _sig(_hip, "genie_hip_last_error", C.c_char_p)
 _sig(_hip, "genie_hip_stream_synchronize", C.c_int, _p)
@@ -232,6 +238,28 @@ class Plan:
         a, b = _u64(), _u64()
         _chk_hip(_hip.genie_hip_plan_bytes(self.handle, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_graph(self, on=True):
+        _chk_hip(_hip.genie_hip_plan_set_graph(self.handle, 1 if on else 0))
+
+    def set_config(self, block, tile_objs, seglen):
+        _chk_hip(_hip.genie_hip_plan_set_config(self.handle, block, tile_objs, seglen))
+
+    def config(self):
+        v = [_u32() for _ in range(4)]
+        _chk_hip(_hip.genie_hip_plan_get_config(self.handle, *[C.byref(x) for x in v]))
+        return dict(block=v[0].value, tile_objs=v[1].value, seglen=v[2].value, lane_bits=v[3].value)
+
+    def autotune(self, reps=2):
+        ms = np.full(16, -1.0)
+        n = _u32()
+        _chk_hip(_hip.genie_hip_plan_autotune(self.handle, reps, ms.ctypes.data_as(_p), C.byref(n)))
+        return ms[: n.value].copy()
+
+    def host_ms(self):
+        v = C.c_double()
+        _chk_hip(_hip.genie_hip_plan_host_ms(self.handle, C.byref(v)))
+        return v.value
 
     def close(self):
         if self.handle:

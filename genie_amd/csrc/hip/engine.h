@@ -55,6 +55,7 @@ struct DevIndex {
     std::vector<uint32_t> h_key_mult;  // max multiplicity of an id within each key's list
     std::vector<uint32_t> h_seg_cpos;
     std::vector<uint32_t> h_seg_key;
+    std::vector<uint32_t> h_key_seg;
 };
 
 void set_error(const std::string& msg);

@@ -17,6 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -27,6 +30,8 @@
 
 #include "codecs_dev.h"
 #include "engine.h"
+#include "kernels.h"
+#include "kernels_batched.h"
 #include "wave.h"
 
 namespace gh {
@@ -71,1032 +76,6 @@ static void dev_free(T*& p)
     if (p)
         (void)hipFree(p);
     p = nullptr;
-}
-
-// ---------------------------------------------------------------------------
-// Small utility kernels
-// ---------------------------------------------------------------------------
-
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n)
-{
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    for (; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
-
-static void fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
-{
-    if (n == 0)
-        return;
-    uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
-    k_fill_u32<<<blocks, 256, 0, s>>>(p, v, n);
-}
-
-// Exclusive scan of n u32 values, 1024 per block (256 threads x 4).
-__global__ void __launch_bounds__(256) k_scan_blocks(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                     uint32_t* block_sums)
-{
-    __shared__ uint32_t s_w[4];
-    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
-    uint32_t v[4], t = 0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        v[j] = (base + j < n) ? in[base + j] : 0u;
-        t += v[j];
-    }
-    uint32_t incl = wave_incl_scan(t);
-    const int w = threadIdx.x >> 6;
-    if (lane_id() == 63)
-        s_w[w] = incl;
-    __syncthreads();
-    uint32_t woff = 0;
-    for (int i = 0; i < w; ++i)
-        woff += s_w[i];
-    uint32_t run = woff + incl - t;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (base + j < n)
-            out[base + j] = run;
-        run += v[j];
-    }
-    if (threadIdx.x == 255)
-        block_sums[blockIdx.x] = woff + incl;
-}
-
-__global__ void __launch_bounds__(256) k_scan_sums(uint32_t* sums, uint32_t nb, uint32_t* total)
-{
-    __shared__ uint32_t s_w[4];
-    __shared__ uint32_t s_carry;
-    if (threadIdx.x == 0)
-        s_carry = 0;
-    __syncthreads();
-    for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
-        uint32_t i = b0 + threadIdx.x;
-        uint32_t v = i < nb ? sums[i] : 0u;
-        uint32_t incl = wave_incl_scan(v);
-        const int w = threadIdx.x >> 6;
-        if (lane_id() == 63)
-            s_w[w] = incl;
-        __syncthreads();
-        uint32_t woff = s_carry;
-        for (int j = 0; j < w; ++j)
-            woff += s_w[j];
-        if (i < nb)
-            sums[i] = woff + incl - v;
-        __syncthreads();
-        if (threadIdx.x == 255)
-            s_carry = woff + incl;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        *total = s_carry;
-}
-
-__global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums)
-{
-    uint32_t i = blockIdx.x * 1024u + threadIdx.x;
-    uint32_t add = sums[blockIdx.x];
-    for (int j = 0; j < 4; ++j, i += 256)
-        if (i < n)
-            out[i] += add;
-}
-
-// ---------------------------------------------------------------------------
-// Index directory: for every long key and every directory-tile boundary b,
-// the position inside the key's list of the first id >= b * kDirTile.
-// ---------------------------------------------------------------------------
-
-__global__ void k_build_dir(const uint32_t* __restrict__ post, const uint32_t* __restrict__ key_start,
-                            const uint32_t* __restrict__ dir_keys, uint32_t n_dir_keys,
-                            uint32_t n_dtiles, uint32_t* __restrict__ dir)
-{
-    const uint64_t per = (uint64_t)n_dtiles + 1;
-    const uint64_t total = (uint64_t)n_dir_keys * per;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t dk = (uint32_t)(i / per), b = (uint32_t)(i % per);
-        const uint32_t k = dir_keys[dk];
-        const uint32_t s = key_start[k], e = key_start[k + 1];
-        const uint64_t target = (uint64_t)b * kDirTile;
-        uint32_t lo = s, hi = e;  // first position with post[p] >= target
-        while (lo < hi) {
-            uint32_t mid = lo + ((hi - lo) >> 1);
-            if ((uint64_t)post[mid] < target)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        dir[i] = lo - s;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Compressed index: segment bounds and segment directory
-// ---------------------------------------------------------------------------
-
-constexpr int kDecodeWaves = 4;
-
-// Decode every segment once: first and last id, decoded length check
-// (against the uncompressed segment positions) and ascending order.
-__global__ void __launch_bounds__(kDecodeWaves * kWave)
-    k_seg_bounds(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
-                 const uint32_t* __restrict__ upos, uint32_t n_seg, uint32_t n_objects,
-                 uint32_t* __restrict__ first, uint32_t* __restrict__ last, uint32_t* __restrict__ err)
-{
-    __shared__ DecodeScratch S[kDecodeWaves];
-    const int wv = threadIdx.x / kWave, lane = lane_id();
-    for (uint32_t sgi = blockIdx.x * kDecodeWaves + wv; sgi < n_seg; sgi += gridDim.x * kDecodeWaves) {
-        const uint32_t w0 = cpos[sgi], nw = cpos[sgi + 1] - w0;
-        const uint32_t want = upos[sgi + 1] - upos[sgi];
-        if (nw > kSegMax || want > kSegMax) {
-            if (lane == 0)
-                atomicOr(err, 1u);
-            continue;
-        }
-        stage_words(S[wv], words, w0, nw);
-        uint32_t v[kPerLane];
-        const uint32_t n = wave_decode(codec, S[wv], nw, v);
-        bool bad = n != want || n == 0;
-        uint32_t prev = lane ? 0u : 0u;
-        // ascending within the segment and inside the object range
-        uint32_t lastv = (uint32_t)__shfl_up((int)v[kPerLane - 1], 1, kWave);
-        if (lane == 0)
-            lastv = 0;
-        for (int j = 0; j < kPerLane; ++j) {
-            const uint32_t i = lane * kPerLane + j;
-            if (i < n) {
-                const uint32_t before = j ? v[j - 1] : lastv;
-                if ((i > 0 && v[j] < before) || v[j] >= n_objects)
-                    bad = true;
-            }
-        }
-        (void)prev;
-        if (ballot(bad) != 0) {
-            if (lane == 0)
-                atomicOr(err, 1u);
-            wave_sync();
-            continue;
-        }
-        if (!bad && n) {
-            if (lane == 0)
-                first[sgi] = v[0];
-            if (lane == (int)((n - 1) / kPerLane))
-                last[sgi] = v[(n - 1) % kPerLane];
-        }
-        wave_sync();
-    }
-}
-
-// For every long key and directory-tile boundary b: the first segment of the
-// key (relative) whose last id is >= b * kDirTile.
-__global__ void k_build_sdir(const uint32_t* __restrict__ seg_last, const uint32_t* __restrict__ key_seg,
-                             const uint32_t* __restrict__ dir_keys, uint32_t n_dir_keys, uint32_t n_dtiles,
-                             uint32_t* __restrict__ sdir)
-{
-    const uint64_t per = (uint64_t)n_dtiles + 1;
-    const uint64_t total = (uint64_t)n_dir_keys * per;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t k = dir_keys[i / per], b = (uint32_t)(i % per);
-        const uint32_t s = key_seg[k], e = key_seg[k + 1];
-        const uint64_t target = (uint64_t)b * kDirTile;
-        uint32_t lo = s, hi = e;
-        while (lo < hi) {
-            const uint32_t mid = lo + ((hi - lo) >> 1);
-            if ((uint64_t)seg_last[mid] < target)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        sdir[i] = lo - s;
-    }
-}
-
-// Stand-alone decode of n arrays (tests, measurement).
-__global__ void __launch_bounds__(kDecodeWaves * kWave)
-    k_decode_arrays(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
-                    uint32_t n_arrays, uint32_t stride, uint32_t* __restrict__ out, uint32_t* __restrict__ out_len)
-{
-    __shared__ DecodeScratch S[kDecodeWaves];
-    const int wv = threadIdx.x / kWave, lane = lane_id();
-    for (uint32_t a = blockIdx.x * kDecodeWaves + wv; a < n_arrays; a += gridDim.x * kDecodeWaves) {
-        const uint32_t w0 = cpos[a], nw = cpos[a + 1] - w0;
-        if (nw > kSegMax) {
-            if (lane == 0)
-                out_len[a] = 0xFFFFFFFFu;
-            continue;
-        }
-        stage_words(S[wv], words, w0, nw);
-        uint32_t v[kPerLane];
-        const uint32_t n = wave_decode(codec, S[wv], nw, v);
-        if (lane == 0)
-            out_len[a] = n;
-        if (n != 0xFFFFFFFFu) {
-            const uint32_t lim = min(n, stride);
-            for (int j = 0; j < kPerLane; ++j) {
-                const uint32_t i = lane * kPerLane + j;
-                if (i < lim)
-                    out[(uint64_t)a * stride + i] = v[j];
-            }
-        }
-        wave_sync();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Device-side item compilation
-// ---------------------------------------------------------------------------
-
-// Last key whose first position is <= pos (key_start is non-decreasing).
-__device__ __forceinline__ uint32_t key_of(const uint32_t* key_start, uint32_t n_keys, uint32_t pos)
-{
-    uint32_t lo = 0, hi = n_keys;  // answer in [0, n_keys)
-    while (hi - lo > 1) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        if (key_start[mid] <= pos)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-__global__ void k_chunk_count(const genie_dim_t* __restrict__ items, uint32_t n_items,
-                              uint32_t* __restrict__ nchunks)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_items)
-        return;
-    const genie_dim_t it = items[i];
-    uint32_t len = it.end_pos > it.start_pos ? (uint32_t)(it.end_pos - it.start_pos) : 0u;
-    nchunks[i] = (len + kChunk - 1) / kChunk;
-}
-
-__global__ void k_chunk_write(const genie_dim_t* __restrict__ items, uint32_t n_items,
-                              const uint32_t* __restrict__ chunk_ptr,
-                              const uint32_t* __restrict__ key_start, uint32_t n_keys,
-                              CChunk* __restrict__ chunks)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_items)
-        return;
-    const genie_dim_t it = items[i];
-    if (it.end_pos <= it.start_pos)
-        return;
-    const uint32_t w = (uint32_t)it.weight;
-    uint32_t c = chunk_ptr[i];
-    for (uint32_t sp = (uint32_t)it.start_pos; sp < (uint32_t)it.end_pos; sp += kChunk, ++c) {
-        uint32_t ep = min((uint32_t)it.end_pos, sp + kChunk);
-        CChunk ch;
-        ch.k0 = key_of(key_start, n_keys, sp);
-        ch.k1 = key_of(key_start, n_keys, ep - 1);
-        ch.sp = sp;
-        ch.ep = ep;
-        ch.w = w;
-        ch.q = (uint32_t)it.query;
-        ch.kp = 0;
-        ch.pad = 0;
-        chunks[c] = ch;
-    }
-}
-
-// Number of keys each chunk covers (for the pair prefix).
-__global__ void k_chunk_nkeys(const CChunk* __restrict__ chunks, const uint32_t* __restrict__ total,
-                              uint32_t* __restrict__ nk)
-{
-    const uint32_t n = *total;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
-        nk[c] = chunks[c].k1 - chunks[c].k0 + 1;
-}
-
-__global__ void k_chunk_set_kp(CChunk* __restrict__ chunks, const uint32_t* __restrict__ total,
-                               const uint32_t* __restrict__ kptr)
-{
-    const uint32_t n = *total;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
-        chunks[c].kp = kptr[c];
-}
-
-// qk_ptr[q] = first pair of query q.
-__global__ void k_query_pair_ptr(const uint32_t* __restrict__ qc_ptr, uint32_t n_queries,
-                                 const uint32_t* __restrict__ kptr, const uint32_t* __restrict__ total_chunks,
-                                 const uint32_t* __restrict__ total_pairs, uint32_t* __restrict__ qk_ptr)
-{
-    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q > n_queries)
-        return;
-    uint32_t c = qc_ptr[q];
-    qk_ptr[q] = c >= *total_chunks ? *total_pairs : kptr[c];
-}
-
-// Compressed items cover whole segments; consecutive items of one query with
-// the same weight and adjacent segments merge into one run (a chunk whose
-// sp/ep are segment indices), so a workgroup jumps over a long list with the
-// segment directory instead of visiting one item per segment.
-__device__ __forceinline__ uint32_t seg_at(const uint32_t* cpos, uint32_t n_seg, uint32_t pos)
-{
-    uint32_t lo = 0, hi = n_seg;  // last s with cpos[s] <= pos
-    while (hi - lo > 1) {
-        const uint32_t mid = lo + ((hi - lo) >> 1);
-        if (cpos[mid] <= pos)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ bool citem_segs(const genie_dim_t& it, const uint32_t* cpos, uint32_t n_seg,
-                                           uint32_t* s0, uint32_t* s1)
-{
-    if (it.end_pos <= it.start_pos)
-        return false;
-    *s0 = seg_at(cpos, n_seg, (uint32_t)it.start_pos);
-    *s1 = seg_at(cpos, n_seg, (uint32_t)it.end_pos - 1) + 1;
-    return true;
-}
-
-__device__ __forceinline__ bool crun_starts(const genie_dim_t* items, uint32_t i, const uint32_t* cpos,
-                                            uint32_t n_seg)
-{
-    uint32_t a0, a1, b0, b1;
-    if (!citem_segs(items[i], cpos, n_seg, &a0, &a1))
-        return false;
-    if (i == 0 || !citem_segs(items[i - 1], cpos, n_seg, &b0, &b1))
-        return true;
-    return items[i].query != items[i - 1].query || items[i].weight != items[i - 1].weight || a0 != b1;
-}
-
-__global__ void k_crun_flags(const genie_dim_t* __restrict__ items, uint32_t n_items,
-                             const uint32_t* __restrict__ cpos, uint32_t n_seg, uint32_t* __restrict__ flag)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_items)
-        flag[i] = crun_starts(items, i, cpos, n_seg) ? 1u : 0u;
-}
-
-__global__ void k_crun_write(const genie_dim_t* __restrict__ items, uint32_t n_items,
-                             const uint32_t* __restrict__ cpos, uint32_t n_seg,
-                             const uint32_t* __restrict__ seg_key, const uint32_t* __restrict__ rptr,
-                             CChunk* __restrict__ runs)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_items)
-        return;
-    uint32_t s0, s1;
-    if (!citem_segs(items[i], cpos, n_seg, &s0, &s1))
-        return;
-    const bool start = crun_starts(items, i, cpos, n_seg);
-    const uint32_t r = start ? rptr[i] : rptr[i] - 1;
-    if (start) {
-        runs[r].sp = s0;
-        runs[r].k0 = seg_key[s0];
-        runs[r].w = (uint32_t)items[i].weight;
-        runs[r].q = (uint32_t)items[i].query;
-        runs[r].kp = 0;
-        runs[r].pad = 0;
-    }
-    if (i + 1 == n_items || crun_starts(items, i + 1, cpos, n_seg) ||
-        items[i + 1].end_pos <= items[i + 1].start_pos) {
-        runs[r].ep = s1;
-        runs[r].k1 = seg_key[s1 - 1];
-    }
-}
-
-// qc_ptr[q] = first chunk of query q (items sorted by query, item_qptr CSR).
-__global__ void k_query_chunk_ptr(const uint32_t* __restrict__ item_qptr, uint32_t n_queries,
-                                  const uint32_t* __restrict__ chunk_ptr, uint32_t n_items,
-                                  const uint32_t* __restrict__ total, uint32_t* __restrict__ qc_ptr)
-{
-    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q > n_queries)
-        return;
-    uint32_t ip = item_qptr[q];
-    qc_ptr[q] = ip >= n_items ? *total : chunk_ptr[ip];
-}
-
-// ---------------------------------------------------------------------------
-// Scan + count + per-tile select (plain form)
-// ---------------------------------------------------------------------------
-
-constexpr int kScanThreads = 512;
-constexpr int kScanWaves = kScanThreads / kWave;
-constexpr int kCntWords = 16384;  // 64 KiB of counters per workgroup
-constexpr uint32_t kEmitFail = 0xFFFFFFFFu;
-
-struct ScanArgs {
-    // compressed index (codec != 0)
-    int codec;
-    const uint32_t* words;
-    const uint32_t* seg_cpos;
-    const uint32_t* seg_first;
-    const uint32_t* seg_last;
-    const uint32_t* key_seg;
-    const uint32_t* sdir_off;
-    const uint32_t* sdir;
-    // uncompressed index
-    const uint32_t* post;
-    const uint32_t* key_start;
-    const uint32_t* dir_off;
-    const uint32_t* dir;
-    const CChunk* chunks;
-    const uint32_t* qc_ptr;
-    const uint32_t* qk_ptr;
-    const uint32_t* kq;
-    uint32_t n_objects, n_queries, n_tiles, tile_objs, dt_per_tile, n_dtiles;
-    uint32_t* gthr;       // per query adaptive threshold (max of finished tiles' k-th counts)
-    uint32_t* cand_cnt;   // per query candidates appended
-    const uint32_t* cand_cap;
-    const uint64_t* cand_base;
-    uint64_t* cand;       // keys: count << 32 | ~id
-    uint32_t* overflow;
-};
-
-template <int LB>
-__device__ __forceinline__ uint32_t lane_count(uint32_t v, int j)
-{
-    if constexpr (LB == 32)
-        return v;
-    else
-        return (v >> (LB * j)) & ((1u << LB) - 1u);
-}
-
-__device__ __forceinline__ uint64_t make_key(uint32_t count, uint32_t id)
-{
-    return ((uint64_t)count << 32) | (uint32_t)(~id);
-}
-
-// Count postings of one key sub-span [a, b) into the tile counters.  Lanes
-// read 16-byte groups of ids, four groups per lane issued back to back (the
-// loads carry no branches, so all four are in flight together: indexes past
-// the end are clamped to the last group and their ids are not counted);
-// the unaligned head and tail go through single-id loads.  FILTER: the span
-// may hold ids outside the tile (short lists without a directory entry).
-template <int LB, bool FILTER>
-__device__ __forceinline__ void count_one(uint32_t id, uint32_t obj0, uint32_t tile_objs, uint32_t w,
-                                          uint32_t* cnt)
-{
-    constexpr int PER = 32 / LB;
-    constexpr int SH = PER == 4 ? 2 : (PER == 2 ? 1 : 0);
-    const uint32_t loc = id - obj0;
-    // Ids outside the tile (and lanes with nothing to add, w == 0) go to a
-    // per-lane scratch word past the counters: no branch, no shared address.
-    const bool ok = !FILTER || loc < tile_objs;
-    const uint32_t word = ok ? (loc >> SH) : (uint32_t)(kCntWords + lane_id());
-    atomicAdd(&cnt[word], ok ? w << ((loc & (PER - 1)) * LB) : 0u);
-}
-
-template <int LB, bool FILTER>
-__device__ __forceinline__ void count_span(const uint32_t* __restrict__ post, uint32_t a, uint32_t b,
-                                           uint32_t obj0, uint32_t tile_objs, uint32_t w, uint32_t* cnt)
-{
-    const int lane = lane_id();
-    const uint32_t a4 = (a + 3u) & ~3u;        // first 16-byte aligned position
-    const uint32_t b4 = b & ~3u;               // end of the aligned body
-    if (a4 >= b4) {                            // short span: one id per lane
-        for (uint32_t p = a + lane; p < b; p += kWave)
-            count_one<LB, FILTER>(post[p], obj0, tile_objs, w, cnt);
-        return;
-    }
-    if (lane < (int)(a4 - a))
-        count_one<LB, FILTER>(post[a + lane], obj0, tile_objs, w, cnt);
-    if (lane < (int)(b - b4))
-        count_one<LB, FILTER>(post[b4 + lane], obj0, tile_objs, w, cnt);
-    const uint4* body = reinterpret_cast<const uint4*>(post + a4);
-    const uint32_t ng = (b4 - a4) >> 2;        // 16-byte groups, >= 1
-    for (uint32_t g0 = 0; g0 < ng; g0 += 4 * kWave) {
-        uint4 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            v[u] = body[min(g0 + u * kWave + lane, ng - 1)];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            // lanes past the end count into the scratch word instead of
-            // branching, which would let the compiler sink their load
-            const bool in = g0 + u * kWave + lane < ng;
-            const uint32_t past = obj0 + tile_objs;  // an id just outside the tile
-            count_one<LB, true>(in ? v[u].x : past, obj0, tile_objs, w, cnt);
-            count_one<LB, true>(in ? v[u].y : past, obj0, tile_objs, w, cnt);
-            count_one<LB, true>(in ? v[u].z : past, obj0, tile_objs, w, cnt);
-            count_one<LB, true>(in ? v[u].w : past, obj0, tile_objs, w, cnt);
-        }
-    }
-}
-
-// Shared scratch of the select passes.
-struct SelShared {
-    uint32_t hist[64];
-    uint32_t wv[kScanWaves];
-    uint32_t res[8];
-};
-
-// Exact per-tile top-k under (count desc, id asc), pruned by the query's
-// adaptive threshold, appended to the query's candidate list.  Called by all
-// threads of the workgroup after the counters are complete.
-template <int LB>
-__device__ __forceinline__ void tile_select(const ScanArgs& a, const uint32_t* cnt, uint32_t q, uint32_t obj0,
-                                            SelShared& S)
-{
-    constexpr int PER = 32 / LB;
-    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-    uint32_t* s_hist = S.hist;
-    uint32_t* s_wv = S.wv;
-    uint32_t* s_res = S.res;
-    // ---- select: exact tile top-k under (count desc, id asc) ------------
-    const uint32_t k = a.kq[q];
-    const uint32_t wpw = kCntWords / kScanWaves;  // words per wave (contiguous)
-    const uint32_t wbeg = wv * wpw;
-
-    uint32_t m = 0;
-    for (uint32_t i = lane; i < wpw; i += kWave) {
-        uint32_t v = cnt[wbeg + i];
-#pragma unroll
-        for (int j = 0; j < PER; ++j)
-            m = max(m, lane_count<LB>(v, j));
-    }
-    m = wave_max(m);
-    if (lane == 0)
-        s_wv[wv] = m;
-    if (tid == 0)  // one read, shared, so every thread takes the same branches
-        s_res[6] = __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    uint32_t M = 0;
-    for (int i = 0; i < kScanWaves; ++i)
-        M = max(M, s_wv[i]);
-    const uint32_t lower = max(1u, s_res[6]);
-    if (M < lower)
-        return;  // nothing in this tile can reach the query's top-k
-
-    // Threshold search in 64-wide count windows from the top.
-    uint32_t top = M, cum = 0, ct = lower, ngt = 0, neq = 0, found = 0;
-    for (;;) {
-        const uint32_t wlo = top >= lower + 63 ? top - 63 : lower;
-        __syncthreads();
-        if (tid < 64)
-            s_hist[tid] = 0;
-        __syncthreads();
-        for (uint32_t i = lane; i < wpw; i += kWave) {
-            uint32_t v = cnt[wbeg + i];
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                uint32_t c = lane_count<LB>(v, j);
-                hist_add_aggregated(s_hist, c - wlo, c >= wlo && c <= top);
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t f = 0, cc = wlo, g = cum, e = 0;
-            for (uint32_t c = top + 1; c-- > wlo;) {
-                uint32_t h = s_hist[c - wlo];
-                if (g + h >= k) {
-                    f = 1;
-                    cc = c;
-                    e = h;
-                    break;
-                }
-                g += h;
-            }
-            s_res[0] = f;
-            s_res[1] = cc;
-            s_res[2] = g;
-            s_res[3] = e;
-        }
-        __syncthreads();
-        found = s_res[0];
-        cum = s_res[2];
-        if (found) {
-            ct = s_res[1];
-            ngt = cum;
-            neq = s_res[3];
-            break;
-        }
-        if (wlo == lower) {
-            ct = lower;  // fewer than k objects reach `lower`: emit them all
-            ngt = cum;
-            neq = 0;
-            break;
-        }
-        top = wlo - 1;
-    }
-
-    const uint32_t need = found ? k - ngt : 0;
-    const bool trim = found && neq > need;
-    const uint32_t nemit = found ? ngt + min(neq, need) : ngt;
-    if (tid == 0) {
-        uint32_t base = atomicAdd(&a.cand_cnt[q], nemit);
-        if ((uint64_t)base + nemit > a.cand_cap[q]) {
-            atomicOr(a.overflow, 1u);
-            base = kEmitFail;
-        }
-        if (found)
-            atomicMax(&a.gthr[q], ct);
-        s_res[4] = base;
-        s_res[5] = 0;  // unordered slot counter
-    }
-    __syncthreads();
-    const uint32_t base = s_res[4];
-    if (base == kEmitFail || nemit == 0)
-        return;
-    uint64_t* out = a.cand + a.cand_base[q] + base;
-
-    // Unordered emission: objects strictly above ct (and, when nothing has to
-    // be trimmed, those equal to ct as well).
-    const uint32_t cut = trim ? ct + 1 : ct;
-    for (uint32_t i = lane; i < wpw; i += kWave) {
-        uint32_t v = cnt[wbeg + i];
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            uint32_t c = lane_count<LB>(v, j);
-            bool take = c >= cut;
-            uint64_t bm = ballot(take);
-            if (bm) {
-                uint32_t wbase = 0;
-                if (lane == __builtin_ctzll(bm))
-                    wbase = atomicAdd(&s_res[5], (uint32_t)__popcll(bm));
-                wbase = (uint32_t)__shfl((int)wbase, __builtin_ctzll(bm), kWave);
-                if (take) {
-                    uint32_t id = obj0 + (wbeg + i) * PER + j;
-                    out[wbase + lane_prefix(bm)] = make_key(c, id);
-                }
-            }
-        }
-    }
-    if (!trim)
-        return;
-
-    // Ordered emission of ties at ct: the `need` smallest ids.  Pass A counts
-    // ties per wave range, pass B ranks them in id order.
-    uint32_t e = 0;
-    for (uint32_t i = lane; i < wpw; i += kWave) {
-        uint32_t v = cnt[wbeg + i];
-#pragma unroll
-        for (int j = 0; j < PER; ++j)
-            e += lane_count<LB>(v, j) == ct;
-    }
-    e = wave_sum(e);
-    __syncthreads();
-    if (lane == 0)
-        s_wv[wv] = e;
-    __syncthreads();
-    uint32_t run = 0;
-    for (int i = 0; i < wv; ++i)
-        run += s_wv[i];
-    for (uint32_t i0 = 0; i0 < wpw && run < need; i0 += kWave) {
-        const uint32_t i = i0 + lane;
-        uint32_t v = i < wpw ? cnt[wbeg + i] : 0u;
-        uint32_t mask = 0;
-#pragma unroll
-        for (int j = 0; j < PER; ++j)
-            mask |= (uint32_t)(i < wpw && lane_count<LB>(v, j) == ct) << j;
-        const uint32_t pc = (uint32_t)__popc(mask);
-        const uint32_t incl = wave_incl_scan(pc);
-        uint32_t r = run + incl - pc;
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            if (mask & (1u << j)) {
-                if (r < need) {
-                    uint32_t id = obj0 + (wbeg + i) * PER + j;
-                    out[ngt + r] = make_key(ct, id);
-                }
-                ++r;
-            }
-        }
-        run += (uint32_t)__shfl((int)incl, 63, kWave);
-    }
-}
-
-// Sub-span of one (chunk, key) pair inside the workgroup's tile.
-struct Span {
-    uint32_t lo, hi;   // posting positions
-    uint32_t wf;       // weight << 1 | filter
-};
-constexpr int kSpanCap = 1024;
-
-template <int LB>
-__global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
-{
-    __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];  // counters + per-lane scratch
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(s_cnt4);
-    __shared__ Span s_span[kSpanCap];
-    __shared__ uint32_t s_next;
-    __shared__ SelShared s_sel;
-
-    const uint32_t t = blockIdx.x / a.n_queries;
-    const uint32_t q = blockIdx.x % a.n_queries;
-    const uint32_t obj0 = t * a.tile_objs;
-    const uint32_t dt0 = t * a.dt_per_tile;
-    const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
-    const int tid = threadIdx.x, lane = lane_id();
-
-    for (int i = tid; i < kCntWords / 4; i += kScanThreads)
-        s_cnt4[i] = make_uint4(0, 0, 0, 0);
-
-    // ---- count ------------------------------------------------------------
-    // Rounds of up to kSpanCap (chunk, key) pairs: every thread resolves the
-    // pairs of its chunks inside the round's window (independent loads, so
-    // the metadata latency is paid once per round), then the waves take the
-    // resolved spans dynamically.
-    const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
-    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
-    for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
-        const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
-        if (tid == 0)
-            s_next = 0;
-        for (uint32_t c = c0 + tid; c < c1; c += kScanThreads) {
-            const CChunk ch = a.chunks[c];
-            const uint32_t cp = ch.kp - pbase;                  // chunk's first pair, query-relative
-            const uint32_t nk = ch.k1 - ch.k0 + 1;
-            if (cp + nk <= w0 || cp >= w0 + wn)
-                continue;
-            const uint32_t j0 = cp < w0 ? w0 - cp : 0;
-            const uint32_t j1 = min(nk, w0 + wn - cp);
-            for (uint32_t j = j0; j < j1; ++j) {
-                const uint32_t k = ch.k0 + j;
-                const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
-                uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
-                const uint32_t doff = a.dir_off[k];
-                uint32_t filter = 1;
-                if (doff != kNoDir) {
-                    const uint32_t* d = a.dir + doff;
-                    lo = max(lo, ks + d[dt0]);
-                    hi = min(hi, ks + d[dt1]);
-                    filter = 0;
-                }
-                Span sp;
-                sp.lo = lo;
-                sp.hi = max(lo, hi);
-                sp.wf = ch.w << 1 | filter;
-                s_span[cp + j - w0] = sp;
-            }
-        }
-        __syncthreads();
-        for (;;) {
-            uint32_t si = 0;
-            if (lane == 0)
-                si = atomicAdd(&s_next, 1u);
-            si = bcast_first(__shfl((int)si, 0, kWave));
-            if (si >= wn)
-                break;
-            const Span sp = s_span[si];
-            if (sp.lo < sp.hi) {
-                if (sp.wf & 1)
-                    count_span<LB, true>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
-                else
-                    count_span<LB, false>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
-            }
-        }
-        __syncthreads();
-    }
-    if (npairs == 0)
-        __syncthreads();  // the zeroing above must finish before the select passes
-
-    tile_select<LB>(a, cnt, q, obj0, s_sel);
-}
-
-// Compressed form of k_scan_tile: pairs resolve to ranges of compressed
-// segments overlapping the tile (segment directory for long keys), the waves
-// take single segments, stage and decode them (codecs_dev.h) and count the
-// decoded ids that fall inside the tile.
-template <int LB>
-__global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
-{
-    __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(s_cnt4);
-    __shared__ Span s_span[kSpanCap];
-    __shared__ uint32_t s_pre[kSpanCap + 1];   // prefix of segments per span
-    __shared__ DecodeScratch s_dec[kScanWaves];
-    __shared__ uint32_t s_next;
-    __shared__ uint32_t s_wsum[kScanWaves];
-    __shared__ SelShared s_sel;
-
-    const uint32_t t = blockIdx.x / a.n_queries;
-    const uint32_t q = blockIdx.x % a.n_queries;
-    const uint32_t obj0 = t * a.tile_objs;
-    const uint32_t obj1 = obj0 + a.tile_objs;
-    const uint32_t dt0 = t * a.dt_per_tile;
-    const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
-    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-
-    for (int i = tid; i < kCntWords / 4; i += kScanThreads)
-        s_cnt4[i] = make_uint4(0, 0, 0, 0);
-
-    const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
-    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
-    for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
-        const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
-        if (tid == 0)
-            s_next = 0;
-        for (uint32_t c = c0 + tid; c < c1; c += kScanThreads) {
-            const CChunk ch = a.chunks[c];
-            const uint32_t cp = ch.kp - pbase;
-            const uint32_t nk = ch.k1 - ch.k0 + 1;
-            if (cp + nk <= w0 || cp >= w0 + wn)
-                continue;
-            const uint32_t j0 = cp < w0 ? w0 - cp : 0;
-            const uint32_t j1 = min(nk, w0 + wn - cp);
-            for (uint32_t j = j0; j < j1; ++j) {
-                const uint32_t k = ch.k0 + j;
-                const uint32_t ks = a.key_seg[k], ke = a.key_seg[k + 1];
-                uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
-                const uint32_t doff = a.sdir_off[k];
-                if (doff != kNoDir) {
-                    const uint32_t* d = a.sdir + doff;
-                    const uint32_t mid = ks + d[dt1];
-                    const uint32_t end = mid + (mid < ke && a.seg_first[mid] < obj1 ? 1u : 0u);
-                    lo = max(lo, ks + d[dt0]);
-                    hi = min(hi, end);
-                }
-                Span sp;
-                sp.lo = lo;
-                sp.hi = max(lo, hi);
-                sp.wf = ch.w << 1 | 1u;
-                s_span[cp + j - w0] = sp;
-            }
-        }
-        __syncthreads();
-        // prefix of segment counts over the window's spans (2 per thread)
-        {
-            const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
-            const uint32_t n0 = i0 < wn ? s_span[i0].hi - s_span[i0].lo : 0u;
-            const uint32_t n1 = i1 < wn ? s_span[i1].hi - s_span[i1].lo : 0u;
-            const uint32_t incl = wave_incl_scan(n0 + n1);
-            if (lane == kWave - 1)
-                s_wsum[wv] = incl;
-            __syncthreads();
-            uint32_t off = 0;
-            for (int i = 0; i < wv; ++i)
-                off += s_wsum[i];
-            const uint32_t ex = off + incl - (n0 + n1);
-            if (i0 <= wn)
-                s_pre[i0] = ex;
-            if (i1 <= wn)
-                s_pre[i1] = ex + n0;
-            if (i1 + 1 == wn)
-                s_pre[wn] = ex + n0 + n1;  // the total, also when wn is the full window
-        }
-        __syncthreads();
-        const uint32_t nsegs = s_pre[wn];
-        DecodeScratch& D = s_dec[wv];
-        for (;;) {
-            uint32_t g = 0;
-            if (lane == 0)
-                g = atomicAdd(&s_next, 1u);
-            g = bcast_first(__shfl((int)g, 0, kWave));
-            if (g >= nsegs)
-                break;
-            uint32_t lo = 0, hi = wn;  // span with s_pre[span] <= g < s_pre[span + 1]
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_pre[mid] <= g)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const Span sp = s_span[lo];
-            const uint32_t sg = sp.lo + (g - s_pre[lo]);
-            if (a.seg_last[sg] < obj0 || a.seg_first[sg] >= obj1)
-                continue;  // segment entirely outside the tile
-            const uint32_t cw0 = a.seg_cpos[sg], nw = a.seg_cpos[sg + 1] - cw0;
-            stage_words(D, a.words, cw0, nw);
-            uint32_t v[kPerLane];
-            const uint32_t n = wave_decode(a.codec, D, nw, v);
-            const uint32_t w = sp.wf >> 1;
-#pragma unroll
-            for (int j = 0; j < kPerLane; ++j) {
-                const uint32_t i = lane * kPerLane + j;
-                count_one<LB, true>(i < n ? v[j] : obj1, obj0, a.tile_objs, w, cnt);
-            }
-            wave_sync();  // the staging buffer is reused by the next segment
-        }
-        __syncthreads();
-    }
-    if (npairs == 0)
-        __syncthreads();
-    tile_select<LB>(a, cnt, q, obj0, s_sel);
-}
-
-// ---------------------------------------------------------------------------
-// Merge: exact top-k of each query's candidate keys, sorted descending.
-// ---------------------------------------------------------------------------
-
-constexpr int kMergeThreads = 256;
-constexpr int kMergeMaxK = GENIE_MAX_K;
-
-struct MergeArgs {
-    const uint64_t* cand;
-    const uint64_t* cand_base;
-    const uint32_t* cand_cnt;
-    const uint32_t* cand_cap;
-    const uint32_t* kq;
-    const uint64_t* out_off;
-    int32_t* out_ids;
-    int32_t* out_counts;
-    uint32_t key_bits;  // significant bits of a key (32 + bits of the count bound)
-};
-
-__global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
-{
-    __shared__ uint64_t s_keys[kMergeMaxK];
-    __shared__ uint32_t s_hist[256];
-    __shared__ uint32_t s_res[4];
-    const uint32_t q = blockIdx.x;
-    const int tid = threadIdx.x;
-    const uint32_t k = a.kq[q];
-    const uint32_t M = min(a.cand_cnt[q], a.cand_cap[q]);
-    const uint64_t* keys = a.cand + a.cand_base[q];
-
-    uint64_t kth = 0;  // every key >= kth is in the result
-    if (M > k) {
-        uint64_t prefix = 0, mask = 0;
-        uint32_t remaining = k;
-        int shift = ((int)((a.key_bits + 7) / 8) - 1) * 8;
-        for (; shift >= 0; shift -= 8) {
-            if (tid < 256)
-                s_hist[tid] = 0;
-            __syncthreads();
-            for (uint32_t i0 = 0; i0 < M; i0 += kMergeThreads) {
-                const uint32_t i = i0 + tid;
-                uint64_t key = i < M ? keys[i] : 0;
-                bool take = i < M && (key & mask) == prefix;
-                hist_add_aggregated(s_hist, (uint32_t)(key >> shift) & 255u, take);
-            }
-            __syncthreads();
-            if (tid == 0) {
-                uint32_t cum = 0, b = 255;
-                for (int d = 255; d >= 0; --d) {
-                    if (cum + s_hist[d] >= remaining) {
-                        b = (uint32_t)d;
-                        break;
-                    }
-                    cum += s_hist[d];
-                }
-                s_res[0] = b;
-                s_res[1] = remaining - cum;
-            }
-            __syncthreads();
-            prefix |= (uint64_t)s_res[0] << shift;
-            mask |= (uint64_t)255 << shift;
-            remaining = s_res[1];
-            __syncthreads();
-        }
-        kth = prefix;
-    }
-    // collect the selected keys
-    if (tid == 0)
-        s_res[2] = 0;
-    uint32_t n2 = 1;
-    const uint32_t nsel = min(M, k);
-    while (n2 < nsel)
-        n2 <<= 1;
-    for (uint32_t i = tid; i < n2; i += kMergeThreads)
-        s_keys[i] = 0;
-    __syncthreads();
-    for (uint32_t i0 = 0; i0 < M; i0 += kMergeThreads) {
-        const uint32_t i = i0 + tid;
-        uint64_t key = i < M ? keys[i] : 0;
-        bool take = i < M && key >= kth;
-        uint64_t bm = ballot(take);
-        if (bm) {
-            uint32_t wb = 0;
-            if (lane_id() == __builtin_ctzll(bm))
-                wb = atomicAdd(&s_res[2], (uint32_t)__popcll(bm));
-            wb = (uint32_t)__shfl((int)wb, __builtin_ctzll(bm), kWave);
-            if (take) {
-                uint32_t slot = wb + lane_prefix(bm);
-                if (slot < n2)
-                    s_keys[slot] = key;
-            }
-        }
-    }
-    __syncthreads();
-    // bitonic sort, descending
-    for (uint32_t size = 2; size <= n2; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t i = tid; i < n2; i += kMergeThreads) {
-                uint32_t j = i ^ stride;
-                if (j > i) {
-                    bool desc = (i & size) == 0;
-                    uint64_t x = s_keys[i], y = s_keys[j];
-                    if (desc ? (x < y) : (x > y)) {
-                        s_keys[i] = y;
-                        s_keys[j] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    const uint64_t o = a.out_off[q];
-    for (uint32_t i = tid; i < k; i += kMergeThreads) {
-        uint64_t key = i < nsel ? s_keys[i] : 0;
-        if (key == 0) {
-            a.out_ids[o + i] = 0;
-            a.out_counts[o + i] = 0;
-        } else {
-            a.out_ids[o + i] = (int32_t)(~(uint32_t)key);
-            a.out_counts[o + i] = (int32_t)(key >> 32);
-        }
-    }
 }
 
 }  // namespace gh
@@ -1147,6 +126,25 @@ struct genie_plan {
     uint64_t out_alloc = 0;
     std::vector<uint32_t> h_kq;
     std::vector<uint64_t> h_out_off;
+    // batched form (mode 1)
+    uint32_t block = 512;            // threads per workgroup
+    uint32_t tile_b = 8192;          // objects per tile (LDS counter capacity / 8 B)
+    uint32_t seglen = 65536;         // longest posting span one wave takes at a time
+    uint32_t lanes = 8, n_groups = 0, n_work = 0;
+    uint64_t n_pairs = 0, pairs_alloc = 0;
+    GPair* d_pairs = nullptr;
+    uint32_t* d_gp_ptr = nullptr;
+    int32_t* d_gq = nullptr;
+    uint32_t* d_queue = nullptr;
+    uint64_t gq_alloc = 0;
+    std::vector<genie_dim_t> h_items;  // sorted by query, kept to rebuild the work queue
+    std::vector<uint32_t> h_qptr;
+    double host_build_ms = 0;
+    // graph replay
+    bool graph_on = false, graph_valid = false;
+    hipStream_t own_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     // timing
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -1161,8 +159,24 @@ static uint32_t bits_for(uint32_t bound)
 
 static uint32_t tile_objs_for(uint32_t lane_bits) { return (uint32_t)kCntWords * (32u / lane_bits); }
 
+static void plan_drop_graph(genie_plan* p)
+{
+    if (p->gexec)
+        (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    p->graph_valid = false;
+}
+
 static void plan_free_buffers(genie_plan* p)
 {
+    plan_drop_graph(p);
+    dev_free(p->d_pairs);
+    dev_free(p->d_gp_ptr);
+    dev_free(p->d_gq);
+    dev_free(p->d_queue);
     dev_free(p->d_items);
     dev_free(p->d_item_qptr);
     dev_free(p->d_nchunks);
@@ -1326,6 +340,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
     d.h_key_mult.assign(K, 1);
     d.h_seg_cpos.assign(cp, cp + S + 1);
     d.h_seg_key = seg_key;
+    d.h_key_seg = key_seg;
     if (desc->postings) {
         for (uint32_t k = 0; k < K; ++k) {
             uint32_t best = 1, run = 1;
@@ -1417,6 +432,188 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
     }
     *out = ix;
     return GENIE_OK;
+}
+
+static uint32_t plan_n_tiles(const genie_plan* p);
+static ScanArgs scan_args(genie_plan* p);
+
+// ---- batched form: host-side work queue ---------------------------------
+
+// Group consecutive queries into lanes of one u64 counter word, resolve every
+// item into per-key posting spans (segment ranges for a compressed index),
+// merge spans a group holds more than once (one read, increments summed over
+// the lanes), split long spans for wave balance, upload.
+static int build_batched(genie_plan* p)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const DevIndex& d = p->ix->d;
+    const uint32_t LB = p->lane_bits, lanes = 64 / LB, nq = p->n_queries;
+    p->lanes = lanes;
+    p->n_groups = (nq + lanes - 1) / lanes;
+    const uint32_t G = p->n_groups;
+    std::vector<int32_t> gq((size_t)G * lanes, -1);
+    for (uint32_t q = 0; q < nq; ++q)
+        gq[q] = (int32_t)q;  // query q -> group q / lanes, lane q % lanes
+    std::vector<std::vector<GPair>> per(G);
+    const uint64_t* ks = d.h_key_start.data();
+    const uint32_t* cp = d.h_seg_cpos.data();
+    auto group_pairs = [&](uint32_t g) {
+        std::vector<GPair>& out = per[g];
+        for (uint32_t l = 0; l < lanes; ++l) {
+            const uint32_t q = g * lanes + l;
+            if (q >= nq)
+                break;
+            const uint64_t unit = 1ull << (LB * l);
+            for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
+                const genie_dim_t& it = p->h_items[i];
+                if (it.end_pos <= it.start_pos)
+                    continue;
+                const uint64_t inc = (uint64_t)(uint32_t)it.weight * unit;
+                if (d.codec == 0) {
+                    const uint32_t sp = (uint32_t)it.start_pos, ep = (uint32_t)it.end_pos;
+                    const uint32_t k0 = (uint32_t)(std::upper_bound(ks, ks + d.n_keys + 1, (uint64_t)sp) - ks - 1);
+                    const uint32_t k1 = (uint32_t)(std::upper_bound(ks, ks + d.n_keys + 1, (uint64_t)ep - 1) - ks - 1);
+                    for (uint32_t k = k0; k <= k1; ++k) {
+                        const uint32_t lo = std::max<uint32_t>(sp, (uint32_t)ks[k]);
+                        const uint32_t hi = std::min<uint32_t>(ep, (uint32_t)ks[k + 1]);
+                        if (lo < hi)
+                            out.push_back({k, lo, hi, 0, inc});
+                    }
+                } else {
+                    const uint32_t s0 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) - cp - 1);
+                    const uint32_t s1 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos - 1) - cp);
+                    for (uint32_t sg = s0; sg < s1;) {
+                        const uint32_t k = d.h_seg_key[sg];
+                        const uint32_t e = std::min(s1, d.h_key_seg[k + 1]);
+                        out.push_back({k, sg, e, 0, inc});
+                        sg = e;
+                    }
+                }
+            }
+        }
+        std::sort(out.begin(), out.end(), [](const GPair& a, const GPair& b) {
+            if (a.key != b.key) return a.key < b.key;
+            if (a.lo != b.lo) return a.lo < b.lo;
+            return a.hi < b.hi;
+        });
+        size_t m = 0;
+        for (size_t i = 0; i < out.size(); ++i) {
+            if (m && out[m - 1].key == out[i].key && out[m - 1].lo == out[i].lo && out[m - 1].hi == out[i].hi)
+                out[m - 1].inc += out[i].inc;  // lanes are disjoint bit fields; repeats add within a lane
+            else
+                out[m++] = out[i];
+        }
+        out.resize(m);
+        if (d.codec == 0 && p->seglen) {
+            std::vector<GPair> split;
+            split.reserve(out.size());
+            for (const GPair& x : out)
+                for (uint32_t a = x.lo; a < x.hi; a += p->seglen)
+                    split.push_back({x.key, a, std::min(x.hi, a + p->seglen), 0, x.inc});
+            out.swap(split);
+        }
+    };
+    {
+        const uint32_t T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        std::atomic<uint32_t> next(0);
+        for (uint32_t t = 0; t < std::min(T, G); ++t)
+            th.emplace_back([&] {
+                for (uint32_t g; (g = next.fetch_add(1)) < G;)
+                    group_pairs(g);
+            });
+        for (auto& x : th)
+            x.join();
+    }
+    std::vector<uint32_t> gp(G + 1, 0);
+    for (uint32_t g = 0; g < G; ++g)
+        gp[g + 1] = gp[g] + (uint32_t)per[g].size();
+    const uint64_t np = gp[G];
+    std::vector<GPair> all(np);
+    for (uint32_t g = 0; g < G; ++g)
+        std::copy(per[g].begin(), per[g].end(), all.begin() + gp[g]);
+    if (np > p->pairs_alloc) {
+        dev_free(p->d_pairs);
+        p->pairs_alloc = 0;
+        if (int rc = dev_alloc(&p->d_pairs, np))
+            return rc;
+        p->pairs_alloc = np;
+    }
+    if ((uint64_t)G * lanes > p->gq_alloc) {
+        dev_free(p->d_gp_ptr);
+        dev_free(p->d_gq);
+        p->gq_alloc = 0;
+        if (int rc = dev_alloc(&p->d_gp_ptr, G + 1))
+            return rc;
+        if (int rc = dev_alloc(&p->d_gq, (uint64_t)G * lanes))
+            return rc;
+        p->gq_alloc = (uint64_t)G * lanes;
+    }
+    if (np)
+        HIP_TRY(hipMemcpy(p->d_pairs, all.data(), np * sizeof(GPair), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_gp_ptr, gp.data(), gp.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_gq, gq.data(), gq.size() * 4, hipMemcpyHostToDevice));
+    p->n_pairs = np;
+    p->n_work = G * plan_n_tiles(p);
+    p->host_build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return GENIE_OK;
+}
+
+template <int LB, bool CMP, int NT>
+static int launch_batched_t(genie_plan* p, const BatchArgs& b, hipStream_t s)
+{
+    const size_t lds = (size_t)(p->tile_b + kWave) * 8;
+    auto kern = k_scan_batched<LB, CMP, NT>;
+    hipFuncAttributes fa;
+    HIP_TRY(hipFuncGetAttributes(&fa, (const void*)kern));
+    if (fa.sharedSizeBytes + lds > 160 * 1024)
+        return fail(GENIE_E_INVALID, "batched kernel with " + std::to_string(NT) + " threads and tile " +
+                                         std::to_string(p->tile_b) + " does not fit in LDS");
+    HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0, n_cu = 0, dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds));
+    if (per_cu < 1)
+        return fail(GENIE_E_INVALID, "batched kernel does not fit a CU with tile " + std::to_string(p->tile_b));
+    const uint32_t grid = std::max(1u, std::min<uint32_t>(b.n_work, (uint32_t)(per_cu * n_cu)));
+    kern<<<grid, NT, lds, s>>>(b);
+    return GENIE_OK;
+}
+
+template <int LB>
+static int launch_batched_lb(genie_plan* p, const BatchArgs& b, hipStream_t s)
+{
+    const bool c = p->ix->d.codec != 0;
+    if (p->block == 256)
+        return c ? launch_batched_t<LB, true, 256>(p, b, s) : launch_batched_t<LB, false, 256>(p, b, s);
+    if (p->block == 1024)
+        return c ? launch_batched_t<LB, true, 1024>(p, b, s) : launch_batched_t<LB, false, 1024>(p, b, s);
+    return c ? launch_batched_t<LB, true, 512>(p, b, s) : launch_batched_t<LB, false, 512>(p, b, s);
+}
+
+static int launch_batched(genie_plan* p, hipStream_t s)
+{
+    BatchArgs b;
+    b.s = scan_args(p);
+    b.s.tile_objs = p->tile_b;
+    b.s.n_tiles = plan_n_tiles(p);
+    b.s.dt_per_tile = p->tile_b / kDirTile;
+    b.pairs = p->d_pairs;
+    b.gp_ptr = p->d_gp_ptr;
+    b.gq = p->d_gq;
+    b.n_groups = p->n_groups;
+    b.lanes = p->lanes;
+    b.n_work = p->n_work;
+    b.queue = p->d_queue;
+    if (b.n_work == 0)
+        return GENIE_OK;
+    switch (p->lane_bits) {
+    case 8: return launch_batched_lb<8>(p, b, s);
+    case 16: return launch_batched_lb<16>(p, b, s);
+    default: return launch_batched_lb<32>(p, b, s);
+    }
 }
 
 extern "C" {
@@ -1615,7 +812,7 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         return fail(GENIE_E_INVALID, "null argument");
     if (max_queries == 0 || max_k == 0 || max_k > GENIE_MAX_K)
         return fail(GENIE_E_INVALID, "max_queries must be > 0 and max_k in [1, GENIE_MAX_K]");
-    if (mode != 0)
+    if (mode != 0 && mode != 1)
         return fail(GENIE_E_INVALID, "unknown plan mode");
     genie_plan* p = new genie_plan();
     p->ix = ix;
@@ -1644,6 +841,11 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         if ((rc = dev_alloc(&p->d_cand_base, max_queries))) break;
         if ((rc = dev_alloc(&p->d_overflow, 1))) break;
         if ((rc = dev_alloc(&p->d_out_off, max_queries + 1))) break;
+        if ((rc = dev_alloc(&p->d_queue, 1))) break;
+        if (hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking) != hipSuccess) {
+            rc = fail(GENIE_E_RUNTIME, "hipStreamCreate");
+            break;
+        }
         for (auto& e : p->ev)
             if (hipEventCreate(&e) != hipSuccess) {
                 rc = fail(GENIE_E_RUNTIME, "hipEventCreate");
@@ -1666,15 +868,26 @@ int genie_hip_plan_destroy(genie_plan* p)
     for (auto& e : p->ev)
         if (e)
             (void)hipEventDestroy(e);
+    if (p->own_stream)
+        (void)hipStreamDestroy(p->own_stream);
     delete p;
     return GENIE_OK;
+}
+
+}  // extern "C"
+
+static uint32_t plan_tile_objs(const genie_plan* p) { return p->mode == 1 ? p->tile_b : tile_objs_for(p->lane_bits); }
+
+static uint32_t plan_n_tiles(const genie_plan* p)
+{
+    const uint32_t t = plan_tile_objs(p);
+    return (p->ix->d.n_objects + t - 1) / t;
 }
 
 // Candidate capacity per query: cap_factor tiles' worth of k, at most all tiles.
 static int plan_size_candidates(genie_plan* p)
 {
-    const DevIndex& d = p->ix->d;
-    const uint32_t n_tiles = (d.n_objects + tile_objs_for(p->lane_bits) - 1) / tile_objs_for(p->lane_bits);
+    const uint32_t n_tiles = plan_n_tiles(p);
     const uint32_t f = std::min<uint32_t>(p->cap_factor, n_tiles);
     std::vector<uint32_t> cap(p->n_queries);
     std::vector<uint64_t> base(p->n_queries);
@@ -1696,6 +909,8 @@ static int plan_size_candidates(genie_plan* p)
     HIP_TRY(hipMemcpy(p->d_cand_base, base.data(), base.size() * 8, hipMemcpyHostToDevice));
     return GENIE_OK;
 }
+
+extern "C" {
 
 int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
                           const uint32_t* k_per_query)
@@ -1804,10 +1019,19 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     HIP_TRY(hipMemcpy(p->d_kq, k_per_query, n_queries * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_out_off, p->h_out_off.data(), p->h_out_off.size() * 8, hipMemcpyHostToDevice));
     p->cap_factor = 8;
+    plan_drop_graph(p);
+    if (p->mode == 1) {
+        p->h_items.swap(sorted);
+        p->h_qptr.swap(qptr);
+        if (int rc = build_batched(p))
+            return rc;
+    }
     return plan_size_candidates(p);
 }
 
-static int launch_scan(genie_plan* p, hipStream_t s)
+}  // extern "C"
+
+static ScanArgs scan_args(genie_plan* p)
 {
     const DevIndex& d = p->ix->d;
     ScanArgs a;
@@ -1839,6 +1063,13 @@ static int launch_scan(genie_plan* p, hipStream_t s)
     a.cand_base = p->d_cand_base;
     a.cand = p->d_cand;
     a.overflow = p->d_overflow;
+    return a;
+}
+
+static int launch_scan(genie_plan* p, hipStream_t s)
+{
+    const DevIndex& d = p->ix->d;
+    const ScanArgs a = scan_args(p);
     const uint64_t blocks = (uint64_t)a.n_tiles * p->n_queries;
     if (blocks >= 0x7FFFFFFFull)
         return fail(GENIE_E_INVALID, "batch too large for one launch");
@@ -1858,6 +1089,8 @@ static int launch_scan(genie_plan* p, hipStream_t s)
     return GENIE_OK;
 }
 
+static int launch_merge(genie_plan* p, hipStream_t s);
+
 static int plan_run_once(genie_plan* p, hipStream_t s)
 {
     const DevIndex& d = p->ix->d;
@@ -1868,6 +1101,17 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     fill_u32(p->d_gthr, 0, nq, s);
     fill_u32(p->d_cand_cnt, 0, nq, s);
     fill_u32(p->d_overflow, 0, 1, s);
+    if (p->mode == 1) {
+        // batched form: the work queue was built at upload
+        fill_u32(p->d_queue, 0, 1, s);
+        if (p->timing)
+            HIP_TRY(hipEventRecord(p->ev[1], s));
+        if (int rc = launch_batched(p, s))
+            return rc;
+        if (p->timing)
+            HIP_TRY(hipEventRecord(p->ev[2], s));
+        return launch_merge(p, s);
+    }
     // compile items into chunks (uncompressed) or segment runs (compressed)
     if (ni && d.codec == 0) {
         k_chunk_count<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, p->d_nchunks);
@@ -1911,6 +1155,12 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
         return rc;
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[2], s));
+    return launch_merge(p, s);
+}
+
+static int launch_merge(genie_plan* p, hipStream_t s)
+{
+    const uint32_t nq = p->n_queries;
     MergeArgs m;
     m.cand = p->d_cand;
     m.cand_base = p->d_cand_base;
@@ -1929,13 +1179,169 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     return GENIE_OK;
 }
 
+extern "C" {
+
 int genie_hip_plan_run(genie_plan* p, void* stream)
 {
     if (!p)
         return fail(GENIE_E_INVALID, "null plan");
     if (p->n_queries == 0)
         return fail(GENIE_E_NOT_BUILT, "no batch uploaded");
-    return plan_run_once(p, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    if (!p->graph_on || p->timing)
+        return plan_run_once(p, s);
+    if (!p->graph_valid) {
+        // capture the whole run (fill kernels, no memset nodes) once
+        plan_drop_graph(p);
+        HIP_TRY(hipStreamBeginCapture(p->own_stream, hipStreamCaptureModeThreadLocal));
+        int rc = plan_run_once(p, p->own_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(p->own_stream, &g);
+        if (rc) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return fail(GENIE_E_RUNTIME, std::string("graph capture: ") + hipGetErrorString(e));
+        p->graph = g;
+        HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+        p->graph_valid = true;
+    }
+    HIP_TRY(hipGraphLaunch(p->gexec, s));
+    p->ran = true;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_set_graph(genie_plan* p, int on)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    p->graph_on = on != 0;
+    if (!p->graph_on)
+        plan_drop_graph(p);
+    return GENIE_OK;
+}
+
+int genie_hip_plan_set_config(genie_plan* p, uint32_t block, uint32_t tile_objs, uint32_t seglen)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (block != 256 && block != 512 && block != 1024)
+        return fail(GENIE_E_INVALID, "block must be 256, 512 or 1024");
+    if (tile_objs == 0 || tile_objs % kDirTile || (uint64_t)(tile_objs + kWave) * 8 > 160 * 1024)
+        return fail(GENIE_E_INVALID, "tile must be a multiple of " + std::to_string(kDirTile) +
+                                         " objects and fit in LDS");
+    if (seglen < 1024)
+        return fail(GENIE_E_INVALID, "segment length must be at least 1024 postings");
+    const bool rebuild = seglen != p->seglen || tile_objs != p->tile_b;
+    p->block = block;
+    p->tile_b = tile_objs;
+    p->seglen = seglen;
+    plan_drop_graph(p);
+    if (p->mode == 1 && p->n_queries && rebuild) {
+        if (int rc = build_batched(p))
+            return rc;
+        p->cap_factor = 8;
+        return plan_size_candidates(p);
+    }
+    return GENIE_OK;
+}
+
+int genie_hip_plan_get_config(genie_plan* p, uint32_t* block, uint32_t* tile_objs, uint32_t* seglen,
+                              uint32_t* lane_bits)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (block)
+        *block = p->block;
+    if (tile_objs)
+        *tile_objs = plan_tile_objs(p);
+    if (seglen)
+        *seglen = p->mode == 1 ? p->seglen : kChunk;
+    if (lane_bits)
+        *lane_bits = p->lane_bits;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_host_ms(genie_plan* p, double* ms)
+{
+    if (!p || !ms)
+        return fail(GENIE_E_INVALID, "null argument");
+    *ms = p->host_build_ms;
+    return GENIE_OK;
+}
+
+// Autotuner of the batched form: every (block, tile, segment length)
+// candidate runs the uploaded batch (one warm-up, `reps` timed runs on the
+// plan's stream); the fastest is kept.  Results go to out_ms (one entry per
+// candidate, in the order block-major, then tile, then segment length) when
+// given; *n_out receives the candidate count.
+int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n_out)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (p->mode != 1 || p->n_queries == 0)
+        return fail(GENIE_E_INVALID, "autotune needs a batched plan with an uploaded batch");
+    const uint32_t blocks[] = {256, 512, 1024};
+    const uint32_t tiles[] = {8192, 16384};
+    const uint32_t segs[] = {16384, 65536};
+    const bool was_graph = p->graph_on;
+    p->graph_on = false;
+    double best = 1e30;
+    uint32_t bb = p->block, bt = p->tile_b, bs = p->seglen, n = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = GENIE_OK;
+    for (uint32_t blk : blocks)
+        for (uint32_t tl : tiles)
+            for (uint32_t sg : segs) {
+                double ms = -1;
+                if ((rc = genie_hip_plan_set_config(p, blk, tl, sg)) == GENIE_OK &&
+                    (rc = plan_run_once(p, p->own_stream)) == GENIE_OK &&
+                    hipStreamSynchronize(p->own_stream) == hipSuccess) {
+                    // a first run may overflow the candidate space: settle it
+                    uint32_t ov = 0;
+                    (void)hipMemcpy(&ov, p->d_overflow, 4, hipMemcpyDeviceToHost);
+                    if (ov) {
+                        p->cap_factor = plan_n_tiles(p);
+                        rc = plan_size_candidates(p);
+                    }
+                    if (rc == GENIE_OK) {
+                        (void)hipEventRecord(e0, p->own_stream);
+                        for (int r = 0; r < std::max(1, reps); ++r)
+                            if ((rc = plan_run_once(p, p->own_stream)))
+                                break;
+                        (void)hipEventRecord(e1, p->own_stream);
+                        (void)hipEventSynchronize(e1);
+                        float f = 0;
+                        (void)hipEventElapsedTime(&f, e0, e1);
+                        ms = f / std::max(1, reps);
+                    }
+                }
+                if (rc == GENIE_E_INVALID)  // a candidate that does not fit this device: skip it
+                    rc = GENIE_OK;
+                if (rc)
+                    break;
+                if (out_ms)
+                    out_ms[n] = ms;
+                ++n;
+                if (ms >= 0 && ms < best) {
+                    best = ms;
+                    bb = blk;
+                    bt = tl;
+                    bs = sg;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (n_out)
+        *n_out = n;
+    p->graph_on = was_graph;
+    if (rc)
+        return rc;
+    return genie_hip_plan_set_config(p, bb, bt, bs);
 }
 
 int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts)
@@ -1953,8 +1359,7 @@ int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32
         HIP_TRY(hipMemcpy(&ov, p->d_overflow, 4, hipMemcpyDeviceToHost));
         if (!ov)
             break;
-        const DevIndex& d = p->ix->d;
-        const uint32_t n_tiles = (d.n_objects + tile_objs_for(p->lane_bits) - 1) / tile_objs_for(p->lane_bits);
+        const uint32_t n_tiles = plan_n_tiles(p);
         if (p->cap_factor >= n_tiles || attempt > 8)
             return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
         p->cap_factor = std::min<uint32_t>(p->cap_factor * 4, n_tiles);

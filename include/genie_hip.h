@@ -121,6 +121,21 @@ int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32
  * count + per-tile select kernel), "merge"; ms is the stage's duration. */
 int genie_hip_plan_set_timing(genie_plan* p, int on);
 int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms);
+/* Replay the run as a captured hipGraph (captured at the next run, again
+ * after any upload or configuration change).  Timing runs bypass the graph. */
+int genie_hip_plan_set_graph(genie_plan* p, int on);
+/* Batched-form configuration: threads per workgroup (256/512/1024), objects
+ * per tile (a multiple of 8192 whose 8-byte counters fit LDS) and the
+ * longest posting span one wave takes at a time. */
+int genie_hip_plan_set_config(genie_plan* p, uint32_t block, uint32_t tile_objs, uint32_t seglen);
+int genie_hip_plan_get_config(genie_plan* p, uint32_t* block, uint32_t* tile_objs, uint32_t* seglen,
+                              uint32_t* lane_bits);
+/* Autotuner of the batched form over block x tile x segment length on the
+ * uploaded batch; keeps the fastest.  out_ms (optional, >= 12 entries)
+ * receives every candidate's time per run (-1 if it could not run). */
+int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n_out);
+/* Host time of the last batched work-queue build (upload), ms. */
+int genie_hip_plan_host_ms(genie_plan* p, double* ms);
 /* Algorithmic bytes of the last uploaded batch: logical = 4 B x sum of
  * item span lengths (uncompressed) or words (compressed); compulsory =
  * the same over distinct keys/segments touched, counted once. */

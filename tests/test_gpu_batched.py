@@ -1,0 +1,117 @@
+"""Batched form (persistent kernel over a (query group, tile) work queue,
+queries packed into count lanes, merged spans, hipGraph replay, autotuner)
+against the plain form and the oracle: identical results."""
+import numpy as np
+import pytest
+
+from helpers import assert_topk_equal
+from oracle import oracle as O
+
+import genie_amd
+from genie_amd import Plan, Table, match_topk, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def plain(table, items, nq, k):
+    return match_topk(table.device(), items, nq, k)
+
+
+def batched(table, items, nq, k, config=None, graph=False, runs=1):
+    kmax = int(np.max(k))
+    p = Plan(table.device(), nq, len(items), kmax, 0, mode=1)
+    if config:
+        p.set_config(*config)
+    p.upload(items, nq, k)
+    if graph:
+        p.set_graph(True)
+    out = None
+    for _ in range(runs):
+        p.run()
+        got = p.download()
+        if out is not None:
+            assert np.array_equal(got[0], out[0]) and np.array_equal(got[1], out[1])
+        out = got
+    return out, p
+
+
+@pytest.mark.parametrize("config", [None, (256, 8192, 16384), (512, 16384, 65536), (1024, 8192, 1024)])
+def test_batched_equals_plain_zipf(config):
+    w = W.cfg1(n=150_000, d=32, q=37, k=50)  # 37 queries: a partly filled last group
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], w["n_queries"])
+    a = plain(t, items, w["n_queries"], w["k"])
+    (b, _) = batched(t, items, w["n_queries"], w["k"], config)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_batched_mixed_k_weights_bijectmap():
+    rp, vals = genie_amd.synth.sets(51, 60_000, 5, 40, 4000, 1.05)
+    qp, qv = genie_amd.synth.sets(52, 70, 1, 64, 4000, 1.05)
+    ranges = W.set_ranges(qp, qv, genie_amd.synth.uniform_int(53, 0, len(qv), 1, 4))
+    kq = np.array([10, 100, 1000] * 23 + [7], np.uint32)
+    t = Table.from_sets(rp, vals)
+    items = t.compile(ranges, 70)
+    (b, p) = batched(t, items, 70, kq)
+    ri, rc = [], []
+    for q in range(70):
+        i, c = O.topk(O.count_exhaustive_biject(rp, vals, ranges[ranges["query"] == q]), int(kq[q]))
+        ri.append(i)
+        rc.append(c)
+    assert_topk_equal(b[0], b[1], np.concatenate(ri), np.concatenate(rc), kq, 70, "batched biject")
+
+
+@pytest.mark.parametrize("wmax,lb", [(1, 8), (300, 16), (5000, 32)])
+def test_batched_lane_widths(wmax, lb):
+    w = W.cfg1(n=40_000, d=24, q=9, k=20)
+    r = w["ranges"].copy()
+    r["weight"] = np.random.default_rng(wmax).integers(1, wmax + 1, len(r))
+    t = Table.from_rows(w["rows"])
+    items = t.compile(r, 9)
+    a = plain(t, items, 9, 20)
+    (b, p) = batched(t, items, 9, 20)
+    assert p.config()["lane_bits"] == lb
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_batched_compressed():
+    w = W.cfg4(n=120_000, d=16, q=20)
+    plain_t = Table.from_rows(w["rows"])
+    comp_t = Table.from_rows(w["rows"], 1024, True, codec=11)
+    a = plain(plain_t, plain_t.compile(w["ranges"], 20), 20, w["k"])
+    (b, _) = batched(comp_t, comp_t.compile(w["ranges"], 20), 20, w["k"])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_graph_replays_match_eager():
+    w = W.cfg1(n=100_000, d=16, q=24, k=32)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 24)
+    eager = plain(t, items, 24, 32)
+    # replays with eager work and synchronisations between them
+    (b, p) = batched(t, items, 24, 32, graph=True, runs=4)
+    assert np.array_equal(eager[0], b[0]) and np.array_equal(eager[1], b[1])
+    p0 = Plan(t.device(), 24, len(items), 32, 0, mode=0)
+    p0.upload(items, 24, 32)
+    p0.set_graph(True)
+    for _ in range(3):
+        p0.run()
+        got = p0.download()
+        assert np.array_equal(eager[0], got[0]) and np.array_equal(eager[1], got[1])
+        plain(t, items[:50], 24, 5)  # eager work in between
+
+
+def test_autotune_picks_a_valid_config():
+    w = W.cfg1(n=200_000, d=32, q=32, k=100)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 32)
+    a = plain(t, items, 32, 100)
+    p = Plan(t.device(), 32, len(items), 100, 0, mode=1)
+    p.upload(items, 32, 100)
+    ms = p.autotune(reps=1)
+    assert len(ms) == 12 and np.any(ms > 0)
+    cfg = p.config()
+    assert cfg["block"] in (256, 512, 1024) and cfg["tile_objs"] in (8192, 16384)
+    p.run()
+    b = p.download()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
