@@ -37,6 +37,10 @@ def main():
     ap.add_argument("--queries", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds of trace per offered load (0 = skip)")
+    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000])
+    ap.add_argument("--serve-executors", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device copy of this size (scales FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -131,6 +135,33 @@ def main():
 
     best = max(forms, key=lambda f: forms[f]["qps"])
 
+    # ---- serving form: Poisson arrivals at fixed offered loads ---------------
+    serving = None
+    if args.serve_seconds > 0:
+        from genie_amd import Server, synth
+        nreq_max = int(max(args.serve_qps) * args.serve_seconds)
+        sq, _ = synth.queries(7001, w["rows"], nreq_max, noise=3, lo=0, hi=255)
+        radius = np.array([0, 2, 5], np.int32)[synth.uniform_int(7002, 0, nreq_max, 0, 2)]
+        sk = np.array([10, 100, 1000], np.uint32)[synth.uniform_int(7003, 1, nreq_max, 0, 2)]
+        sranges = W.single_ranges(sq, radius)
+        sitems = table.compile(sranges, nreq_max)
+        sptr = np.searchsorted(sitems["query"], np.arange(nreq_max + 1)).astype(np.uint64)
+        serving = {"index": "the 1M x 128 index above, resident", "requests": "ranges {0,+-2,+-5}, k {10,100,1000}",
+                   "loads": []}
+        srv = Server(index, executors=args.serve_executors, max_batch=args.serve_batch, max_k=1000)
+        srv.replay(sitems[: int(sptr[64])], sptr[:65], sk[:64], np.zeros(64))  # warm
+        for qps in args.serve_qps:
+            n = int(qps * args.serve_seconds)
+            arr = np.cumsum(synth.exponential(7004 + qps, n, float(qps)))
+            lat, _, _, wall = srv.replay(sitems[: int(sptr[n])], sptr[: n + 1], sk[:n], arr)
+            ent = dict(offered_qps=qps, requests=n, achieved_qps=n / wall, p50_ms=float(np.percentile(lat, 50)),
+                       p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()))
+            serving["loads"].append(ent)
+            log(f"[bench] serving @ {qps} QPS offered: achieved {ent['achieved_qps']:.0f} QPS, "
+                f"p50 {ent['p50_ms']:.2f} ms, p99 {ent['p99_ms']:.2f} ms")
+        serving["stats"] = srv.stats()
+        srv.close()
+
     # ---- CPU baseline + exactness on the sample ------------------------------
     cpu = None
     exact = None
@@ -178,6 +209,7 @@ def main():
             "bytes_logical": logical,
             "bytes_compulsory": compulsory,
             "bit_exact_vs_oracle_sample": exact,
+            "serving": serving,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -76,6 +76,22 @@ _sig(_hip, "genie_hip_plan_get_config", C.c_int, _p, C.POINTER(_u32), C.POINTER(
      C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_autotune", C.c_int, _p, C.c_int, _p, C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
+_sig(_hip, "genie_hip_plan_download_async", C.c_int, _p, _p, _p, _p, _p)
+_sig(_hip, "genie_hip_host_alloc", C.c_int, C.POINTER(_p), _u64)
+_sig(_hip, "genie_hip_host_free", C.c_int, _p)
+
+
+class _SrvCfg(C.Structure):
+    _fields_ = [("executors", _u32), ("max_batch", _u32), ("max_batch_items", _u64), ("max_k", _u32),
+                ("memory_budget", _u64), ("mode", C.c_int)]
+
+
+_sig(_hip, "genie_hip_server_create", C.c_int, _p, C.POINTER(_SrvCfg), C.POINTER(_p))
+_sig(_hip, "genie_hip_server_destroy", C.c_int, _p)
+_sig(_hip, "genie_hip_server_submit", C.c_int, _p, _p, _u32, _u32, C.POINTER(_u64))
+_sig(_hip, "genie_hip_server_wait", C.c_int, _p, _u64, _p, _p, C.POINTER(C.c_double))
+_sig(_hip, "genie_hip_server_replay", C.c_int, _p, _p, _p, _p, _p, _u32, _p, _p, _p, C.POINTER(C.c_double))
+_sig(_hip, "genie_hip_server_stats", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64))
 _sig(_hip, "genie_hip_decode", C.c_int, C.c_int, _p, _p, _u32, _u32, _p, _p)
 _sig(_hip, "genie_hip_last_error", C.c_char_p)
 _sig(_hip, "genie_hip_stream_synchronize", C.c_int, _p)
@@ -264,6 +280,63 @@ class Plan:
     def close(self):
         if self.handle:
             _hip.genie_hip_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop over a resident index (genie_hip_server_*)."""
+
+    def __init__(self, index, executors=4, max_batch=256, max_batch_items=1 << 20, max_k=1000,
+                 memory_budget=1 << 30, mode=0):
+        self.index = index
+        cfg = _SrvCfg(executors, max_batch, max_batch_items, max_k, memory_budget, mode)
+        h = _p()
+        _chk_hip(_hip.genie_hip_server_create(index.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+
+    def submit(self, items, k):
+        items = _c(items, DIM_DTYPE)
+        t = _u64()
+        _chk_hip(_hip.genie_hip_server_submit(self.handle, _ptr(items), len(items), k, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, k):
+        ids = np.zeros(k, np.int32)
+        cnt = np.zeros(k, np.int32)
+        lat = C.c_double()
+        _chk_hip(_hip.genie_hip_server_wait(self.handle, ticket, _ptr(ids), _ptr(cnt), C.byref(lat)))
+        return ids, cnt, lat.value
+
+    def replay(self, items, item_ptr, k, arrival_s):
+        """Open-loop replay; returns (latency_ms, ids, counts, wall_s)."""
+        items = _c(items, DIM_DTYPE)
+        item_ptr = _c(item_ptr, np.uint64)
+        n = len(item_ptr) - 1
+        kq = _c(np.broadcast_to(np.asarray(k, np.uint32), (n,)), np.uint32)
+        arr = _c(arrival_s, np.float64)
+        lat = np.zeros(n, np.float64)
+        tot = int(kq.sum())
+        ids = np.zeros(tot, np.int32)
+        cnt = np.zeros(tot, np.int32)
+        wall = C.c_double()
+        _chk_hip(_hip.genie_hip_server_replay(self.handle, _ptr(items), _ptr(item_ptr), _ptr(kq), _ptr(arr), n,
+                                              _ptr(lat), _ptr(ids), _ptr(cnt), C.byref(wall)))
+        return lat, ids, cnt, wall.value
+
+    def stats(self):
+        a, b, c = _u64(), _u64(), _u64()
+        _chk_hip(_hip.genie_hip_server_stats(self.handle, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(batches=a.value, requests=b.value, max_inflight=c.value)
+
+    def close(self):
+        if self.handle:
+            _hip.genie_hip_server_destroy(self.handle)
         self.handle = None
 
     def __del__(self):

@@ -30,7 +30,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wall", "-Wno-unused-result", f"-I{INC}"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{INC}"]
 
-HIP_SOURCES = ["hip/engine.hip"]
+HIP_SOURCES = ["hip/engine.hip", "hip/server.hip"]
 HOST_SOURCES = ["host/codecs.cc", "host/inv_list.cc", "host/inv_table.cc", "host/query.cc",
                 "host/knn.cc", "host/original.cc", "host/interface.cc", "host/file_reader.cc",
                 "host/capi.cc"]

@@ -1383,6 +1383,35 @@ int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32
     return GENIE_OK;
 }
 
+int genie_hip_plan_download_async(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts,
+                                  uint32_t* overflow)
+{
+    if (!p || !out_ids || !out_counts || !overflow)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (!p->ran)
+        return fail(GENIE_E_NOT_BUILT, "plan has not run");
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(out_ids, p->d_out_ids, p->total_k * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out_counts, p->d_out_counts, p->total_k * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(overflow, p->d_overflow, 4, hipMemcpyDeviceToHost, s));
+    return GENIE_OK;
+}
+
+int genie_hip_host_alloc(void** ptr, uint64_t bytes)
+{
+    if (!ptr)
+        return fail(GENIE_E_INVALID, "null argument");
+    HIP_TRY(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return GENIE_OK;
+}
+
+int genie_hip_host_free(void* ptr)
+{
+    if (ptr)
+        HIP_TRY(hipHostFree(ptr));
+    return GENIE_OK;
+}
+
 int genie_hip_plan_set_timing(genie_plan* p, int on)
 {
     if (!p)

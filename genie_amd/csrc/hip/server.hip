@@ -1,0 +1,450 @@
+// Serving form: a continuous-batching request loop over an HBM-resident
+// index (C ABI in include/genie_hip.h).
+//
+// One scheduler thread owns E executors (a plan and a non-blocking stream
+// each).  Whenever an executor is free and requests are pending, it packs the
+// pending requests (FIFO) into a batch -- as many as fit the batch limits and
+// the device-memory budget of in-flight requests -- uploads it, launches the
+// search and an asynchronous copy of the results into pinned host memory, and
+// goes on; so requests that arrive while other batches are on the device are
+// admitted into the next free executor and their work overlaps on the GPU.
+// Completion is detected with events; results go back per request.
+//
+// The reference serves one JSON batch at a time from a polled queue
+// (src/dist_genie/online.cu:144-180, scheduler.cc:20-64) and admits whole
+// tables by free memory (knn.cu:207-243); here admission is per request.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+#include "genie_hip.h"
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Request {
+    uint64_t ticket = 0;
+    std::vector<genie_dim_t> items;
+    uint32_t k = 0;
+    uint64_t mem = 0;
+    Clock::time_point t_submit, t_done;
+    std::vector<int32_t> ids, counts;
+    int rc = GENIE_OK;
+    std::string err;
+    bool done = false;
+};
+
+struct Executor {
+    genie_plan* plan = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    int32_t* h_ids = nullptr;
+    int32_t* h_counts = nullptr;
+    uint32_t* h_ovf = nullptr;
+    uint64_t host_cap = 0;
+    std::vector<Request*> batch;
+    uint64_t mem = 0;
+    bool busy = false;
+};
+
+// Device scratch one request holds while in flight: its work items, the
+// candidate space (8 tiles' worth of k keys, grown on overflow) and its
+// output rows.
+uint64_t request_bytes(uint64_t n_items, uint32_t k)
+{
+    return n_items * sizeof(genie_dim_t) + (uint64_t)k * 8 * 8 + (uint64_t)k * 8 + 64;
+}
+
+}  // namespace
+
+struct genie_server {
+    genie_index* ix = nullptr;
+    genie_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::vector<Executor> ex;
+    uint64_t next_ticket = 1, mem_in_use = 0;
+    uint64_t n_batches = 0, n_done = 0, max_inflight = 0, inflight = 0;
+    bool stop = false;
+    std::thread sched;
+    int device = 0;
+
+    void run();
+    int launch(Executor& e);
+    void finish(Executor& e);
+};
+
+static int fail_s(int code, const std::string& m)
+{
+    gh::set_error(m);
+    return code;
+}
+
+int genie_server::launch(Executor& e)
+{
+    const uint32_t nq = (uint32_t)e.batch.size();
+    std::vector<genie_dim_t> items;
+    std::vector<uint32_t> kq(nq);
+    for (uint32_t q = 0; q < nq; ++q) {
+        Request* r = e.batch[q];
+        for (genie_dim_t it : r->items) {
+            it.query = (int32_t)q;
+            items.push_back(it);
+        }
+        kq[q] = r->k;
+    }
+    uint64_t tot = 0;
+    for (uint32_t k : kq)
+        tot += k;
+    if (tot > e.host_cap) {
+        genie_hip_host_free(e.h_ids);
+        genie_hip_host_free(e.h_counts);
+        e.h_ids = e.h_counts = nullptr;
+        e.host_cap = 0;
+        if (int rc = genie_hip_host_alloc((void**)&e.h_ids, tot * 4))
+            return rc;
+        if (int rc = genie_hip_host_alloc((void**)&e.h_counts, tot * 4))
+            return rc;
+        e.host_cap = tot;
+    }
+    if (int rc = genie_hip_plan_upload(e.plan, items.data(), items.size(), nq, kq.data()))
+        return rc;
+    if (int rc = genie_hip_plan_run(e.plan, e.stream))
+        return rc;
+    if (int rc = genie_hip_plan_download_async(e.plan, e.stream, e.h_ids, e.h_counts, e.h_ovf))
+        return rc;
+    if (hipEventRecord(e.ev, e.stream) != hipSuccess)
+        return fail_s(GENIE_E_RUNTIME, "hipEventRecord");
+    return GENIE_OK;
+}
+
+void genie_server::finish(Executor& e)
+{
+    int rc = GENIE_OK;
+    if (*e.h_ovf)  // candidate space too small: grow, rerun, copy synchronously
+        rc = genie_hip_plan_download(e.plan, e.stream, e.h_ids, e.h_counts);
+    const std::string err = rc ? genie_hip_last_error() : "";
+    const auto now = Clock::now();
+    uint64_t off = 0;
+    std::lock_guard<std::mutex> lk(mu);
+    for (Request* r : e.batch) {
+        r->rc = rc;
+        r->err = err;
+        if (!rc) {
+            r->ids.assign(e.h_ids + off, e.h_ids + off + r->k);
+            r->counts.assign(e.h_counts + off, e.h_counts + off + r->k);
+        }
+        off += r->k;
+        r->t_done = now;
+        r->done = true;
+    }
+    n_done += e.batch.size();
+    inflight -= e.batch.size();
+    mem_in_use -= e.mem;
+    e.batch.clear();
+    e.mem = 0;
+    e.busy = false;
+    cv_done.notify_all();
+}
+
+void genie_server::run()
+{
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        bool progressed = false;
+        // dispatch pending requests to free executors
+        for (Executor& e : ex) {
+            if (e.busy || pending.empty())
+                continue;
+            uint64_t items = 0, mem = 0;
+            while (!pending.empty() && e.batch.size() < cfg.max_batch) {
+                Request* r = pending.front();
+                const bool first = e.batch.empty() && inflight == 0;  // always admit one when idle
+                if (!first && (items + r->items.size() > cfg.max_batch_items ||
+                               mem_in_use + mem + r->mem > cfg.memory_budget))
+                    break;
+                if (first && r->items.size() > cfg.max_batch_items)
+                    break;
+                items += r->items.size();
+                mem += r->mem;
+                e.batch.push_back(r);
+                pending.pop_front();
+            }
+            if (e.batch.empty())
+                continue;
+            e.mem = mem;
+            e.busy = true;
+            mem_in_use += mem;
+            inflight += e.batch.size();
+            max_inflight = std::max(max_inflight, inflight);
+            ++n_batches;
+            lk.unlock();
+            const int rc = launch(e);
+            if (rc) {  // report the failure on every request of the batch
+                const std::string err = genie_hip_last_error();
+                lk.lock();
+                for (Request* r : e.batch) {
+                    r->rc = rc;
+                    r->err = err;
+                    r->t_done = Clock::now();
+                    r->done = true;
+                }
+                n_done += e.batch.size();
+                inflight -= e.batch.size();
+                mem_in_use -= e.mem;
+                e.batch.clear();
+                e.busy = false;
+                cv_done.notify_all();
+                lk.unlock();
+            }
+            lk.lock();
+            progressed = true;
+        }
+        // completions
+        for (Executor& e : ex) {
+            if (!e.busy || e.batch.empty())
+                continue;
+            lk.unlock();
+            const hipError_t q = hipEventQuery(e.ev);
+            if (q == hipSuccess) {
+                finish(e);
+                progressed = true;
+            } else if (q != hipErrorNotReady) {
+                hipGetLastError();
+                std::lock_guard<std::mutex> g(mu);
+                for (Request* r : e.batch) {
+                    r->rc = GENIE_E_RUNTIME;
+                    r->err = hipGetErrorString(q);
+                    r->t_done = Clock::now();
+                    r->done = true;
+                }
+                n_done += e.batch.size();
+                inflight -= e.batch.size();
+                mem_in_use -= e.mem;
+                e.batch.clear();
+                e.busy = false;
+                cv_done.notify_all();
+            }
+            lk.lock();
+        }
+        if (stop && pending.empty() && inflight == 0)
+            break;
+        if (!progressed) {
+            // idle: wake on a new request, or poll the in-flight events again soon
+            if (inflight == 0)
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+            else
+                cv_work.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+}
+
+extern "C" {
+
+int genie_hip_server_create(genie_index* ix, const genie_server_config* cfg, genie_server** out)
+{
+    if (!ix || !cfg || !out)
+        return fail_s(GENIE_E_INVALID, "null argument");
+    if (cfg->executors == 0 || cfg->executors > 16 || cfg->max_batch == 0 || cfg->max_batch_items == 0 ||
+        cfg->max_k == 0 || cfg->max_k > GENIE_MAX_K || (cfg->mode != 0 && cfg->mode != 1))
+        return fail_s(GENIE_E_INVALID, "server configuration out of range");
+    auto* s = new genie_server();
+    s->ix = ix;
+    s->cfg = *cfg;
+    if (s->cfg.memory_budget == 0)
+        s->cfg.memory_budget = 1ull << 30;
+    (void)hipGetDevice(&s->device);
+    s->ex.resize(cfg->executors);
+    int rc = GENIE_OK;
+    for (Executor& e : s->ex) {
+        if ((rc = genie_hip_plan_create(ix, cfg->max_batch, cfg->max_batch_items, cfg->max_k, 0, cfg->mode,
+                                        &e.plan)))
+            break;
+        if (hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e.ev, hipEventDisableTiming) != hipSuccess) {
+            rc = fail_s(GENIE_E_RUNTIME, "stream/event creation");
+            break;
+        }
+        if ((rc = genie_hip_host_alloc((void**)&e.h_ovf, 4)))
+            break;
+        *e.h_ovf = 0;
+    }
+    if (rc) {
+        genie_hip_server_destroy(s);
+        return rc;
+    }
+    s->sched = std::thread([s] { s->run(); });
+    *out = s;
+    return GENIE_OK;
+}
+
+int genie_hip_server_destroy(genie_server* s)
+{
+    if (!s)
+        return GENIE_OK;
+    if (s->sched.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->sched.join();
+    }
+    for (Executor& e : s->ex) {
+        if (e.stream)
+            (void)hipStreamSynchronize(e.stream);
+        genie_hip_plan_destroy(e.plan);
+        if (e.ev)
+            (void)hipEventDestroy(e.ev);
+        if (e.stream)
+            (void)hipStreamDestroy(e.stream);
+        genie_hip_host_free(e.h_ids);
+        genie_hip_host_free(e.h_counts);
+        genie_hip_host_free(e.h_ovf);
+    }
+    delete s;
+    return GENIE_OK;
+}
+
+int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t n_items, uint32_t k,
+                            uint64_t* ticket)
+{
+    if (!s || (!items && n_items) || !ticket)
+        return fail_s(GENIE_E_INVALID, "null argument");
+    if (k == 0 || k > s->cfg.max_k)
+        return fail_s(GENIE_E_INVALID, "k outside [1, max_k]");
+    if (n_items > s->cfg.max_batch_items)
+        return fail_s(GENIE_E_INVALID, "request has more work items than a batch holds");
+    auto r = std::make_unique<Request>();
+    r->items.assign(items, items + n_items);
+    for (genie_dim_t& it : r->items)
+        it.query = 0;
+    r->k = k;
+    r->mem = request_bytes(n_items, k);
+    // validate alone so that one bad request cannot fail a whole batch
+    uint32_t bound = 0;
+    if (int rc = genie_hip_count_bounds(s->ix, r->items.data(), n_items, 1, &bound))
+        return rc;
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->ticket = s->next_ticket++;
+        *ticket = r->ticket;
+        s->pending.push_back(r.get());
+        s->reqs.emplace(r->ticket, std::move(r));
+    }
+    s->cv_work.notify_one();
+    return GENIE_OK;
+}
+
+int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, int32_t* out_counts,
+                          double* latency_ms)
+{
+    if (!s)
+        return fail_s(GENIE_E_INVALID, "null server");
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end())
+            return fail_s(GENIE_E_INVALID, "unknown ticket");
+        Request* rp = it->second.get();
+        s->cv_done.wait(lk, [&] { return rp->done; });
+        r = std::move(it->second);
+        s->reqs.erase(it);
+    }
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->rc)
+        return fail_s(r->rc, r->err);
+    if (out_ids)
+        std::memcpy(out_ids, r->ids.data(), r->k * 4);
+    if (out_counts)
+        std::memcpy(out_counts, r->counts.data(), r->k * 4);
+    return GENIE_OK;
+}
+
+int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uint64_t* item_ptr,
+                            const uint32_t* k_per_request, const double* arrival_s, uint32_t n_requests,
+                            double* out_latency_ms, int32_t* out_ids, int32_t* out_counts, double* wall_s)
+{
+    if (!s || !item_ptr || !k_per_request || !arrival_s || !out_latency_ms)
+        return fail_s(GENIE_E_INVALID, "null argument");
+    std::vector<uint64_t> tickets(n_requests, 0);
+    std::vector<std::atomic<int>> submitted(n_requests);
+    for (auto& a : submitted)
+        a.store(0);
+    std::atomic<int> gen_rc(GENIE_OK);
+    std::string gen_err;
+    const auto t0 = Clock::now();
+    std::thread gen([&] {
+        for (uint32_t r = 0; r < n_requests; ++r) {
+            std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
+                                                   std::chrono::duration<double>(arrival_s[r])));
+            const int rc = genie_hip_server_submit(s, items + item_ptr[r], (uint32_t)(item_ptr[r + 1] - item_ptr[r]),
+                                                   k_per_request[r], &tickets[r]);
+            if (rc) {
+                gen_rc = rc;
+                gen_err = genie_hip_last_error();
+                submitted[r].store(-1);
+                // later requests are still submitted so the trace keeps its shape
+                continue;
+            }
+            submitted[r].store(1);
+        }
+    });
+    int rc = GENIE_OK;
+    uint64_t off = 0;
+    for (uint32_t r = 0; r < n_requests; ++r) {
+        while (submitted[r].load() == 0)  // the generator has not reached request r yet
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        const uint32_t k = k_per_request[r];
+        if (submitted[r].load() < 0) {
+            out_latency_ms[r] = -1;
+        } else {
+            const int w = genie_hip_server_wait(s, tickets[r], out_ids ? out_ids + off : nullptr,
+                                                out_counts ? out_counts + off : nullptr, &out_latency_ms[r]);
+            if (w && !rc)
+                rc = w;
+        }
+        off += k;
+    }
+    gen.join();
+    if (wall_s)
+        *wall_s = std::chrono::duration<double>(Clock::now() - t0).count();
+    if (gen_rc != GENIE_OK)
+        return fail_s(gen_rc, gen_err);
+    return rc;
+}
+
+int genie_hip_server_stats(genie_server* s, uint64_t* batches, uint64_t* requests, uint64_t* max_inflight)
+{
+    if (!s)
+        return fail_s(GENIE_E_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (batches)
+        *batches = s->n_batches;
+    if (requests)
+        *requests = s->n_done;
+    if (max_inflight)
+        *max_inflight = s->max_inflight;
+    return GENIE_OK;
+}
+
+}  // extern "C"

@@ -141,6 +141,51 @@ int genie_hip_plan_host_ms(genie_plan* p, double* ms);
  * the same over distinct keys/segments touched, counted once. */
 int genie_hip_plan_bytes(genie_plan* p, uint64_t* logical, uint64_t* compulsory);
 
+/* Asynchronous result copy into caller-owned pinned host memory
+ * (genie_hip_host_alloc): ids, counts and the candidate-overflow flag.  When
+ * *overflow reads nonzero after the stream completes, call
+ * genie_hip_plan_download, which grows the candidate space and reruns. */
+int genie_hip_plan_download_async(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts,
+                                  uint32_t* overflow);
+int genie_hip_host_alloc(void** ptr, uint64_t bytes);
+int genie_hip_host_free(void* ptr);
+
+/* ---- serving form ------------------------------------------------------- */
+/* A request loop over a resident index: requests (the work items of ONE
+ * query plus its k) are admitted while the estimated count-table memory of
+ * the in-flight ones stays within the budget, packed into batches as they
+ * arrive and run on several executors (plan + stream each), so batches that
+ * arrive while others are on the device overlap with them.  Results are per
+ * request, identical to running the request alone. */
+typedef struct genie_server_config {
+    uint32_t executors;       /* concurrent batches on the device (streams), >= 1 */
+    uint32_t max_batch;       /* queries per batch */
+    uint64_t max_batch_items; /* work items per batch */
+    uint32_t max_k;           /* largest k accepted */
+    uint64_t memory_budget;   /* bytes of device scratch for in-flight requests */
+    int mode;                 /* plan mode of the executors: 0 plain, 1 batched */
+} genie_server_config;
+
+typedef struct genie_server genie_server;
+int genie_hip_server_create(genie_index* ix, const genie_server_config* cfg, genie_server** out);
+int genie_hip_server_destroy(genie_server* s);
+/* Thread-safe.  items: the query's work items (their .query is ignored). */
+int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t n_items, uint32_t k,
+                            uint64_t* ticket);
+/* Block until the request is done; copies its k results and its latency
+ * (submit to completion, ms).  Each ticket can be waited once. */
+int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, int32_t* out_counts,
+                          double* latency_ms);
+/* Open-loop trace replay: request r (items [item_ptr[r], item_ptr[r+1]),
+ * k_per_request[r]) is submitted at arrival_s[r] seconds after the start by
+ * a generator thread; returns per-request latency (ms), all results
+ * (out_off[r] = sum of earlier k), and the wall time of the whole replay. */
+int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uint64_t* item_ptr,
+                            const uint32_t* k_per_request, const double* arrival_s, uint32_t n_requests,
+                            double* out_latency_ms, int32_t* out_ids, int32_t* out_counts, double* wall_s);
+/* Batches run and requests completed so far. */
+int genie_hip_server_stats(genie_server* s, uint64_t* batches, uint64_t* requests, uint64_t* max_inflight);
+
 /* ---- codec decode (tests and measurement) ------------------------------ */
 /* Decode n_arrays compressed arrays ([cpos[i], cpos[i+1]) words of `words`)
  * on the device with the engine's wave decoders; output i goes to

@@ -1,0 +1,78 @@
+"""Serving form: continuous batching over a resident index.  Every request's
+result is the same alone and in the mix (bit-exact), the memory budget
+admission holds, and bad requests are refused without failing others."""
+import numpy as np
+import pytest
+
+import genie_amd
+from genie_amd import GenieError, Server, Table, match_topk, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def requests_of(table, w):
+    items = table.compile(w["ranges"], w["n_queries"])
+    q = items["query"]
+    ptr = np.searchsorted(q, np.arange(w["n_queries"] + 1)).astype(np.uint64)
+    return items, ptr
+
+
+@pytest.mark.parametrize("mode,codec", [(0, 0), (1, 0), (0, 11)])
+def test_alone_equals_in_the_mix(mode, codec):
+    w = W.cfg4(n=200_000, d=32, q=60)
+    t = Table.from_rows(w["rows"], 1024, codec != 0, codec=codec)
+    items, ptr = requests_of(t, w)
+    k = w["k"]
+    srv = Server(t.device(), executors=3, max_batch=16, max_k=1000, mode=mode)
+    alone = []
+    for r in range(w["n_queries"]):
+        tk = srv.submit(items[ptr[r]:ptr[r + 1]], int(k[r]))
+        ids, cnt, lat = srv.wait(tk, int(k[r]))
+        alone.append((ids, cnt))
+        assert lat >= 0
+    # all at once (t = 0) and as a Poisson trace: batches form and overlap
+    for arrivals in (np.zeros(w["n_queries"]), np.cumsum(genie_amd.synth.exponential(9, w["n_queries"], 4000.0))):
+        lat, ids, cnt, wall = srv.replay(items, ptr, k, arrivals)
+        off = 0
+        for r in range(w["n_queries"]):
+            kr = int(k[r])
+            assert np.array_equal(ids[off:off + kr], alone[r][0]) and np.array_equal(cnt[off:off + kr], alone[r][1]), r
+            off += kr
+        assert np.all(lat > 0) and wall > 0
+    st = srv.stats()
+    assert st["requests"] == 3 * w["n_queries"] and st["batches"] < st["requests"]
+    assert st["max_inflight"] > 1
+    # and the same as one plain batch of everything
+    pi, pc = match_topk(t.device(), items, w["n_queries"], k)
+    assert np.array_equal(pi, ids) and np.array_equal(pc, cnt)
+    srv.close()
+
+
+def test_memory_budget_admits_one_at_a_time():
+    w = W.cfg1(n=50_000, d=16, q=12, k=100)
+    t = Table.from_rows(w["rows"])
+    items, ptr = requests_of(t, w)
+    srv = Server(t.device(), executors=4, max_batch=8, max_k=100, memory_budget=1)
+    lat, ids, cnt, wall = srv.replay(items, ptr, 100, np.zeros(12))
+    pi, pc = match_topk(t.device(), items, 12, 100)
+    assert np.array_equal(pi, ids) and np.array_equal(pc, cnt)
+    assert srv.stats()["max_inflight"] == 1
+    srv.close()
+
+
+def test_bad_request_is_refused():
+    w = W.cfg0(n=500, d=8, q=2)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 2)
+    srv = Server(t.device(), executors=2, max_batch=4, max_k=50)
+    bad = items[:3].copy()
+    bad["weight"][0] = 0.5
+    with pytest.raises(GenieError):
+        srv.submit(bad, 10)
+    with pytest.raises(GenieError):
+        srv.submit(items[:3], 51)
+    tk = srv.submit(items[items["query"] == 0], 10)
+    ids, cnt, _ = srv.wait(tk, 10)
+    ref = match_topk(t.device(), items[items["query"] == 0], 1, 10)
+    assert np.array_equal(ids, ref[0]) and np.array_equal(cnt, ref[1])
+    srv.close()
