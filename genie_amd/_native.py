@@ -75,6 +75,8 @@ _sig(_hip, "genie_hip_plan_set_config", C.c_int, _p, _u32, _u32, _u32)
 _sig(_hip, "genie_hip_plan_get_config", C.c_int, _p, C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u32),
      C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_autotune", C.c_int, _p, C.c_int, _p, C.POINTER(_u32))
+_sig(_hip, "genie_hip_plan_set_dense", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_dense_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_plan_download_async", C.c_int, _p, _p, _p, _p, _p)
 _sig(_hip, "genie_hip_host_alloc", C.c_int, C.POINTER(_p), _u64)
@@ -264,7 +266,18 @@ class Plan:
     def config(self):
         v = [_u32() for _ in range(4)]
         _chk_hip(_hip.genie_hip_plan_get_config(self.handle, *[C.byref(x) for x in v]))
-        return dict(block=v[0].value, tile_objs=v[1].value, seglen=v[2].value, lane_bits=v[3].value)
+        return dict(block=v[0].value, tile_objs=v[1].value, seglen=v[2].value, lane_bits=v[3].value,
+                    dense=self.dense_active())
+
+    def set_dense(self, on=True):
+        """Batched plans: count whole-key unit-weight batches of a single-value
+        index from per-object key codes (bit-sliced, 32 queries per lookup)."""
+        _chk_hip(_hip.genie_hip_plan_set_dense(self.handle, 1 if on else 0))
+
+    def dense_active(self):
+        v = C.c_int()
+        _chk_hip(_hip.genie_hip_plan_dense_active(self.handle, C.byref(v)))
+        return bool(v.value)
 
     def autotune(self, reps=2):
         ms = np.full(16, -1.0)

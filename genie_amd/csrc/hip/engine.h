@@ -48,6 +48,12 @@ struct DevIndex {
     uint32_t* seg_last = nullptr;   // [n_segments] last id
     uint32_t* key_seg = nullptr;    // [n_keys + 1] first segment of each key
     uint32_t* sdir = nullptr;       // [n_dir_keys * (n_dtiles + 1)] segment directory (compressed)
+    // forward codes of a single-value table (dense form of the batched search)
+    bool dense = false;
+    uint32_t dense_dpad = 0;        // dims rounded up to 16
+    uint32_t dense_npad = 0;        // objects rounded up to 64 (row stride of codes)
+    uint32_t* codes = nullptr;      // [dense_dpad / 4][dense_npad] four 8-bit codes per word
+    uint32_t* key_slot = nullptr;   // [n_keys] dim * 256 + code
     uint64_t bytes = 0;
     // host copies used for validation and count bounds
     std::vector<uint64_t> h_key_start;

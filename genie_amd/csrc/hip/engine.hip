@@ -32,6 +32,7 @@
 #include "engine.h"
 #include "kernels.h"
 #include "kernels_batched.h"
+#include "kernels_dense.h"
 #include "wave.h"
 
 namespace gh {
@@ -140,6 +141,16 @@ struct genie_plan {
     std::vector<genie_dim_t> h_items;  // sorted by query, kept to rebuild the work queue
     std::vector<uint32_t> h_qptr;
     double host_build_ms = 0;
+    bool lanes_built = false;        // work queue of the lane form built for this batch
+    // dense form of the batched search (single-value index, whole-key unit-weight batch)
+    bool dense_on = true;            // use it when the batch allows
+    bool dense_ok = false;           // the uploaded batch allows it
+    uint32_t dn_groups = 0;
+    uint32_t* d_dtab = nullptr;      // [dn_groups][dpad * 256] query masks
+    uint64_t dtab_alloc = 0;
+    uint32_t* d_dhist = nullptr;     // [max_queries][kDenseBins]
+    uint32_t* d_dlist = nullptr;     // sample tiles
+    uint32_t n_dlist = 0, dlist_alloc = 0;
     // graph replay
     bool graph_on = false, graph_valid = false;
     hipStream_t own_stream = nullptr;
@@ -148,7 +159,8 @@ struct genie_plan {
     // timing
     bool timing = false;
     hipEvent_t ev[8] = {};
-    double ms_compile = 0, ms_scan = 0, ms_merge = 0;
+    double ms_compile = 0, ms_scan = 0, ms_merge = 0, ms_sample = 0;
+    bool timed_dense = false;
     bool ran = false;
 };
 
@@ -200,6 +212,9 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_out_off);
     dev_free(p->d_out_ids);
     dev_free(p->d_out_counts);
+    dev_free(p->d_dtab);
+    dev_free(p->d_dhist);
+    dev_free(p->d_dlist);
 }
 
 // Count bound of every query: per (query, order) group the largest weight
@@ -616,6 +631,78 @@ static int launch_batched(genie_plan* p, hipStream_t s)
     }
 }
 
+// Forward codes for the dense form: only for a single-value table of at most
+// kDenseMaxDims dims with at most 256 keys per dim, keys ordered by dim, in
+// which (checked on the device) every object appears exactly once per dim.
+// Any other index simply has no dense form; only device errors fail.
+static int build_dense(const genie_index_desc* desc, DevIndex& d)
+{
+    const uint32_t D = desc->n_dims, N = desc->n_objects, K = desc->n_keys;
+    if (!desc->single_value || D == 0 || D > kDenseMaxDims || !desc->postings)
+        return GENIE_OK;
+    std::vector<uint32_t> key_base(D, 0xFFFFFFFFu), per_dim(D, 0);
+    std::vector<uint64_t> dim_post(D, 0);
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t dm = desc->key_dim[k];
+        if (dm >= D || (k && dm < desc->key_dim[k - 1]))
+            return GENIE_OK;
+        if (key_base[dm] == 0xFFFFFFFFu)
+            key_base[dm] = k;
+        if (++per_dim[dm] > kDenseKeysPerDim)
+            return GENIE_OK;
+        dim_post[dm] += desc->key_start[k + 1] - desc->key_start[k];
+    }
+    for (uint32_t dm = 0; dm < D; ++dm)
+        if (dim_post[dm] != N)
+            return GENIE_OK;
+    std::vector<uint32_t> slot(K);
+    for (uint32_t k = 0; k < K; ++k)
+        slot[k] = desc->key_dim[k] * kDenseKeysPerDim + (k - key_base[desc->key_dim[k]]);
+    const uint32_t dpad = (D + 15) / 16 * 16, npad = (N + 63) / 64 * 64;
+    const uint32_t bm_words = (D + 31) / 32;
+    uint32_t *codes = nullptr, *key_slot = nullptr, *bm = nullptr, *flag = nullptr, *dim = nullptr;
+    uint64_t bytes = 0;
+    int rc = GENIE_OK;
+    uint32_t h_flag = 0;
+    do {
+        if ((rc = dev_alloc(&codes, (uint64_t)dpad / 4 * npad, &bytes))) break;
+        if ((rc = dev_alloc(&key_slot, K, &bytes))) break;
+        if ((rc = dev_alloc(&bm, (uint64_t)N * bm_words))) break;
+        if ((rc = dev_alloc(&flag, 1))) break;
+        if ((rc = dev_alloc(&dim, K))) break;
+        hipError_t e;
+        if ((e = hipMemset(codes, 0, (uint64_t)dpad * npad)) || (e = hipMemset(bm, 0, (uint64_t)N * bm_words * 4)) ||
+            (e = hipMemset(flag, 0, 4)) ||
+            (e = hipMemcpy(key_slot, slot.data(), K * 4ull, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(dim, desc->key_dim, K * 4ull, hipMemcpyHostToDevice))) {
+            rc = fail(GENIE_E_RUNTIME, std::string("dense codes: ") + hipGetErrorString(e));
+            break;
+        }
+        k_dense_codes<<<K, 256>>>(d.post, d.key_start, dim, key_slot, npad, bm_words,
+                                  reinterpret_cast<uint8_t*>(codes), bm, flag);
+        k_dense_holes<<<(N + 255) / 256, 256>>>(bm, N, D, bm_words, flag);
+        if ((e = hipGetLastError()) || (e = hipMemcpy(&h_flag, flag, 4, hipMemcpyDeviceToHost))) {
+            rc = fail(GENIE_E_RUNTIME, std::string("dense codes: ") + hipGetErrorString(e));
+            break;
+        }
+    } while (0);
+    dev_free(bm);
+    dev_free(flag);
+    dev_free(dim);
+    if (rc || h_flag) {
+        dev_free(codes);
+        dev_free(key_slot);
+        return rc;
+    }
+    d.dense = true;
+    d.dense_dpad = dpad;
+    d.dense_npad = npad;
+    d.codes = codes;
+    d.key_slot = key_slot;
+    d.bytes += bytes;
+    return GENIE_OK;
+}
+
 extern "C" {
 
 const char* genie_hip_last_error(void) { return g_err.c_str(); }
@@ -763,6 +850,8 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
         }
     } while (0);
     dev_free(d_dir_keys);
+    if (!rc)
+        rc = build_dense(desc, d);
     if (rc) {
         genie_hip_index_destroy(ix);
         return rc;
@@ -788,6 +877,8 @@ int genie_hip_index_destroy(genie_index* ix)
     dev_free(d.seg_last);
     dev_free(d.key_seg);
     dev_free(d.sdir);
+    dev_free(d.codes);
+    dev_free(d.key_slot);
     delete ix;
     return GENIE_OK;
 }
@@ -876,7 +967,14 @@ int genie_hip_plan_destroy(genie_plan* p)
 
 }  // extern "C"
 
-static uint32_t plan_tile_objs(const genie_plan* p) { return p->mode == 1 ? p->tile_b : tile_objs_for(p->lane_bits); }
+static bool dense_active(const genie_plan* p) { return p->mode == 1 && p->dense_on && p->dense_ok; }
+
+static uint32_t plan_tile_objs(const genie_plan* p)
+{
+    if (dense_active(p))
+        return kDenseTile;
+    return p->mode == 1 ? p->tile_b : tile_objs_for(p->lane_bits);
+}
 
 static uint32_t plan_n_tiles(const genie_plan* p)
 {
@@ -884,16 +982,32 @@ static uint32_t plan_n_tiles(const genie_plan* p)
     return (p->ix->d.n_objects + t - 1) / t;
 }
 
-// Candidate capacity per query: cap_factor tiles' worth of k, at most all tiles.
+// Largest useful cap_factor: the tile forms emit at most k per tile, the
+// dense form at most every object once per query.
+static uint32_t max_cap_factor(const genie_plan* p)
+{
+    if (!dense_active(p))
+        return plan_n_tiles(p);
+    uint32_t kmin = 0xFFFFFFFFu;
+    for (uint32_t q = 0; q < p->n_queries; ++q)
+        kmin = std::min(kmin, p->h_kq[q]);
+    return (p->ix->d.n_objects + kmin - 1) / kmin;
+}
+
+static uint32_t initial_cap_factor(const genie_plan* p) { return dense_active(p) ? 64u : 8u; }
+
+// Candidate capacity per query: cap_factor x k, at most what the form can emit.
 static int plan_size_candidates(genie_plan* p)
 {
+    const bool dn = dense_active(p);
     const uint32_t n_tiles = plan_n_tiles(p);
-    const uint32_t f = std::min<uint32_t>(p->cap_factor, n_tiles);
+    const uint32_t f = std::min<uint32_t>(p->cap_factor, max_cap_factor(p));
     std::vector<uint32_t> cap(p->n_queries);
     std::vector<uint64_t> base(p->n_queries);
     uint64_t tot = 0;
     for (uint32_t q = 0; q < p->n_queries; ++q) {
-        cap[q] = f * p->h_kq[q];
+        const uint64_t limit = dn ? (uint64_t)p->ix->d.n_objects : (uint64_t)n_tiles * p->h_kq[q];
+        cap[q] = (uint32_t)std::min<uint64_t>((uint64_t)f * p->h_kq[q], limit);
         base[q] = tot;
         tot += cap[q];
     }
@@ -908,6 +1022,103 @@ static int plan_size_candidates(genie_plan* p)
     HIP_TRY(hipMemcpy(p->d_cand_cap, cap.data(), cap.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_cand_base, base.data(), base.size() * 8, hipMemcpyHostToDevice));
     return GENIE_OK;
+}
+
+// Whether the uploaded batch (kept on the host by mode-1 plans) can run the
+// dense form: every item with postings and a nonzero weight covers whole keys
+// with weight 1, and no query covers a key twice.
+static bool dense_batch_ok(const genie_plan* p)
+{
+    const DevIndex& d = p->ix->d;
+    if (p->mode != 1 || !d.dense)
+        return false;
+    const uint64_t* ks = d.h_key_start.data();
+    const uint32_t nk = d.n_keys;
+    std::vector<std::pair<uint32_t, uint32_t>> iv;
+    for (uint32_t q = 0; q < p->n_queries; ++q) {
+        iv.clear();
+        for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
+            const genie_dim_t& it = p->h_items[i];
+            if (it.end_pos <= it.start_pos || it.weight == 0.0f)
+                continue;
+            if (it.weight != 1.0f)
+                return false;
+            const uint64_t sp = (uint64_t)it.start_pos, ep = (uint64_t)it.end_pos;
+            const uint32_t ka = (uint32_t)(std::upper_bound(ks, ks + nk + 1, sp) - ks - 1);
+            const uint32_t kb = (uint32_t)(std::lower_bound(ks, ks + nk + 1, ep) - ks);
+            if (ks[ka] != sp || kb > nk || ks[kb] != ep)
+                return false;
+            iv.push_back({ka, kb});
+        }
+        std::sort(iv.begin(), iv.end());
+        for (size_t i = 1; i < iv.size(); ++i)
+            if (iv[i].first < iv[i - 1].second)
+                return false;
+    }
+    return true;
+}
+
+// Device buffers of the dense form for the uploaded batch: the query mask
+// tables, the sample histogram and the sample tile list (evenly spread tiles,
+// at least 16 and at least 1/16 of all tiles).
+static int prepare_dense(genie_plan* p)
+{
+    const DevIndex& d = p->ix->d;
+    p->dn_groups = (p->n_queries + 31) / 32;
+    const uint64_t tw = (uint64_t)d.dense_dpad * kDenseKeysPerDim;
+    if (p->dn_groups * tw > p->dtab_alloc) {
+        dev_free(p->d_dtab);
+        p->dtab_alloc = 0;
+        if (int rc = dev_alloc(&p->d_dtab, p->dn_groups * tw))
+            return rc;
+        p->dtab_alloc = p->dn_groups * tw;
+    }
+    if (!p->d_dhist)
+        if (int rc = dev_alloc(&p->d_dhist, (uint64_t)p->max_queries * kDenseBins))
+            return rc;
+    const uint32_t n_tiles = (d.n_objects + kDenseTile - 1) / kDenseTile;
+    const uint32_t ns = std::min(n_tiles, std::max(16u, (n_tiles + 15) / 16));
+    std::vector<uint32_t> list(ns);
+    for (uint32_t i = 0; i < ns; ++i)
+        list[i] = (uint32_t)((uint64_t)i * n_tiles / ns);
+    if (ns > p->dlist_alloc) {
+        dev_free(p->d_dlist);
+        p->dlist_alloc = 0;
+        if (int rc = dev_alloc(&p->d_dlist, ns))
+            return rc;
+        p->dlist_alloc = ns;
+    }
+    HIP_TRY(hipMemcpy(p->d_dlist, list.data(), ns * 4ull, hipMemcpyHostToDevice));
+    p->n_dlist = ns;
+    return GENIE_OK;
+}
+
+// The lane form's work queue is built when the lane form is first needed.
+static int ensure_lanes(genie_plan* p)
+{
+    if (p->lanes_built)
+        return GENIE_OK;
+    if (int rc = build_batched(p))
+        return rc;
+    p->lanes_built = true;
+    return GENIE_OK;
+}
+
+// After a change of form or configuration: the active form's buffers and a
+// fresh candidate space.
+static int plan_settle_form(genie_plan* p)
+{
+    plan_drop_graph(p);
+    if (p->mode != 1 || p->n_queries == 0)
+        return GENIE_OK;
+    if (dense_active(p)) {
+        if (int rc = prepare_dense(p))
+            return rc;
+    } else if (int rc = ensure_lanes(p)) {
+        return rc;
+    }
+    p->cap_factor = initial_cap_factor(p);
+    return plan_size_candidates(p);
 }
 
 extern "C" {
@@ -1018,14 +1229,15 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     HIP_TRY(hipMemcpy(p->d_item_qptr, qptr.data(), qptr.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_kq, k_per_query, n_queries * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_out_off, p->h_out_off.data(), p->h_out_off.size() * 8, hipMemcpyHostToDevice));
-    p->cap_factor = 8;
     plan_drop_graph(p);
     if (p->mode == 1) {
         p->h_items.swap(sorted);
         p->h_qptr.swap(qptr);
-        if (int rc = build_batched(p))
-            return rc;
+        p->lanes_built = false;
+        p->dense_ok = dense_batch_ok(p);
+        return plan_settle_form(p);
     }
+    p->cap_factor = initial_cap_factor(p);
     return plan_size_candidates(p);
 }
 
@@ -1091,6 +1303,112 @@ static int launch_scan(genie_plan* p, hipStream_t s)
 
 static int launch_merge(genie_plan* p, hipStream_t s);
 
+// Grid of the dense kernels: every CU's share of resident workgroups, a
+// multiple of the 8 XCDs the workgroup ids are dealt to.
+template <int NBLK, bool SAMPLE>
+static int launch_dense_t(const DenseArgs& a, hipStream_t s)
+{
+    const size_t lds = (size_t)NBLK * 16 * kDenseKeysPerDim * 4 + (SAMPLE ? 32u * (NBLK * 16 + 1) * 4 : 0u);
+    auto kern = k_dense_count<NBLK, SAMPLE>;
+    HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0, n_cu = 0, dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDenseThreads, lds));
+    if (per_cu < 1)
+        return fail(GENIE_E_INVALID, "dense kernel does not fit a CU");
+    const uint32_t grid = std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8);
+    kern<<<grid, kDenseThreads, lds, s>>>(a);
+    return GENIE_OK;
+}
+
+template <bool SAMPLE>
+static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s)
+{
+    switch (nblk) {
+    case 1: return launch_dense_t<1, SAMPLE>(a, s);
+    case 2: return launch_dense_t<2, SAMPLE>(a, s);
+    case 3: return launch_dense_t<3, SAMPLE>(a, s);
+    case 4: return launch_dense_t<4, SAMPLE>(a, s);
+    case 5: return launch_dense_t<5, SAMPLE>(a, s);
+    case 6: return launch_dense_t<6, SAMPLE>(a, s);
+    case 7: return launch_dense_t<7, SAMPLE>(a, s);
+    case 8: return launch_dense_t<8, SAMPLE>(a, s);
+    }
+    return fail(GENIE_E_INVALID, "dense form supports at most 128 dims");
+}
+
+// Dense form: query masks, sample pass + thresholds, main pass, merge.
+static int run_dense(genie_plan* p, hipStream_t s)
+{
+    const DevIndex& d = p->ix->d;
+    const uint32_t nq = p->n_queries, ni = (uint32_t)p->n_items;
+    const uint32_t tw = d.dense_dpad * kDenseKeysPerDim;
+    fill_u32(p->d_dtab, 0, (uint64_t)p->dn_groups * tw, s);
+    fill_u32(p->d_dhist, 0, (uint64_t)nq * kDenseBins, s);
+    if (ni)
+        k_dense_masks<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, d.key_start, d.n_keys, d.key_slot, tw,
+                                                       p->d_dtab);
+    if (p->timing)
+        HIP_TRY(hipEventRecord(p->ev[1], s));
+    DenseArgs a;
+    a.codes = d.codes;
+    a.n_pad = d.dense_npad;
+    a.tab = p->d_dtab;
+    a.D_pad = d.dense_dpad;
+    a.G = p->dn_groups;
+    a.n_queries = nq;
+    a.n_objects = d.n_objects;
+    a.n_tiles = (d.n_objects + kDenseTile - 1) / kDenseTile;
+    a.list = p->d_dlist;
+    a.n_list = p->n_dlist;
+    a.gthr = p->d_gthr;
+    a.hist = p->d_dhist;
+    a.cand_cnt = p->d_cand_cnt;
+    a.cand_cap = p->d_cand_cap;
+    a.cand_base = p->d_cand_base;
+    a.cand = p->d_cand;
+    a.overflow = p->d_overflow;
+    const uint32_t nblk = d.dense_dpad / 16;
+    if (int rc = launch_dense<true>(nblk, a, s))
+        return rc;
+    k_dense_thresh<<<(nq + 255) / 256, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->d_gthr);
+    if (p->timing)
+        HIP_TRY(hipEventRecord(p->ev[4], s));
+    a.list = nullptr;
+    a.n_list = 0;
+    if (int rc = launch_dense<false>(nblk, a, s))
+        return rc;
+    if (p->timing)
+        HIP_TRY(hipEventRecord(p->ev[2], s));
+    return launch_merge(p, s);
+}
+
+static int plan_run_once(genie_plan* p, hipStream_t s);
+
+// Candidate overflow after a completed run on stream s: grow the capacity
+// and run again (the reference retries on hash-table overflow in the same
+// way, match.cu:356-421).
+static int settle_overflow(genie_plan* p, hipStream_t s)
+{
+    for (int attempt = 0;; ++attempt) {
+        uint32_t ov = 0;
+        HIP_TRY(hipMemcpy(&ov, p->d_overflow, 4, hipMemcpyDeviceToHost));
+        if (!ov)
+            return GENIE_OK;
+        const uint32_t fmax = max_cap_factor(p);
+        if (p->cap_factor >= fmax || attempt > 12)
+            return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
+        p->cap_factor = (uint32_t)std::min<uint64_t>((uint64_t)p->cap_factor * 4, fmax);
+        plan_drop_graph(p);
+        if (int rc = plan_size_candidates(p))
+            return rc;
+        if (int rc = plan_run_once(p, s))
+            return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+}
+
 static int plan_run_once(genie_plan* p, hipStream_t s)
 {
     const DevIndex& d = p->ix->d;
@@ -1101,6 +1419,9 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     fill_u32(p->d_gthr, 0, nq, s);
     fill_u32(p->d_cand_cnt, 0, nq, s);
     fill_u32(p->d_overflow, 0, 1, s);
+    p->timed_dense = dense_active(p);
+    if (p->timed_dense)
+        return run_dense(p, s);
     if (p->mode == 1) {
         // batched form: the work queue was built at upload
         fill_u32(p->d_queue, 0, 1, s);
@@ -1239,12 +1560,30 @@ int genie_hip_plan_set_config(genie_plan* p, uint32_t block, uint32_t tile_objs,
     p->tile_b = tile_objs;
     p->seglen = seglen;
     plan_drop_graph(p);
-    if (p->mode == 1 && p->n_queries && rebuild) {
-        if (int rc = build_batched(p))
-            return rc;
-        p->cap_factor = 8;
-        return plan_size_candidates(p);
-    }
+    if (rebuild)
+        p->lanes_built = false;
+    if (p->mode == 1 && p->n_queries && rebuild && !dense_active(p))
+        return plan_settle_form(p);
+    return GENIE_OK;
+}
+
+int genie_hip_plan_set_dense(genie_plan* p, int on)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (p->mode != 1)
+        return fail(GENIE_E_INVALID, "the dense form belongs to batched plans");
+    if (p->dense_on == (on != 0))
+        return GENIE_OK;
+    p->dense_on = on != 0;
+    return plan_settle_form(p);
+}
+
+int genie_hip_plan_dense_active(genie_plan* p, int* active)
+{
+    if (!p || !active)
+        return fail(GENIE_E_INVALID, "null argument");
+    *active = dense_active(p) ? 1 : 0;
     return GENIE_OK;
 }
 
@@ -1286,40 +1625,46 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
     const uint32_t blocks[] = {256, 512, 1024};
     const uint32_t tiles[] = {8192, 16384};
     const uint32_t segs[] = {16384, 65536};
-    const bool was_graph = p->graph_on;
+    const bool was_graph = p->graph_on, try_dense = p->dense_on && p->dense_ok;
     p->graph_on = false;
     double best = 1e30;
     uint32_t bb = p->block, bt = p->tile_b, bs = p->seglen, n = 0;
+    bool best_dense = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
+    // one settled run (the first may overflow the candidate space), then reps
+    auto time_it = [&](double* ms) -> int {
+        *ms = -1;
+        if (int rc = plan_run_once(p, p->own_stream))
+            return rc;
+        HIP_TRY(hipStreamSynchronize(p->own_stream));
+        if (int rc = settle_overflow(p, p->own_stream))
+            return rc;
+        HIP_TRY(hipEventRecord(e0, p->own_stream));
+        for (int r = 0; r < std::max(1, reps); ++r)
+            if (int rc = plan_run_once(p, p->own_stream))
+                return rc;
+        HIP_TRY(hipEventRecord(e1, p->own_stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float f = 0;
+        HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+        *ms = f / std::max(1, reps);
+        return GENIE_OK;
+    };
     int rc = GENIE_OK;
+    if (try_dense) {
+        p->dense_on = false;
+        rc = plan_settle_form(p);
+    }
     for (uint32_t blk : blocks)
         for (uint32_t tl : tiles)
             for (uint32_t sg : segs) {
+                if (rc)
+                    break;
                 double ms = -1;
-                if ((rc = genie_hip_plan_set_config(p, blk, tl, sg)) == GENIE_OK &&
-                    (rc = plan_run_once(p, p->own_stream)) == GENIE_OK &&
-                    hipStreamSynchronize(p->own_stream) == hipSuccess) {
-                    // a first run may overflow the candidate space: settle it
-                    uint32_t ov = 0;
-                    (void)hipMemcpy(&ov, p->d_overflow, 4, hipMemcpyDeviceToHost);
-                    if (ov) {
-                        p->cap_factor = plan_n_tiles(p);
-                        rc = plan_size_candidates(p);
-                    }
-                    if (rc == GENIE_OK) {
-                        (void)hipEventRecord(e0, p->own_stream);
-                        for (int r = 0; r < std::max(1, reps); ++r)
-                            if ((rc = plan_run_once(p, p->own_stream)))
-                                break;
-                        (void)hipEventRecord(e1, p->own_stream);
-                        (void)hipEventSynchronize(e1);
-                        float f = 0;
-                        (void)hipEventElapsedTime(&f, e0, e1);
-                        ms = f / std::max(1, reps);
-                    }
-                }
+                if ((rc = genie_hip_plan_set_config(p, blk, tl, sg)) == GENIE_OK)
+                    rc = time_it(&ms);
                 if (rc == GENIE_E_INVALID)  // a candidate that does not fit this device: skip it
                     rc = GENIE_OK;
                 if (rc)
@@ -1334,6 +1679,22 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
                     bs = sg;
                 }
             }
+    if (!rc && try_dense) {
+        // the dense form is the last candidate
+        double ms = -1;
+        p->dense_on = true;
+        if ((rc = plan_settle_form(p)) == GENIE_OK)
+            rc = time_it(&ms);
+        if (!rc) {
+            if (out_ms)
+                out_ms[n] = ms;
+            ++n;
+            if (ms >= 0 && ms < best) {
+                best = ms;
+                best_dense = true;
+            }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (n_out)
@@ -1341,7 +1702,12 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
     p->graph_on = was_graph;
     if (rc)
         return rc;
-    return genie_hip_plan_set_config(p, bb, bt, bs);
+    if ((rc = genie_hip_plan_set_config(p, bb, bt, bs)))
+        return rc;
+    if (!try_dense)
+        return GENIE_OK;
+    p->dense_on = !best_dense;  // force a settle below
+    return genie_hip_plan_set_dense(p, best_dense ? 1 : 0);
 }
 
 int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts)
@@ -1352,29 +1718,20 @@ int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32
         return fail(GENIE_E_NOT_BUILT, "plan has not run");
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipStreamSynchronize(s));
-    // candidate overflow: grow the capacity and run again (the reference
-    // retries on hash-table overflow in the same way, match.cu:356-421)
-    for (int attempt = 0;; ++attempt) {
-        uint32_t ov = 0;
-        HIP_TRY(hipMemcpy(&ov, p->d_overflow, 4, hipMemcpyDeviceToHost));
-        if (!ov)
-            break;
-        const uint32_t n_tiles = plan_n_tiles(p);
-        if (p->cap_factor >= n_tiles || attempt > 8)
-            return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
-        p->cap_factor = std::min<uint32_t>(p->cap_factor * 4, n_tiles);
-        if (int rc = plan_size_candidates(p))
-            return rc;
-        if (int rc = plan_run_once(p, s))
-            return rc;
-        HIP_TRY(hipStreamSynchronize(s));
-    }
+    if (int rc = settle_overflow(p, s))
+        return rc;
     if (p->timing) {
-        float a = 0, b = 0, c = 0;
+        float a = 0, b = 0, c = 0, e = 0;
         HIP_TRY(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
-        HIP_TRY(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
+        if (p->timed_dense) {
+            HIP_TRY(hipEventElapsedTime(&e, p->ev[1], p->ev[4]));
+            HIP_TRY(hipEventElapsedTime(&b, p->ev[4], p->ev[2]));
+        } else {
+            HIP_TRY(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
+        }
         HIP_TRY(hipEventElapsedTime(&c, p->ev[2], p->ev[3]));
         p->ms_compile = a;
+        p->ms_sample = e;
         p->ms_scan = b;
         p->ms_merge = c;
     }
@@ -1427,8 +1784,13 @@ int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms)
     if (p->timing && p->ran) {
         // refresh from the events of the last run
         float v = 0;
+        const hipEvent_t scan0 = p->timed_dense ? p->ev[4] : p->ev[1];
         if (!strcmp(name, "compile") && hipEventElapsedTime(&v, p->ev[0], p->ev[1]) == hipSuccess) { *ms = v; return GENIE_OK; }
-        if (!strcmp(name, "scan") && hipEventElapsedTime(&v, p->ev[1], p->ev[2]) == hipSuccess) { *ms = v; return GENIE_OK; }
+        if (!strcmp(name, "sample")) {
+            if (!p->timed_dense) { *ms = 0; return GENIE_OK; }
+            if (hipEventElapsedTime(&v, p->ev[1], p->ev[4]) == hipSuccess) { *ms = v; return GENIE_OK; }
+        }
+        if (!strcmp(name, "scan") && hipEventElapsedTime(&v, scan0, p->ev[2]) == hipSuccess) { *ms = v; return GENIE_OK; }
         if (!strcmp(name, "merge") && hipEventElapsedTime(&v, p->ev[2], p->ev[3]) == hipSuccess) { *ms = v; return GENIE_OK; }
     }
     return fail(GENIE_E_INVALID, std::string("no timing for kernel ") + name);

@@ -117,8 +117,10 @@ int genie_hip_plan_run(genie_plan* p, void* stream);
 /* Copy the results of the last run to host buffers (synchronises stream). */
 int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts);
 /* Per-stage timing of the last run when timing is on (HIP events recorded
- * on the run's stream): "compile" (item -> chunk kernels), "scan" (the
- * count + per-tile select kernel), "merge"; ms is the stage's duration. */
+ * on the run's stream): "compile" (item -> chunk kernels; query masks for
+ * the dense form), "sample" (the dense form's sample pass and thresholds,
+ * 0 otherwise), "scan" (the count + per-tile select kernel; the dense main
+ * pass), "merge"; ms is the stage's duration. */
 int genie_hip_plan_set_timing(genie_plan* p, int on);
 int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms);
 /* Replay the run as a captured hipGraph (captured at the next run, again
@@ -130,10 +132,23 @@ int genie_hip_plan_set_graph(genie_plan* p, int on);
 int genie_hip_plan_set_config(genie_plan* p, uint32_t block, uint32_t tile_objs, uint32_t seglen);
 int genie_hip_plan_get_config(genie_plan* p, uint32_t* block, uint32_t* tile_objs, uint32_t* seglen,
                               uint32_t* lane_bits);
-/* Autotuner of the batched form over block x tile x segment length on the
- * uploaded batch; keeps the fastest.  out_ms (optional, >= 12 entries)
- * receives every candidate's time per run (-1 if it could not run). */
+/* Autotuner of the batched form over block x tile x segment length of the
+ * lane form, plus the dense form as a 13th candidate when the batch allows it
+ * and it is switched on; keeps the fastest (form and configuration).  out_ms
+ * (optional, >= 13 entries) receives every candidate's time per run (-1 if it
+ * could not run); *n_out the number of candidates. */
 int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n_out);
+/* Dense form of the batched search (mode 1 only; on by default).  When the
+ * index is a single-value table of at most 128 dims with at most 256 keys per
+ * dim, and every item of the uploaded batch covers whole keys with weight 1
+ * (never one key twice for a query), the batch is counted from per-object key
+ * codes built at index creation, 32 queries per LDS lookup with bit-sliced
+ * counters, instead of from the posting lists; results are identical.
+ * No counterpart in the reference (its matching always walks the lists,
+ * match.cu:161-220).  on = 0 forces the lane form. */
+int genie_hip_plan_set_dense(genie_plan* p, int on);
+/* Whether the last uploaded batch runs the dense form. */
+int genie_hip_plan_dense_active(genie_plan* p, int* active);
 /* Host time of the last batched work-queue build (upload), ms. */
 int genie_hip_plan_host_ms(genie_plan* p, double* ms);
 /* Algorithmic bytes of the last uploaded batch: logical = 4 B x sum of

@@ -1,6 +1,7 @@
-"""Batched form (persistent kernel over a (query group, tile) work queue,
-queries packed into count lanes, merged spans, hipGraph replay, autotuner)
-against the plain form and the oracle: identical results."""
+"""Batched form (lane form: persistent kernel over a (query group, tile) work
+queue, queries packed into count lanes, merged spans; dense form: bit-sliced
+counting over per-object key codes; hipGraph replay, autotuner) against the
+plain form and the oracle: identical results."""
 import numpy as np
 import pytest
 
@@ -17,9 +18,10 @@ def plain(table, items, nq, k):
     return match_topk(table.device(), items, nq, k)
 
 
-def batched(table, items, nq, k, config=None, graph=False, runs=1):
+def batched(table, items, nq, k, config=None, graph=False, runs=1, dense=False):
     kmax = int(np.max(k))
     p = Plan(table.device(), nq, len(items), kmax, 0, mode=1)
+    p.set_dense(dense)
     if config:
         p.set_config(*config)
     p.upload(items, nq, k)
@@ -107,6 +109,7 @@ def test_autotune_picks_a_valid_config():
     items = t.compile(w["ranges"], 32)
     a = plain(t, items, 32, 100)
     p = Plan(t.device(), 32, len(items), 100, 0, mode=1)
+    p.set_dense(False)
     p.upload(items, 32, 100)
     ms = p.autotune(reps=1)
     assert len(ms) == 12 and np.any(ms > 0)

@@ -1,0 +1,182 @@
+"""Dense form of the batched search (bit-sliced counting over per-object key
+codes built from the inverted lists) against the exhaustive oracle and the
+plain form: identical results; and the batches and indexes that must take the
+lane form instead."""
+import numpy as np
+import pytest
+
+from helpers import assert_topk_equal, oracle_topk_batch
+from oracle import oracle as O
+
+import genie_amd
+from genie_amd import Plan, Table, match_topk, workloads as W
+from genie_amd._native import RANGE_DTYPE
+
+pytestmark = pytest.mark.gpu
+
+
+def run(table, items, nq, k, dense=True, graph=False, runs=1):
+    kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
+    p = Plan(table.device(), nq, max(1, len(items)), int(kq.max()), 0, mode=1)
+    p.set_dense(dense)
+    p.upload(items, nq, kq)
+    if graph:
+        p.set_graph(True)
+    out = None
+    for _ in range(runs):
+        p.run()
+        got = p.download()
+        if out is not None:
+            assert np.array_equal(got[0], out[0]) and np.array_equal(got[1], out[1])
+        out = got
+    return out, p
+
+
+def oracle_single(rows, ranges, nq, k):
+    q_of = ranges["query"]
+    return oracle_topk_batch(lambda q: O.count_exhaustive_single(rows, ranges[q_of == q]), nq, k)
+
+
+@pytest.mark.parametrize("n,d,q,k", [
+    (150_000, 32, 37, 50),    # a partly filled last query group
+    (70_001, 20, 64, 10),     # dims not a multiple of 16, objects not a multiple of the tile
+    (5_000, 128, 33, 100),    # fewer tiles than the sample: every tile is sampled
+    (300_000, 128, 96, 100),  # full width, three query groups
+    (40_000, 7, 5, 1000),     # few dims, large k
+])
+def test_dense_equals_oracle(n, d, q, k):
+    w = W.cfg1(n=n, d=d, q=q, k=k)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], q)
+    (b, p) = run(t, items, q, k)
+    assert p.dense_active()
+    ri, rc = oracle_single(w["rows"], w["ranges"], q, k)
+    assert_topk_equal(b[0], b[1], ri, rc, k, q, f"dense n={n} d={d}")
+
+
+def test_dense_equals_plain_mixed_k_and_normal_data():
+    w = W.cfg0(n=20_000, q=70)
+    t = Table.from_rows(w["rows"])
+    r = w["ranges"].copy()
+    r["low"] -= 3  # ranges of 7 values
+    r["up"] += 3
+    items = t.compile(r, 70)
+    kq = np.array([10, 100, 1000] * 23 + [3], np.uint32)
+    a = match_topk(t.device(), items, 70, kq)
+    (b, p) = run(t, items, 70, kq)
+    assert p.dense_active()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_dense_empty_queries_pad():
+    w = W.cfg1(n=30_000, d=16, q=10, k=20)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 12)  # queries 10 and 11 have no ranges
+    (b, p) = run(t, items, 12, 20)
+    assert p.dense_active()
+    ids, cnt = b[0].reshape(12, 20), b[1].reshape(12, 20)
+    assert np.all(ids[10:] == 0) and np.all(cnt[10:] == 0)
+    ri, rc = oracle_single(w["rows"], w["ranges"], 12, 20)
+    assert_topk_equal(b[0], b[1], ri, rc, 20, 12, "dense empty")
+
+
+def test_dense_candidate_regrowth():
+    """One wide range on one dim per query: most objects count 1, the sample
+    bound stays at 1, and the candidate space has to grow several times."""
+    rows = genie_amd.synth.rows(91, 200_000, 4, genie_amd.synth.UNIFORM, 0.0, 0.0, 0, 15, 16)
+    nq = 40
+    r = np.zeros(nq, RANGE_DTYPE)
+    r["query"] = np.arange(nq)
+    r["dim"] = np.arange(nq) % 4
+    r["low"] = np.arange(nq) % 3
+    r["up"] = 12
+    r["weight"] = 1
+    r["order"] = 0
+    t = Table.from_rows(rows)
+    items = t.compile(r, nq)
+    (b, p) = run(t, items, nq, 10)
+    assert p.dense_active()
+    ri, rc = oracle_single(rows, r, nq, 10)
+    assert_topk_equal(b[0], b[1], ri, rc, 10, nq, "dense regrowth")
+
+
+def test_dense_graph_replays_match_eager():
+    w = W.cfg1(n=100_000, d=16, q=40, k=32)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 40)
+    eager = match_topk(t.device(), items, 40, 32)
+    kq = np.full(40, 32, np.uint32)
+    p = Plan(t.device(), 40, len(items), 32, 0, mode=1)
+    p.upload(items, 40, kq)
+    assert p.dense_active()
+    p.set_graph(True)
+    for _ in range(4):  # replays with eager work and synchronisations between them
+        p.run()
+        got = p.download()
+        assert np.array_equal(eager[0], got[0]) and np.array_equal(eager[1], got[1])
+        match_topk(t.device(), items[:50], 40, 5)
+
+
+def test_dense_autotune_includes_dense():
+    w = W.cfg1(n=200_000, d=32, q=32, k=100)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 32)
+    a = match_topk(t.device(), items, 32, 100)
+    p = Plan(t.device(), 32, len(items), 100, 0, mode=1)
+    p.upload(items, 32, np.full(32, 100, np.uint32))
+    ms = p.autotune(reps=1)
+    assert len(ms) == 13 and ms[12] > 0
+    p.run()
+    b = p.download()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_dense_not_used_where_it_cannot_be_exact():
+    w = W.cfg1(n=60_000, d=24, q=10, k=20)
+    t = Table.from_rows(w["rows"])
+    # weights other than 1
+    r = w["ranges"].copy()
+    r["weight"][::3] = 2
+    (b, p) = run(t, t.compile(r, 10), 10, 20)
+    assert not p.dense_active()
+    ri, rc = oracle_single(w["rows"], r, 10, 20)
+    assert_topk_equal(b[0], b[1], ri, rc, 20, 10, "weights")
+    # load-balanced items split keys
+    (b, p) = run(t, t.compile(w["ranges"], 10, load_balance=True, max_load=64), 10, 20)
+    assert not p.dense_active()
+    ri, rc = oracle_single(w["rows"], w["ranges"], 10, 20)
+    assert_topk_equal(b[0], b[1], ri, rc, 20, 10, "load balance")
+    # a query covering keys twice (ranges repeated on the same dims)
+    r2 = np.concatenate([w["ranges"], w["ranges"][:3]])
+    r2 = r2[np.argsort(r2["query"], kind="stable")]
+    (b, p) = run(t, t.compile(r2, 10), 10, 20)
+    assert not p.dense_active()
+    ri, rc = oracle_single(w["rows"], r2, 10, 20)
+    assert_topk_equal(b[0], b[1], ri, rc, 20, 10, "overlap")
+    # dense switched off
+    (b, p) = run(t, t.compile(w["ranges"], 10), 10, 20, dense=False)
+    assert not p.dense_active()
+
+
+def test_dense_not_built_for_other_indexes():
+    # more than 128 dims
+    w = W.cfg1(n=20_000, d=130, q=4, k=10)
+    t = Table.from_rows(w["rows"])
+    (b, p) = run(t, t.compile(w["ranges"], 4), 4, 10)
+    assert not p.dense_active()
+    ri, rc = oracle_single(w["rows"], w["ranges"], 4, 10)
+    assert_topk_equal(b[0], b[1], ri, rc, 10, 4, "130 dims")
+    # set-valued (bijectMap) table
+    rp, vals = genie_amd.synth.sets(61, 20_000, 5, 40, 3000, 1.05)
+    qp, qv = genie_amd.synth.sets(62, 6, 1, 30, 3000, 1.05)
+    sr = W.set_ranges(qp, qv, np.ones(len(qv), np.float32))
+    ts = Table.from_sets(rp, vals)
+    (b, p) = run(ts, ts.compile(sr, 6), 6, 10)
+    assert not p.dense_active()
+    # compressed table
+    wc = W.cfg1(n=30_000, d=16, q=6, k=10)
+    tc = Table.from_rows(wc["rows"], 1024, True, codec=11)
+    (b, p) = run(tc, tc.compile(wc["ranges"], 6), 6, 10)
+    assert not p.dense_active()
+    ri, rc = oracle_single(wc["rows"], wc["ranges"], 6, 10)
+    assert_topk_equal(b[0], b[1], ri, rc, 10, 6, "compressed")
