@@ -52,7 +52,7 @@ struct DevIndex {
     bool dense = false;
     uint32_t dense_dpad = 0;        // dims rounded up to 16
     uint32_t dense_npad = 0;        // objects rounded up to 64 (row stride of codes)
-    uint32_t* codes = nullptr;      // [dense_dpad / 4][dense_npad] four 8-bit codes per word
+    uint32_t* codes = nullptr;      // [dense_words(dpad)][dense_npad] packed 10-bit fields (code * 4)
     uint32_t* key_slot = nullptr;   // [n_keys] dim * 256 + code
     uint64_t bytes = 0;
     // host copies used for validation and count bounds

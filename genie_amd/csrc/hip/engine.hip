@@ -655,37 +655,65 @@ static int build_dense(const genie_index_desc* desc, DevIndex& d)
     for (uint32_t dm = 0; dm < D; ++dm)
         if (dim_post[dm] != N)
             return GENIE_OK;
+    // slot of a key within its dim: rank by list length, longest first, so
+    // the 32 most frequent keys of a dim sit in 32 different LDS banks
     std::vector<uint32_t> slot(K);
-    for (uint32_t k = 0; k < K; ++k)
-        slot[k] = desc->key_dim[k] * kDenseKeysPerDim + (k - key_base[desc->key_dim[k]]);
+    for (uint32_t dm = 0; dm < D; ++dm) {
+        std::vector<uint32_t> ks(per_dim[dm]);
+        std::iota(ks.begin(), ks.end(), key_base[dm]);
+        std::stable_sort(ks.begin(), ks.end(), [&](uint32_t a, uint32_t b) {
+            return desc->key_start[a + 1] - desc->key_start[a] > desc->key_start[b + 1] - desc->key_start[b];
+        });
+        for (uint32_t r = 0; r < ks.size(); ++r)
+            slot[ks[r]] = dm * kDenseKeysPerDim + r;
+    }
     const uint32_t dpad = (D + 15) / 16 * 16, npad = (N + 63) / 64 * 64;
+    const uint32_t nw = (uint32_t)dense_words((int)dpad);
+    if ((uint64_t)nw * npad * 4 >= 0x80000000ull)
+        return GENIE_OK;  // 32-bit buffer offsets
+    std::vector<uint32_t> dim_word(dpad, 0), dim_shift(dpad, 0);
+    for (uint32_t w = 0; w < nw; ++w)
+        for (int f = 0; f < 3; ++f) {
+            const int dd = dense_field_dim((int)dpad, (int)w, f);
+            if (dd >= 0) {
+                dim_word[dd] = w;
+                dim_shift[dd] = (uint32_t)dense_field_shift(f);
+            }
+        }
     const uint32_t bm_words = (D + 31) / 32;
     uint32_t *codes = nullptr, *key_slot = nullptr, *bm = nullptr, *flag = nullptr, *dim = nullptr;
+    uint32_t *d_word = nullptr, *d_shift = nullptr;
     uint64_t bytes = 0;
     int rc = GENIE_OK;
     uint32_t h_flag = 0;
     do {
-        if ((rc = dev_alloc(&codes, (uint64_t)dpad / 4 * npad, &bytes))) break;
+        if ((rc = dev_alloc(&codes, (uint64_t)nw * npad, &bytes))) break;
         if ((rc = dev_alloc(&key_slot, K, &bytes))) break;
         if ((rc = dev_alloc(&bm, (uint64_t)N * bm_words))) break;
         if ((rc = dev_alloc(&flag, 1))) break;
         if ((rc = dev_alloc(&dim, K))) break;
+        if ((rc = dev_alloc(&d_word, dpad))) break;
+        if ((rc = dev_alloc(&d_shift, dpad))) break;
         hipError_t e;
-        if ((e = hipMemset(codes, 0, (uint64_t)dpad * npad)) || (e = hipMemset(bm, 0, (uint64_t)N * bm_words * 4)) ||
+        if ((e = hipMemset(codes, 0, (uint64_t)nw * npad * 4)) || (e = hipMemset(bm, 0, (uint64_t)N * bm_words * 4)) ||
             (e = hipMemset(flag, 0, 4)) ||
             (e = hipMemcpy(key_slot, slot.data(), K * 4ull, hipMemcpyHostToDevice)) ||
-            (e = hipMemcpy(dim, desc->key_dim, K * 4ull, hipMemcpyHostToDevice))) {
+            (e = hipMemcpy(dim, desc->key_dim, K * 4ull, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_word, dim_word.data(), dpad * 4ull, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_shift, dim_shift.data(), dpad * 4ull, hipMemcpyHostToDevice))) {
             rc = fail(GENIE_E_RUNTIME, std::string("dense codes: ") + hipGetErrorString(e));
             break;
         }
-        k_dense_codes<<<K, 256>>>(d.post, d.key_start, dim, key_slot, npad, bm_words,
-                                  reinterpret_cast<uint8_t*>(codes), bm, flag);
+        k_dense_codes<<<K, 256>>>(d.post, d.key_start, dim, key_slot, d_word, d_shift, npad, bm_words, codes, bm,
+                                  flag);
         k_dense_holes<<<(N + 255) / 256, 256>>>(bm, N, D, bm_words, flag);
         if ((e = hipGetLastError()) || (e = hipMemcpy(&h_flag, flag, 4, hipMemcpyDeviceToHost))) {
             rc = fail(GENIE_E_RUNTIME, std::string("dense codes: ") + hipGetErrorString(e));
             break;
         }
     } while (0);
+    dev_free(d_word);
+    dev_free(d_shift);
     dev_free(bm);
     dev_free(flag);
     dev_free(dim);
@@ -1308,17 +1336,15 @@ static int launch_merge(genie_plan* p, hipStream_t s);
 template <int NBLK, bool SAMPLE>
 static int launch_dense_t(const DenseArgs& a, hipStream_t s)
 {
-    const size_t lds = (size_t)NBLK * 16 * kDenseKeysPerDim * 4 + (SAMPLE ? 32u * (NBLK * 16 + 1) * 4 : 0u);
-    auto kern = k_dense_count<NBLK, SAMPLE>;
-    HIP_TRY(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    auto kern = k_dense_count<NBLK, SAMPLE>;  // static LDS (dense_lds_bytes)
     int per_cu = 0, n_cu = 0, dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDenseThreads, lds));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDenseThreads, 0));
     if (per_cu < 1)
         return fail(GENIE_E_INVALID, "dense kernel does not fit a CU");
     const uint32_t grid = std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8);
-    kern<<<grid, kDenseThreads, lds, s>>>(a);
+    kern<<<grid, kDenseThreads, 0, s>>>(a);
     return GENIE_OK;
 }
 
@@ -1372,7 +1398,7 @@ static int run_dense(genie_plan* p, hipStream_t s)
     const uint32_t nblk = d.dense_dpad / 16;
     if (int rc = launch_dense<true>(nblk, a, s))
         return rc;
-    k_dense_thresh<<<(nq + 255) / 256, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->d_gthr);
+    k_dense_thresh<<<(nq + 3) / 4, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->d_gthr);
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[4], s));
     a.list = nullptr;

@@ -4,7 +4,7 @@
 // For a single-value table every object holds exactly one key per dim, so
 // the inverted lists of one dim are a permutation of all objects.  At index
 // creation the lists are turned around into codes[d][o] = key of object o in
-// dim d, relative to the dim's first key (one byte, four dims per word).  A
+// dim d, as its slot within the dim (packed three per word, see below).  A
 // batch whose items cover whole keys with unit weight, and never the same key
 // twice for one query, then has the exact per-(query, key) weights 0/1.  For
 // 32 queries at a time those weights are a 32-bit mask per key (`tab`,
@@ -34,8 +34,53 @@ constexpr uint32_t kDenseMaxDims = 128;   // 128 dims x 256 keys x 4 B = 128 KiB
 constexpr uint32_t kDenseKeysPerDim = 256;
 constexpr uint32_t kDenseBins = 256;      // histogram bins per query (counts <= 128)
 
+// Packed codes: three 10-bit fields per word, at bits 0, 11 and 22, each
+// holding code * 4 -- the byte offset of the key's mask within its dim's
+// 1 KiB table row -- so every LDS address takes one VALU op.  Rows of dims
+// 0..63 start below 64 KiB and are reached through the 16-bit offset of
+// ds_read; rows of dims 64..127 lie 64 KiB higher, which an op can add only
+// for the outer fields ((x & 0x3FC) | 0x10000 and alignbit(0x40, x, 22)), so
+// those dims occupy outer fields only and the middle fields hold low dims.
+__host__ __device__ constexpr int dense_words(int dpad)
+{
+    const int L = dpad < 64 ? dpad : 64, U = dpad - L;
+    return U / 2 + (L - U / 2 + 2) / 3;
+}
+
+// Dim stored in field f (0, 1, 2) of word w, or -1 for an unused field.
+__host__ __device__ constexpr int dense_field_dim(int dpad, int w, int f)
+{
+    const int L = dpad < 64 ? dpad : 64, U = dpad - L;
+    if (w < U / 2)
+        return f == 0 ? L + 2 * w : (f == 2 ? L + 2 * w + 1 : w);
+    const int i = U / 2 + 3 * (w - U / 2) + f;
+    return i < L ? i : -1;
+}
+
+__host__ __device__ constexpr int dense_field_shift(int f) { return f == 0 ? 0 : (f == 1 ? 11 : 22); }
+
+// The fields of a layout in load order, for the unrolled accumulation.
+template <int DPAD>
+struct DenseLayout {
+    int w[DPAD], f[DPAD], d[DPAD];
+    constexpr DenseLayout() : w(), f(), d()
+    {
+        int n = 0;
+        for (int i = 0; i < dense_words(DPAD); ++i)
+            for (int ff = 0; ff < 3; ++ff) {
+                const int dd = dense_field_dim(DPAD, i, ff);
+                if (dd >= 0) {
+                    w[n] = i;
+                    f[n] = ff;
+                    d[n] = dd;
+                    ++n;
+                }
+            }
+    }
+};
+
 struct DenseArgs {
-    const uint32_t* codes;  // [D_pad / 4][n_pad] words, byte d % 4 = code of dim d
+    const uint32_t* codes;  // [dense_words(D_pad)][n_pad] packed fields
     uint32_t n_pad;
     const uint32_t* tab;    // [G][D_pad * 256] masks, bit j = query 32 g + j
     uint32_t D_pad, G, n_queries, n_objects, n_tiles;
@@ -52,21 +97,24 @@ struct DenseArgs {
 
 // ---- index creation --------------------------------------------------------
 
-// One workgroup per key: write the key's code into the byte of every object
-// in its list, and mark (object, dim) in a bitmap; a bit found already set
-// means an object listed twice in one dim (flag |= 1).
+// One workgroup per key: or the key's field (code * 4 at the dim's word and
+// shift) into the packed codes of every object in its list, and mark
+// (object, dim) in a bitmap; a bit found already set means an object listed
+// twice in one dim (flag |= 1).
 __global__ void k_dense_codes(const uint32_t* __restrict__ post, const uint32_t* __restrict__ key_start,
                               const uint32_t* __restrict__ key_dim, const uint32_t* __restrict__ key_slot,
-                              uint32_t n_pad, uint32_t bm_words, uint8_t* __restrict__ codes8,
+                              const uint32_t* __restrict__ dim_word, const uint32_t* __restrict__ dim_shift,
+                              uint32_t n_pad, uint32_t bm_words, uint32_t* __restrict__ codes,
                               uint32_t* __restrict__ bm, uint32_t* __restrict__ flag)
 {
     const uint32_t k = blockIdx.x;
     const uint32_t d = key_dim[k];
-    const uint8_t code = (uint8_t)(key_slot[k] & 255u);
+    const uint32_t field = (key_slot[k] & 255u) * 4u << dim_shift[d];
+    uint32_t* row = codes + (uint64_t)dim_word[d] * n_pad;
     const uint32_t s = key_start[k], e = key_start[k + 1];
     for (uint32_t p = s + threadIdx.x; p < e; p += blockDim.x) {
         const uint32_t o = post[p];
-        codes8[((uint64_t)(d >> 2) * n_pad + o) * 4 + (d & 3)] = code;
+        atomicOr(&row[o], field);
         const uint32_t bit = 1u << (d & 31);
         const uint32_t old = atomicOr(&bm[(uint64_t)o * bm_words + (d >> 5)], bit);
         if (old & bit)
@@ -129,24 +177,33 @@ __global__ void k_dense_masks(const genie_dim_t* __restrict__ items, uint32_t n_
 
 // Threshold of every query from its sample histogram: the largest count c
 // with at least k sample objects counting >= c, and 1 when fewer than k
-// sample objects count at all.
-__global__ void k_dense_thresh(const uint32_t* __restrict__ hist, const uint32_t* __restrict__ kq,
-                               uint32_t n_queries, uint32_t* __restrict__ gthr)
+// sample objects count at all.  One wave per query, four bins per lane.
+__global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ kq, uint32_t n_queries,
+                                                      uint32_t* __restrict__ gthr)
 {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    static_assert(kDenseBins == 4 * kWave, "four bins per lane");
+    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= n_queries)
-        return;
+        return;  // whole waves leave together
+    const uint32_t lane = lane_id();
+    const uint4 h = reinterpret_cast<const uint4*>(hist + (uint64_t)q * kDenseBins)[lane];
+    const uint32_t mine = h.x + h.y + h.z + h.w;
+    // objects counting above this lane's bins: sum over higher lanes
+    const uint32_t incl = wave_incl_scan(mine);
+    const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
+    const uint32_t above = total - incl;
     const uint32_t k = kq[q];
-    const uint32_t* h = hist + (uint64_t)q * kDenseBins;
-    uint32_t cum = 0, thr = 1;
-    for (uint32_t c = kDenseBins - 1; c >= 1; --c) {
-        cum += h[c];
-        if (cum >= k) {
-            thr = c;
-            break;
-        }
-    }
-    gthr[q] = thr;
+    const uint32_t s3 = above + h.w, s2 = s3 + h.z, s1 = s2 + h.y, s0 = s1 + h.x;
+    const uint32_t b = 4 * lane;
+    uint32_t best = 0;
+    if (s3 >= k) best = b + 3;
+    else if (s2 >= k) best = b + 2;
+    else if (s1 >= k) best = b + 1;
+    else if (s0 >= k) best = b;
+    best = wave_max(best);
+    if (lane == 0)
+        gthr[q] = max(best, 1u);
 }
 
 __device__ __forceinline__ void csa(uint32_t& hi, uint32_t& lo, uint32_t a, uint32_t b, uint32_t c)
@@ -155,26 +212,56 @@ __device__ __forceinline__ void csa(uint32_t& hi, uint32_t& lo, uint32_t a, uint
     lo = __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);  // a ^ b ^ c
 }
 
-// Bit-sliced counts of object o over NBLK blocks of 16 dims: c[b] bit j =
-// bit b of query j's count.  Harley-Seal: ones/twos/fours/eights absorb 16
-// inputs per step, the weight-16 carry ripples into the upper planes.
-template <int NBLK>
-__device__ __forceinline__ void dense_accumulate(const uint32_t* __restrict__ codes, uint32_t n_pad, uint32_t o,
-                                                 const uint32_t* s_tab, uint32_t (&c)[8])
+// (v & mask) | bits in one VALU op: the mask from a VGPR and the bits from an
+// SGPR, since a gfx9 VOP3 encoding holds no literal.  Pure register op.
+__device__ __forceinline__ uint32_t and_or_vs(uint32_t v, uint32_t mask_v, uint32_t bits_s)
 {
-    uint32_t w[4 * NBLK];
+    uint32_t r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(v), "v"(mask_v), "s"(bits_s));
+    return r;
+}
+
+// Buffer descriptor of the code matrix: raw 32-bit loads, row offsets in
+// scalar registers, the lane's object offset in one vector register.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t codes_rsrc(const uint32_t* codes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(codes), (short)0, 0x7FFFFFFF, 0x00020000);
+}
+
+// Bit-sliced counts of object o: c[b] bit j = bit b of query j's count.
+// Harley-Seal: ones/twos/fours/eights absorb 16 inputs per step, the
+// weight-16 carry ripples into the upper planes.  s_lds holds the group's
+// table at LDS address 0 (dim d's row at d KiB).
+template <int DPAD>
+__device__ __forceinline__ void dense_accumulate(__amdgpu_buffer_rsrc_t rs, uint32_t row_bytes, uint32_t o,
+                                                 const uint32_t* s_lds, uint32_t (&c)[8])
+{
+    constexpr int NW = dense_words(DPAD);
+    constexpr DenseLayout<DPAD> L{};
+    const char* tab = reinterpret_cast<const char*>(s_lds);
+    uint32_t x[NW];
 #pragma unroll
-    for (int i = 0; i < 4 * NBLK; ++i)
-        w[i] = __builtin_nontemporal_load(codes + (uint64_t)i * n_pad + o);
+    for (int i = 0; i < NW; ++i)
+        x[i] = __builtin_amdgcn_raw_buffer_load_b32(rs, o * 4u, (uint32_t)i * row_bytes, 0);
     uint32_t ones = 0, twos = 0, fours = 0, eights = 0, h16 = 0, h32 = 0, h64 = 0, h128 = 0;
 #pragma unroll
-    for (int blk = 0; blk < NBLK; ++blk) {
+    for (int blk = 0; blk < DPAD / 16; ++blk) {
         uint32_t m[16];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                m[4 * i + j] = s_tab[(blk * 16 + 4 * i + j) * kDenseKeysPerDim + ((w[4 * blk + i] >> (8 * j)) & 255u)];
+        for (int i = 0; i < 16; ++i) {
+            const int e = blk * 16 + i;
+            const uint32_t v = x[L.w[e]];
+            const bool up = L.d[e] >= 64;
+            uint32_t a;
+            if (L.f[e] == 0)
+                a = up ? and_or_vs(v, 0x3FCu, 0x10000u) : (v & 0x3FCu);
+            else if (L.f[e] == 1)
+                a = (v >> 11) & 0x3FFu;
+            else
+                a = up ? __builtin_amdgcn_alignbit(0x40u, v, 22u) : (v >> 22);
+            const uint32_t off = (uint32_t)(up ? L.d[e] - 64 : L.d[e]) * 1024u;
+            m[i] = *reinterpret_cast<const uint32_t*>(tab + a + off);
+        }
         uint32_t tA, tB, fA, fB, eA, eB, sx;
         csa(tA, ones, ones, m[0], m[1]);
         csa(tB, ones, ones, m[2], m[3]);
@@ -211,6 +298,28 @@ __device__ __forceinline__ void dense_accumulate(const uint32_t* __restrict__ co
     c[7] = h128;
 }
 
+// All 32 counts of the lane at once: byte column q of the eight planes forms
+// an 8 x 8 bit matrix (row b = plane b); its transpose (three delta swaps)
+// holds in byte j the count of query 8 q + j.  out[q] = counts of queries
+// 8 q .. 8 q + 7, one per byte.
+__device__ __forceinline__ void dense_all_counts(const uint32_t (&c)[8], uint64_t (&out)[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint64_t x = 0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+            x |= (uint64_t)((c[b] >> (8 * q)) & 255u) << (8 * b);
+        uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+        x = x ^ t ^ (t << 7);
+        t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+        x = x ^ t ^ (t << 14);
+        t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+        x = x ^ t ^ (t << 28);
+        out[q] = x;
+    }
+}
+
 __device__ __forceinline__ uint32_t dense_lane_count(const uint32_t (&c)[8], uint32_t j)
 {
     uint32_t v = 0;
@@ -220,16 +329,25 @@ __device__ __forceinline__ uint32_t dense_lane_count(const uint32_t (&c)[8], uin
     return v;
 }
 
+// LDS bytes of the dense kernels.
+__host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample)
+{
+    return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 8 + (sample ? 32 * (nblk * 16 + 1) : 0)) * 4u;
+}
+
 // Workgroup -> (query group, tiles) so that the groups of one tile run on one
 // XCD (workgroup ids are dealt to the 8 XCDs round robin) and each
 // workgroup loads one group's mask table once.
 template <int NBLK, bool SAMPLE>
 __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
 {
-    extern __shared__ uint32_t s_lds[];
-    uint32_t* s_tab = s_lds;                          // D_pad * 256
-    uint32_t* s_hist = s_lds + NBLK * 16 * kDenseKeysPerDim;  // SAMPLE: 32 x (D_pad + 1)
-    __shared__ uint32_t s_thr[8];
+    // the kernel's only LDS, static so that the table sits at address 0 and
+    // every table offset folds into the ds_read instruction
+    constexpr int DPAD = NBLK * 16;
+    __shared__ uint32_t s_lds[dense_lds_bytes(NBLK, SAMPLE) / 4];
+    uint32_t* s_tab = s_lds;                             // DPAD * 256 masks
+    uint32_t* s_thr = s_lds + DPAD * kDenseKeysPerDim;   // 8 threshold planes
+    uint32_t* s_hist = s_thr + 8;                        // SAMPLE: 32 x (DPAD + 1)
     constexpr uint32_t HB = NBLK * 16 + 1;
     const uint32_t tid = threadIdx.x;
     const uint32_t P = gridDim.x / 8, x = blockIdx.x % 8, s = blockIdx.x / 8;
@@ -250,6 +368,8 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     }
     const uint32_t n_list = a.list ? a.n_list : a.n_tiles;
     const uint32_t tab_words = NBLK * 16 * kDenseKeysPerDim;
+    const __amdgpu_buffer_rsrc_t rs = codes_rsrc(a.codes);
+    const uint32_t row_bytes = a.n_pad * 4u;
     for (uint32_t g = g_first; g < a.G; g += g_step) {
         __syncthreads();
         {
@@ -291,13 +411,18 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             const uint32_t o1 = min(o0 + kDenseTile, a.n_objects);
             for (uint32_t o = o0 + tid; o < o1; o += kDenseThreads) {
                 uint32_t c[8];
-                dense_accumulate<NBLK>(a.codes, a.n_pad, o, s_tab, c);
+                dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
                 if constexpr (SAMPLE) {
-                    for (uint32_t j = 0; j < 32; ++j) {
-                        const uint32_t v = dense_lane_count(c, j);
-                        if (v)
-                            atomicAdd(&s_hist[j * HB + v], 1u);
-                    }
+                    uint64_t cnt[4];
+                    dense_all_counts(c, cnt);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const uint32_t v = (uint32_t)(cnt[q] >> (8 * j)) & 255u;
+                            if (v)
+                                atomicAdd(&s_hist[(8 * q + j) * HB + v], 1u);
+                        }
                 } else {
                     // count >= threshold, per lane, most significant plane first
                     uint32_t gt = 0, eq = 0xFFFFFFFFu;
