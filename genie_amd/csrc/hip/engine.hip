@@ -150,7 +150,9 @@ struct genie_plan {
     uint64_t dtab_alloc = 0;
     uint32_t* d_dhist = nullptr;     // [max_queries][kDenseBins]
     uint32_t* d_dlist = nullptr;     // sample tiles
-    uint32_t n_dlist = 0, dlist_alloc = 0;
+    uint32_t n_dlist = 0, dlist_alloc = 0, n_dsample = 0;
+    uint32_t* d_gsafe = nullptr;     // [max_queries] safe threshold
+    uint32_t* d_redo = nullptr;      // [groups] second count of a group
     // graph replay
     bool graph_on = false, graph_valid = false;
     hipStream_t own_stream = nullptr;
@@ -215,6 +217,8 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_dtab);
     dev_free(p->d_dhist);
     dev_free(p->d_dlist);
+    dev_free(p->d_gsafe);
+    dev_free(p->d_redo);
 }
 
 // Count bound of every query: per (query, order) group the largest weight
@@ -1104,11 +1108,21 @@ static int prepare_dense(genie_plan* p)
     if (!p->d_dhist)
         if (int rc = dev_alloc(&p->d_dhist, (uint64_t)p->max_queries * kDenseBins))
             return rc;
+    if (!p->d_gsafe)
+        if (int rc = dev_alloc(&p->d_gsafe, p->max_queries))
+            return rc;
+    if (!p->d_redo)
+        if (int rc = dev_alloc(&p->d_redo, (p->max_queries + 31) / 32))
+            return rc;
     const uint32_t n_tiles = (d.n_objects + kDenseTile - 1) / kDenseTile;
-    const uint32_t ns = std::min(n_tiles, std::max(16u, (n_tiles + 15) / 16));
+    const uint32_t ns = std::min(n_tiles, std::max(8u, (n_tiles + 31) / 32));
     std::vector<uint32_t> list(ns);
-    for (uint32_t i = 0; i < ns; ++i)
+    uint32_t n_sample = 0;
+    for (uint32_t i = 0; i < ns; ++i) {
         list[i] = (uint32_t)((uint64_t)i * n_tiles / ns);
+        n_sample += std::min(kDenseTile, d.n_objects - list[i] * kDenseTile);
+    }
+    p->n_dsample = n_sample;
     if (ns > p->dlist_alloc) {
         dev_free(p->d_dlist);
         p->dlist_alloc = 0;
@@ -1389,6 +1403,7 @@ static int run_dense(genie_plan* p, hipStream_t s)
     a.list = p->d_dlist;
     a.n_list = p->n_dlist;
     a.gthr = p->d_gthr;
+    a.redo = nullptr;
     a.hist = p->d_dhist;
     a.cand_cnt = p->d_cand_cnt;
     a.cand_cap = p->d_cand_cap;
@@ -1398,11 +1413,17 @@ static int run_dense(genie_plan* p, hipStream_t s)
     const uint32_t nblk = d.dense_dpad / 16;
     if (int rc = launch_dense<true>(nblk, a, s))
         return rc;
-    k_dense_thresh<<<(nq + 3) / 4, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->d_gthr);
+    k_dense_thresh<<<(nq + 3) / 4, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->n_dsample, d.n_objects, p->d_gthr,
+                                                p->d_gsafe);
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[4], s));
     a.list = nullptr;
     a.n_list = 0;
+    if (int rc = launch_dense<false>(nblk, a, s))
+        return rc;
+    // queries the speculative threshold left short: their groups again
+    k_dense_check<<<p->dn_groups, 64, 0, s>>>(p->d_kq, nq, p->d_gthr, p->d_gsafe, p->d_cand_cnt, p->d_redo);
+    a.redo = p->d_redo;
     if (int rc = launch_dense<false>(nblk, a, s))
         return rc;
     if (p->timing)

@@ -87,6 +87,7 @@ struct DenseArgs {
     const uint32_t* list;   // tiles to visit (sample pass), null = every tile
     uint32_t n_list;
     uint32_t* gthr;         // [n_queries] emission threshold (>= 1)
+    const uint32_t* redo;   // [G] groups to count again (null = every group)
     uint32_t* hist;         // [n_queries][kDenseBins] sample histogram
     uint32_t* cand_cnt;
     const uint32_t* cand_cap;
@@ -175,12 +176,20 @@ __global__ void k_dense_masks(const genie_dim_t* __restrict__ items, uint32_t n_
         atomicOr(&t[key_slot[k]], bit);
 }
 
-// Threshold of every query from its sample histogram: the largest count c
-// with at least k sample objects counting >= c, and 1 when fewer than k
-// sample objects count at all.  One wave per query, four bins per lane.
+// Thresholds of every query from its sample histogram (S(c) = sample
+// objects counting >= c):
+//   safe: the largest c with S(c) >= k -- the k-th largest sample count, a
+//         lower bound of the final k-th count, or 1;
+//   spec: the largest c with S(c) >= j, j = min(k, 4 k n_sample / N + 4):
+//         where the sample holds 1/16 of the objects, some 4 k objects of the
+//         whole index are expected to reach it.
+// The main pass emits from spec; a query left with fewer than k candidates
+// is counted again from safe (k_dense_check).  One wave per query, four bins
+// per lane.
 __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ kq, uint32_t n_queries,
-                                                      uint32_t* __restrict__ gthr)
+                                                      uint32_t n_sample, uint32_t n_objects,
+                                                      uint32_t* __restrict__ gthr, uint32_t* __restrict__ gsafe)
 {
     static_assert(kDenseBins == 4 * kWave, "four bins per lane");
     const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -194,16 +203,44 @@ __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict
     const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
     const uint32_t above = total - incl;
     const uint32_t k = kq[q];
+    const uint64_t jj = (4ull * k * n_sample + n_objects - 1) / n_objects + 4;
+    const uint32_t j = (uint32_t)min<uint64_t>(k, jj);
     const uint32_t s3 = above + h.w, s2 = s3 + h.z, s1 = s2 + h.y, s0 = s1 + h.x;
     const uint32_t b = 4 * lane;
-    uint32_t best = 0;
-    if (s3 >= k) best = b + 3;
-    else if (s2 >= k) best = b + 2;
-    else if (s1 >= k) best = b + 1;
-    else if (s0 >= k) best = b;
-    best = wave_max(best);
+    auto largest = [&](uint32_t need) {
+        return s3 >= need ? b + 3 : (s2 >= need ? b + 2 : (s1 >= need ? b + 1 : (s0 >= need ? b : 0u)));
+    };
+    const uint32_t safe = max(wave_max(largest(k)), 1u);
+    const uint32_t spec = max(wave_max(largest(j)), safe);
+    if (lane == 0) {
+        gthr[q] = spec;
+        gsafe[q] = safe;
+    }
+}
+
+// After the main pass: a query whose speculative threshold left it fewer than
+// k candidates may miss objects counting between safe and spec.  Its group is
+// marked for a second count; the query restarts its list from the safe
+// threshold, the other queries of the group keep theirs (threshold 255: no
+// count reaches it).  One wave per group.
+__global__ void __launch_bounds__(64) k_dense_check(const uint32_t* __restrict__ kq, uint32_t n_queries,
+                                                    uint32_t* __restrict__ gthr, const uint32_t* __restrict__ gsafe,
+                                                    uint32_t* __restrict__ cand_cnt, uint32_t* __restrict__ redo)
+{
+    const uint32_t g = blockIdx.x, lane = threadIdx.x, q = g * 32 + lane;
+    const bool live = lane < 32 && q < n_queries;
+    const bool flagged = live && gthr[q] > gsafe[q] && cand_cnt[q] < kq[q];
+    const bool any = ballot(flagged) != 0;
     if (lane == 0)
-        gthr[q] = max(best, 1u);
+        redo[g] = any ? 1u : 0u;
+    if (any && live) {
+        if (flagged) {
+            cand_cnt[q] = 0;
+            gthr[q] = gsafe[q];
+        } else {
+            gthr[q] = 255u;
+        }
+    }
 }
 
 __device__ __forceinline__ void csa(uint32_t& hi, uint32_t& lo, uint32_t a, uint32_t b, uint32_t c)
@@ -371,6 +408,8 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     const __amdgpu_buffer_rsrc_t rs = codes_rsrc(a.codes);
     const uint32_t row_bytes = a.n_pad * 4u;
     for (uint32_t g = g_first; g < a.G; g += g_step) {
+        if (a.redo && !a.redo[g])
+            continue;  // uniform across the workgroup
         __syncthreads();
         {
             const uint4* src = reinterpret_cast<const uint4*>(a.tab + (uint64_t)g * tab_words);

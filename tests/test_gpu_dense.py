@@ -180,3 +180,31 @@ def test_dense_not_built_for_other_indexes():
     assert not p.dense_active()
     ri, rc = oracle_single(wc["rows"], wc["ranges"], 6, 10)
     assert_topk_equal(b[0], b[1], ri, rc, 10, 6, "compressed")
+
+
+def test_dense_speculative_threshold_redo():
+    """The sample sees 80 objects at the top count, the whole index only 90
+    (k = 100): the speculative threshold leaves the query short, so its group
+    is counted again from the safe threshold.  A second query of the same
+    group keeps the candidates of the first count."""
+    n, d = 200_000, 4
+    rows = np.zeros((n, d), np.int32)
+    rows[:, 0] = 5
+    rows[:, 1] = np.where(np.arange(n) % 7 == 0, 3, 0)
+    rows[np.r_[0:80, 4096:4106], :] = 5  # tile 0 is a sample tile, tile 1 is not
+    r = np.zeros(d + 1, RANGE_DTYPE)
+    r["query"] = [0, 0, 0, 0, 1]
+    r["dim"] = [0, 1, 2, 3, 1]
+    r["low"] = [5, 5, 5, 5, 3]
+    r["up"] = [5, 5, 5, 5, 3]
+    r["weight"] = 1
+    r["order"] = [0, 1, 2, 3, 0]
+    t = Table.from_rows(rows)
+    items = t.compile(r, 2)
+    (b, p) = run(t, items, 2, 100)
+    assert p.dense_active()
+    ri, rc = oracle_single(rows, r, 2, 100)
+    assert_topk_equal(b[0], b[1], ri, rc, 100, 2, "dense redo")
+    ids, cnt = b[0].reshape(2, 100), b[1].reshape(2, 100)
+    assert np.all(cnt[0, :90] == 4) and np.all(cnt[0, 90:] == 1)
+    assert list(ids[0, 80:90]) == list(range(4096, 4106)) and list(ids[0, 90:]) == list(range(80, 90))
