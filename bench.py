@@ -28,6 +28,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated): lines of the form
+    `traffic_bytes_per_dispatch <kernel> <bytes>` in profiles/*.md."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None, None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith(".md"):
+            continue
+        with open(os.path.join(d, f)) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_dispatch" and parts[1] == kernel:
+                    return int(parts[2]), "profiles/" + f
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +59,7 @@ def main():
     ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000])
     ap.add_argument("--serve-executors", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
+    ap.add_argument("--serve-mode", type=int, default=1, help="plan form of the server (0 plain, 1 batched)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device copy of this size (scales FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -150,8 +169,10 @@ def main():
         sitems = table.compile(sranges, nreq_max)
         sptr = np.searchsorted(sitems["query"], np.arange(nreq_max + 1)).astype(np.uint64)
         serving = {"index": "the 1M x 128 index above, resident", "requests": "ranges {0,+-2,+-5}, k {10,100,1000}",
-                   "loads": []}
-        srv = Server(index, executors=args.serve_executors, max_batch=args.serve_batch, max_k=1000)
+                   "mode": "batched" if args.serve_mode == 1 else "plain", "executors": args.serve_executors,
+                   "max_batch": args.serve_batch, "loads": []}
+        srv = Server(index, executors=args.serve_executors, max_batch=args.serve_batch, max_k=1000,
+                     mode=args.serve_mode)
         srv.replay(sitems[: int(sptr[64])], sptr[:65], sk[:64], np.zeros(64))  # warm
         for qps in args.serve_qps:
             n = int(qps * args.serve_seconds)
@@ -194,6 +215,7 @@ def main():
                        else "k_scan_batched (count + select)")
     else:
         scan_kernel = "k_scan_tile (count + select)"
+    traffic, traffic_src = profiled_traffic(scan_kernel.split()[0])
     achieved = compulsory / (scan_ms * 1e-3) / 1e9
     out = {
         "metric": "queries/s (1M x 128, k=100) batched vs plain GPU; p50/p99 latency at fixed QPS",
@@ -220,7 +242,7 @@ def main():
             "serving": serving,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": scan_kernel,
                      "effective_logical_GBs": logical / (scan_ms * 1e-3) / 1e9},
         "cpu_baseline": cpu,

@@ -138,7 +138,8 @@ struct genie_plan {
     int32_t* d_gq = nullptr;
     uint32_t* d_queue = nullptr;
     uint64_t gq_alloc = 0;
-    std::vector<genie_dim_t> h_items;  // sorted by query, kept to rebuild the work queue
+    std::vector<genie_dim_t> h_items;  // the uploaded batch sorted by query (work queue, byte figures)
+    bool bytes_valid = false;
     std::vector<uint32_t> h_qptr;
     double host_build_ms = 0;
     bool lanes_built = false;        // work queue of the lane form built for this batch
@@ -224,70 +225,90 @@ static void plan_free_buffers(genie_plan* p)
 // Count bound of every query: per (query, order) group the largest weight
 // times how often one object can occur in the group's spans (1 per distinct
 // dim for single-value tables; the sum of per-key multiplicities for
-// bijectMap tables), summed over groups.
+// bijectMap tables), summed over groups.  Items are bucketed by query and
+// each query's entries sorted on their own (one entry per item when the item
+// lies in one dim of a single-value table, one per key otherwise).
 static int compute_bounds(const DevIndex& d, const genie_dim_t* items, uint64_t n_items,
                           uint32_t n_queries, uint32_t* out)
 {
-    struct G {
-        uint32_t q;
-        int32_t order;
-        uint32_t w;
-        uint32_t key;
-    };
-    std::vector<G> v;
-    v.reserve(n_items * 2);
+    std::vector<uint32_t> qptr(n_queries + 1, 0), idx(n_items);
+    for (uint64_t i = 0; i < n_items; ++i)
+        qptr[items[i].query + 1]++;
+    for (uint32_t q = 0; q < n_queries; ++q)
+        qptr[q + 1] += qptr[q];
+    {
+        std::vector<uint32_t> fill(qptr.begin(), qptr.end() - 1);
+        for (uint64_t i = 0; i < n_items; ++i)
+            idx[fill[items[i].query]++] = (uint32_t)i;
+    }
     const uint64_t* ks = d.h_key_start.data();
     const uint32_t nk = d.n_keys;
     auto key_of_pos = [&](uint64_t pos) -> uint32_t {
         return (uint32_t)(std::upper_bound(ks, ks + nk + 1, pos) - ks - 1);
     };
-    for (uint64_t i = 0; i < n_items; ++i) {
-        const genie_dim_t& it = items[i];
-        if (it.end_pos <= it.start_pos)
-            continue;
-        uint32_t k0, k1;
-        if (d.codec == 0) {
-            k0 = key_of_pos((uint64_t)it.start_pos);
-            k1 = key_of_pos((uint64_t)it.end_pos - 1);
-        } else {
-            const uint32_t* cp = d.h_seg_cpos.data();
-            uint32_t s0 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) - cp - 1);
-            uint32_t s1 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos - 1) - cp - 1);
-            k0 = d.h_seg_key[s0];
-            k1 = d.h_seg_key[s1];
-        }
-        for (uint32_t k = k0; k <= k1; ++k)
-            v.push_back({(uint32_t)it.query, it.order, (uint32_t)it.weight, k});
-    }
-    std::sort(v.begin(), v.end(), [](const G& a, const G& b) {
-        if (a.q != b.q) return a.q < b.q;
-        if (a.order != b.order) return a.order < b.order;
-        return a.key < b.key;
-    });
-    std::vector<uint64_t> acc(n_queries, 0);
-    for (size_t i = 0; i < v.size();) {
-        size_t j = i;
-        uint32_t wmax = 0;
-        uint64_t mult = 0;
-        std::unordered_set<uint32_t> dims;
-        uint32_t last_key = 0xFFFFFFFFu;
-        for (; j < v.size() && v[j].q == v[i].q && v[j].order == v[i].order; ++j) {
-            wmax = std::max(wmax, v[j].w);
-            if (v[j].key == last_key)
+    auto one_query = [&](uint32_t q) {
+        struct E {
+            int32_t order;
+            uint32_t x, w;
+        };
+        std::vector<E> v;
+        for (uint32_t j = qptr[q]; j < qptr[q + 1]; ++j) {
+            const genie_dim_t& it = items[idx[j]];
+            if (it.end_pos <= it.start_pos)
                 continue;
-            last_key = v[j].key;
-            if (d.single_value)
-                dims.insert(d.h_key_dim[v[j].key]);
+            uint32_t k0, k1;
+            if (d.codec == 0) {
+                k0 = key_of_pos((uint64_t)it.start_pos);
+                k1 = key_of_pos((uint64_t)it.end_pos - 1);
+            } else {
+                const uint32_t* cp = d.h_seg_cpos.data();
+                const uint32_t s0 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) - cp - 1);
+                const uint32_t s1 = (uint32_t)(std::upper_bound(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos - 1) - cp - 1);
+                k0 = d.h_seg_key[s0];
+                k1 = d.h_seg_key[s1];
+            }
+            const uint32_t w = (uint32_t)it.weight;
+            if (d.single_value && d.h_key_dim[k0] == d.h_key_dim[k1])
+                v.push_back({it.order, d.h_key_dim[k0], w});
             else
-                mult += d.h_key_mult[v[j].key];
+                for (uint32_t k = k0; k <= k1; ++k)
+                    v.push_back({it.order, d.single_value ? d.h_key_dim[k] : k, w});
         }
-        if (d.single_value)
-            mult = dims.size();
-        acc[v[i].q] += (uint64_t)wmax * mult;
-        i = j;
+        std::sort(v.begin(), v.end(), [](const E& a, const E& b) {
+            return a.order != b.order ? a.order < b.order : a.x < b.x;
+        });
+        uint64_t acc = 0;
+        for (size_t i = 0; i < v.size();) {
+            size_t j = i;
+            uint32_t wmax = 0;
+            uint64_t mult = 0;
+            for (; j < v.size() && v[j].order == v[i].order; ++j) {
+                wmax = std::max(wmax, v[j].w);
+                if (j > i && v[j].x == v[j - 1].x)
+                    continue;  // the same dim (key) again
+                mult += d.single_value ? 1u : d.h_key_mult[v[j].x];
+            }
+            acc += (uint64_t)wmax * mult;
+            i = j;
+        }
+        out[q] = (uint32_t)std::min<uint64_t>(acc, 0xFFFFFFFFull);
+    };
+    if (n_items < 65536) {
+        for (uint32_t q = 0; q < n_queries; ++q)
+            one_query(q);
+    } else {
+        const uint32_t T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        std::atomic<uint32_t> next(0);
+        for (uint32_t t = 0; t < T; ++t)
+            th.emplace_back([&] {
+                for (uint32_t q; (q = next.fetch_add(64)) < n_queries;)
+                    for (uint32_t e = q; e < std::min(n_queries, q + 64); ++e)
+                        one_query(e);
+            });
+        for (auto& x : th)
+            x.join();
     }
-    for (uint32_t q = 0; q < n_queries; ++q)
-        out[q] = (uint32_t)std::min<uint64_t>(acc[q], 0xFFFFFFFFull);
     return GENIE_OK;
 }
 
@@ -1059,7 +1080,7 @@ static int plan_size_candidates(genie_plan* p)
 // Whether the uploaded batch (kept on the host by mode-1 plans) can run the
 // dense form: every item with postings and a nonzero weight covers whole keys
 // with weight 1, and no query covers a key twice.
-static bool dense_batch_ok(const genie_plan* p)
+static bool dense_batch_ok(const genie_plan* p, uint32_t n_queries)
 {
     const DevIndex& d = p->ix->d;
     if (p->mode != 1 || !d.dense)
@@ -1067,7 +1088,7 @@ static bool dense_batch_ok(const genie_plan* p)
     const uint64_t* ks = d.h_key_start.data();
     const uint32_t nk = d.n_keys;
     std::vector<std::pair<uint32_t, uint32_t>> iv;
-    for (uint32_t q = 0; q < p->n_queries; ++q) {
+    for (uint32_t q = 0; q < n_queries; ++q) {
         iv.clear();
         for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
             const genie_dim_t& it = p->h_items[i];
@@ -1190,48 +1211,44 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
         for (uint64_t i = 0; i < n_items; ++i)
             sorted[fillp[items[i].query]++] = items[i];
     }
+    // the batch stays on the host: the lane form's work queue and the byte
+    // figures of genie_hip_plan_bytes are built from it when needed
+    p->h_items.swap(sorted);
+    p->h_qptr.swap(qptr);
+    p->bytes_valid = false;
+    p->dense_ok = dense_batch_ok(p, n_queries);
     // count bound of this batch
     if (p->count_bound == 0) {
-        std::vector<uint32_t> b(n_queries);
-        compute_bounds(d, sorted.data(), n_items, n_queries, b.data());
-        uint32_t mx = 0;
-        for (uint32_t v : b)
-            mx = std::max(mx, v);
-        p->lane_bits = bits_for(mx);
+        if (p->dense_ok) {
+            p->lane_bits = 8;  // unit weights, each dim once: counts <= 128
+        } else {
+            std::vector<uint32_t> b(n_queries);
+            compute_bounds(d, p->h_items.data(), n_items, n_queries, b.data());
+            uint32_t mx = 0;
+            for (uint32_t v : b)
+                mx = std::max(mx, v);
+            p->lane_bits = bits_for(mx);
+        }
     }
-    // chunks and algorithmic bytes
-    uint64_t nch = 0, logical = 0;
-    std::vector<std::pair<uint32_t, uint32_t>> spans;
-    spans.reserve(n_items);
+    // chunks of the plain and lane forms
+    uint64_t nch = 0;
     bool prev_ok = false;
     for (uint64_t i = 0; i < n_items; ++i) {
-        const genie_dim_t& it = sorted[i];
+        const genie_dim_t& it = p->h_items[i];
         if (it.end_pos > it.start_pos) {
             uint64_t len = (uint64_t)(it.end_pos - it.start_pos);
             if (d.codec == 0) {
                 nch += (len + kChunk - 1) / kChunk;
             } else {
                 // runs of contiguous segments (k_crun_flags uses the same rule)
-                const genie_dim_t& pv = sorted[i - (i ? 1 : 0)];
+                const genie_dim_t& pv = p->h_items[i - (i ? 1 : 0)];
                 const bool starts = i == 0 || !prev_ok || pv.query != it.query || pv.weight != it.weight ||
                                     pv.end_pos != it.start_pos;
                 nch += starts ? 1 : 0;
             }
-            logical += 4 * len;
-            spans.push_back({(uint32_t)it.start_pos, (uint32_t)it.end_pos});
         }
         prev_ok = it.end_pos > it.start_pos;
     }
-    std::sort(spans.begin(), spans.end());
-    uint64_t compulsory = 0, cur_e = 0;
-    for (auto& s : spans) {  // union of position ranges
-        uint64_t a = std::max<uint64_t>(s.first, cur_e);
-        if (s.second > a)
-            compulsory += 4 * (s.second - a);
-        cur_e = std::max<uint64_t>(cur_e, s.second);
-    }
-    p->logical_bytes = logical;
-    p->compulsory_bytes = compulsory;
     if (nch > p->chunks_cap) {
         dev_free(p->d_chunks);
         dev_free(p->d_nk);
@@ -1267,16 +1284,13 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
         p->out_alloc = p->total_k;
     }
     if (n_items)
-        HIP_TRY(hipMemcpy(p->d_items, sorted.data(), n_items * sizeof(genie_dim_t), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_item_qptr, qptr.data(), qptr.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(p->d_items, p->h_items.data(), n_items * sizeof(genie_dim_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_item_qptr, p->h_qptr.data(), p->h_qptr.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_kq, k_per_query, n_queries * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_out_off, p->h_out_off.data(), p->h_out_off.size() * 8, hipMemcpyHostToDevice));
     plan_drop_graph(p);
     if (p->mode == 1) {
-        p->h_items.swap(sorted);
-        p->h_qptr.swap(qptr);
         p->lanes_built = false;
-        p->dense_ok = dense_batch_ok(p);
         return plan_settle_form(p);
     }
     p->cap_factor = initial_cap_factor(p);
@@ -1847,6 +1861,29 @@ int genie_hip_plan_bytes(genie_plan* p, uint64_t* logical, uint64_t* compulsory)
 {
     if (!p || !logical || !compulsory)
         return fail(GENIE_E_INVALID, "null argument");
+    if (!p->bytes_valid) {
+        uint64_t logical_b = 0;
+        std::vector<std::pair<uint32_t, uint32_t>> spans;
+        spans.reserve(p->n_items);
+        for (uint64_t i = 0; i < p->n_items; ++i) {
+            const genie_dim_t& it = p->h_items[i];
+            if (it.end_pos > it.start_pos) {
+                logical_b += 4ull * (uint32_t)(it.end_pos - it.start_pos);
+                spans.push_back({(uint32_t)it.start_pos, (uint32_t)it.end_pos});
+            }
+        }
+        std::sort(spans.begin(), spans.end());
+        uint64_t comp = 0, cur_e = 0;
+        for (auto& sp : spans) {  // union of position ranges
+            const uint64_t a = std::max<uint64_t>(sp.first, cur_e);
+            if (sp.second > a)
+                comp += 4 * (sp.second - a);
+            cur_e = std::max<uint64_t>(cur_e, sp.second);
+        }
+        p->logical_bytes = logical_b;
+        p->compulsory_bytes = comp;
+        p->bytes_valid = true;
+    }
     *logical = p->logical_bytes;
     *compulsory = p->compulsory_bytes;
     return GENIE_OK;

@@ -1,14 +1,15 @@
 // Serving form: a continuous-batching request loop over an HBM-resident
 // index (C ABI in include/genie_hip.h).
 //
-// One scheduler thread owns E executors (a plan and a non-blocking stream
-// each).  Whenever an executor is free and requests are pending, it packs the
-// pending requests (FIFO) into a batch -- as many as fit the batch limits and
-// the device-memory budget of in-flight requests -- uploads it, launches the
-// search and an asynchronous copy of the results into pinned host memory, and
-// goes on; so requests that arrive while other batches are on the device are
-// admitted into the next free executor and their work overlaps on the GPU.
-// Completion is detected with events; results go back per request.
+// A scheduler thread and E executors (a plan, a non-blocking stream and a
+// host thread each).  Whenever an executor is free and requests are pending,
+// the scheduler packs the pending requests (FIFO) into a batch -- as many as
+// fit the batch limits and the device-memory budget of in-flight requests --
+// and hands it to the executor, whose thread uploads it, launches the search
+// and an asynchronous copy of the results into pinned host memory, waits for
+// its event and returns the results per request.  Requests that arrive while
+// other batches are on the device go to the next free executor, so host-side
+// batch preparation and device work of different batches overlap.
 //
 // The reference serves one JSON batch at a time from a polled queue
 // (src/dist_genie/online.cu:144-180, scheduler.cc:20-64) and admits whole
@@ -57,7 +58,10 @@ struct Executor {
     uint64_t host_cap = 0;
     std::vector<Request*> batch;
     uint64_t mem = 0;
-    bool busy = false;
+    bool busy = false;      // holds a batch (set by the scheduler, cleared by finish)
+    bool assigned = false;  // a batch waits for the executor's thread
+    std::condition_variable cv;
+    std::thread th;
 };
 
 // Device scratch one request holds while in flight: its work items, the
@@ -77,16 +81,18 @@ struct genie_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
-    std::vector<Executor> ex;
+    std::deque<Executor> ex;  // stable addresses: each executor has its own thread
     uint64_t next_ticket = 1, mem_in_use = 0;
     uint64_t n_batches = 0, n_done = 0, max_inflight = 0, inflight = 0;
-    bool stop = false;
+    uint64_t events = 0;  // submissions and completions: what the scheduler waits for
+    bool stop = false, stop_exec = false;
     std::thread sched;
     int device = 0;
 
     void run();
+    void exec_loop(Executor& e);
     int launch(Executor& e);
-    void finish(Executor& e);
+    void finish(Executor& e, int rc, const std::string& err);
 };
 
 static int fail_s(int code, const std::string& m)
@@ -133,42 +139,76 @@ int genie_server::launch(Executor& e)
     return GENIE_OK;
 }
 
-void genie_server::finish(Executor& e)
+void genie_server::finish(Executor& e, int rc, const std::string& err_in)
 {
-    int rc = GENIE_OK;
-    if (*e.h_ovf)  // candidate space too small: grow, rerun, copy synchronously
+    std::string err = err_in;
+    if (!rc && *e.h_ovf) {  // candidate space too small: grow, rerun, copy synchronously
         rc = genie_hip_plan_download(e.plan, e.stream, e.h_ids, e.h_counts);
-    const std::string err = rc ? genie_hip_last_error() : "";
+        if (rc)
+            err = genie_hip_last_error();
+    }
     const auto now = Clock::now();
     uint64_t off = 0;
-    std::lock_guard<std::mutex> lk(mu);
-    for (Request* r : e.batch) {
-        r->rc = rc;
-        r->err = err;
-        if (!rc) {
-            r->ids.assign(e.h_ids + off, e.h_ids + off + r->k);
-            r->counts.assign(e.h_counts + off, e.h_counts + off + r->k);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Request* r : e.batch) {
+            r->rc = rc;
+            r->err = err;
+            if (!rc) {
+                r->ids.assign(e.h_ids + off, e.h_ids + off + r->k);
+                r->counts.assign(e.h_counts + off, e.h_counts + off + r->k);
+            }
+            off += r->k;
+            r->t_done = now;
+            r->done = true;
         }
-        off += r->k;
-        r->t_done = now;
-        r->done = true;
+        n_done += e.batch.size();
+        inflight -= e.batch.size();
+        mem_in_use -= e.mem;
+        e.batch.clear();
+        e.mem = 0;
+        e.busy = false;
+        ++events;
     }
-    n_done += e.batch.size();
-    inflight -= e.batch.size();
-    mem_in_use -= e.mem;
-    e.batch.clear();
-    e.mem = 0;
-    e.busy = false;
     cv_done.notify_all();
+    cv_work.notify_one();  // an executor is free
 }
 
-void genie_server::run()
+// Executor thread: prepare and launch each batch handed over, wait for its
+// results (event polling), return them.
+void genie_server::exec_loop(Executor& e)
 {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
-        bool progressed = false;
-        // dispatch pending requests to free executors
+        e.cv.wait(lk, [&] { return e.assigned || stop_exec; });
+        if (!e.assigned)
+            break;
+        e.assigned = false;
+        lk.unlock();
+        int rc = launch(e);
+        std::string err = rc ? genie_hip_last_error() : "";
+        if (!rc) {
+            hipError_t q;
+            while ((q = hipEventQuery(e.ev)) == hipErrorNotReady)
+                std::this_thread::sleep_for(std::chrono::microseconds(5));
+            if (q != hipSuccess) {
+                (void)hipGetLastError();
+                rc = GENIE_E_RUNTIME;
+                err = hipGetErrorString(q);
+            }
+        }
+        finish(e, rc, err);
+        lk.lock();
+    }
+}
+
+// Scheduler thread: FIFO admission of pending requests into free executors.
+void genie_server::run()
+{
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        const uint64_t seen = events;
         for (Executor& e : ex) {
             if (e.busy || pending.empty())
                 continue;
@@ -190,68 +230,19 @@ void genie_server::run()
                 continue;
             e.mem = mem;
             e.busy = true;
+            e.assigned = true;
             mem_in_use += mem;
             inflight += e.batch.size();
             max_inflight = std::max(max_inflight, inflight);
             ++n_batches;
-            lk.unlock();
-            const int rc = launch(e);
-            if (rc) {  // report the failure on every request of the batch
-                const std::string err = genie_hip_last_error();
-                lk.lock();
-                for (Request* r : e.batch) {
-                    r->rc = rc;
-                    r->err = err;
-                    r->t_done = Clock::now();
-                    r->done = true;
-                }
-                n_done += e.batch.size();
-                inflight -= e.batch.size();
-                mem_in_use -= e.mem;
-                e.batch.clear();
-                e.busy = false;
-                cv_done.notify_all();
-                lk.unlock();
-            }
-            lk.lock();
-            progressed = true;
-        }
-        // completions
-        for (Executor& e : ex) {
-            if (!e.busy || e.batch.empty())
-                continue;
-            lk.unlock();
-            const hipError_t q = hipEventQuery(e.ev);
-            if (q == hipSuccess) {
-                finish(e);
-                progressed = true;
-            } else if (q != hipErrorNotReady) {
-                hipGetLastError();
-                std::lock_guard<std::mutex> g(mu);
-                for (Request* r : e.batch) {
-                    r->rc = GENIE_E_RUNTIME;
-                    r->err = hipGetErrorString(q);
-                    r->t_done = Clock::now();
-                    r->done = true;
-                }
-                n_done += e.batch.size();
-                inflight -= e.batch.size();
-                mem_in_use -= e.mem;
-                e.batch.clear();
-                e.busy = false;
-                cv_done.notify_all();
-            }
-            lk.lock();
+            e.cv.notify_one();
         }
         if (stop && pending.empty() && inflight == 0)
             break;
-        if (!progressed) {
-            // idle: wake on a new request, or poll the in-flight events again soon
-            if (inflight == 0)
-                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
-            else
-                cv_work.wait_for(lk, std::chrono::microseconds(20));
-        }
+        // sleep (releasing the lock) until a request arrives or a batch
+        // completes; free executors alone are no reason to look again, since
+        // the budget may hold the pending requests back until a completion
+        cv_work.wait(lk, [&] { return events != seen || (stop && pending.empty() && inflight == 0); });
     }
 }
 
@@ -289,6 +280,10 @@ int genie_hip_server_create(genie_index* ix, const genie_server_config* cfg, gen
         genie_hip_server_destroy(s);
         return rc;
     }
+    for (Executor& e : s->ex) {
+        Executor* ep = &e;
+        e.th = std::thread([s, ep] { s->exec_loop(*ep); });
+    }
     s->sched = std::thread([s] { s->run(); });
     *out = s;
     return GENIE_OK;
@@ -305,6 +300,15 @@ int genie_hip_server_destroy(genie_server* s)
         }
         s->cv_work.notify_all();
         s->sched.join();
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop_exec = true;
+    }
+    for (Executor& e : s->ex) {
+        e.cv.notify_all();
+        if (e.th.joinable())
+            e.th.join();
     }
     for (Executor& e : s->ex) {
         if (e.stream)
@@ -348,6 +352,7 @@ int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t 
         *ticket = r->ticket;
         s->pending.push_back(r.get());
         s->reqs.emplace(r->ticket, std::move(r));
+        ++s->events;
     }
     s->cv_work.notify_one();
     return GENIE_OK;
