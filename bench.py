@@ -59,7 +59,8 @@ def main():
     ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000])
     ap.add_argument("--serve-executors", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
-    ap.add_argument("--serve-mode", type=int, default=1, help="plan form of the server (0 plain, 1 batched)")
+    ap.add_argument("--serve-mode", type=int, default=2,
+                    help="plan mode of the server (0 plain, 1 batched, 2 automatic: dense when it applies, else plain)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device copy of this size (scales FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -169,7 +170,7 @@ def main():
         sitems = table.compile(sranges, nreq_max)
         sptr = np.searchsorted(sitems["query"], np.arange(nreq_max + 1)).astype(np.uint64)
         serving = {"index": "the 1M x 128 index above, resident", "requests": "ranges {0,+-2,+-5}, k {10,100,1000}",
-                   "mode": "batched" if args.serve_mode == 1 else "plain", "executors": args.serve_executors,
+                   "mode": ["plain", "batched", "automatic"][args.serve_mode], "executors": args.serve_executors,
                    "max_batch": args.serve_batch, "loads": []}
         srv = Server(index, executors=args.serve_executors, max_batch=args.serve_batch, max_k=1000,
                      mode=args.serve_mode)

@@ -956,7 +956,7 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         return fail(GENIE_E_INVALID, "null argument");
     if (max_queries == 0 || max_k == 0 || max_k > GENIE_MAX_K)
         return fail(GENIE_E_INVALID, "max_queries must be > 0 and max_k in [1, GENIE_MAX_K]");
-    if (mode != 0 && mode != 1)
+    if (mode != 0 && mode != 1 && mode != 2)
         return fail(GENIE_E_INVALID, "unknown plan mode");
     genie_plan* p = new genie_plan();
     p->ix = ix;
@@ -1020,7 +1020,9 @@ int genie_hip_plan_destroy(genie_plan* p)
 
 }  // extern "C"
 
-static bool dense_active(const genie_plan* p) { return p->mode == 1 && p->dense_on && p->dense_ok; }
+// Mode 1 (batched) and mode 2 (automatic) run the dense form when the batch
+// allows it; otherwise mode 1 runs the lane form and mode 2 the plain form.
+static bool dense_active(const genie_plan* p) { return p->mode != 0 && p->dense_on && p->dense_ok; }
 
 static uint32_t plan_tile_objs(const genie_plan* p)
 {
@@ -1083,7 +1085,7 @@ static int plan_size_candidates(genie_plan* p)
 static bool dense_batch_ok(const genie_plan* p, uint32_t n_queries)
 {
     const DevIndex& d = p->ix->d;
-    if (p->mode != 1 || !d.dense)
+    if (p->mode == 0 || !d.dense)
         return false;
     const uint64_t* ks = d.h_key_start.data();
     const uint32_t nk = d.n_keys;
@@ -1172,13 +1174,14 @@ static int ensure_lanes(genie_plan* p)
 static int plan_settle_form(genie_plan* p)
 {
     plan_drop_graph(p);
-    if (p->mode != 1 || p->n_queries == 0)
+    if (p->mode == 0 || p->n_queries == 0)
         return GENIE_OK;
     if (dense_active(p)) {
         if (int rc = prepare_dense(p))
             return rc;
-    } else if (int rc = ensure_lanes(p)) {
-        return rc;
+    } else if (p->mode == 1) {
+        if (int rc = ensure_lanes(p))
+            return rc;
     }
     p->cap_factor = initial_cap_factor(p);
     return plan_size_candidates(p);
@@ -1289,7 +1292,7 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     HIP_TRY(hipMemcpy(p->d_kq, k_per_query, n_queries * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_out_off, p->h_out_off.data(), p->h_out_off.size() * 8, hipMemcpyHostToDevice));
     plan_drop_graph(p);
-    if (p->mode == 1) {
+    if (p->mode != 0) {
         p->lanes_built = false;
         return plan_settle_form(p);
     }
@@ -1632,8 +1635,8 @@ int genie_hip_plan_set_dense(genie_plan* p, int on)
 {
     if (!p)
         return fail(GENIE_E_INVALID, "null plan");
-    if (p->mode != 1)
-        return fail(GENIE_E_INVALID, "the dense form belongs to batched plans");
+    if (p->mode == 0)
+        return fail(GENIE_E_INVALID, "the dense form belongs to batched and automatic plans");
     if (p->dense_on == (on != 0))
         return GENIE_OK;
     p->dense_on = on != 0;
@@ -1903,7 +1906,7 @@ int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_i
     if (max_k == 0 || max_k > GENIE_MAX_K)
         return fail(GENIE_E_INVALID, "k outside [1, GENIE_MAX_K]");
     genie_plan* p = nullptr;
-    int rc = genie_hip_plan_create(ix, n_queries, n_items, max_k, count_bound, 0, &p);
+    int rc = genie_hip_plan_create(ix, n_queries, n_items, max_k, count_bound, 2, &p);
     if (rc)
         return rc;
     if (!(rc = genie_hip_plan_upload(p, items, n_items, n_queries, k_per_query)) &&

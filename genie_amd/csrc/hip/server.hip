@@ -253,7 +253,7 @@ int genie_hip_server_create(genie_index* ix, const genie_server_config* cfg, gen
     if (!ix || !cfg || !out)
         return fail_s(GENIE_E_INVALID, "null argument");
     if (cfg->executors == 0 || cfg->executors > 16 || cfg->max_batch == 0 || cfg->max_batch_items == 0 ||
-        cfg->max_k == 0 || cfg->max_k > GENIE_MAX_K || (cfg->mode != 0 && cfg->mode != 1))
+        cfg->max_k == 0 || cfg->max_k > GENIE_MAX_K || (cfg->mode < 0 || cfg->mode > 2))
         return fail_s(GENIE_E_INVALID, "server configuration out of range");
     auto* s = new genie_server();
     s->ix = ix;

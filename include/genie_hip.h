@@ -88,9 +88,11 @@ uint64_t genie_hip_index_bytes(const genie_index* ix);
 int genie_hip_count_bounds(const genie_index* ix, const genie_dim_t* items, uint64_t n_items,
                            uint32_t n_queries, uint32_t* out_bounds);
 
-/* Plain per-batch search: replaces knn() -> match()/match_integrated() ->
+/* Per-batch search: replaces knn() -> match()/match_integrated() ->
  * heap_count_topk() -> extract_index_and_count() (knn.cu:106-183,
- * match.cu:224-464, heap_count.cu:161-282, knn.cu:48-55).  items in HOST
+ * match.cu:224-464, heap_count.cu:161-282, knn.cu:48-55).  Runs as an
+ * automatic plan (mode 2): the dense form when the batch qualifies, the
+ * plain form otherwise; the results are the same either way.  items in HOST
  * memory (any order; item.query in [0, n_queries)); k_per_query[q] in
  * [1, GENIE_MAX_K]; count_bound 0 = derive from the items.  Output is
  * query-major with out_offsets[q] = sum of k over earlier queries, k_q slots
@@ -103,7 +105,10 @@ int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_i
 /* ---- device-resident form (inputs already in HBM) ---------------------- */
 /* A plan holds device scratch for batches up to the given shape, so the hot
  * call allocates nothing.  mode: 0 plain (one query per workgroup),
- * 1 batched (queries packed into count lanes, persistent work queue). */
+ * 1 batched (the dense form when the batch qualifies -- see
+ * genie_hip_plan_set_dense -- else queries packed into count lanes over a
+ * persistent work queue), 2 automatic (the dense form when the batch
+ * qualifies, else plain).  All modes give identical results. */
 typedef struct genie_plan genie_plan;
 int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_items,
                           uint32_t max_k, uint32_t count_bound, int mode, genie_plan** out);
@@ -138,7 +143,7 @@ int genie_hip_plan_get_config(genie_plan* p, uint32_t* block, uint32_t* tile_obj
  * (optional, >= 13 entries) receives every candidate's time per run (-1 if it
  * could not run); *n_out the number of candidates. */
 int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n_out);
-/* Dense form of the batched search (mode 1 only; on by default).  When the
+/* Dense form of the batched search (modes 1 and 2; on by default).  When the
  * index is a single-value table of at most 128 dims with at most 256 keys per
  * dim, and every item of the uploaded batch covers whole keys with weight 1
  * (never one key twice for a query), the batch is counted from per-object key
@@ -178,7 +183,7 @@ typedef struct genie_server_config {
     uint64_t max_batch_items; /* work items per batch */
     uint32_t max_k;           /* largest k accepted */
     uint64_t memory_budget;   /* bytes of device scratch for in-flight requests */
-    int mode;                 /* plan mode of the executors: 0 plain, 1 batched */
+    int mode;                 /* plan mode of the executors: 0 plain, 1 batched, 2 automatic */
 } genie_server_config;
 
 typedef struct genie_server genie_server;

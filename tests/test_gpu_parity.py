@@ -36,17 +36,34 @@ def oracle_sets(row_ptr, vals, ranges, n_queries, k):
     return np.concatenate(res_i), np.concatenate(res_c)
 
 
-def run_engine(table, ranges, n_queries, k, lb=False, max_load=0, shuffle=False):
+def engine(table, items, n_queries, k, mode):
+    """mode "auto": genie_hip_match_topk (dense form when the batch allows,
+    else plain); 0: an explicit plain plan, so the per-query scan kernel
+    itself is checked whatever the batch."""
+    if mode == "auto":
+        return match_topk(table.device(), items, n_queries, k)
+    kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (n_queries,)))
+    p = Plan(table.device(), n_queries, max(1, len(items)), int(kq.max()), 0, mode=mode)
+    p.upload(items, n_queries, kq)
+    p.run()
+    return p.download()
+
+
+def run_engine(table, ranges, n_queries, k, lb=False, max_load=0, shuffle=False, mode="auto"):
     items = table.compile(ranges, n_queries, lb, max_load)
     if shuffle:
         items = items[np.random.default_rng(0).permutation(len(items))]
-    return match_topk(table.device(), items, n_queries, k)
+    return engine(table, items, n_queries, k, mode)
 
 
-def test_cfg0_plain():
+MODES = [0, "auto"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_cfg0_plain(mode):
     w = W.cfg0()
     t = Table.from_rows(w["rows"])
-    ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"])
+    ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"], mode=mode)
     ri, rc = oracle_single(w["rows"], w["ranges"], w["n_queries"], w["k"])
     assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], "cfg0")
     # every query is a copy of a row with U{-3..3} noise per dim, so the row
@@ -78,11 +95,12 @@ def test_tweets20_exact_counts():
     assert_topk_equal(ids, cnt, ri, rc, 5, 20, "tweets20")
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("n", [70_000, 200_003])
-def test_many_tiles_zipf_ranges(n):
+def test_many_tiles_zipf_ranges(n, mode):
     w = W.cfg1(n=n, d=32, q=24, k=100)
     t = Table.from_rows(w["rows"])
-    ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"])
+    ids, cnt = run_engine(t, w["ranges"], w["n_queries"], w["k"], mode=mode)
     ri, rc = oracle_single(w["rows"], w["ranges"], w["n_queries"], w["k"])
     assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], f"zipf n={n}")
 
@@ -99,7 +117,8 @@ def test_weights_widen_count_lanes():
         assert_topk_equal(ids, cnt, ri, rc, w["k"], w["n_queries"], f"weights<={wmax}")
 
 
-def test_edge_cases_padding_empty_and_k():
+@pytest.mark.parametrize("mode", MODES)
+def test_edge_cases_padding_empty_and_k(mode):
     w = W.cfg0(n=3000, d=16, q=6)
     t = Table.from_rows(w["rows"])
     r = w["ranges"]
@@ -110,14 +129,14 @@ def test_edge_cases_padding_empty_and_k():
     r["low"][m4] = 1000                           # query 4: disjoint ranges only
     r["up"][m4] = 2000
     for k in (1, 7, 300, 4096):
-        ids, cnt = run_engine(t, r, 6, k)
+        ids, cnt = run_engine(t, r, 6, k, mode=mode)
         ri, rc = oracle_single(w["rows"], r, 6, k)
         assert_topk_equal(ids, cnt, ri, rc, k, 6, f"edge k={k}")
         c = cnt.reshape(6, k)
         assert np.all(c[2] == 0) and np.all(c[4] == 0)
     # per-query k
     kq = np.array([1, 10, 100, 1000, 5, 4096], np.uint32)
-    ids, cnt = run_engine(t, r, 6, kq)
+    ids, cnt = run_engine(t, r, 6, kq, mode=mode)
     ri, rc = oracle_single(w["rows"], r, 6, kq)
     assert_topk_equal(ids, cnt, ri, rc, kq, 6, "per-query k")
 

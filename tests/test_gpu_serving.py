@@ -17,7 +17,7 @@ def requests_of(table, w):
     return items, ptr
 
 
-@pytest.mark.parametrize("mode,codec", [(0, 0), (1, 0), (0, 11)])
+@pytest.mark.parametrize("mode,codec", [(0, 0), (1, 0), (2, 0), (0, 11)])
 def test_alone_equals_in_the_mix(mode, codec):
     w = W.cfg4(n=200_000, d=32, q=60)
     t = Table.from_rows(w["rows"], 1024, codec != 0, codec=codec)
