@@ -153,6 +153,8 @@ struct genie_plan {
     uint32_t* d_dlist = nullptr;     // sample tiles
     uint32_t n_dlist = 0, dlist_alloc = 0, n_dsample = 0;
     uint32_t* d_gsafe = nullptr;     // [max_queries] safe threshold
+    uint32_t* d_dkeys = nullptr;     // [max_items][2] key range of every item
+    std::vector<uint32_t> h_dkeys;
     uint32_t* d_redo = nullptr;      // [groups] second count of a group
     // graph replay
     bool graph_on = false, graph_valid = false;
@@ -220,6 +222,7 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_dlist);
     dev_free(p->d_gsafe);
     dev_free(p->d_redo);
+    dev_free(p->d_dkeys);
 }
 
 // Count bound of every query: per (query, order) group the largest weight
@@ -1082,13 +1085,16 @@ static int plan_size_candidates(genie_plan* p)
 // Whether the uploaded batch (kept on the host by mode-1 plans) can run the
 // dense form: every item with postings and a nonzero weight covers whole keys
 // with weight 1, and no query covers a key twice.
-static bool dense_batch_ok(const genie_plan* p, uint32_t n_queries)
+// The key range of every item is kept for the device-side mask build
+// (h_dkeys: first key | query % 32 << 24, end key; empty for skipped items).
+static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
 {
     const DevIndex& d = p->ix->d;
     if (p->mode == 0 || !d.dense)
         return false;
     const uint64_t* ks = d.h_key_start.data();
     const uint32_t nk = d.n_keys;
+    p->h_dkeys.assign(2 * p->h_items.size(), 0);
     std::vector<std::pair<uint32_t, uint32_t>> iv;
     for (uint32_t q = 0; q < n_queries; ++q) {
         iv.clear();
@@ -1104,6 +1110,8 @@ static bool dense_batch_ok(const genie_plan* p, uint32_t n_queries)
             if (ks[ka] != sp || kb > nk || ks[kb] != ep)
                 return false;
             iv.push_back({ka, kb});
+            p->h_dkeys[2 * i] = ka | (q & 31u) << 24;  // keys < 2^15 in a dense index
+            p->h_dkeys[2 * i + 1] = kb;
         }
         std::sort(iv.begin(), iv.end());
         for (size_t i = 1; i < iv.size(); ++i)
@@ -1131,6 +1139,11 @@ static int prepare_dense(genie_plan* p)
     if (!p->d_dhist)
         if (int rc = dev_alloc(&p->d_dhist, (uint64_t)p->max_queries * kDenseBins))
             return rc;
+    if (!p->d_dkeys)
+        if (int rc = dev_alloc(&p->d_dkeys, 2 * std::max<uint64_t>(1, p->max_items)))
+            return rc;
+    if (p->n_items)
+        HIP_TRY(hipMemcpy(p->d_dkeys, p->h_dkeys.data(), p->n_items * 8, hipMemcpyHostToDevice));
     if (!p->d_gsafe)
         if (int rc = dev_alloc(&p->d_gsafe, p->max_queries))
             return rc;
@@ -1400,15 +1413,16 @@ static int run_dense(genie_plan* p, hipStream_t s)
 {
     const DevIndex& d = p->ix->d;
     const uint32_t nq = p->n_queries, ni = (uint32_t)p->n_items;
-    const uint32_t tw = d.dense_dpad * kDenseKeysPerDim;
-    fill_u32(p->d_dtab, 0, (uint64_t)p->dn_groups * tw, s);
     fill_u32(p->d_dhist, 0, (uint64_t)nq * kDenseBins, s);
-    if (ni)
-        k_dense_masks<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, d.key_start, d.n_keys, d.key_slot, tw,
-                                                       p->d_dtab);
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[1], s));
+    // the sample pass builds every group's mask table in LDS from the items
+    // and writes it out for the main pass
     DenseArgs a;
+    a.items = p->d_items;
+    a.item_qptr = p->d_item_qptr;
+    a.dkeys = p->d_dkeys;
+    a.key_slot = d.key_slot;
     a.codes = d.codes;
     a.n_pad = d.dense_npad;
     a.tab = p->d_dtab;

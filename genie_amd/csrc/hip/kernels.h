@@ -962,8 +962,15 @@ struct MergeArgs {
     uint32_t key_bits;  // significant bits of a key (32 + bits of the count bound)
 };
 
+// Per query: select the k largest candidate keys (radix select over the key
+// bytes, most significant first), sort them descending, pad with (0, 0).
+// Candidates are staged in LDS when they fit; the bin holding the k-th key
+// is found with a wave-parallel suffix scan; the select stops as soon as a
+// bin holds exactly the keys still needed.
 __global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
 {
+    constexpr int PER = kMergeMaxK / kMergeThreads;  // staged keys per thread
+    static_assert(kMergeThreads == 256, "one histogram bin per thread");
     __shared__ uint64_t s_keys[kMergeMaxK];
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_res[4];
@@ -972,68 +979,102 @@ __global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
     const uint32_t k = a.kq[q];
     const uint32_t M = min(a.cand_cnt[q], a.cand_cap[q]);
     const uint64_t* keys = a.cand + a.cand_base[q];
+    const bool staged = M <= (uint32_t)kMergeMaxK;
+    if (staged)
+        for (uint32_t i = tid; i < M; i += kMergeThreads)
+            s_keys[i] = keys[i];
+    __syncthreads();
 
     uint64_t kth = 0;  // every key >= kth is in the result
     if (M > k) {
         uint64_t prefix = 0, mask = 0;
         uint32_t remaining = k;
-        int shift = ((int)((a.key_bits + 7) / 8) - 1) * 8;
-        for (; shift >= 0; shift -= 8) {
-            if (tid < 256)
-                s_hist[tid] = 0;
+        for (int shift = ((int)((a.key_bits + 7) / 8) - 1) * 8; shift >= 0; shift -= 8) {
+            s_hist[tid] = 0;  // kMergeThreads == 256 bins
             __syncthreads();
-            for (uint32_t i0 = 0; i0 < M; i0 += kMergeThreads) {
-                const uint32_t i = i0 + tid;
-                uint64_t key = i < M ? keys[i] : 0;
-                bool take = i < M && (key & mask) == prefix;
-                hist_add_aggregated(s_hist, (uint32_t)(key >> shift) & 255u, take);
+            for (uint32_t i = tid; i < M; i += kMergeThreads) {
+                const uint64_t key = staged ? s_keys[i] : keys[i];
+                if ((key & mask) == prefix)
+                    atomicAdd(&s_hist[(uint32_t)(key >> shift) & 255u], 1u);
             }
             __syncthreads();
-            if (tid == 0) {
-                uint32_t cum = 0, b = 255;
-                for (int d = 255; d >= 0; --d) {
-                    if (cum + s_hist[d] >= remaining) {
-                        b = (uint32_t)d;
-                        break;
+            if (tid < kWave) {
+                // lane l holds bins 255 - 4l .. 252 - 4l, counted from the top
+                const uint32_t top = 255u - 4u * (uint32_t)tid;
+                uint32_t h[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    h[j] = s_hist[top - j];
+                const uint32_t sum = h[0] + h[1] + h[2] + h[3];
+                uint32_t above = wave_incl_scan(sum) - sum;  // keys in higher bins
+                if (above < remaining && above + sum >= remaining) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (above + h[j] >= remaining) {
+                            s_res[0] = top - j;
+                            s_res[1] = remaining - above;
+                            s_res[2] = h[j];
+                            break;
+                        }
+                        above += h[j];
                     }
-                    cum += s_hist[d];
                 }
-                s_res[0] = b;
-                s_res[1] = remaining - cum;
             }
             __syncthreads();
-            prefix |= (uint64_t)s_res[0] << shift;
+            const uint32_t bin = s_res[0], rem = s_res[1], inbin = s_res[2];
+            prefix |= (uint64_t)bin << shift;
             mask |= (uint64_t)255 << shift;
-            remaining = s_res[1];
+            remaining = rem;
             __syncthreads();
+            if (inbin == rem)
+                break;  // the whole bin is needed: prefix with zero low bytes bounds it
         }
         kth = prefix;
     }
-    // collect the selected keys
-    if (tid == 0)
-        s_res[2] = 0;
-    uint32_t n2 = 1;
+    // collect the selected keys (read first, then overwrite the stage)
     const uint32_t nsel = min(M, k);
+    uint32_t n2 = 1;
     while (n2 < nsel)
         n2 <<= 1;
+    uint64_t mine[PER];
+    if (staged) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t i = tid + j * kMergeThreads;
+            mine[j] = i < M ? s_keys[i] : 0;
+        }
+    }
+    if (tid == 0)
+        s_res[3] = 0;
+    __syncthreads();
     for (uint32_t i = tid; i < n2; i += kMergeThreads)
         s_keys[i] = 0;
     __syncthreads();
-    for (uint32_t i0 = 0; i0 < M; i0 += kMergeThreads) {
-        const uint32_t i = i0 + tid;
-        uint64_t key = i < M ? keys[i] : 0;
-        bool take = i < M && key >= kth;
-        uint64_t bm = ballot(take);
-        if (bm) {
-            uint32_t wb = 0;
-            if (lane_id() == __builtin_ctzll(bm))
-                wb = atomicAdd(&s_res[2], (uint32_t)__popcll(bm));
-            wb = (uint32_t)__shfl((int)wb, __builtin_ctzll(bm), kWave);
-            if (take) {
-                uint32_t slot = wb + lane_prefix(bm);
-                if (slot < n2)
-                    s_keys[slot] = key;
-            }
+    auto put = [&](uint64_t key, bool take) {
+        const uint64_t bm = ballot(take);
+        if (!bm)
+            return;
+        uint32_t wb = 0;
+        if (lane_id() == __builtin_ctzll(bm))
+            wb = atomicAdd(&s_res[3], (uint32_t)__popcll(bm));
+        wb = (uint32_t)__shfl((int)wb, __builtin_ctzll(bm), kWave);
+        if (take) {
+            const uint32_t slot = wb + lane_prefix(bm);
+            if (slot < n2)
+                s_keys[slot] = key;
+        }
+    };
+    if (staged) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t i = tid + j * kMergeThreads;
+            put(mine[j], i < M && mine[j] >= kth);
+        }
+    } else {
+        for (uint32_t i0 = 0; i0 < M; i0 += kMergeThreads) {
+            const uint32_t i = i0 + tid;
+            const uint64_t key = i < M ? keys[i] : 0;
+            put(key, i < M && key >= kth);
         }
     }
     __syncthreads();

@@ -82,7 +82,11 @@ struct DenseLayout {
 struct DenseArgs {
     const uint32_t* codes;  // [dense_words(D_pad)][n_pad] packed fields
     uint32_t n_pad;
-    const uint32_t* tab;    // [G][D_pad * 256] masks, bit j = query 32 g + j
+    uint32_t* tab;          // [G][D_pad * 256] masks, bit j = query 32 g + j (written by the sample pass)
+    const genie_dim_t* items;     // the batch, sorted by query
+    const uint32_t* item_qptr;    // [n_queries + 1]
+    const uint32_t* dkeys;        // [n_items][2]: first key | (query % 32) << 24, end key
+    const uint32_t* key_slot;     // [n_keys] dim * 256 + slot
     uint32_t D_pad, G, n_queries, n_objects, n_tiles;
     const uint32_t* list;   // tiles to visit (sample pass), null = every tile
     uint32_t n_list;
@@ -142,39 +146,6 @@ __global__ void k_dense_holes(const uint32_t* __restrict__ bm, uint32_t n_object
 }
 
 // ---- per run ---------------------------------------------------------------
-
-// One thread per item: set the query's bit in the mask of every key the item
-// covers (the batch was checked at upload: whole keys, unit weight, no key
-// twice per query).
-__global__ void k_dense_masks(const genie_dim_t* __restrict__ items, uint32_t n_items,
-                              const uint32_t* __restrict__ key_start, uint32_t n_keys,
-                              const uint32_t* __restrict__ key_slot, uint32_t tab_words, uint32_t* __restrict__ tab)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_items)
-        return;
-    const genie_dim_t it = items[i];
-    if (it.end_pos <= it.start_pos || it.weight == 0.0f)
-        return;
-    // key containing position `pos`: last k with key_start[k] <= pos
-    auto key_of = [&](uint32_t pos) {
-        uint32_t lo = 0, hi = n_keys;  // key_start[lo] <= pos < key_start[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (key_start[mid] <= pos)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        return lo;
-    };
-    const uint32_t ka = key_of((uint32_t)it.start_pos), kb = key_of((uint32_t)it.end_pos - 1);
-    const uint32_t q = (uint32_t)it.query;
-    uint32_t* t = tab + (uint64_t)(q >> 5) * tab_words;
-    const uint32_t bit = 1u << (q & 31);
-    for (uint32_t k = ka; k <= kb; ++k)
-        atomicOr(&t[key_slot[k]], bit);
-}
 
 // Thresholds of every query from its sample histogram (S(c) = sample
 // objects counting >= c):
@@ -411,14 +382,42 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
         if (a.redo && !a.redo[g])
             continue;  // uniform across the workgroup
         __syncthreads();
-        {
+        if constexpr (SAMPLE) {
+            // build the group's mask table from its items: one bit per
+            // (query, covered key), at the key's slot
+            uint4* dst = reinterpret_cast<uint4*>(s_tab);
+            for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
+                dst[i] = make_uint4(0, 0, 0, 0);
+            for (uint32_t i = tid; i < 32 * HB; i += kDenseThreads)
+                s_hist[i] = 0;
+            __syncthreads();
+            const uint32_t i0 = a.item_qptr[g * 32], i1 = a.item_qptr[min(g * 32 + 32, a.n_queries)];
+            for (uint32_t i = i0 + tid; i < i1; i += kDenseThreads) {
+                const uint32_t k0 = a.dkeys[2 * i], k1 = a.dkeys[2 * i + 1];
+                const uint32_t bit = 1u << (k0 >> 24);
+                // up to 16 slot loads in flight per item, then the LDS ors
+                for (uint32_t kb = k0 & 0xFFFFFFu; kb < k1; kb += 16) {
+                    uint32_t sl[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        sl[j] = kb + j < k1 ? a.key_slot[kb + j] : 0u;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j)
+                        if (kb + j < k1)
+                            atomicOr(&s_tab[sl[j]], bit);
+                }
+            }
+            __syncthreads();
+            if (x == 0 && (a.G > P || s < a.G)) {  // one workgroup per group keeps it for the main pass
+                uint4* out = reinterpret_cast<uint4*>(a.tab + (uint64_t)g * tab_words);
+                for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
+                    out[i] = dst[i];
+            }
+        } else {
             const uint4* src = reinterpret_cast<const uint4*>(a.tab + (uint64_t)g * tab_words);
             uint4* dst = reinterpret_cast<uint4*>(s_tab);
             for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
                 dst[i] = src[i];
-            if constexpr (SAMPLE)
-                for (uint32_t i = tid; i < 32 * HB; i += kDenseThreads)
-                    s_hist[i] = 0;
         }
         uint32_t tp[8] = {};
         for (uint32_t li = t_first; li < n_list; li += t_step) {
