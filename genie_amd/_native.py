@@ -129,6 +129,8 @@ _sig(_host, "genie_table_sizes", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64), 
 _sig(_host, "genie_table_export", C.c_int, _p, _p, _p, _p, _p, _p, _p, _p)
 _sig(_host, "genie_table_compile", C.c_int, _p, _p, _u64, _u32, C.c_int, _i32, _p, _u64, C.POINTER(_u64))
 _sig(_host, "genie_table_device", C.c_int, _p, C.POINTER(_p))
+_sig(_host, "genie_table_save", C.c_int, _p, C.c_char_p)
+_sig(_host, "genie_table_load", C.c_int, C.c_char_p, C.POINTER(_p))
 _sig(_host, "genie_table_knn", C.c_int, _p, _p, _u64, _u32, _u32, C.c_int, _i32, _p, _p)
 _sig(_host, "genie_table_last_error", C.c_char_p)
 
@@ -381,6 +383,17 @@ class Table:
         h = _p()
         _chk_host(_host.genie_table_build_sets(_ptr(row_ptr), _ptr(vals), len(row_ptr) - 1, max_length,
                                                int(load_balance), codec, int(via_lists), C.byref(h)))
+        return cls(h)
+
+    def save(self, path):
+        """genie::utility::SaveTable (native index file format)."""
+        _chk_host(_host.genie_table_save(self.handle, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """genie::utility::LoadTable: inv_table or inv_compr_table as saved."""
+        h = _p()
+        _chk_host(_host.genie_table_load(os.fsencode(path), C.byref(h)))
         return cls(h)
 
     def sizes(self):

@@ -3,6 +3,7 @@
 
 #include <genie/matching/knn.h>
 #include <genie/original/interface.h>
+#include <genie/utility/serialization.h>
 
 #include <memory>
 #include <new>
@@ -11,7 +12,7 @@
 #include "host_common.h"
 
 struct genie_table {
-    std::unique_ptr<genie::table::inv_table> t;
+    std::shared_ptr<genie::table::inv_table> t;
 };
 
 namespace {
@@ -181,6 +182,30 @@ int genie_table_compile(genie_table* tt, const genie_range_t* ranges, uint64_t n
         static_assert(sizeof(genie::query::Query::dim) == sizeof(genie_dim_t), "layout");
         std::copy(dims.begin(), dims.end(), reinterpret_cast<genie::query::Query::dim*>(out));
         *n_out = dims.size();
+    });
+}
+
+int genie_table_save(genie_table* tt, const char* path)
+{
+    if (!tt || !tt->t || !path)
+        return GENIE_E_INVALID;
+    return guard([&] { genie::utility::SaveTable(path, tt->t); });
+}
+
+int genie_table_load(const char* path, genie_table** out)
+{
+    if (!path || !out)
+        return GENIE_E_INVALID;
+    *out = nullptr;
+    return guard([&] {
+        auto* g = new genie_table();
+        try {
+            g->t = genie::utility::LoadTable(path);
+        } catch (...) {
+            delete g;
+            throw;
+        }
+        *out = g;
     });
 }
 

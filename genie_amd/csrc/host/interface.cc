@@ -6,6 +6,7 @@
 // matching/interface.cc:9-26).
 #include <genie/genie.h>
 #include <genie/original/interface.h>
+#include <genie/utility/serialization.h>
 
 #include <cstdio>
 #include <mutex>
@@ -224,6 +225,16 @@ SearchResult SingleValueExecutionPolicy::Match(
 }
 
 }  // namespace execution_policy
+
+std::shared_ptr<genie::table::inv_table> LoadTableFromBinary(const std::string& filename)
+{
+    return genie::utility::LoadTable(filename);
+}
+
+void SaveTableToBinary(const std::string& filename, const std::shared_ptr<const genie::table::inv_table>& table)
+{
+    genie::utility::SaveTable(filename, table);
+}
 
 TableData LoadTableDataFromCsv(const std::string& filename)
 {

@@ -5,6 +5,7 @@
 #include <genie/exception/exception.h>
 #include <genie/interface/config.h>
 #include <genie/interface/execution_policy.h>
+#include <genie/interface/io.h>
 #include <genie/interface/search.h>
 #include <genie/interface/types.h>
 

@@ -29,6 +29,9 @@ public:
     std::vector<int>* uncompressedInv() { return inv_table::inv(); }
     std::vector<int>* uncompressedInvPos() { return inv_table::inv_pos(); }
 
+    void save_fields(std::string& out) const override;
+    void load_fields(const std::string& in, size_t& at) override;
+
 protected:
     int upload(genie_index** out) override;
     void after_build(size_t max_length) override;

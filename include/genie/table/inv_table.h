@@ -12,6 +12,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -88,6 +89,28 @@ public:
     // Engine handle, uploading on first use.
     genie_index* device_index();
 
+    // Fields of the reference's sequence search (search_type 2,
+    // inv_table.h:176-199, 230): carried through save/load so that a saved
+    // table holds the reference's full field list; this build does not run
+    // sequence search.
+    unsigned int _shift_bits_subsequence() const { return shift_bits_subsequence; }
+    void set_shift_bits_subsequence(unsigned int v) { shift_bits_subsequence = v; }
+    int get_min_value_sequence() const { return min_value_sequence; }
+    void set_min_value_sequence(int v) { min_value_sequence = v; }
+    int get_max_value_sequence() const { return max_value_sequence; }
+    void set_max_value_sequence(int v) { max_value_sequence = v; }
+    int get_gram_length_sequence() const { return gram_length_sequence; }
+    void set_gram_length_sequence(int v) { gram_length_sequence = v; }
+    std::unordered_map<int, int>* get_distinct_map(int dim);
+
+    // Persistence (genie::utility::SaveTable / LoadTable): the table's fields
+    // in the order of the reference's archive (serialization.cc:68-146),
+    // appended to / read from a byte buffer at `at`; finish_load validates the
+    // layout and rebuilds the derived arrays.
+    virtual void save_fields(std::string& out) const;
+    virtual void load_fields(const std::string& in, size_t& at);
+    void finish_load(bool single_value);
+
 protected:
     status _build_status = not_builded;
     int _shifter = 16;
@@ -107,6 +130,15 @@ protected:
     uint32_t _n_dense_keys = 0;
     std::vector<uint64_t> _dense_key_start;
     std::vector<uint32_t> _dense_key_dim;
+    unsigned int shift_bits_subsequence = 0;
+    int min_value_sequence = 0;
+    int max_value_sequence = 0;
+    int gram_length_sequence = 1;
+    std::vector<std::unordered_map<int, int> > _distinct_map;
+
+    // _ck and the dense key arrays from the bounds, _inv_index_map and
+    // _inv_pos (after a load).
+    void rebuild_derived();
 
     virtual int upload(genie_index** out);
     // Runs after every build; compressed tables encode their segments here.

@@ -58,6 +58,12 @@ int genie_table_compile(genie_table* t, const genie_range_t* ranges, uint64_t n_
 /* Engine handle of the table (cpy_data_to_gpu); owned by the table. */
 int genie_table_device(genie_table* t, genie_index** out);
 
+/* Save / load a built table (genie::utility::SaveTable / LoadTable,
+ * src/genie/utility/serialization.cc:18-56; the file format is described in
+ * genie/utility/serialization.h).  A loaded table is not yet on the device. */
+int genie_table_save(genie_table* t, const char* path);
+int genie_table_load(const char* path, genie_table** out);
+
 /* Legacy search: queries from ranges, genie::original::knn_search(table,
  * queries, ids, counts, config); k slots per query. */
 int genie_table_knn(genie_table* t, const genie_range_t* ranges, uint64_t n_ranges, uint32_t n_queries, uint32_t k,

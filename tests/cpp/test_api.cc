@@ -1,15 +1,21 @@
 // The reference's own API tests, restated against this build's C++ API
 // (test/test_csv_input_sift_data.cc, test_refactoring_new_interface.cc,
 // test_csv_input_tweets.cc, test_multiple_search.cc, test_hashtable_overflow.cc,
-// test_csv_input_multiload.cc, test_csv_multiload_tweets.cc).  Counts are
+// test_csv_input_multiload.cc, test_csv_multiload_tweets.cc,
+// test_serialization.cc).  Counts are
 // the exact ones (the reference's 16s on tweets_20 are wrapped 4-bit
 // counters).  Usage: test_api <golden dir> [test name]
 #include <genie/genie.h>
 #include <genie/original/interface.h>
 
+#include <genie/table/inv_compr_table.h>
+#include <genie/utility/serialization.h>
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <memory>
 #include <map>
 #include <string>
 #include <vector>
@@ -257,6 +263,85 @@ static void test_compressed_sift()
     }
 }
 
+// test_serialization.cc:25-81: save, load, compare every field the
+// reference compares; then the loaded table answers the same queries.
+static void test_serialization()
+{
+    for (int codec : {0, 11}) {
+        vector<vector<int> > data, queries;
+        GPUGenie_Config config = base_config(data, queries, 5, 5);
+        if (codec) {
+            config.use_load_balance = true;
+            config.posting_list_max_length = 1024;
+            config.compression = (genie::compression::COMPRESSION_TYPE)codec;
+        }
+        genie::utility::read_file(data, (G + "/sift_20.csv").c_str(), -1);
+        genie::utility::read_file(queries, (G + "/sift_20.csv").c_str(), config.num_of_queries);
+        init_genie(config);
+        inv_table* raw = nullptr;
+        preprocess_for_knn_csv(config, raw);
+        CHECK(raw != nullptr && raw->build_status() == inv_table::builded);
+        std::shared_ptr<inv_table> table(raw, std::default_delete<inv_table[]>());
+        const string path = "/tmp/genie_test_api_" + std::to_string(getpid()) + "_" + std::to_string(codec) + ".gix";
+        genie::SaveTableToBinary(path, table);
+        std::shared_ptr<inv_table> loaded = genie::LoadTableFromBinary(path);
+        std::remove(path.c_str());
+        CHECK(loaded->d_inv_p == nullptr);
+        CHECK(loaded->is_stored_in_gpu == false);
+        CHECK(loaded->get_table_index() == 0);
+        CHECK(loaded->get_total_num_of_table() == 1);
+        CHECK(table->shift_bits_sequence == loaded->shift_bits_sequence);
+        CHECK(table->m_size() == loaded->m_size());
+        CHECK(table->i_size() == loaded->i_size());
+        CHECK(table->shifter() == loaded->shifter());
+        CHECK(table->_shift_bits_subsequence() == loaded->_shift_bits_subsequence());
+        CHECK(table->build_status() == loaded->build_status());
+        CHECK(*table->inv() == *loaded->inv());
+        CHECK(*table->inv_pos() == *loaded->inv_pos());
+        CHECK(*table->inv_index_map() == *loaded->inv_index_map());
+        CHECK(table->get_min_value_sequence() == loaded->get_min_value_sequence());
+        CHECK(table->get_max_value_sequence() == loaded->get_max_value_sequence());
+        CHECK(table->get_gram_length_sequence() == loaded->get_gram_length_sequence());
+        if (codec) {
+            auto* c = dynamic_cast<genie::table::inv_compr_table*>(table.get());
+            auto* cl = dynamic_cast<genie::table::inv_compr_table*>(loaded.get());
+            CHECK(c && cl);
+            if (c && cl) {
+                CHECK(*c->compressedInv() == *cl->compressedInv());
+                CHECK(*c->uncompressedInv() == *cl->uncompressedInv());
+                CHECK(*c->compressedInvPos() == *cl->compressedInvPos());
+                CHECK(*c->uncompressedInvPos() == *cl->uncompressedInvPos());
+                CHECK(c->getUncompressedPostingListMaxLength() == cl->getUncompressedPostingListMaxLength());
+                CHECK(c->getCompression() == cl->getCompression());
+            }
+        }
+        // the loaded table answers like the original (sift_20 goldens)
+        vector<int> r1, c1, r2, c2;
+        inv_table* tp = table.get();
+        inv_table* lp = loaded.get();
+        knn_search_after_preprocess(config, tp, r1, c1);
+        knn_search_after_preprocess(config, lp, r2, c2);
+        CHECK(r1 == r2 && c1 == c2);
+        CHECK(r2[0] == 0 && c2[0] == 5 && r2[1] == 4 && c2[1] == 2);
+    }
+    // a table of a multi-table set is refused, as by the reference
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 5, 5);
+    genie::utility::read_file(data, (G + "/sift_20.csv").c_str(), -1);
+    inv_table* raw = nullptr;
+    preprocess_for_knn_csv(config, raw);
+    raw->set_table_index(1);
+    raw->set_total_num_of_table(2);
+    std::shared_ptr<inv_table> t(raw, std::default_delete<inv_table[]>());
+    bool threw = false;
+    try {
+        genie::utility::SaveTable("/tmp/genie_test_api_refused.gix", t);
+    } catch (genie::exception::cpu_runtime_error&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) {
@@ -274,6 +359,7 @@ int main(int argc, char** argv)
         {"multiload_tweets", test_multiload_tweets},
         {"multiload_sift", test_multiload_sift},
         {"compressed_sift", test_compressed_sift},
+        {"serialization", test_serialization},
     };
     for (auto& kv : tests) {
         if (argc > 2 && kv.first != argv[2])

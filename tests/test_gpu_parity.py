@@ -203,3 +203,22 @@ def test_cpp_api_reference_tests():
     r = subprocess.run([exe, GOLDEN], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
+
+
+def test_loaded_tables_answer_the_same(tmp_path):
+    """SaveTable / LoadTable: the reloaded table's device index gives the same
+    results as the original, for rows, sets and a compressed table."""
+    w = W.cfg1(n=60_000, d=16, q=12, k=30)
+    rp, vals = genie_amd.synth.sets(71, 20_000, 5, 40, 2000, 1.05)
+    qp, qv = genie_amd.synth.sets(72, 6, 1, 20, 2000, 1.05)
+    sr = W.set_ranges(qp, qv, np.ones(len(qv), np.float32))
+    cases = [(Table.from_rows(w["rows"]), w["ranges"], 12),
+             (Table.from_rows(w["rows"], 1024, True, codec=13), w["ranges"], 12),
+             (Table.from_sets(rp, vals), sr, 6)]
+    for i, (t, ranges, nq) in enumerate(cases):
+        path = str(tmp_path / f"t{i}.gix")
+        t.save(path)
+        u = Table.load(path)
+        a = match_topk(t.device(), t.compile(ranges, nq), nq, 30)
+        b = match_topk(u.device(), u.compile(ranges, nq), nq, 30)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
