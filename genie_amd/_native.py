@@ -130,6 +130,11 @@ _sig(_host, "genie_table_export", C.c_int, _p, _p, _p, _p, _p, _p, _p, _p)
 _sig(_host, "genie_table_compile", C.c_int, _p, _p, _u64, _u32, C.c_int, _i32, _p, _u64, C.POINTER(_u64))
 _sig(_host, "genie_table_device", C.c_int, _p, C.POINTER(_p))
 _sig(_host, "genie_table_save", C.c_int, _p, C.c_char_p)
+_sig(_host, "genie_wire_parse_query", C.c_int, C.c_char_p, C.POINTER(_i32), C.POINTER(_u32))
+_sig(_host, "genie_wire_parse_config", C.c_int, C.c_char_p, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32),
+     C.POINTER(_i32), C.POINTER(C.c_float))
+_sig(_host, "genie_wire_search", C.c_int, C.POINTER(_p), _u32, C.c_char_p, C.c_char_p, C.c_char_p, _u64,
+     C.POINTER(_u64))
 _sig(_host, "genie_table_load", C.c_int, C.c_char_p, C.POINTER(_p))
 _sig(_host, "genie_table_knn", C.c_int, _p, _p, _u64, _u32, _u32, C.c_int, _i32, _p, _p)
 _sig(_host, "genie_table_last_error", C.c_char_p)
@@ -456,6 +461,44 @@ class Table:
             self.close()
         except Exception:
             pass
+
+
+class wire:
+    """The serving precedent's JSON documents and multi-cluster search
+    (genie/interface/wire.h)."""
+
+    @staticmethod
+    def parse_query(doc):
+        topk, n = _i32(), _u32()
+        _chk_host(_host.genie_wire_parse_query(doc.encode(), C.byref(topk), C.byref(n)))
+        return topk.value, n.value
+
+    @staticmethod
+    def parse_config(doc):
+        v = [_i32() for _ in range(4)]
+        sel = C.c_float()
+        _chk_host(_host.genie_wire_parse_config(doc.encode(), *[C.byref(x) for x in v], C.byref(sel)))
+        return dict(dim=v[0].value, radius=v[1].value, num_of_cluster=v[2].value, data_format=v[3].value,
+                    selectivity=sel.value)
+
+    @staticmethod
+    def search(tables, config_doc, query_doc):
+        """GenerateOutput lines of the query document over `tables` (one per
+        cluster): [(id, count), ...], topk per query."""
+        arr = (_p * len(tables))(*[t.handle for t in tables])
+        need = _u64()
+        cap = 1 << 16
+        while True:
+            buf = C.create_string_buffer(cap)
+            rc = _host.genie_wire_search(arr, len(tables), config_doc.encode(), query_doc.encode(), buf, cap,
+                                         C.byref(need))
+            if rc != 0 and need.value > cap:
+                cap = need.value
+                continue
+            _chk_host(rc)
+            break
+        lines = buf.value.decode().splitlines()
+        return [tuple(int(x) for x in l.split(",")) for l in lines]
 
 
 class synth:

@@ -33,7 +33,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{INC}"]
 HIP_SOURCES = ["hip/engine.hip", "hip/server.hip"]
 HOST_SOURCES = ["host/codecs.cc", "host/inv_list.cc", "host/inv_table.cc", "host/query.cc",
                 "host/knn.cc", "host/original.cc", "host/interface.cc", "host/file_reader.cc",
-                "host/capi.cc", "host/serialization.cc"]
+                "host/capi.cc", "host/serialization.cc", "host/json.cc", "host/wire.cc"]
 
 
 def _run(cmd):

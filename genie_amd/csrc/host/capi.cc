@@ -3,7 +3,10 @@
 
 #include <genie/matching/knn.h>
 #include <genie/original/interface.h>
+#include <genie/interface/wire.h>
 #include <genie/utility/serialization.h>
+
+#include <cstring>
 
 #include <memory>
 #include <new>
@@ -206,6 +209,73 @@ int genie_table_load(const char* path, genie_table** out)
             throw;
         }
         *out = g;
+    });
+}
+
+int genie_wire_parse_query(const char* json, int32_t* topk, uint32_t* n_queries)
+{
+    if (!json || !topk || !n_queries)
+        return GENIE_E_INVALID;
+    return guard([&] {
+        genie::original::GPUGenie_Config config;
+        genie::wire::QueryBatch batch;
+        std::string err;
+        if (!genie::wire::ParseQuery(json, config, batch, &err))
+            throw genie::exception::cpu_runtime_error(err);
+        *topk = batch.topk;
+        *n_queries = (uint32_t)batch.content.size();
+    });
+}
+
+int genie_wire_parse_config(const char* json, int32_t* dim, int32_t* radius, int32_t* num_of_cluster,
+                            int32_t* data_format, float* selectivity)
+{
+    if (!json)
+        return GENIE_E_INVALID;
+    return guard([&] {
+        genie::original::GPUGenie_Config config;
+        genie::wire::DistConfig extra;
+        std::string err;
+        if (!genie::wire::ParseConfiguration(json, config, extra, &err))
+            throw genie::exception::cpu_runtime_error(err);
+        if (dim)
+            *dim = config.dim;
+        if (radius)
+            *radius = config.query_radius;
+        if (num_of_cluster)
+            *num_of_cluster = extra.num_of_cluster;
+        if (data_format)
+            *data_format = extra.data_format;
+        if (selectivity)
+            *selectivity = config.selectivity;
+    });
+}
+
+int genie_wire_search(genie_table* const* tables, uint32_t n_tables, const char* config_json, const char* query_json,
+                      char* out, uint64_t cap, uint64_t* needed)
+{
+    if (!tables || !config_json || !query_json || !needed)
+        return GENIE_E_INVALID;
+    return guard([&] {
+        genie::original::GPUGenie_Config config;
+        genie::wire::DistConfig extra;
+        genie::wire::QueryBatch batch;
+        std::string err;
+        if (!genie::wire::ParseConfiguration(config_json, config, extra, &err) ||
+            !genie::wire::ParseQuery(query_json, config, batch, &err))
+            throw genie::exception::cpu_runtime_error(err);
+        std::vector<std::shared_ptr<genie::table::inv_table> > ts;
+        for (uint32_t i = 0; i < n_tables; ++i) {
+            if (!tables[i] || !tables[i]->t)
+                throw genie::exception::cpu_runtime_error("null table");
+            ts.push_back(tables[i]->t);
+        }
+        std::vector<genie::wire::Result> res = genie::wire::ExecuteMultitableQuery(config, ts, batch);
+        const std::string text = genie::wire::GenerateOutput(res, config.num_of_topk);
+        *needed = text.size() + 1;
+        if (!out || cap < text.size() + 1)
+            throw genie::exception::cpu_runtime_error("output buffer too small");
+        std::memcpy(out, text.c_str(), text.size() + 1);
     });
 }
 

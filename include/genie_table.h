@@ -64,6 +64,20 @@ int genie_table_device(genie_table* t, genie_index** out);
 int genie_table_save(genie_table* t, const char* path);
 int genie_table_load(const char* path, genie_table** out);
 
+/* The serving precedent's JSON documents (genie/interface/wire.h;
+ * src/dist_genie/parser.cu:16-217).  Invalid documents return
+ * GENIE_E_INVALID with the reference's message in genie_table_last_error(). */
+int genie_wire_parse_query(const char* json, int32_t* topk, uint32_t* n_queries);
+int genie_wire_parse_config(const char* json, int32_t* dim, int32_t* radius, int32_t* num_of_cluster,
+                            int32_t* data_format, float* selectivity);
+/* Search a query document over the clusters tables[0..n_tables) with the
+ * configuration document's settings (ExecuteMultitableQuery + MergeResult,
+ * search.cu:34-52, sorting.cc:20-64) and write GenerateOutput's "id,count"
+ * lines (file.cu:42-53) into out (NUL-terminated, cap bytes); *needed gets
+ * the bytes required.  GENIE_E_INVALID when out is too small. */
+int genie_wire_search(genie_table* const* tables, uint32_t n_tables, const char* config_json, const char* query_json,
+                      char* out, uint64_t cap, uint64_t* needed);
+
 /* Legacy search: queries from ranges, genie::original::knn_search(table,
  * queries, ids, counts, config); k slots per query. */
 int genie_table_knn(genie_table* t, const genie_range_t* ranges, uint64_t n_ranges, uint32_t n_queries, uint32_t k,

@@ -8,6 +8,7 @@
 #include <genie/genie.h>
 #include <genie/original/interface.h>
 
+#include <genie/interface/wire.h>
 #include <genie/table/inv_compr_table.h>
 #include <genie/utility/serialization.h>
 #include <unistd.h>
@@ -342,6 +343,58 @@ static void test_serialization()
     CHECK(threw);
 }
 
+// The serving precedent's path end to end: cluster files -> ReadData ->
+// query document -> ExecuteMultitableQuery -> GenerateOutput; the csv and
+// the table-file (data_format 1) forms of the same clusters answer alike.
+static void test_wire()
+{
+    vector<vector<int> > rows;
+    genie::utility::read_file(rows, (G + "/sift_20.csv").c_str(), -1);
+    const string base = "/tmp/genie_test_wire_" + std::to_string(getpid());
+    // clusters: rows 0..9 and 10..19, five dims each
+    for (int c = 0; c < 2; ++c) {
+        FILE* f = std::fopen((base + "_" + std::to_string(c) + "_0.csv").c_str(), "w");
+        for (int r = 10 * c; r < 10 * c + 10; ++r)
+            std::fprintf(f, "%d,%d,%d,%d,%d\n", rows[r][0], rows[r][1], rows[r][2], rows[r][3], rows[r][4]);
+        std::fclose(f);
+    }
+    const string cfg = "{\"data_file\": \"" + base + "\", \"dim\": 5, \"count_threshold\": 5, \"data_type\": 0, "
+                       "\"search_type\": 0, \"max_data_size\": 0, \"num_of_cluster\": 2, \"data_format\": 0, "
+                       "\"radius\": 0, \"selectivity\": 0.0}";
+    char q[512];
+    std::snprintf(q, sizeof(q),
+                  "{\"topk\": 3, \"queries\": [{\"content\": [%d, %d, %d, %d, %d], \"clusters\": [0, 1]}, "
+                  "{\"content\": [%d, %d, %d, %d, %d], \"clusters\": [1]}]}",
+                  rows[4][0], rows[4][1], rows[4][2], rows[4][3], rows[4][4], rows[12][0], rows[12][1], rows[12][2],
+                  rows[12][3], rows[12][4]);
+    string outputs[2];
+    for (int fmt = 0; fmt < 2; ++fmt) {
+        GPUGenie_Config config;
+        genie::wire::DistConfig extra;
+        genie::wire::QueryBatch batch;
+        string err;
+        CHECK(genie::wire::ParseConfiguration(cfg, config, extra, &err));
+        extra.data_format = fmt;
+        CHECK(genie::wire::ParseQuery(q, config, batch, &err));
+        auto tables = genie::wire::ReadData(config, extra);
+        CHECK(tables.size() == 2);
+        if (fmt == 0)  // keep the csv-built tables as table files for the second pass
+            for (int c = 0; c < 2; ++c)
+                genie::utility::SaveTable(base + "_" + std::to_string(c) + "_0.dat", tables[c]);
+        auto res = genie::wire::ExecuteMultitableQuery(config, tables, batch);
+        outputs[fmt] = genie::wire::GenerateOutput(res, config.num_of_topk);
+    }
+    CHECK(outputs[0] == outputs[1]);
+    // query 0 is row 4 of cluster 0: itself with count 5 first; query 1 is
+    // row 12, i.e. row 2 of cluster 1, global id 10 + 2 = 12
+    CHECK(outputs[0].rfind("4,5\n", 0) == 0);
+    CHECK(outputs[0].find("\n12,5\n") != string::npos);
+    for (int c = 0; c < 2; ++c) {
+        std::remove((base + "_" + std::to_string(c) + "_0.csv").c_str());
+        std::remove((base + "_" + std::to_string(c) + "_0.dat").c_str());
+    }
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) {
@@ -360,6 +413,7 @@ int main(int argc, char** argv)
         {"multiload_sift", test_multiload_sift},
         {"compressed_sift", test_compressed_sift},
         {"serialization", test_serialization},
+        {"wire", test_wire},
     };
     for (auto& kv : tests) {
         if (argc > 2 && kv.first != argv[2])
