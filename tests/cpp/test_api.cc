@@ -13,6 +13,7 @@
 #include <genie/utility/serialization.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
@@ -395,6 +396,103 @@ static void test_wire()
     }
 }
 
+// Adaptive query range (query.cc:196-261): ranges widened one value each
+// side until the selectivity's posting count is reached.  Known answers
+// worked out by hand from the reference's loop on a table whose dim 0 holds
+// values 0..9 with list sizes {5,0,3,2,7,1,0,4,6,2} and dim 1 values 3..9
+// with sizes {4,5,6,5,4,3,3}.  Host only: runs without a GPU.
+static void test_adaptive_range()
+{
+    const int size0[10] = {5, 0, 3, 2, 7, 1, 0, 4, 6, 2};
+    const int size1[7] = {4, 5, 6, 5, 4, 3, 3};
+    vector<int> col0, col1;
+    for (int v = 0; v < 10; ++v)
+        col0.insert(col0.end(), size0[v], v);
+    for (int v = 0; v < 7; ++v)
+        col1.insert(col1.end(), size1[v], v + 3);
+    CHECK(col0.size() == 30 && col1.size() == 30);
+    genie::table::inv_table table;
+    genie::table::inv_list l0, l1;
+    l0.invert(col0);
+    l1.invert(col1);
+    table.append(l0);
+    table.append(l1);
+    table.build(0, false);
+    struct Case {
+        int dim, value, threshold, low, up;
+    };
+    const Case cases[] = {
+        {0, 5, 12, 3, 7},   // both sides grow in step: 1 -> 8 -> 14 postings
+        {0, 1, 20, 0, 7},   // the low side stops at 0, the high side goes on
+        {0, 9, 10, 7, 9},   // nothing above the dim's max: only the low side grows
+        {1, 4, 25, 0, 8},   // below the dim's min the high side waits for low to reach 0
+        {0, 5, 40, 0, 9},   // unreachable: the whole dim
+    };
+    for (const Case& c : cases) {
+        genie::query::Query q(table, 0);
+        // selectivity s with ceil(s * 30) == threshold
+        q.attr(c.dim, c.value, 1.0f, (c.threshold - 0.5f) / 30.0f, 0);
+        q.apply_adaptive_query_range();
+        vector<genie::query::Query::range> rs;
+        q.dump(rs);
+        CHECK(rs.size() == 1);
+        if (rs.size() == 1) {
+            if (rs[0].low != c.low || rs[0].up != c.up)
+                std::printf("  dim %d value %d threshold %d: [%d, %d], expected [%d, %d]\n", c.dim, c.value,
+                            c.threshold, rs[0].low, rs[0].up, c.low, c.up);
+            CHECK(rs[0].low == c.low && rs[0].up == c.up);
+        }
+    }
+}
+
+// Multirange query file "qid dim value selectivity weight"
+// (FileReader.cc:85-130, interface.cc:427-472): weighted exact matches on
+// sift_20, checked against a brute-force count over the rows.
+static void test_multirange()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 5, 4);
+    genie::utility::read_file(data, (G + "/sift_20.csv").c_str(), -1);
+    const string path = "/tmp/genie_test_multirange_" + std::to_string(getpid()) + ".txt";
+    FILE* f = std::fopen(path.c_str(), "w");
+    // query 0: row 4 with dim 0 weighted 2; query 1: dims 1 and 3 of row 12
+    std::fprintf(f, "0 0 %d 0.0 2\n0 1 %d 0.0 1\n0 2 %d 0.0 1\n0 3 %d 0.0 1\n0 4 %d 0.0 1\n", data[4][0],
+                 data[4][1], data[4][2], data[4][3], data[4][4]);
+    std::fprintf(f, "1 1 %d 0.0 3\n1 3 %d 0.0 1\n", data[12][1], data[12][3]);
+    std::fclose(f);
+    vector<genie::utility::attr_t> attrs;
+    genie::utility::read_query(attrs, path.c_str(), -1);
+    std::remove(path.c_str());
+    CHECK(attrs.size() == 7);
+    config.use_multirange = true;
+    config.multirange_query_points = &attrs;
+    config.num_of_queries = 2;
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    delete[] table;
+    // brute force: per query the weighted number of matching dims
+    for (int q = 0; q < 2; ++q) {
+        vector<std::pair<int, int> > score;  // (-count, id)
+        for (int r = 0; r < (int)data.size(); ++r) {
+            int c = 0;
+            for (const auto& a : attrs)
+                if (a.qid == q && data[r][a.dim] == a.value)
+                    c += (int)a.weight;
+            if (c > 0)
+                score.push_back({-c, r});
+        }
+        std::sort(score.begin(), score.end());
+        for (int i = 0; i < 4; ++i) {
+            const int want_id = i < (int)score.size() ? score[i].second : 0;
+            const int want_c = i < (int)score.size() ? -score[i].first : 0;
+            CHECK(result[q * 4 + i] == want_id && count[q * 4 + i] == want_c);
+        }
+    }
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 2) {
@@ -414,6 +512,8 @@ int main(int argc, char** argv)
         {"compressed_sift", test_compressed_sift},
         {"serialization", test_serialization},
         {"wire", test_wire},
+        {"adaptive_range", test_adaptive_range},
+        {"multirange", test_multirange},
     };
     for (auto& kv : tests) {
         if (argc > 2 && kv.first != argv[2])

@@ -118,3 +118,14 @@ def test_synth_is_deterministic():
     # Zipf(1.1) over 256 values: the most frequent value holds about a fifth
     top = np.bincount(a[:, 0], minlength=256).max() / 1000
     assert 0.12 < top < 0.3
+
+
+def test_adaptive_query_range_known_answers():
+    """apply_adaptive_query_range (query.cc:196-261) on hand-worked cases;
+    the C++ test driver's host-only case."""
+    import subprocess
+    from conftest import GOLDEN
+    exe = os.path.join(ROOT, "genie_amd", "lib", "test_api")
+    r = subprocess.run([exe, GOLDEN, "adaptive_range"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and "[  OK  ] adaptive_range" in r.stdout, r.stdout
