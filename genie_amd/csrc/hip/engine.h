@@ -66,6 +66,10 @@ struct DevIndex {
 
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
+// Checks a batch's items against the index: query ids, spans inside the
+// posting (or compressed word) array, integral weights in [0, 65535],
+// compressed spans on segment boundaries.
+int validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries);
 
 }  // namespace gh
 

@@ -315,7 +315,7 @@ static int compute_bounds(const DevIndex& d, const genie_dim_t* items, uint64_t 
     return GENIE_OK;
 }
 
-static int validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries)
+int gh::validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries)
 {
     const uint64_t limit = d.codec == 0 ? d.n_postings : d.n_words;
     for (uint64_t i = 0; i < n_items; ++i) {

@@ -342,8 +342,7 @@ int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t 
     r->k = k;
     r->mem = request_bytes(n_items, k);
     // validate alone so that one bad request cannot fail a whole batch
-    uint32_t bound = 0;
-    if (int rc = genie_hip_count_bounds(s->ix, r->items.data(), n_items, 1, &bound))
+    if (int rc = gh::validate_items(s->ix->d, r->items.data(), n_items, 1))
         return rc;
     r->t_submit = Clock::now();
     {
