@@ -62,6 +62,7 @@ struct DevIndex {
     std::vector<uint32_t> h_seg_cpos;
     std::vector<uint32_t> h_seg_key;
     std::vector<uint32_t> h_key_seg;
+    std::vector<uint64_t> h_key_wstart;  // [n_keys + 1] first word of each key (compressed)
 };
 
 void set_error(const std::string& msg);

@@ -340,6 +340,9 @@ int gh::validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_i
     return GENIE_OK;
 }
 
+static bool dense_shape(const genie_index_desc* desc, std::vector<uint32_t>& key_base, std::vector<uint32_t>& per_dim);
+static int build_dense(const genie_index_desc* desc, DevIndex& d, const uint32_t* post, const uint32_t* key_start);
+
 // Compressed index: words, segment offsets, key -> segment map, per-segment
 // first/last ids (one device decode pass that also validates every segment)
 // and the segment directory of long keys.
@@ -384,6 +387,9 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
     d.h_seg_cpos.assign(cp, cp + S + 1);
     d.h_seg_key = seg_key;
     d.h_key_seg = key_seg;
+    d.h_key_wstart.resize(K + 1);
+    for (uint32_t k = 0; k <= K; ++k)
+        d.h_key_wstart[k] = cp[key_seg[k]];
     if (desc->postings) {
         for (uint32_t k = 0; k < K; ++k) {
             uint32_t best = 1, run = 1;
@@ -404,9 +410,27 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         delete ix;
         return fail(GENIE_E_INVALID, "segment directory too large");
     }
+    // a table that can have the dense form is also decoded once into a
+    // temporary posting array, from which its forward codes are built
+    std::vector<uint32_t> kbase, kper;
+    const bool want_dense = dense_shape(desc, kbase, kper);
+    std::vector<uint32_t> ks32;
+    if (want_dense) {
+        ks32.resize(K + 1);
+        for (uint32_t k = 0; k <= K; ++k)
+            ks32[k] = (uint32_t)desc->key_start[k];
+    }
     int rc = GENIE_OK;
-    uint32_t *d_upos = nullptr, *d_dir_keys = nullptr, *d_err = nullptr;
+    uint32_t *d_upos = nullptr, *d_dir_keys = nullptr, *d_err = nullptr, *d_post = nullptr, *d_ks = nullptr;
     do {
+        if (want_dense) {
+            if ((rc = dev_alloc(&d_post, d.n_postings))) break;
+            if ((rc = dev_alloc(&d_ks, K + 1))) break;
+            if (hipError_t e = hipMemcpy(d_ks, ks32.data(), (K + 1) * 4ull, hipMemcpyHostToDevice)) {
+                rc = fail(GENIE_E_RUNTIME, std::string("index upload: ") + hipGetErrorString(e));
+                break;
+            }
+        }
         if ((rc = dev_alloc(&d.words, d.n_words, &d.bytes))) break;
         if ((rc = dev_alloc(&d.seg_cpos, S + 1, &d.bytes))) break;
         if ((rc = dev_alloc(&d.seg_key, S, &d.bytes))) break;
@@ -433,7 +457,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         }
         const uint32_t blocks = std::min<uint32_t>((S + kDecodeWaves - 1) / kDecodeWaves, 65535);
         k_seg_bounds<<<blocks, kDecodeWaves * kWave>>>(d.codec, d.words, d.seg_cpos, d_upos, S, d.n_objects,
-                                                        d.seg_first, d.seg_last, d_err);
+                                                        d.seg_first, d.seg_last, d_err, d_post);
         uint32_t err = 0;
         if ((e = hipGetLastError()) || (e = hipDeviceSynchronize()) ||
             (e = hipMemcpy(&err, d_err, 4, hipMemcpyDeviceToHost))) {
@@ -465,10 +489,14 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
                 break;
             }
         }
+        if (want_dense)
+            rc = build_dense(desc, d, d_post, d_ks);
     } while (0);
     dev_free(d_upos);
     dev_free(d_dir_keys);
     dev_free(d_err);
+    dev_free(d_post);
+    dev_free(d_ks);
     if (rc) {
         genie_hip_index_destroy(ix);
         return rc;
@@ -659,30 +687,45 @@ static int launch_batched(genie_plan* p, hipStream_t s)
     }
 }
 
-// Forward codes for the dense form: only for a single-value table of at most
-// kDenseMaxDims dims with at most 256 keys per dim, keys ordered by dim, in
-// which (checked on the device) every object appears exactly once per dim.
-// Any other index simply has no dense form; only device errors fail.
-static int build_dense(const genie_index_desc* desc, DevIndex& d)
+// Whether an index can have the dense form, from its key table alone: a
+// single-value table of at most kDenseMaxDims dims with at most 256 keys per
+// dim, keys ordered by dim, and N postings in every dim.  Fills the first key
+// and the key count of every dim.
+static bool dense_shape(const genie_index_desc* desc, std::vector<uint32_t>& key_base, std::vector<uint32_t>& per_dim)
 {
     const uint32_t D = desc->n_dims, N = desc->n_objects, K = desc->n_keys;
-    if (!desc->single_value || D == 0 || D > kDenseMaxDims || !desc->postings)
-        return GENIE_OK;
-    std::vector<uint32_t> key_base(D, 0xFFFFFFFFu), per_dim(D, 0);
+    if (!desc->single_value || D == 0 || D > kDenseMaxDims)
+        return false;
+    key_base.assign(D, 0xFFFFFFFFu);
+    per_dim.assign(D, 0);
     std::vector<uint64_t> dim_post(D, 0);
     for (uint32_t k = 0; k < K; ++k) {
         const uint32_t dm = desc->key_dim[k];
         if (dm >= D || (k && dm < desc->key_dim[k - 1]))
-            return GENIE_OK;
+            return false;
         if (key_base[dm] == 0xFFFFFFFFu)
             key_base[dm] = k;
         if (++per_dim[dm] > kDenseKeysPerDim)
-            return GENIE_OK;
+            return false;
         dim_post[dm] += desc->key_start[k + 1] - desc->key_start[k];
     }
     for (uint32_t dm = 0; dm < D; ++dm)
         if (dim_post[dm] != N)
-            return GENIE_OK;
+            return false;
+    return true;
+}
+
+// Forward codes for the dense form, from the uncompressed posting lists on
+// the device (`post`, `key_start` as 32-bit positions), for an index that
+// passes dense_shape and in which (checked on the device) every object
+// appears exactly once per dim.  Any other index simply has no dense form;
+// only device errors fail.
+static int build_dense(const genie_index_desc* desc, DevIndex& d, const uint32_t* post, const uint32_t* key_start)
+{
+    const uint32_t D = desc->n_dims, N = desc->n_objects, K = desc->n_keys;
+    std::vector<uint32_t> key_base, per_dim;
+    if (!post || !dense_shape(desc, key_base, per_dim))
+        return GENIE_OK;
     // slot of a key within its dim: rank by list length, longest first, so
     // the 32 most frequent keys of a dim sit in 32 different LDS banks
     std::vector<uint32_t> slot(K);
@@ -732,7 +775,7 @@ static int build_dense(const genie_index_desc* desc, DevIndex& d)
             rc = fail(GENIE_E_RUNTIME, std::string("dense codes: ") + hipGetErrorString(e));
             break;
         }
-        k_dense_codes<<<K, 256>>>(d.post, d.key_start, dim, key_slot, d_word, d_shift, npad, bm_words, codes, bm,
+        k_dense_codes<<<K, 256>>>(post, key_start, dim, key_slot, d_word, d_shift, npad, bm_words, codes, bm,
                                   flag);
         k_dense_holes<<<(N + 255) / 256, 256>>>(bm, N, D, bm_words, flag);
         if ((e = hipGetLastError()) || (e = hipMemcpy(&h_flag, flag, 4, hipMemcpyDeviceToHost))) {
@@ -907,7 +950,7 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
     } while (0);
     dev_free(d_dir_keys);
     if (!rc)
-        rc = build_dense(desc, d);
+        rc = build_dense(desc, d, d.post, d.key_start);
     if (rc) {
         genie_hip_index_destroy(ix);
         return rc;
@@ -1083,19 +1126,29 @@ static int plan_size_candidates(genie_plan* p)
 }
 
 // Whether the uploaded batch (kept on the host by mode-1 plans) can run the
-// dense form: every item with postings and a nonzero weight covers whole keys
-// with weight 1, and no query covers a key twice.
-// The key range of every item is kept for the device-side mask build
-// (h_dkeys: first key | query % 32 << 24, end key; empty for skipped items).
+// dense form: every item with postings and a nonzero weight has weight 1, a
+// query's items never overlap, and each run of a query's items that follow
+// one another without a gap covers whole keys (a run is one item of a plain
+// batch, the pieces of a load-balanced range, or the segments of a range of
+// a compressed index).
+// The key range of every run is kept for the device-side mask build on the
+// run's first item (h_dkeys: first key | query % 32 << 24, end key; empty for
+// every other item).
 static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
 {
     const DevIndex& d = p->ix->d;
     if (p->mode == 0 || !d.dense)
         return false;
-    const uint64_t* ks = d.h_key_start.data();
+    // items of a compressed index span words: keys start at their first
+    // segment's first word
+    const uint64_t* ks = d.codec == 0 ? d.h_key_start.data() : d.h_key_wstart.data();
     const uint32_t nk = d.n_keys;
     p->h_dkeys.assign(2 * p->h_items.size(), 0);
-    std::vector<std::pair<uint32_t, uint32_t>> iv;
+    struct Span {
+        uint64_t sp, ep;
+        uint32_t item;
+    };
+    std::vector<Span> iv;
     for (uint32_t q = 0; q < n_queries; ++q) {
         iv.clear();
         for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
@@ -1104,19 +1157,25 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
                 continue;
             if (it.weight != 1.0f)
                 return false;
-            const uint64_t sp = (uint64_t)it.start_pos, ep = (uint64_t)it.end_pos;
-            const uint32_t ka = (uint32_t)(std::upper_bound(ks, ks + nk + 1, sp) - ks - 1);
-            const uint32_t kb = (uint32_t)(std::lower_bound(ks, ks + nk + 1, ep) - ks);
-            if (ks[ka] != sp || kb > nk || ks[kb] != ep)
-                return false;
-            iv.push_back({ka, kb});
-            p->h_dkeys[2 * i] = ka | (q & 31u) << 24;  // keys < 2^15 in a dense index
-            p->h_dkeys[2 * i + 1] = kb;
+            iv.push_back({(uint64_t)it.start_pos, (uint64_t)it.end_pos, i});
         }
-        std::sort(iv.begin(), iv.end());
-        for (size_t i = 1; i < iv.size(); ++i)
-            if (iv[i].first < iv[i - 1].second)
+        std::sort(iv.begin(), iv.end(), [](const Span& x, const Span& y) { return x.sp < y.sp; });
+        for (size_t r = 0; r < iv.size();) {
+            size_t e = r + 1;
+            uint64_t end = iv[r].ep;
+            while (e < iv.size() && iv[e].sp == end)
+                end = iv[e++].ep;
+            if (e < iv.size() && iv[e].sp < end)
+                return false;  // overlapping items
+            const uint64_t sp = iv[r].sp;
+            const uint32_t ka = (uint32_t)(std::upper_bound(ks, ks + nk + 1, sp) - ks - 1);
+            const uint32_t kb = (uint32_t)(std::lower_bound(ks, ks + nk + 1, end) - ks);
+            if (ks[ka] != sp || kb > nk || ks[kb] != end)
                 return false;
+            p->h_dkeys[2 * iv[r].item] = ka | (q & 31u) << 24;  // keys < 2^15 in a dense index
+            p->h_dkeys[2 * iv[r].item + 1] = kb;
+            r = e;
+        }
     }
     return true;
 }

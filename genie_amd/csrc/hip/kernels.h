@@ -134,11 +134,14 @@ __global__ void k_build_dir(const uint32_t* __restrict__ post, const uint32_t* _
 constexpr int kDecodeWaves = 4;
 
 // Decode every segment once: first and last id, decoded length check
-// (against the uncompressed segment positions) and ascending order.
+// (against the uncompressed segment positions) and ascending order.  With
+// `post` set, the decoded ids of a valid segment are also written to
+// post[upos[s] .. upos[s + 1]) (the uncompressed posting array).
 __global__ void __launch_bounds__(kDecodeWaves * kWave)
     k_seg_bounds(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
                  const uint32_t* __restrict__ upos, uint32_t n_seg, uint32_t n_objects,
-                 uint32_t* __restrict__ first, uint32_t* __restrict__ last, uint32_t* __restrict__ err)
+                 uint32_t* __restrict__ first, uint32_t* __restrict__ last, uint32_t* __restrict__ err,
+                 uint32_t* __restrict__ post)
 {
     __shared__ DecodeScratch S[kDecodeWaves];
     const int wv = threadIdx.x / kWave, lane = lane_id();
@@ -179,6 +182,14 @@ __global__ void __launch_bounds__(kDecodeWaves * kWave)
                 first[sgi] = v[0];
             if (lane == (int)((n - 1) / kPerLane))
                 last[sgi] = v[(n - 1) % kPerLane];
+            if (post) {
+                const uint32_t base = upos[sgi];
+                for (int j = 0; j < kPerLane; ++j) {
+                    const uint32_t i = lane * kPerLane + j;
+                    if (i < n)
+                        post[base + i] = v[j];
+                }
+            }
         }
         wave_sync();
     }

@@ -147,8 +147,9 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
  * index is a single-value table of at most 128 dims with at most 256 keys per
  * dim, and every item of the uploaded batch covers whole keys with weight 1
  * (never one key twice for a query), the batch is counted from per-object key
- * codes built at index creation, 32 queries per LDS lookup with bit-sliced
- * counters, instead of from the posting lists; results are identical.
+ * codes built at index creation (a compressed table is decoded once on the
+ * device for this), 32 queries per LDS lookup with bit-sliced counters,
+ * instead of from the posting lists; results are identical.
  * No counterpart in the reference (its matching always walks the lists,
  * match.cu:161-220).  on = 0 forces the lane form. */
 int genie_hip_plan_set_dense(genie_plan* p, int on);

@@ -141,11 +141,6 @@ def test_dense_not_used_where_it_cannot_be_exact():
     assert not p.dense_active()
     ri, rc = oracle_single(w["rows"], r, 10, 20)
     assert_topk_equal(b[0], b[1], ri, rc, 20, 10, "weights")
-    # load-balanced items split keys
-    (b, p) = run(t, t.compile(w["ranges"], 10, load_balance=True, max_load=64), 10, 20)
-    assert not p.dense_active()
-    ri, rc = oracle_single(w["rows"], w["ranges"], 10, 20)
-    assert_topk_equal(b[0], b[1], ri, rc, 20, 10, "load balance")
     # a query covering keys twice (ranges repeated on the same dims)
     r2 = np.concatenate([w["ranges"], w["ranges"][:3]])
     r2 = r2[np.argsort(r2["query"], kind="stable")]
@@ -173,13 +168,68 @@ def test_dense_not_built_for_other_indexes():
     ts = Table.from_sets(rp, vals)
     (b, p) = run(ts, ts.compile(sr, 6), 6, 10)
     assert not p.dense_active()
-    # compressed table
-    wc = W.cfg1(n=30_000, d=16, q=6, k=10)
-    tc = Table.from_rows(wc["rows"], 1024, True, codec=11)
-    (b, p) = run(tc, tc.compile(wc["ranges"], 6), 6, 10)
+
+
+def plain(table, items, nq, k):
+    kq = np.full(nq, k, np.uint32)
+    p = Plan(table.device(), nq, max(1, len(items)), k, 0, mode=0)
+    p.upload(items, nq, kq)
+    p.run()
+    return p.download()
+
+
+def test_dense_load_balanced_pieces():
+    """Load-balanced items split a range into pieces of max_load postings;
+    the pieces of one range together cover whole keys."""
+    w = W.cfg1(n=60_000, d=24, q=10, k=20)
+    t = Table.from_rows(w["rows"])
+    (b, p) = run(t, t.compile(w["ranges"], 10, load_balance=True, max_load=64), 10, 20)
+    assert p.dense_active()
+    ri, rc = oracle_single(w["rows"], w["ranges"], 10, 20)
+    assert_topk_equal(b[0], b[1], ri, rc, 20, 10, "load balance")
+
+
+@pytest.mark.parametrize("codec", [1, 3, 6, 11, 13])
+def test_dense_compressed_equals_oracle(codec):
+    """A compressed single-value table is decoded once at index creation for
+    its codes; a batch's items (one per segment) run the dense form."""
+    w = W.cfg1(n=120_000, d=24, q=40, k=50)
+    t = Table.from_rows(w["rows"], 1024, True, codec=codec)
+    items = t.compile(w["ranges"], 40)
+    (b, p) = run(t, items, 40, 50)
+    assert p.dense_active()
+    ri, rc = oracle_single(w["rows"], w["ranges"], 40, 50)
+    assert_topk_equal(b[0], b[1], ri, rc, 50, 40, f"compressed codec {codec}")
+
+
+def test_dense_not_used_for_part_of_a_key():
+    w = W.cfg1(n=60_000, d=16, q=8, k=20)
+    # uncompressed: one item shortened by a posting
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 8).copy()
+    i = int(np.argmax(items["end_pos"] - items["start_pos"]))
+    items["end_pos"][i] -= 1
+    (b, p) = run(t, items, 8, 20)
     assert not p.dense_active()
-    ri, rc = oracle_single(wc["rows"], wc["ranges"], 6, 10)
-    assert_topk_equal(b[0], b[1], ri, rc, 10, 6, "compressed")
+    a = plain(t, items, 8, 20)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # compressed: one segment of a key left out (4 values per dim: every key
+    # has about 60 segments of 256 ids, so no segment is a whole key)
+    rows = genie_amd.synth.rows(93, 60_000, 16, genie_amd.synth.UNIFORM, 0.0, 0.0, 0, 3, 4)
+    r = np.zeros(8 * 16, RANGE_DTYPE)
+    r["query"] = np.repeat(np.arange(8), 16)
+    r["dim"] = np.tile(np.arange(16), 8)
+    r["low"] = r["up"] = rows[:8].reshape(-1)
+    r["weight"] = 1
+    r["order"] = r["dim"]
+    tc = Table.from_rows(rows, 256, True, codec=11)
+    items = tc.compile(r, 8)
+    assert len(items) > 50 * 8 * 16
+    items = np.delete(items, len(items) // 2)
+    (b, p) = run(tc, items, 8, 20)
+    assert not p.dense_active()
+    a = plain(tc, items, 8, 20)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
 def test_dense_speculative_threshold_redo():
