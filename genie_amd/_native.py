@@ -77,6 +77,8 @@ _sig(_hip, "genie_hip_plan_get_config", C.c_int, _p, C.POINTER(_u32), C.POINTER(
 _sig(_hip, "genie_hip_plan_autotune", C.c_int, _p, C.c_int, _p, C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_set_dense", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_dense_active", C.c_int, _p, C.POINTER(C.c_int))
+_sig(_hip, "genie_hip_plan_set_sparse", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_sparse_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_plan_download_async", C.c_int, _p, _p, _p, _p, _p)
 _sig(_hip, "genie_hip_host_alloc", C.c_int, C.POINTER(_p), _u64)
@@ -284,6 +286,16 @@ class Plan:
     def dense_active(self):
         v = C.c_int()
         _chk_hip(_hip.genie_hip_plan_dense_active(self.handle, C.byref(v)))
+        return bool(v.value)
+
+    def set_sparse(self, mode=-1):
+        """Plain form: -1 chooses the sparse select from the batch's postings
+        per (query, tile), 0 never, 1 always (results are identical)."""
+        _chk_hip(_hip.genie_hip_plan_set_sparse(self.handle, int(mode)))
+
+    def sparse_active(self):
+        v = C.c_int()
+        _chk_hip(_hip.genie_hip_plan_sparse_active(self.handle, C.byref(v)))
         return bool(v.value)
 
     def autotune(self, reps=2):

@@ -98,6 +98,10 @@ struct genie_plan {
     uint64_t n_items = 0, n_chunks = 0, total_k = 0, total_cap = 0;
     uint32_t cap_factor = 8;
     uint64_t logical_bytes = 0, compulsory_bytes = 0;
+    // plain form: postings of the batch (estimated from words when
+    // compressed), which choose the sparse select
+    uint64_t est_postings = 0;
+    int sparse_mode = -1;  // -1 automatic, 0 off, 1 on
     // device buffers
     genie_dim_t* d_items = nullptr;
     uint32_t* d_item_qptr = nullptr;
@@ -1306,12 +1310,13 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
         }
     }
     // chunks of the plain and lane forms
-    uint64_t nch = 0;
+    uint64_t nch = 0, span_total = 0;
     bool prev_ok = false;
     for (uint64_t i = 0; i < n_items; ++i) {
         const genie_dim_t& it = p->h_items[i];
         if (it.end_pos > it.start_pos) {
             uint64_t len = (uint64_t)(it.end_pos - it.start_pos);
+            span_total += len;
             if (d.codec == 0) {
                 nch += (len + kChunk - 1) / kChunk;
             } else {
@@ -1343,6 +1348,9 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     p->n_queries = n_queries;
     p->n_items = n_items;
     p->n_chunks = nch;
+    p->est_postings = d.codec == 0 || d.n_words == 0
+                          ? span_total
+                          : (uint64_t)((double)span_total * (double)d.n_postings / (double)d.n_words);
     p->h_kq.assign(k_per_query, k_per_query + n_queries);
     p->h_out_off.assign(n_queries + 1, 0);
     for (uint32_t q = 0; q < n_queries; ++q)
@@ -1409,26 +1417,46 @@ static ScanArgs scan_args(genie_plan* p)
     return a;
 }
 
+// Whether the plain form marks touched counter words and selects sparsely:
+// on average fewer postings per (query, tile) than a quarter of the tile's
+// counter words (each tile still falls back to the dense select when it
+// turns out busier, kSparseWords).
+static bool plain_sparse(const genie_plan* p, uint32_t n_tiles)
+{
+    if (p->sparse_mode >= 0)
+        return p->sparse_mode != 0;
+    const double per = (double)p->est_postings / ((double)p->n_queries * (double)n_tiles);
+    return per <= (double)kCntWords / 4;
+}
+
+template <bool SPARSE>
+static void launch_scan_t(const genie_plan* p, const ScanArgs& a, uint32_t blocks, hipStream_t s)
+{
+    if (p->ix->d.codec == 0) {
+        switch (p->lane_bits) {
+        case 8: k_scan_tile<8, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
+        case 16: k_scan_tile<16, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
+        default: k_scan_tile<32, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
+        }
+    } else {
+        switch (p->lane_bits) {
+        case 8: k_scan_tile_c<8, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
+        case 16: k_scan_tile_c<16, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
+        default: k_scan_tile_c<32, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
+        }
+    }
+}
+
 static int launch_scan(genie_plan* p, hipStream_t s)
 {
-    const DevIndex& d = p->ix->d;
     const ScanArgs a = scan_args(p);
     const uint64_t blocks = (uint64_t)a.n_tiles * p->n_queries;
     if (blocks >= 0x7FFFFFFFull)
         return fail(GENIE_E_INVALID, "batch too large for one launch");
-    if (d.codec == 0) {
-        switch (p->lane_bits) {
-        case 8: k_scan_tile<8><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-        case 16: k_scan_tile<16><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-        default: k_scan_tile<32><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-        }
-    } else {
-        switch (p->lane_bits) {
-        case 8: k_scan_tile_c<8><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-        case 16: k_scan_tile_c<16><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-        default: k_scan_tile_c<32><<<(uint32_t)blocks, kScanThreads, 0, s>>>(a); break;
-        }
-    }
+    if (plain_sparse(p, a.n_tiles))
+        launch_scan_t<true>(p, a, (uint32_t)blocks, s);
+    else
+        launch_scan_t<false>(p, a, (uint32_t)blocks, s);
     return GENIE_OK;
 }
 
@@ -1471,7 +1499,7 @@ static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s)
 static int run_dense(genie_plan* p, hipStream_t s)
 {
     const DevIndex& d = p->ix->d;
-    const uint32_t nq = p->n_queries, ni = (uint32_t)p->n_items;
+    const uint32_t nq = p->n_queries;
     fill_u32(p->d_dhist, 0, (uint64_t)nq * kDenseBins, s);
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[1], s));
@@ -1714,6 +1742,27 @@ int genie_hip_plan_set_dense(genie_plan* p, int on)
         return GENIE_OK;
     p->dense_on = on != 0;
     return plan_settle_form(p);
+}
+
+int genie_hip_plan_set_sparse(genie_plan* p, int mode)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (mode < -1 || mode > 1)
+        return fail(GENIE_E_INVALID, "sparse select mode must be -1, 0 or 1");
+    plan_drop_graph(p);
+    p->sparse_mode = mode;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_sparse_active(genie_plan* p, int* active)
+{
+    if (!p || !active)
+        return fail(GENIE_E_INVALID, "null argument");
+    const DevIndex& d = p->ix->d;
+    const uint32_t to = tile_objs_for(p->lane_bits);
+    *active = p->n_queries && plain_sparse(p, (d.n_objects + to - 1) / to) ? 1 : 0;
+    return GENIE_OK;
 }
 
 int genie_hip_plan_dense_active(genie_plan* p, int* active)

@@ -480,9 +480,26 @@ __device__ __forceinline__ uint64_t make_key(uint32_t count, uint32_t id)
 // the end are clamped to the last group and their ids are not counted);
 // the unaligned head and tail go through single-id loads.  FILTER: the span
 // may hold ids outside the tile (short lists without a directory entry).
-template <int LB, bool FILTER>
+// Touched-word bitmap of the sparse select: counter word c of the tile is
+// bit (c >> 6) & 31 of bitmap word (c >> 11) * 64 + (c & 63), so that wave v
+// lane l of the select owns bitmap word 64 v + l, i.e. the counter words
+// 2048 v + 64 i + l (i = 0..31): the words the dense passes give that lane,
+// always in LDS bank l.
+constexpr int kTouchWords = kCntWords / 32;
+static_assert(kTouchWords == kScanThreads, "one bitmap word per select thread");
+// A tile with at most this many touched counter words takes the sparse
+// select; a busier one (a query with long lists in a sparse batch) the dense.
+constexpr uint32_t kSparseWords = kCntWords / 8;
+
+__device__ __forceinline__ void mark_touched(uint32_t* bm, uint32_t word)
+{
+    atomicOr(&bm[((word >> 11) << 6) | (word & 63u)], 1u << ((word >> 6) & 31u));
+}
+
+// MARK: the first increment of a counter word also sets its bit in `bm`.
+template <int LB, bool FILTER, bool MARK = false>
 __device__ __forceinline__ void count_one(uint32_t id, uint32_t obj0, uint32_t tile_objs, uint32_t w,
-                                          uint32_t* cnt)
+                                          uint32_t* cnt, uint32_t* bm = nullptr)
 {
     constexpr int PER = 32 / LB;
     constexpr int SH = PER == 4 ? 2 : (PER == 2 ? 1 : 0);
@@ -491,25 +508,33 @@ __device__ __forceinline__ void count_one(uint32_t id, uint32_t obj0, uint32_t t
     // per-lane scratch word past the counters: no branch, no shared address.
     const bool ok = !FILTER || loc < tile_objs;
     const uint32_t word = ok ? (loc >> SH) : (uint32_t)(kCntWords + lane_id());
-    atomicAdd(&cnt[word], ok ? w << ((loc & (PER - 1)) * LB) : 0u);
+    const uint32_t inc = ok ? w << ((loc & (PER - 1)) * LB) : 0u;
+    if constexpr (MARK) {
+        const uint32_t old = atomicAdd(&cnt[word], inc);
+        if (old == 0 && inc != 0 && word < (uint32_t)kCntWords)
+            mark_touched(bm, word);
+    } else {
+        atomicAdd(&cnt[word], inc);
+    }
 }
 
-template <int LB, bool FILTER>
+template <int LB, bool FILTER, bool MARK = false>
 __device__ __forceinline__ void count_span(const uint32_t* __restrict__ post, uint32_t a, uint32_t b,
-                                           uint32_t obj0, uint32_t tile_objs, uint32_t w, uint32_t* cnt)
+                                           uint32_t obj0, uint32_t tile_objs, uint32_t w, uint32_t* cnt,
+                                           uint32_t* bm = nullptr)
 {
     const int lane = lane_id();
     const uint32_t a4 = (a + 3u) & ~3u;        // first 16-byte aligned position
     const uint32_t b4 = b & ~3u;               // end of the aligned body
     if (a4 >= b4) {                            // short span: one id per lane
         for (uint32_t p = a + lane; p < b; p += kWave)
-            count_one<LB, FILTER>(post[p], obj0, tile_objs, w, cnt);
+            count_one<LB, FILTER, MARK>(post[p], obj0, tile_objs, w, cnt, bm);
         return;
     }
     if (lane < (int)(a4 - a))
-        count_one<LB, FILTER>(post[a + lane], obj0, tile_objs, w, cnt);
+        count_one<LB, FILTER, MARK>(post[a + lane], obj0, tile_objs, w, cnt, bm);
     if (lane < (int)(b - b4))
-        count_one<LB, FILTER>(post[b4 + lane], obj0, tile_objs, w, cnt);
+        count_one<LB, FILTER, MARK>(post[b4 + lane], obj0, tile_objs, w, cnt, bm);
     const uint4* body = reinterpret_cast<const uint4*>(post + a4);
     const uint32_t ng = (b4 - a4) >> 2;        // 16-byte groups, >= 1
     for (uint32_t g0 = 0; g0 < ng; g0 += 4 * kWave) {
@@ -523,10 +548,10 @@ __device__ __forceinline__ void count_span(const uint32_t* __restrict__ post, ui
             // branching, which would let the compiler sink their load
             const bool in = g0 + u * kWave + lane < ng;
             const uint32_t past = obj0 + tile_objs;  // an id just outside the tile
-            count_one<LB, true>(in ? v[u].x : past, obj0, tile_objs, w, cnt);
-            count_one<LB, true>(in ? v[u].y : past, obj0, tile_objs, w, cnt);
-            count_one<LB, true>(in ? v[u].z : past, obj0, tile_objs, w, cnt);
-            count_one<LB, true>(in ? v[u].w : past, obj0, tile_objs, w, cnt);
+            count_one<LB, true, MARK>(in ? v[u].x : past, obj0, tile_objs, w, cnt, bm);
+            count_one<LB, true, MARK>(in ? v[u].y : past, obj0, tile_objs, w, cnt, bm);
+            count_one<LB, true, MARK>(in ? v[u].z : past, obj0, tile_objs, w, cnt, bm);
+            count_one<LB, true, MARK>(in ? v[u].w : past, obj0, tile_objs, w, cnt, bm);
         }
     }
 }
@@ -744,6 +769,205 @@ __device__ __forceinline__ void tile_select(const ScanArgs& a, const Acc& acc, u
     }
 }
 
+// Sparse form of tile_select for the plain kernels' packed counters: the same
+// threshold search and the same emitted keys, but every pass visits only the
+// counter words marked in `bm` (kTouchWords bitmap words; the lane owning
+// bitmap word 64 v + l handles counter words 2048 v + 64 i + l).  Unmarked
+// words hold only zero counts, which no pass would use (every window, cut
+// and tie count is >= 1).  Passes over a lane's marked words run per lane;
+// the ordered tie emission keeps the dense passes' word order.
+template <int LB>
+__device__ __forceinline__ void tile_select_sparse(const ScanArgs& a, const uint32_t* cnt, const uint32_t* bm,
+                                                   uint32_t q, uint32_t obj0, SelShared& S)
+{
+    constexpr int PER = 32 / LB;
+    constexpr uint32_t kWpw = kCntWords / kScanWaves;  // counter words per wave
+    static_assert(kWpw == 64 * 32, "one bitmap word per lane covers 32 counter words");
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    uint32_t* s_hist = S.hist;
+    uint32_t* s_wv = S.wv;
+    uint32_t* s_res = S.res;
+    const uint32_t k = a.kq[q];
+    const uint32_t mine = bm[tid];
+    const uint32_t* cw = cnt + wv * kWpw + lane;  // word i of this lane: cw[64 * i]
+
+    uint32_t m = 0;
+    for (uint32_t b = mine; b; b &= b - 1) {
+        const uint32_t v = cw[64 * __builtin_ctz(b)];
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            m = max(m, lane_count<LB>(v, j));
+    }
+    m = wave_max(m);
+    if (lane == 0)
+        s_wv[wv] = m;
+    if (tid == 0)
+        s_res[6] = __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    uint32_t M = 0;
+    for (int i = 0; i < kScanWaves; ++i)
+        M = max(M, s_wv[i]);
+    const uint32_t lower = max(1u, s_res[6]);
+    if (M < lower)
+        return;
+
+    uint32_t top = M, cum = 0, ct = lower, ngt = 0, neq = 0, found = 0;
+    for (;;) {
+        const uint32_t wlo = top >= lower + 63 ? top - 63 : lower;
+        __syncthreads();
+        if (tid < 64)
+            s_hist[tid] = 0;
+        __syncthreads();
+        for (uint32_t b = mine; b; b &= b - 1) {
+            const uint32_t v = cw[64 * __builtin_ctz(b)];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t c = lane_count<LB>(v, j);
+                if (c >= wlo && c <= top)
+                    atomicAdd(&s_hist[c - wlo], 1u);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t f = 0, cc = wlo, g = cum, e = 0;
+            for (uint32_t c = top + 1; c-- > wlo;) {
+                const uint32_t h = s_hist[c - wlo];
+                if (g + h >= k) {
+                    f = 1;
+                    cc = c;
+                    e = h;
+                    break;
+                }
+                g += h;
+            }
+            s_res[0] = f;
+            s_res[1] = cc;
+            s_res[2] = g;
+            s_res[3] = e;
+        }
+        __syncthreads();
+        found = s_res[0];
+        cum = s_res[2];
+        if (found) {
+            ct = s_res[1];
+            ngt = cum;
+            neq = s_res[3];
+            break;
+        }
+        if (wlo == lower) {
+            ct = lower;
+            ngt = cum;
+            neq = 0;
+            break;
+        }
+        top = wlo - 1;
+    }
+
+    const uint32_t need = found ? k - ngt : 0;
+    const bool trim = found && neq > need;
+    const uint32_t nemit = found ? ngt + min(neq, need) : ngt;
+    if (tid == 0) {
+        uint32_t base = atomicAdd(&a.cand_cnt[q], nemit);
+        if ((uint64_t)base + nemit > a.cand_cap[q]) {
+            atomicOr(a.overflow, 1u);
+            base = kEmitFail;
+        }
+        if (found)
+            atomicMax(&a.gthr[q], ct);
+        s_res[4] = base;
+        s_res[5] = 0;
+    }
+    __syncthreads();
+    const uint32_t base = s_res[4];
+    if (base == kEmitFail || nemit == 0)
+        return;
+    uint64_t* out = a.cand + a.cand_base[q] + base;
+
+    // unordered emission of the keys at or above the cut
+    const uint32_t cut = trim ? ct + 1 : ct;
+    for (uint32_t b = mine; b; b &= b - 1) {
+        const uint32_t i = (uint32_t)__builtin_ctz(b);
+        const uint32_t v = cw[64 * i];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const uint32_t c = lane_count<LB>(v, j);
+            if (c >= cut) {
+                const uint32_t slot = atomicAdd(&s_res[5], 1u);
+                const uint32_t id = obj0 + (wv * kWpw + 64 * i + lane) * PER + j;
+                out[slot] = make_key(c, id);
+            }
+        }
+    }
+    if (!trim)
+        return;
+
+    // Ordered emission of the `need` smallest ids tied at ct, in the word
+    // order of the dense passes (wave ranges, then i, then lane).
+    uint32_t e = 0;
+    for (uint32_t b = mine; b; b &= b - 1) {
+        const uint32_t v = cw[64 * __builtin_ctz(b)];
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            e += lane_count<LB>(v, j) == ct;
+    }
+    e = wave_sum(e);
+    __syncthreads();
+    if (lane == 0)
+        s_wv[wv] = e;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int i = 0; i < wv; ++i)
+        run += s_wv[i];
+    uint32_t any = mine;  // iterations where some lane of the wave has a marked word
+    for (int off = 32; off > 0; off >>= 1)
+        any |= (uint32_t)__shfl_xor((int)any, off, kWave);
+    for (uint32_t bb = any; bb && run < need; bb &= bb - 1) {
+        const uint32_t i = (uint32_t)__builtin_ctz(bb);
+        const uint32_t v = (mine >> i) & 1u ? cw[64 * i] : 0u;
+        uint32_t mask = 0;
+#pragma unroll
+        for (int j = 0; j < PER; ++j)
+            mask |= (uint32_t)(lane_count<LB>(v, j) == ct) << j;
+        const uint32_t pc = (uint32_t)__popc(mask);
+        const uint32_t incl = wave_incl_scan(pc);
+        uint32_t r = run + incl - pc;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (mask & (1u << j)) {
+                if (r < need) {
+                    const uint32_t id = obj0 + (wv * kWpw + 64 * i + lane) * PER + j;
+                    out[ngt + r] = make_key(ct, id);
+                }
+                ++r;
+            }
+        }
+        run += (uint32_t)__shfl((int)incl, 63, kWave);
+    }
+}
+
+// Select of a plain kernel: sparse when few counter words were touched.
+template <int LB, bool SPARSE>
+__device__ __forceinline__ void plain_select(const ScanArgs& a, const uint32_t* cnt, const uint32_t* bm, uint32_t q,
+                                             uint32_t obj0, SelShared& S)
+{
+    if constexpr (SPARSE) {
+        const int lane = lane_id(), wv = threadIdx.x >> 6;
+        const uint32_t pc = wave_sum((uint32_t)__popc(bm[threadIdx.x]));
+        if (lane == 0)
+            S.wv[wv] = pc;
+        __syncthreads();
+        uint32_t touched = 0;
+        for (int i = 0; i < kScanWaves; ++i)
+            touched += S.wv[i];
+        __syncthreads();  // S.wv is reused by the select
+        if (touched <= kSparseWords) {
+            tile_select_sparse<LB>(a, cnt, bm, q, obj0, S);
+            return;
+        }
+    }
+    tile_select(a, PackedAcc<LB>{cnt, (uint32_t)kCntWords}, q, obj0, S);
+}
+
 // Sub-span of one (chunk, key) pair inside the workgroup's tile.
 struct Span {
     uint32_t lo, hi;   // posting positions
@@ -751,7 +975,10 @@ struct Span {
 };
 constexpr int kSpanCap = 1024;
 
-template <int LB>
+// SPARSE: counting marks the touched counter words and the select visits
+// only those (tile_select_sparse); for batches whose queries hold few
+// postings per tile.
+template <int LB, bool SPARSE>
 __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
 {
     __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];  // counters + per-lane scratch
@@ -759,16 +986,21 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
     __shared__ Span s_span[kSpanCap];
     __shared__ uint32_t s_next;
     __shared__ SelShared s_sel;
+    __shared__ uint32_t s_touch[SPARSE ? kTouchWords : 1];
+    __shared__ uint32_t s_wsum[kScanWaves];
 
     const uint32_t t = blockIdx.x / a.n_queries;
     const uint32_t q = blockIdx.x % a.n_queries;
     const uint32_t obj0 = t * a.tile_objs;
     const uint32_t dt0 = t * a.dt_per_tile;
     const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
-    const int tid = threadIdx.x, lane = lane_id();
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
 
     for (int i = tid; i < kCntWords / 4; i += kScanThreads)
         s_cnt4[i] = make_uint4(0, 0, 0, 0);
+    if constexpr (SPARSE)
+        for (int i = tid; i < kTouchWords; i += kScanThreads)
+            s_touch[i] = 0;
 
     // ---- count ------------------------------------------------------------
     // Rounds of up to kSpanCap (chunk, key) pairs: every thread resolves the
@@ -809,19 +1041,69 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
             }
         }
         __syncthreads();
-        for (;;) {
-            uint32_t si = 0;
-            if (lane == 0)
-                si = atomicAdd(&s_next, 1u);
-            si = bcast_first(__shfl((int)si, 0, kWave));
-            if (si >= wn)
-                break;
-            const Span sp = s_span[si];
-            if (sp.lo < sp.hi) {
-                if (sp.wf & 1)
-                    count_span<LB, true>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
-                else
-                    count_span<LB, false>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
+        if constexpr (SPARSE) {
+            // Few postings per span: flatten the window.  A prefix of span
+            // lengths replaces each span's end (2 spans per thread), then
+            // every thread takes postings g = tid, tid + 512, ... four loads
+            // at a time, each finding its span by binary search.
+            const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+            const uint32_t n0 = i0 < wn ? s_span[i0].hi - s_span[i0].lo : 0u;
+            const uint32_t n1 = i1 < wn ? s_span[i1].hi - s_span[i1].lo : 0u;
+            const uint32_t incl = wave_incl_scan(n0 + n1);
+            if (lane == kWave - 1)
+                s_wsum[wv] = incl;
+            __syncthreads();
+            uint32_t off = 0, total = 0;
+            for (int i = 0; i < kScanWaves; ++i) {
+                off += i < wv ? s_wsum[i] : 0u;
+                total += s_wsum[i];
+            }
+            const uint32_t ex = off + incl - (n0 + n1);
+            if (i0 < wn)
+                s_span[i0].hi = ex;  // hi now holds the span's first flat index
+            if (i1 < wn)
+                s_span[i1].hi = ex + n0;
+            __syncthreads();
+            for (uint32_t g0 = 0; g0 < total; g0 += 4 * kScanThreads) {
+                uint32_t id[4], w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t g = g0 + u * kScanThreads + tid;
+                    id[u] = obj0 + a.tile_objs;  // outside the tile: counts nothing
+                    w[u] = 0;
+                    if (g < total) {
+                        uint32_t lo = 0, hi = wn;  // last span whose first index is <= g
+                        while (hi - lo > 1) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s_span[mid].hi <= g)
+                                lo = mid;
+                            else
+                                hi = mid;
+                        }
+                        const Span sp = s_span[lo];
+                        id[u] = a.post[sp.lo + (g - sp.hi)];
+                        w[u] = sp.wf >> 1;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    count_one<LB, true, true>(id[u], obj0, a.tile_objs, w[u], cnt, s_touch);
+            }
+        } else {
+            for (;;) {
+                uint32_t si = 0;
+                if (lane == 0)
+                    si = atomicAdd(&s_next, 1u);
+                si = bcast_first(__shfl((int)si, 0, kWave));
+                if (si >= wn)
+                    break;
+                const Span sp = s_span[si];
+                if (sp.lo < sp.hi) {
+                    if (sp.wf & 1)
+                        count_span<LB, true>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
+                    else
+                        count_span<LB, false>(a.post, sp.lo, sp.hi, obj0, a.tile_objs, sp.wf >> 1, cnt);
+                }
             }
         }
         __syncthreads();
@@ -829,14 +1111,14 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
     if (npairs == 0)
         __syncthreads();  // the zeroing above must finish before the select passes
 
-    tile_select(a, PackedAcc<LB>{cnt, (uint32_t)kCntWords}, q, obj0, s_sel);
+    plain_select<LB, SPARSE>(a, cnt, s_touch, q, obj0, s_sel);
 }
 
 // Compressed form of k_scan_tile: pairs resolve to ranges of compressed
 // segments overlapping the tile (segment directory for long keys), the waves
 // take single segments, stage and decode them (codecs_dev.h) and count the
 // decoded ids that fall inside the tile.
-template <int LB>
+template <int LB, bool SPARSE>
 __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
 {
     __shared__ uint4 s_cnt4[kCntWords / 4 + kWave / 4];
@@ -847,6 +1129,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
     __shared__ uint32_t s_next;
     __shared__ uint32_t s_wsum[kScanWaves];
     __shared__ SelShared s_sel;
+    __shared__ uint32_t s_touch[SPARSE ? kTouchWords : 1];
 
     const uint32_t t = blockIdx.x / a.n_queries;
     const uint32_t q = blockIdx.x % a.n_queries;
@@ -858,6 +1141,9 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
 
     for (int i = tid; i < kCntWords / 4; i += kScanThreads)
         s_cnt4[i] = make_uint4(0, 0, 0, 0);
+    if constexpr (SPARSE)
+        for (int i = tid; i < kTouchWords; i += kScanThreads)
+            s_touch[i] = 0;
 
     const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
     const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
@@ -943,7 +1229,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
 #pragma unroll
             for (int j = 0; j < kPerLane; ++j) {
                 const uint32_t i = lane * kPerLane + j;
-                count_one<LB, true>(i < n ? v[j] : obj1, obj0, a.tile_objs, w, cnt);
+                count_one<LB, true, SPARSE>(i < n ? v[j] : obj1, obj0, a.tile_objs, w, cnt, s_touch);
             }
             wave_sync();  // the staging buffer is reused by the next segment
         }
@@ -951,7 +1237,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
     }
     if (npairs == 0)
         __syncthreads();
-    tile_select(a, PackedAcc<LB>{cnt, (uint32_t)kCntWords}, q, obj0, s_sel);
+    plain_select<LB, SPARSE>(a, cnt, s_touch, q, obj0, s_sel);
 }
 
 // ---------------------------------------------------------------------------

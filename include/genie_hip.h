@@ -155,6 +155,17 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
 int genie_hip_plan_set_dense(genie_plan* p, int on);
 /* Whether the last uploaded batch runs the dense form. */
 int genie_hip_plan_dense_active(genie_plan* p, int* active);
+/* Sparse select of the plain form: counting marks the counter words a
+ * (query, tile) touches and the per-tile top-k visits only those.  mode -1
+ * (default) chooses it when the batch averages fewer postings per (query,
+ * tile) than a quarter of the tile's counter words; 0 never, 1 always.  A
+ * tile that touched more than an eighth of its counter words takes the
+ * dense select either way.  Results are identical.  The reference has no
+ * counterpart: its per-query c-PQ hash tables hold only the ids a query
+ * touches (match.cu:332-341, grown on overflow at 397-404). */
+int genie_hip_plan_set_sparse(genie_plan* p, int mode);
+/* Whether the plain form of the last uploaded batch selects sparsely. */
+int genie_hip_plan_sparse_active(genie_plan* p, int* active);
 /* Host time of the last batched work-queue build (upload), ms. */
 int genie_hip_plan_host_ms(genie_plan* p, double* ms);
 /* Algorithmic bytes of the last uploaded batch: logical = 4 B x sum of

@@ -38,12 +38,17 @@ def oracle_sets(row_ptr, vals, ranges, n_queries, k):
 
 def engine(table, items, n_queries, k, mode):
     """mode "auto": genie_hip_match_topk (dense form when the batch allows,
-    else plain); 0: an explicit plain plan, so the per-query scan kernel
-    itself is checked whatever the batch."""
+    else plain); "plain-sparse" / "plain-dense": an explicit plain plan with
+    the sparse select forced on / off, so the per-query scan kernel and both
+    of its selects are checked whatever the batch; other values: a plan of
+    that mode."""
     if mode == "auto":
         return match_topk(table.device(), items, n_queries, k)
     kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (n_queries,)))
-    p = Plan(table.device(), n_queries, max(1, len(items)), int(kq.max()), 0, mode=mode)
+    pm = 0 if mode in ("plain-sparse", "plain-dense") else mode
+    p = Plan(table.device(), n_queries, max(1, len(items)), int(kq.max()), 0, mode=pm)
+    if mode in ("plain-sparse", "plain-dense"):
+        p.set_sparse(1 if mode == "plain-sparse" else 0)
     p.upload(items, n_queries, kq)
     p.run()
     return p.download()
@@ -56,7 +61,7 @@ def run_engine(table, ranges, n_queries, k, lb=False, max_load=0, shuffle=False,
     return engine(table, items, n_queries, k, mode)
 
 
-MODES = [0, "auto"]
+MODES = ["plain-sparse", "plain-dense", "auto"]
 
 
 @pytest.mark.parametrize("mode", MODES)
