@@ -79,6 +79,8 @@ _sig(_hip, "genie_hip_plan_set_dense", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_dense_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_set_sparse", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_sparse_active", C.c_int, _p, C.POINTER(C.c_int))
+_sig(_hip, "genie_hip_plan_set_hash", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_hash_queries", C.c_int, _p, C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_plan_download_async", C.c_int, _p, _p, _p, _p, _p)
 _sig(_hip, "genie_hip_host_alloc", C.c_int, C.POINTER(_p), _u64)
@@ -297,6 +299,16 @@ class Plan:
         v = C.c_int()
         _chk_hip(_hip.genie_hip_plan_sparse_active(self.handle, C.byref(v)))
         return bool(v.value)
+
+    def set_hash(self, mode=-1):
+        """Plain form: -1 runs selective queries on the LDS hash-table form,
+        0 never, 1 every query whose counts fit (results are identical)."""
+        _chk_hip(_hip.genie_hip_plan_set_hash(self.handle, int(mode)))
+
+    def hash_queries(self):
+        v = _u32()
+        _chk_hip(_hip.genie_hip_plan_hash_queries(self.handle, C.byref(v)))
+        return int(v.value)
 
     def autotune(self, reps=2):
         ms = np.full(16, -1.0)

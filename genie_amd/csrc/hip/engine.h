@@ -63,6 +63,7 @@ struct DevIndex {
     std::vector<uint32_t> h_seg_key;
     std::vector<uint32_t> h_key_seg;
     std::vector<uint64_t> h_key_wstart;  // [n_keys + 1] first word of each key (compressed)
+    std::vector<uint32_t> h_seg_upos;    // [n_segments + 1] first posting of each segment (compressed)
 };
 
 void set_error(const std::string& msg);

@@ -33,6 +33,7 @@
 #include "kernels.h"
 #include "kernels_batched.h"
 #include "kernels_dense.h"
+#include "kernels_hash.h"
 #include "wave.h"
 
 namespace gh {
@@ -98,10 +99,26 @@ struct genie_plan {
     uint64_t n_items = 0, n_chunks = 0, total_k = 0, total_cap = 0;
     uint32_t cap_factor = 8;
     uint64_t logical_bytes = 0, compulsory_bytes = 0;
-    // plain form: postings of the batch (estimated from words when
-    // compressed), which choose the sparse select
-    uint64_t est_postings = 0;
-    int sparse_mode = -1;  // -1 automatic, 0 off, 1 on
+    // plain form: per query, the hash form (selective queries) or the
+    // direct-mapped kernels; the direct queries' postings choose the sparse
+    // select
+    uint64_t est_postings = 0;  // postings of the direct queries
+    int sparse_mode = -1;       // -1 automatic, 0 off, 1 on
+    int hash_mode = -1;         // -1 automatic, 0 off, 1 every query whose counts fit
+    std::vector<uint64_t> h_qpost;        // postings of every query
+    std::vector<uint32_t> h_qbound;       // count bound of every query
+    std::vector<uint8_t> h_force_direct;  // queries whose table filled up in an earlier run
+    std::vector<uint32_t> h_hwork;        // (query, tile) of the hash kernel
+    std::vector<uint4> h_hq;              // per query: directory tiles per tile, key bits, log2 slots
+    std::vector<uint32_t> h_dq;           // queries of the direct-mapped kernels
+    uint32_t max_hash_tiles = 0;
+    bool plain_stats = false;             // h_qpost / h_qbound hold this batch's figures
+    uint32_t* d_hwork = nullptr;
+    uint64_t hwork_alloc = 0;
+    uint4* d_hq = nullptr;
+    uint32_t* d_dq = nullptr;
+    uint32_t* d_hov = nullptr;
+    uint32_t* d_hov_any = nullptr;
     // device buffers
     genie_dim_t* d_items = nullptr;
     uint32_t* d_item_qptr = nullptr;
@@ -227,6 +244,11 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_gsafe);
     dev_free(p->d_redo);
     dev_free(p->d_dkeys);
+    dev_free(p->d_hwork);
+    dev_free(p->d_hq);
+    dev_free(p->d_dq);
+    dev_free(p->d_hov);
+    dev_free(p->d_hov_any);
 }
 
 // Count bound of every query: per (query, order) group the largest weight
@@ -389,6 +411,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
     d.h_key_dim.assign(desc->key_dim, desc->key_dim + K);
     d.h_key_mult.assign(K, 1);
     d.h_seg_cpos.assign(cp, cp + S + 1);
+    d.h_seg_upos.assign(up, up + S + 1);
     d.h_seg_key = seg_key;
     d.h_key_seg = key_seg;
     d.h_key_wstart.resize(K + 1);
@@ -1036,6 +1059,10 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         if ((rc = dev_alloc(&p->d_overflow, 1))) break;
         if ((rc = dev_alloc(&p->d_out_off, max_queries + 1))) break;
         if ((rc = dev_alloc(&p->d_queue, 1))) break;
+        if ((rc = dev_alloc(&p->d_hq, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_dq, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_hov, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_hov_any, 1))) break;
         if (hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking) != hipSuccess) {
             rc = fail(GENIE_E_RUNTIME, "hipStreamCreate");
             break;
@@ -1092,7 +1119,7 @@ static uint32_t plan_n_tiles(const genie_plan* p)
 static uint32_t max_cap_factor(const genie_plan* p)
 {
     if (!dense_active(p))
-        return plan_n_tiles(p);
+        return std::max(plan_n_tiles(p), p->mode == 1 ? 0u : p->max_hash_tiles);
     uint32_t kmin = 0xFFFFFFFFu;
     for (uint32_t q = 0; q < p->n_queries; ++q)
         kmin = std::min(kmin, p->h_kq[q]);
@@ -1110,8 +1137,16 @@ static int plan_size_candidates(genie_plan* p)
     std::vector<uint32_t> cap(p->n_queries);
     std::vector<uint64_t> base(p->n_queries);
     uint64_t tot = 0;
+    const bool plain = !dn && p->mode != 1;
     for (uint32_t q = 0; q < p->n_queries; ++q) {
-        const uint64_t limit = dn ? (uint64_t)p->ix->d.n_objects : (uint64_t)n_tiles * p->h_kq[q];
+        // the dense form emits each object at most once; the tile forms at
+        // most k per tile (a hash-form query: per tile of its own)
+        uint64_t tiles = n_tiles;
+        if (plain && q < p->h_hq.size() && p->h_hq[q].x) {
+            const uint64_t tobj = (uint64_t)p->h_hq[q].x * kDirTile;
+            tiles = (p->ix->d.n_objects + tobj - 1) / tobj;
+        }
+        const uint64_t limit = dn ? (uint64_t)p->ix->d.n_objects : tiles * p->h_kq[q];
         cap[q] = (uint32_t)std::min<uint64_t>((uint64_t)f * p->h_kq[q], limit);
         base[q] = tot;
         tot += cap[q];
@@ -1245,6 +1280,113 @@ static int ensure_lanes(genie_plan* p)
     return GENIE_OK;
 }
 
+// Per-query figures of the plain form: count bounds (unless the plan has a
+// fixed bound) and postings (from the segment offsets when compressed).
+static void plain_query_stats(genie_plan* p)
+{
+    if (p->plain_stats)
+        return;
+    const DevIndex& d = p->ix->d;
+    const uint32_t nq = p->n_queries;
+    if (p->count_bound != 0) {
+        p->h_qbound.assign(nq, p->count_bound);
+    } else if (p->h_qbound.size() != nq || p->dense_ok) {  // upload skips the bounds of a dense batch
+        p->h_qbound.resize(nq);
+        compute_bounds(d, p->h_items.data(), p->n_items, nq, p->h_qbound.data());
+    }
+    p->h_qpost.assign(nq, 0);
+    const uint32_t* cp = d.h_seg_cpos.data();
+    for (uint32_t q = 0; q < nq; ++q) {
+        uint64_t P = 0;
+        for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
+            const genie_dim_t& it = p->h_items[i];
+            if (it.end_pos <= it.start_pos)
+                continue;
+            if (d.codec == 0) {
+                P += (uint64_t)(it.end_pos - it.start_pos);
+            } else {  // compressed spans start and end on segment boundaries
+                const uint32_t s0 = (uint32_t)(std::lower_bound(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) - cp);
+                const uint32_t s1 = (uint32_t)(std::lower_bound(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos) - cp);
+                P += d.h_seg_upos[s1] - d.h_seg_upos[s0];
+            }
+        }
+        p->h_qpost[q] = P;
+    }
+    p->plain_stats = true;
+}
+
+// The plain form's split of the batch: a query with at most N / 16 postings
+// (every query whose counts fit, when the hash form is forced) runs the hash
+// kernel over tiles of whole directory tiles holding about kHashTarget of its
+// postings each, with 2x that many table slots; the others, and queries
+// whose table filled up in an earlier run, run the direct-mapped kernels.
+// Key bits: key = tile-local id + 1, so tiles shrink until the counts fit in
+// the remaining bits.
+static int plan_split_plain(genie_plan* p)
+{
+    plain_query_stats(p);
+    const DevIndex& d = p->ix->d;
+    const uint32_t nq = p->n_queries;
+    const uint64_t N = d.n_objects;
+    p->h_hwork.clear();
+    p->h_dq.clear();
+    p->h_hq.assign(nq, make_uint4(0, 0, 0, 0));
+    p->max_hash_tiles = 0;
+    p->est_postings = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint64_t P = p->h_qpost[q];
+        bool hashed = false;
+        if (p->hash_mode != 0 && !p->h_force_direct[q] && (p->hash_mode == 1 || P * 16 <= N)) {
+            const uint64_t nt0 = std::max<uint64_t>(1, (P + kHashTarget - 1) / kHashTarget);
+            uint64_t dtpt = std::max<uint64_t>(1, ((N + nt0 - 1) / nt0 + kDirTile - 1) / kDirTile);
+            uint32_t kb = 0;
+            for (;;) {
+                const uint64_t tobj = dtpt * kDirTile;
+                kb = 64u - (uint32_t)__builtin_clzll(tobj);  // bits of tobj (keys are 1..tobj)
+                if (kb < 32 && p->h_qbound[q] < (1u << (32 - kb)))
+                    break;
+                if (dtpt == 1) {
+                    dtpt = 0;
+                    break;
+                }
+                dtpt = (dtpt + 1) / 2;
+            }
+            if (dtpt) {
+                const uint64_t tobj = dtpt * kDirTile;
+                const uint64_t nt = (N + tobj - 1) / tobj;
+                const uint64_t per = (P + nt - 1) / nt;
+                uint32_t lhs = 9;
+                while (lhs < (uint32_t)kHashLogMax && (1ull << lhs) < 2 * per)
+                    ++lhs;
+                p->h_hq[q] = make_uint4((uint32_t)dtpt, kb, lhs, 0);
+                for (uint64_t t = 0; t < nt; ++t) {
+                    p->h_hwork.push_back(q);
+                    p->h_hwork.push_back((uint32_t)t);
+                }
+                p->max_hash_tiles = std::max(p->max_hash_tiles, (uint32_t)nt);
+                hashed = true;
+            }
+        }
+        if (!hashed) {
+            p->h_dq.push_back(q);
+            p->est_postings += P;
+        }
+    }
+    if (p->h_hwork.size() > p->hwork_alloc) {
+        dev_free(p->d_hwork);
+        p->hwork_alloc = 0;
+        if (int rc = dev_alloc(&p->d_hwork, p->h_hwork.size()))
+            return rc;
+        p->hwork_alloc = p->h_hwork.size();
+    }
+    if (!p->h_hwork.empty())
+        HIP_TRY(hipMemcpy(p->d_hwork, p->h_hwork.data(), p->h_hwork.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_hq, p->h_hq.data(), nq * sizeof(uint4), hipMemcpyHostToDevice));
+    if (!p->h_dq.empty())
+        HIP_TRY(hipMemcpy(p->d_dq, p->h_dq.data(), p->h_dq.size() * 4, hipMemcpyHostToDevice));
+    return GENIE_OK;
+}
+
 // After a change of form or configuration: the active form's buffers and a
 // fresh candidate space.
 static int plan_settle_form(genie_plan* p)
@@ -1257,6 +1399,9 @@ static int plan_settle_form(genie_plan* p)
             return rc;
     } else if (p->mode == 1) {
         if (int rc = ensure_lanes(p))
+            return rc;
+    } else {
+        if (int rc = plan_split_plain(p))
             return rc;
     }
     p->cap_factor = initial_cap_factor(p);
@@ -1296,15 +1441,21 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     p->h_qptr.swap(qptr);
     p->bytes_valid = false;
     p->dense_ok = dense_batch_ok(p, n_queries);
+    p->plain_stats = false;
+    p->h_force_direct.assign(n_queries, 0);
+    p->h_hwork.clear();
+    p->h_dq.clear();
+    p->h_hq.clear();
+    p->max_hash_tiles = 0;
     // count bound of this batch
     if (p->count_bound == 0) {
         if (p->dense_ok) {
             p->lane_bits = 8;  // unit weights, each dim once: counts <= 128
         } else {
-            std::vector<uint32_t> b(n_queries);
-            compute_bounds(d, p->h_items.data(), n_items, n_queries, b.data());
+            p->h_qbound.resize(n_queries);
+            compute_bounds(d, p->h_items.data(), n_items, n_queries, p->h_qbound.data());
             uint32_t mx = 0;
-            for (uint32_t v : b)
+            for (uint32_t v : p->h_qbound)
                 mx = std::max(mx, v);
             p->lane_bits = bits_for(mx);
         }
@@ -1376,6 +1527,8 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
         p->lanes_built = false;
         return plan_settle_form(p);
     }
+    if (int rc = plan_split_plain(p))
+        return rc;
     p->cap_factor = initial_cap_factor(p);
     return plan_size_candidates(p);
 }
@@ -1402,6 +1555,8 @@ static ScanArgs scan_args(genie_plan* p)
     a.qc_ptr = p->d_qc_ptr;
     a.qk_ptr = p->d_qk_ptr;
     a.kq = p->d_kq;
+    a.qlist = nullptr;
+    a.n_qlist = 0;
     a.n_objects = d.n_objects;
     a.n_queries = p->n_queries;
     a.tile_objs = tile_objs_for(p->lane_bits);
@@ -1425,7 +1580,9 @@ static bool plain_sparse(const genie_plan* p, uint32_t n_tiles)
 {
     if (p->sparse_mode >= 0)
         return p->sparse_mode != 0;
-    const double per = (double)p->est_postings / ((double)p->n_queries * (double)n_tiles);
+    if (p->h_dq.empty())
+        return false;
+    const double per = (double)p->est_postings / ((double)p->h_dq.size() * (double)n_tiles);
     return per <= (double)kCntWords / 4;
 }
 
@@ -1447,16 +1604,36 @@ static void launch_scan_t(const genie_plan* p, const ScanArgs& a, uint32_t block
     }
 }
 
+// The plain form's scan: the hash kernel over the (query, tile) work of the
+// selective queries, then the direct-mapped kernel over the other queries.
 static int launch_scan(genie_plan* p, hipStream_t s)
 {
-    const ScanArgs a = scan_args(p);
-    const uint64_t blocks = (uint64_t)a.n_tiles * p->n_queries;
-    if (blocks >= 0x7FFFFFFFull)
+    ScanArgs a = scan_args(p);
+    const uint64_t nwork = p->h_hwork.size() / 2;
+    const uint64_t ndq = p->h_dq.size();
+    const uint64_t blocks = (uint64_t)a.n_tiles * ndq;
+    if (blocks >= 0x7FFFFFFFull || nwork >= 0x7FFFFFFFull)
         return fail(GENIE_E_INVALID, "batch too large for one launch");
-    if (plain_sparse(p, a.n_tiles))
-        launch_scan_t<true>(p, a, (uint32_t)blocks, s);
-    else
-        launch_scan_t<false>(p, a, (uint32_t)blocks, s);
+    if (nwork) {
+        HashArgs h;
+        h.s = a;
+        h.work = p->d_hwork;
+        h.hq = p->d_hq;
+        h.hov = p->d_hov;
+        h.hov_any = p->d_hov_any;
+        if (p->ix->d.codec == 0)
+            k_scan_hash<false><<<(uint32_t)nwork, kScanThreads, 0, s>>>(h);
+        else
+            k_scan_hash<true><<<(uint32_t)nwork, kScanThreads, 0, s>>>(h);
+    }
+    if (ndq) {
+        a.qlist = ndq == p->n_queries ? nullptr : p->d_dq;
+        a.n_qlist = (uint32_t)ndq;
+        if (plain_sparse(p, a.n_tiles))
+            launch_scan_t<true>(p, a, (uint32_t)blocks, s);
+        else
+            launch_scan_t<false>(p, a, (uint32_t)blocks, s);
+    }
     return GENIE_OK;
 }
 
@@ -1554,18 +1731,34 @@ static int plan_run_once(genie_plan* p, hipStream_t s);
 // Candidate overflow after a completed run on stream s: grow the capacity
 // and run again (the reference retries on hash-table overflow in the same
 // way, match.cu:356-421).
+// A hash-form query whose table filled up is run on the direct-mapped form
+// instead, in the same way.
 static int settle_overflow(genie_plan* p, hipStream_t s)
 {
     for (int attempt = 0;; ++attempt) {
-        uint32_t ov = 0;
+        uint32_t ov = 0, hv = 0;
         HIP_TRY(hipMemcpy(&ov, p->d_overflow, 4, hipMemcpyDeviceToHost));
-        if (!ov)
+        HIP_TRY(hipMemcpy(&hv, p->d_hov_any, 4, hipMemcpyDeviceToHost));
+        if (!ov && !hv)
             return GENIE_OK;
-        const uint32_t fmax = max_cap_factor(p);
-        if (p->cap_factor >= fmax || attempt > 12)
+        if (attempt > 12)
             return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
-        p->cap_factor = (uint32_t)std::min<uint64_t>((uint64_t)p->cap_factor * 4, fmax);
         plan_drop_graph(p);
+        if (hv) {
+            std::vector<uint32_t> flags(p->n_queries);
+            HIP_TRY(hipMemcpy(flags.data(), p->d_hov, p->n_queries * 4ull, hipMemcpyDeviceToHost));
+            for (uint32_t q = 0; q < p->n_queries; ++q)
+                if (flags[q])
+                    p->h_force_direct[q] = 1;
+            if (int rc = plan_split_plain(p))
+                return rc;
+        }
+        if (ov) {
+            const uint32_t fmax = max_cap_factor(p);
+            if (p->cap_factor >= fmax)
+                return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
+            p->cap_factor = (uint32_t)std::min<uint64_t>((uint64_t)p->cap_factor * 4, fmax);
+        }
         if (int rc = plan_size_candidates(p))
             return rc;
         if (int rc = plan_run_once(p, s))
@@ -1584,6 +1777,8 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     fill_u32(p->d_gthr, 0, nq, s);
     fill_u32(p->d_cand_cnt, 0, nq, s);
     fill_u32(p->d_overflow, 0, 1, s);
+    fill_u32(p->d_hov, 0, nq, s);
+    fill_u32(p->d_hov_any, 0, 1, s);
     p->timed_dense = dense_active(p);
     if (p->timed_dense)
         return run_dense(p, s);
@@ -1752,6 +1947,35 @@ int genie_hip_plan_set_sparse(genie_plan* p, int mode)
         return fail(GENIE_E_INVALID, "sparse select mode must be -1, 0 or 1");
     plan_drop_graph(p);
     p->sparse_mode = mode;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_set_hash(genie_plan* p, int mode)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (mode < -1 || mode > 1)
+        return fail(GENIE_E_INVALID, "hash form mode must be -1, 0 or 1");
+    plan_drop_graph(p);
+    p->hash_mode = mode;
+    if (p->n_queries && p->mode != 1 && !dense_active(p)) {
+        std::fill(p->h_force_direct.begin(), p->h_force_direct.end(), 0);
+        if (int rc = plan_split_plain(p))
+            return rc;
+        return plan_size_candidates(p);
+    }
+    return GENIE_OK;
+}
+
+int genie_hip_plan_hash_queries(genie_plan* p, uint32_t* n)
+{
+    if (!p || !n)
+        return fail(GENIE_E_INVALID, "null argument");
+    uint32_t c = 0;
+    if (p->mode != 1 && !dense_active(p))
+        for (const uint4& h : p->h_hq)
+            c += h.x != 0;
+    *n = c;
     return GENIE_OK;
 }
 

@@ -451,6 +451,8 @@ struct ScanArgs {
     const uint32_t* qc_ptr;
     const uint32_t* qk_ptr;
     const uint32_t* kq;
+    const uint32_t* qlist;  // queries of the direct-mapped kernels (null: all)
+    uint32_t n_qlist;
     uint32_t n_objects, n_queries, n_tiles, tile_objs, dt_per_tile, n_dtiles;
     uint32_t* gthr;       // per query adaptive threshold (max of finished tiles' k-th counts)
     uint32_t* cand_cnt;   // per query candidates appended
@@ -975,9 +977,90 @@ struct Span {
 };
 constexpr int kSpanCap = 1024;
 
+// Resolves the window [w0, w0 + wn) of a query's (chunk, key) pairs into
+// posting spans inside directory tiles [dt0, dt1): every thread takes the
+// pairs of its chunks (independent loads, so the metadata latency is paid
+// once per window).  Spans of short keys (no directory entry) cover the whole
+// list and carry the filter flag.  Called by all threads.
+__device__ __forceinline__ void resolve_spans(const ScanArgs& a, uint32_t c0, uint32_t c1, uint32_t pbase, uint32_t w0,
+                                              uint32_t wn, uint32_t dt0, uint32_t dt1, Span* s_span)
+{
+    for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const CChunk ch = a.chunks[c];
+        const uint32_t cp = ch.kp - pbase;  // chunk's first pair, query-relative
+        const uint32_t nk = ch.k1 - ch.k0 + 1;
+        if (cp + nk <= w0 || cp >= w0 + wn)
+            continue;
+        const uint32_t j0 = cp < w0 ? w0 - cp : 0;
+        const uint32_t j1 = min(nk, w0 + wn - cp);
+        for (uint32_t j = j0; j < j1; ++j) {
+            const uint32_t k = ch.k0 + j;
+            const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
+            uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
+            const uint32_t doff = a.dir_off[k];
+            uint32_t filter = 1;
+            if (doff != kNoDir) {
+                const uint32_t* d = a.dir + doff;
+                lo = max(lo, ks + d[dt0]);
+                hi = min(hi, ks + d[dt1]);
+                filter = 0;
+            }
+            Span sp;
+            sp.lo = lo;
+            sp.hi = max(lo, hi);
+            sp.wf = ch.w << 1 | filter;
+            s_span[cp + j - w0] = sp;
+        }
+    }
+}
+
+// Flattens a resolved window of wn <= 2 * blockDim.x spans: each span's end
+// is replaced by the flat index of its first posting (a prefix of the span
+// lengths, 2 spans per thread); returns the window's posting count.  Called
+// by all threads after the spans are complete; ends with a barrier.
+__device__ __forceinline__ uint32_t flat_prefix(Span* s_span, uint32_t wn, uint32_t* s_wsum)
+{
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6, nw = blockDim.x / kWave;
+    const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
+    const uint32_t n0 = i0 < wn ? s_span[i0].hi - s_span[i0].lo : 0u;
+    const uint32_t n1 = i1 < wn ? s_span[i1].hi - s_span[i1].lo : 0u;
+    const uint32_t incl = wave_incl_scan(n0 + n1);
+    if (lane == kWave - 1)
+        s_wsum[wv] = incl;
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (int i = 0; i < nw; ++i) {
+        off += i < wv ? s_wsum[i] : 0u;
+        total += s_wsum[i];
+    }
+    const uint32_t ex = off + incl - (n0 + n1);
+    if (i0 < wn)
+        s_span[i0].hi = ex;
+    if (i1 < wn)
+        s_span[i1].hi = ex + n0;
+    __syncthreads();
+    return total;
+}
+
+// The flattened span holding flat posting index g (last span whose first
+// flat index is <= g).
+__device__ __forceinline__ uint32_t flat_span(const Span* s_span, uint32_t wn, uint32_t g)
+{
+    uint32_t lo = 0, hi = wn;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_span[mid].hi <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // SPARSE: counting marks the touched counter words and the select visits
 // only those (tile_select_sparse); for batches whose queries hold few
-// postings per tile.
+// postings per tile.  With a.qlist set, the grid covers the listed queries
+// only (the others run the hash form).
 template <int LB, bool SPARSE>
 __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
 {
@@ -989,12 +1072,13 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
     __shared__ uint32_t s_touch[SPARSE ? kTouchWords : 1];
     __shared__ uint32_t s_wsum[kScanWaves];
 
-    const uint32_t t = blockIdx.x / a.n_queries;
-    const uint32_t q = blockIdx.x % a.n_queries;
+    const uint32_t nql = a.qlist ? a.n_qlist : a.n_queries;
+    const uint32_t t = blockIdx.x / nql;
+    const uint32_t q = a.qlist ? a.qlist[blockIdx.x % nql] : blockIdx.x % nql;
     const uint32_t obj0 = t * a.tile_objs;
     const uint32_t dt0 = t * a.dt_per_tile;
     const uint32_t dt1 = min(dt0 + a.dt_per_tile, a.n_dtiles);
-    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    const int tid = threadIdx.x, lane = lane_id();
 
     for (int i = tid; i < kCntWords / 4; i += kScanThreads)
         s_cnt4[i] = make_uint4(0, 0, 0, 0);
@@ -1003,67 +1087,20 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
             s_touch[i] = 0;
 
     // ---- count ------------------------------------------------------------
-    // Rounds of up to kSpanCap (chunk, key) pairs: every thread resolves the
-    // pairs of its chunks inside the round's window (independent loads, so
-    // the metadata latency is paid once per round), then the waves take the
-    // resolved spans dynamically.
+    // Windows of up to kSpanCap (chunk, key) pairs, resolved into spans, then
+    // counted: by the waves taking spans dynamically, or (SPARSE) flattened.
     const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
     const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
     for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
         const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
         if (tid == 0)
             s_next = 0;
-        for (uint32_t c = c0 + tid; c < c1; c += kScanThreads) {
-            const CChunk ch = a.chunks[c];
-            const uint32_t cp = ch.kp - pbase;                  // chunk's first pair, query-relative
-            const uint32_t nk = ch.k1 - ch.k0 + 1;
-            if (cp + nk <= w0 || cp >= w0 + wn)
-                continue;
-            const uint32_t j0 = cp < w0 ? w0 - cp : 0;
-            const uint32_t j1 = min(nk, w0 + wn - cp);
-            for (uint32_t j = j0; j < j1; ++j) {
-                const uint32_t k = ch.k0 + j;
-                const uint32_t ks = a.key_start[k], ke = a.key_start[k + 1];
-                uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
-                const uint32_t doff = a.dir_off[k];
-                uint32_t filter = 1;
-                if (doff != kNoDir) {
-                    const uint32_t* d = a.dir + doff;
-                    lo = max(lo, ks + d[dt0]);
-                    hi = min(hi, ks + d[dt1]);
-                    filter = 0;
-                }
-                Span sp;
-                sp.lo = lo;
-                sp.hi = max(lo, hi);
-                sp.wf = ch.w << 1 | filter;
-                s_span[cp + j - w0] = sp;
-            }
-        }
+        resolve_spans(a, c0, c1, pbase, w0, wn, dt0, dt1, s_span);
         __syncthreads();
         if constexpr (SPARSE) {
-            // Few postings per span: flatten the window.  A prefix of span
-            // lengths replaces each span's end (2 spans per thread), then
-            // every thread takes postings g = tid, tid + 512, ... four loads
-            // at a time, each finding its span by binary search.
-            const uint32_t i0 = 2 * tid, i1 = 2 * tid + 1;
-            const uint32_t n0 = i0 < wn ? s_span[i0].hi - s_span[i0].lo : 0u;
-            const uint32_t n1 = i1 < wn ? s_span[i1].hi - s_span[i1].lo : 0u;
-            const uint32_t incl = wave_incl_scan(n0 + n1);
-            if (lane == kWave - 1)
-                s_wsum[wv] = incl;
-            __syncthreads();
-            uint32_t off = 0, total = 0;
-            for (int i = 0; i < kScanWaves; ++i) {
-                off += i < wv ? s_wsum[i] : 0u;
-                total += s_wsum[i];
-            }
-            const uint32_t ex = off + incl - (n0 + n1);
-            if (i0 < wn)
-                s_span[i0].hi = ex;  // hi now holds the span's first flat index
-            if (i1 < wn)
-                s_span[i1].hi = ex + n0;
-            __syncthreads();
+            // Few postings per span: every thread takes postings g = tid,
+            // tid + 512, ... of the flattened window, four loads at a time.
+            const uint32_t total = flat_prefix(s_span, wn, s_wsum);
             for (uint32_t g0 = 0; g0 < total; g0 += 4 * kScanThreads) {
                 uint32_t id[4], w[4];
 #pragma unroll
@@ -1072,15 +1109,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile(ScanArgs a)
                     id[u] = obj0 + a.tile_objs;  // outside the tile: counts nothing
                     w[u] = 0;
                     if (g < total) {
-                        uint32_t lo = 0, hi = wn;  // last span whose first index is <= g
-                        while (hi - lo > 1) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (s_span[mid].hi <= g)
-                                lo = mid;
-                            else
-                                hi = mid;
-                        }
-                        const Span sp = s_span[lo];
+                        const Span sp = s_span[flat_span(s_span, wn, g)];
                         id[u] = a.post[sp.lo + (g - sp.hi)];
                         w[u] = sp.wf >> 1;
                     }
@@ -1131,8 +1160,9 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
     __shared__ SelShared s_sel;
     __shared__ uint32_t s_touch[SPARSE ? kTouchWords : 1];
 
-    const uint32_t t = blockIdx.x / a.n_queries;
-    const uint32_t q = blockIdx.x % a.n_queries;
+    const uint32_t nql = a.qlist ? a.n_qlist : a.n_queries;
+    const uint32_t t = blockIdx.x / nql;
+    const uint32_t q = a.qlist ? a.qlist[blockIdx.x % nql] : blockIdx.x % nql;
     const uint32_t obj0 = t * a.tile_objs;
     const uint32_t obj1 = obj0 + a.tile_objs;
     const uint32_t dt0 = t * a.dt_per_tile;

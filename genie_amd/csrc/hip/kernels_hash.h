@@ -1,0 +1,314 @@
+// Hash-table form of the plain scan, for selective queries (a few postings
+// per object of the index): one workgroup per (query, object tile), where a
+// query's tiles are sized to hold about kHashTarget of its postings each.  The
+// tile's counts live in an open-addressing table of twice that many slots in
+// LDS instead of one counter per object, so a workgroup covers hundreds of
+// thousands of objects; an exact radix select then takes the tile's top k
+// under (count desc, id asc).  A table that fills up flags its query and
+// emits nothing for it; the host runs such queries again on the
+// direct-mapped form.  The reference keeps a per-query c-PQ hash table in
+// global memory for the same purpose (match_device_utils.h:161-419), with
+// approximate eviction; these tables are exact and hold every touched id.
+#pragma once
+#include "kernels.h"
+
+namespace gh {
+
+constexpr int kHashLogMax = 14;         // 16,384 slots = 64 KiB of LDS
+constexpr uint32_t kHashTarget = 8192;  // postings per (query, tile) aimed at
+
+struct HashArgs {
+    ScanArgs s;
+    const uint32_t* work;  // [n_work][2]: query, tile
+    const uint4* hq;       // [n_queries]: directory tiles per tile, key bits, log2 slots, 0
+    uint32_t* hov;         // [n_queries] set when a table of the query filled up
+    uint32_t* hov_any;
+};
+
+// Adds w to the count of tile-local id `loc`.  A slot word holds key = loc + 1
+// in its high kb bits and the count in the low 32 - kb bits (the host sizes
+// kb and checks that every count fits); 0 is an empty slot.  Linear probing
+// from a multiplicative hash; false when the table is full.
+__device__ __forceinline__ bool hash_add(uint32_t* tab, uint32_t lhs, uint32_t kb, uint32_t loc, uint32_t w)
+{
+    const uint32_t key = (loc + 1u) << (32u - kb);
+    const uint32_t kmask = ~0u << (32u - kb);
+    const uint32_t smask = (1u << lhs) - 1u;
+    uint32_t h = (loc * 0x9E3779B1u) >> (32u - lhs);
+    for (uint32_t probe = 0; probe <= smask; ++probe) {
+        uint32_t v = tab[h];
+        if (v == 0) {
+            v = atomicCAS(&tab[h], 0u, key | w);
+            if (v == 0)
+                return true;
+        }
+        if ((v & kmask) == key) {
+            atomicAdd(&tab[h], w);
+            return true;
+        }
+        h = (h + 1u) & smask;
+    }
+    return false;
+}
+
+// Sum over the workgroup (every thread gets it); s_w holds one word per wave.
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_w)
+{
+    const int lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x / kWave;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0)
+        s_w[wv] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int i = 0; i < nw; ++i)
+        t += s_w[i];
+    return t;
+}
+
+// Exact top-k of one (query, tile) from its table: the occupied slots with a
+// count at or above the query's adaptive threshold are ordered by
+// rank = count << kb | (2^kb - 1 - key), i.e. count descending, then id
+// ascending (keys are unique, so ranks are too); a radix select over the
+// rank bytes finds the k-th.  Appends the selected keys to the query's
+// candidates and raises the threshold to the tile's k-th count.
+__device__ __forceinline__ void hash_select(const ScanArgs& a, const uint32_t* tab, uint32_t ns, uint32_t kb,
+                                            uint32_t q, uint32_t obj0, uint32_t* s_hist, uint32_t* s_res,
+                                            uint32_t* s_w)
+{
+    const int tid = threadIdx.x;
+    const uint32_t k = a.kq[q];
+    const uint32_t cb = 32u - kb;
+    const uint32_t cmask = (1u << cb) - 1u;
+    const uint32_t kfull = (1u << kb) - 1u;
+    if (tid == 0)
+        s_res[0] = __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const uint32_t lower = max(1u, s_res[0]);
+    uint32_t e = 0;
+    for (uint32_t i = tid; i < ns; i += blockDim.x)
+        e += (tab[i] & cmask) >= lower;
+    const uint32_t E = block_sum(e, s_w);
+    if (E == 0)
+        return;
+
+    uint32_t kth = 0;  // every eligible rank >= kth is selected
+    if (E > k) {
+        uint32_t prefix = 0, mask = 0, remaining = k;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            if (tid < 256)
+                s_hist[tid] = 0;
+            __syncthreads();
+            for (uint32_t i = tid; i < ns; i += blockDim.x) {
+                const uint32_t v = tab[i];
+                const uint32_t r = ((v & cmask) << kb) | (kfull - (v >> cb));
+                if ((v & cmask) >= lower && (r & mask) == prefix)
+                    atomicAdd(&s_hist[(r >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            if (tid < kWave) {
+                // lane l holds bins 255 - 4l .. 252 - 4l, counted from the top
+                const uint32_t top = 255u - 4u * (uint32_t)tid;
+                uint32_t hb[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    hb[j] = s_hist[top - j];
+                const uint32_t sum = hb[0] + hb[1] + hb[2] + hb[3];
+                uint32_t above = wave_incl_scan(sum) - sum;
+                if (above < remaining && above + sum >= remaining) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (above + hb[j] >= remaining) {
+                            s_res[1] = top - j;
+                            s_res[2] = remaining - above;
+                            s_res[3] = hb[j];
+                            break;
+                        }
+                        above += hb[j];
+                    }
+                }
+            }
+            __syncthreads();
+            const uint32_t bin = s_res[1], rem = s_res[2], inbin = s_res[3];
+            prefix |= bin << shift;
+            mask |= 255u << shift;
+            remaining = rem;
+            __syncthreads();
+            if (inbin == rem)
+                break;  // the whole bin is selected: the prefix bounds it
+        }
+        kth = prefix;
+    }
+    const uint32_t nemit = min(E, k);
+    if (tid == 0) {
+        uint32_t base = atomicAdd(&a.cand_cnt[q], nemit);
+        if ((uint64_t)base + nemit > a.cand_cap[q]) {
+            atomicOr(a.overflow, 1u);
+            base = kEmitFail;
+        }
+        s_res[4] = base;
+        s_res[5] = 0;            // slot counter
+        s_res[6] = 0xFFFFFFFFu;  // smallest selected count
+    }
+    __syncthreads();
+    const uint32_t base = s_res[4];
+    if (base == kEmitFail)
+        return;
+    uint64_t* out = a.cand + a.cand_base[q] + base;
+    for (uint32_t i = tid; i < ns; i += blockDim.x) {
+        const uint32_t v = tab[i];
+        const uint32_t c = v & cmask;
+        const uint32_t r = (c << kb) | (kfull - (v >> cb));
+        if (c >= lower && r >= kth) {
+            const uint32_t slot = atomicAdd(&s_res[5], 1u);
+            out[slot] = make_key(c, obj0 + (v >> cb) - 1u);
+            atomicMin(&s_res[6], c);
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && E >= k)
+        atomicMax(&a.gthr[q], s_res[6]);
+}
+
+// Compressed spans: the segment range of each (chunk, key) pair that can hold
+// ids of [obj0, obj1) (segment directory for long keys).
+__device__ __forceinline__ void resolve_seg_spans(const ScanArgs& a, uint32_t c0, uint32_t c1, uint32_t pbase,
+                                                  uint32_t w0, uint32_t wn, uint32_t dt0, uint32_t dt1,
+                                                  uint32_t obj1, Span* s_span)
+{
+    for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const CChunk ch = a.chunks[c];
+        const uint32_t cp = ch.kp - pbase;
+        const uint32_t nk = ch.k1 - ch.k0 + 1;
+        if (cp + nk <= w0 || cp >= w0 + wn)
+            continue;
+        const uint32_t j0 = cp < w0 ? w0 - cp : 0;
+        const uint32_t j1 = min(nk, w0 + wn - cp);
+        for (uint32_t j = j0; j < j1; ++j) {
+            const uint32_t k = ch.k0 + j;
+            const uint32_t ks = a.key_seg[k], ke = a.key_seg[k + 1];
+            uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
+            const uint32_t doff = a.sdir_off[k];
+            if (doff != kNoDir) {
+                const uint32_t* d = a.sdir + doff;
+                const uint32_t mid = ks + d[dt1];
+                const uint32_t end = mid + (mid < ke && a.seg_first[mid] < obj1 ? 1u : 0u);
+                lo = max(lo, ks + d[dt0]);
+                hi = min(hi, end);
+            }
+            Span sp;
+            sp.lo = lo;
+            sp.hi = max(lo, hi);
+            sp.wf = ch.w << 1 | 1u;
+            s_span[cp + j - w0] = sp;
+        }
+    }
+}
+
+// C: compressed index (the waves take single segments, stage and decode
+// them, codecs_dev.h); otherwise the window's postings are flattened over
+// the workgroup, four loads per thread at a time.
+template <bool C>
+__global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
+{
+    __shared__ uint32_t s_tab[1 << kHashLogMax];
+    __shared__ Span s_span[kSpanCap];
+    __shared__ uint32_t s_wsum[kScanWaves];
+    __shared__ uint32_t s_hist[256];
+    __shared__ uint32_t s_res[8];
+    __shared__ uint32_t s_next;
+    __shared__ DecodeScratch s_dec[C ? kScanWaves : 1];
+
+    const ScanArgs& a = h.s;
+    const uint32_t q = h.work[2 * blockIdx.x], t = h.work[2 * blockIdx.x + 1];
+    const uint4 prm = h.hq[q];
+    const uint32_t kb = prm.y, lhs = prm.z, ns = 1u << lhs;
+    const uint32_t tobj = prm.x * kDirTile;
+    const uint32_t obj0 = t * tobj;
+    const uint32_t nobj = min(tobj, a.n_objects - obj0);  // objects of this tile
+    const uint32_t dt0 = t * prm.x, dt1 = min(dt0 + prm.x, a.n_dtiles);
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+
+    for (uint32_t i = tid; i < ns; i += kScanThreads)
+        s_tab[i] = 0;
+    if (tid == 0)
+        s_res[7] = 0;  // a table insert failed
+    __syncthreads();
+
+    bool ok = true;
+    const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
+    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
+    for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
+        const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
+        if constexpr (C) {
+            if (tid == 0)
+                s_next = 0;
+            resolve_seg_spans(a, c0, c1, pbase, w0, wn, dt0, dt1, obj0 + nobj, s_span);
+            __syncthreads();
+            const uint32_t nsegs = flat_prefix(s_span, wn, s_wsum);
+            DecodeScratch& D = s_dec[wv];
+            for (;;) {
+                uint32_t g = 0;
+                if (lane == 0)
+                    g = atomicAdd(&s_next, 1u);
+                g = bcast_first(__shfl((int)g, 0, kWave));
+                if (g >= nsegs)
+                    break;
+                const Span sp = s_span[flat_span(s_span, wn, g)];
+                const uint32_t sg = sp.lo + (g - sp.hi);
+                if (a.seg_last[sg] < obj0 || a.seg_first[sg] >= obj0 + nobj)
+                    continue;  // segment entirely outside the tile
+                const uint32_t cw0 = a.seg_cpos[sg], nw = a.seg_cpos[sg + 1] - cw0;
+                stage_words(D, a.words, cw0, nw);
+                uint32_t v[kPerLane];
+                const uint32_t n = wave_decode(a.codec, D, nw, v);
+                const uint32_t w = sp.wf >> 1;
+#pragma unroll
+                for (int j = 0; j < kPerLane; ++j) {
+                    const uint32_t loc = v[j] - obj0;
+                    if (lane * kPerLane + j < n && loc < nobj && w)
+                        ok &= hash_add(s_tab, lhs, kb, loc, w);
+                }
+                wave_sync();  // the staging buffer is reused by the next segment
+            }
+        } else {
+            resolve_spans(a, c0, c1, pbase, w0, wn, dt0, dt1, s_span);
+            __syncthreads();
+            const uint32_t total = flat_prefix(s_span, wn, s_wsum);
+            for (uint32_t g0 = 0; g0 < total; g0 += 4 * kScanThreads) {
+                uint32_t id[4], w[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t g = g0 + u * kScanThreads + tid;
+                    id[u] = obj0 + nobj;  // outside the tile: counts nothing
+                    w[u] = 0;
+                    if (g < total) {
+                        const Span sp = s_span[flat_span(s_span, wn, g)];
+                        id[u] = a.post[sp.lo + (g - sp.hi)];
+                        w[u] = sp.wf >> 1;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t loc = id[u] - obj0;
+                    if (loc < nobj && w[u])
+                        ok &= hash_add(s_tab, lhs, kb, loc, w[u]);
+                }
+            }
+        }
+        __syncthreads();  // the next window rewrites the spans
+    }
+    if (!ok)
+        s_res[7] = 1;
+    __syncthreads();
+    if (s_res[7]) {
+        if (tid == 0) {
+            h.hov[q] = 1u;
+            *h.hov_any = 1u;
+        }
+        return;
+    }
+    hash_select(a, s_tab, ns, kb, q, obj0, s_hist, s_res, s_wsum);
+}
+
+}  // namespace gh

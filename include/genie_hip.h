@@ -166,6 +166,18 @@ int genie_hip_plan_dense_active(genie_plan* p, int* active);
 int genie_hip_plan_set_sparse(genie_plan* p, int mode);
 /* Whether the plain form of the last uploaded batch selects sparsely. */
 int genie_hip_plan_sparse_active(genie_plan* p, int* active);
+/* Hash form of the plain scan (modes 0 and 2).  A selective query (mode -1,
+ * the default: at most one posting per 16 objects; mode 1: every query whose
+ * counts fit) is counted per tile of about 8,192 of its postings in an LDS
+ * open-addressing table instead of one counter per object, and the tile's
+ * top k taken by an exact radix select.  A query whose table fills up is run
+ * on the direct-mapped form at download.  mode 0: never.  Results are
+ * identical in every mode.  It replaces the reference's per-query c-PQ hash
+ * table (match_device_utils.h:161-419), which evicts entries and can lose
+ * the k-th count; these tables keep every touched id. */
+int genie_hip_plan_set_hash(genie_plan* p, int mode);
+/* Number of queries of the last uploaded batch on the hash form. */
+int genie_hip_plan_hash_queries(genie_plan* p, uint32_t* n);
 /* Host time of the last batched work-queue build (upload), ms. */
 int genie_hip_plan_host_ms(genie_plan* p, double* ms);
 /* Algorithmic bytes of the last uploaded batch: logical = 4 B x sum of

@@ -36,19 +36,24 @@ def oracle_sets(row_ptr, vals, ranges, n_queries, k):
     return np.concatenate(res_i), np.concatenate(res_c)
 
 
+PLAIN = {"plain-sparse": (0, 1), "plain-dense": (0, 0), "plain-hash": (1, -1)}  # (hash form, sparse select)
+
+
 def engine(table, items, n_queries, k, mode):
     """mode "auto": genie_hip_match_topk (dense form when the batch allows,
-    else plain); "plain-sparse" / "plain-dense": an explicit plain plan with
-    the sparse select forced on / off, so the per-query scan kernel and both
-    of its selects are checked whatever the batch; other values: a plan of
-    that mode."""
+    else plain); "plain-sparse" / "plain-dense": an explicit plain plan on
+    the direct-mapped kernels with the sparse select forced on / off;
+    "plain-hash": every query on the hash form; so each plain kernel and
+    select is checked whatever the batch; other values: a plan of that
+    mode."""
     if mode == "auto":
         return match_topk(table.device(), items, n_queries, k)
     kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (n_queries,)))
-    pm = 0 if mode in ("plain-sparse", "plain-dense") else mode
+    pm = 0 if mode in PLAIN else mode
     p = Plan(table.device(), n_queries, max(1, len(items)), int(kq.max()), 0, mode=pm)
-    if mode in ("plain-sparse", "plain-dense"):
-        p.set_sparse(1 if mode == "plain-sparse" else 0)
+    if mode in PLAIN:
+        p.set_hash(PLAIN[mode][0])
+        p.set_sparse(PLAIN[mode][1])
     p.upload(items, n_queries, kq)
     p.run()
     return p.download()
@@ -61,7 +66,7 @@ def run_engine(table, ranges, n_queries, k, lb=False, max_load=0, shuffle=False,
     return engine(table, items, n_queries, k, mode)
 
 
-MODES = ["plain-sparse", "plain-dense", "auto"]
+MODES = ["plain-sparse", "plain-dense", "plain-hash", "auto"]
 
 
 @pytest.mark.parametrize("mode", MODES)

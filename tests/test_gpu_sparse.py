@@ -18,8 +18,10 @@ pytestmark = pytest.mark.gpu
 
 
 def plain(table, items, nq, k, sparse):
+    """The direct-mapped kernels (hash form off) with the given select."""
     kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
     p = Plan(table.device(), nq, max(1, len(items)), int(kq.max()), 0, mode=0)
+    p.set_hash(0)
     p.set_sparse(sparse)
     p.upload(items, nq, kq)
     p.run()
