@@ -36,12 +36,11 @@ __device__ __forceinline__ bool hash_add(uint32_t* tab, uint32_t lhs, uint32_t k
     const uint32_t smask = (1u << lhs) - 1u;
     uint32_t h = (loc * 0x9E3779B1u) >> (32u - lhs);
     for (uint32_t probe = 0; probe <= smask; ++probe) {
-        uint32_t v = tab[h];
-        if (v == 0) {
-            v = atomicCAS(&tab[h], 0u, key | w);
-            if (v == 0)
-                return true;
-        }
+        // claim an empty slot outright (most ids of a selective query arrive
+        // once); a taken slot returns its word
+        const uint32_t v = atomicCAS(&tab[h], 0u, key | w);
+        if (v == 0)
+            return true;
         if ((v & kmask) == key) {
             atomicAdd(&tab[h], w);
             return true;
@@ -51,75 +50,97 @@ __device__ __forceinline__ bool hash_add(uint32_t* tab, uint32_t lhs, uint32_t k
     return false;
 }
 
-// Sum over the workgroup (every thread gets it); s_w holds one word per wave.
-__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_w)
-{
-    const int lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x / kWave;
-    v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0)
-        s_w[wv] = v;
-    __syncthreads();
-    uint32_t t = 0;
-    for (int i = 0; i < nw; ++i)
-        t += s_w[i];
-    return t;
-}
-
 // Exact top-k of one (query, tile) from its table: the occupied slots with a
 // count at or above the query's adaptive threshold are ordered by
 // rank = count << kb | (2^kb - 1 - key), i.e. count descending, then id
-// ascending (keys are unique, so ranks are too); a radix select over the
-// rank bytes finds the k-th.  Appends the selected keys to the query's
+// ascending (ranks are unique and nonzero).  Every thread loads its slots
+// (tid + 512 j) once as ranks into registers (0 = not eligible); a radix
+// select over the rank bytes, from the highest byte in which the eligible
+// ranks differ, finds the k-th.  Appends the selected keys to the query's
 // candidates and raises the threshold to the tile's k-th count.
+constexpr int kHashPer = (1 << kHashLogMax) / kScanThreads;  // slots per thread at most
+
 __device__ __forceinline__ void hash_select(const ScanArgs& a, const uint32_t* tab, uint32_t ns, uint32_t kb,
                                             uint32_t q, uint32_t obj0, uint32_t* s_hist, uint32_t* s_res,
-                                            uint32_t* s_w)
+                                            uint32_t* s_red)
 {
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6, nw = blockDim.x / kWave;
     const uint32_t k = a.kq[q];
     const uint32_t cb = 32u - kb;
     const uint32_t cmask = (1u << cb) - 1u;
     const uint32_t kfull = (1u << kb) - 1u;
+    const uint32_t per = ns / blockDim.x;  // uniform, 1..kHashPer
     if (tid == 0)
         s_res[0] = __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const uint32_t lower = max(1u, s_res[0]);
-    uint32_t e = 0;
-    for (uint32_t i = tid; i < ns; i += blockDim.x)
-        e += (tab[i] & cmask) >= lower;
-    const uint32_t E = block_sum(e, s_w);
+    uint32_t rk[kHashPer];
+    uint32_t e = 0, rmax = 0, rmin = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < kHashPer; ++j) {
+        rk[j] = 0;
+        if ((uint32_t)j < per) {
+            const uint32_t v = tab[tid + j * blockDim.x];
+            const uint32_t c = v & cmask;
+            if (c >= lower) {
+                rk[j] = (c << kb) | (kfull - (v >> cb));
+                ++e;
+                rmax = max(rmax, rk[j]);
+                rmin = min(rmin, rk[j]);
+            }
+        }
+    }
+    // eligible count, largest and smallest rank of the tile
+    e = wave_sum(e);
+    rmax = wave_max(rmax);
+    rmin = wave_min(rmin);
+    if (lane == 0) {
+        s_red[wv] = e;
+        s_red[kMaxWaves + wv] = rmax;
+        s_red[2 * kMaxWaves + wv] = rmin;
+    }
+    __syncthreads();
+    uint32_t E = 0, Rmax = 0, Rmin = 0xFFFFFFFFu;
+    for (int i = 0; i < nw; ++i) {
+        E += s_red[i];
+        Rmax = max(Rmax, s_red[kMaxWaves + i]);
+        Rmin = min(Rmin, s_red[2 * kMaxWaves + i]);
+    }
     if (E == 0)
         return;
 
-    uint32_t kth = 0;  // every eligible rank >= kth is selected
+    uint32_t kth = 0;  // every nonzero rank >= kth is selected
     if (E > k) {
-        uint32_t prefix = 0, mask = 0, remaining = k;
-        for (int shift = 24; shift >= 0; shift -= 8) {
+        // bits above the highest one in which Rmax and Rmin differ are common
+        // to every eligible rank
+        const int top = 31 - __builtin_clz(Rmax ^ Rmin);  // Rmax != Rmin since E > k >= 1
+        int shift = top & ~7;
+        uint32_t mask = shift + 8 >= 32 ? 0u : ~0u << (shift + 8);
+        uint32_t prefix = Rmax & mask, remaining = k;
+#pragma unroll 1
+        for (; shift >= 0; shift -= 8) {
             if (tid < 256)
                 s_hist[tid] = 0;
             __syncthreads();
-            for (uint32_t i = tid; i < ns; i += blockDim.x) {
-                const uint32_t v = tab[i];
-                const uint32_t r = ((v & cmask) << kb) | (kfull - (v >> cb));
-                if ((v & cmask) >= lower && (r & mask) == prefix)
-                    atomicAdd(&s_hist[(r >> shift) & 255u], 1u);
-            }
+#pragma unroll
+            for (int j = 0; j < kHashPer; ++j)
+                if ((uint32_t)j < per)
+                    hist_add_leader(s_hist, (rk[j] >> shift) & 255u, rk[j] != 0 && (rk[j] & mask) == prefix);
             __syncthreads();
             if (tid < kWave) {
                 // lane l holds bins 255 - 4l .. 252 - 4l, counted from the top
-                const uint32_t top = 255u - 4u * (uint32_t)tid;
+                const uint32_t tb = 255u - 4u * (uint32_t)tid;
                 uint32_t hb[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    hb[j] = s_hist[top - j];
+                    hb[j] = s_hist[tb - j];
                 const uint32_t sum = hb[0] + hb[1] + hb[2] + hb[3];
                 uint32_t above = wave_incl_scan(sum) - sum;
                 if (above < remaining && above + sum >= remaining) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         if (above + hb[j] >= remaining) {
-                            s_res[1] = top - j;
+                            s_res[1] = tb - j;
                             s_res[2] = remaining - above;
                             s_res[3] = hb[j];
                             break;
@@ -155,16 +176,27 @@ __device__ __forceinline__ void hash_select(const ScanArgs& a, const uint32_t* t
     if (base == kEmitFail)
         return;
     uint64_t* out = a.cand + a.cand_base[q] + base;
-    for (uint32_t i = tid; i < ns; i += blockDim.x) {
-        const uint32_t v = tab[i];
-        const uint32_t c = v & cmask;
-        const uint32_t r = (c << kb) | (kfull - (v >> cb));
-        if (c >= lower && r >= kth) {
-            const uint32_t slot = atomicAdd(&s_res[5], 1u);
-            out[slot] = make_key(c, obj0 + (v >> cb) - 1u);
-            atomicMin(&s_res[6], c);
+    uint32_t cmin = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < kHashPer; ++j) {
+        const uint32_t r = (uint32_t)j < per ? rk[j] : 0u;
+        const bool take = r != 0 && r >= kth;
+        const uint64_t bm = ballot(take);
+        if (bm) {
+            uint32_t wb = 0;
+            if (lane == __builtin_ctzll(bm))
+                wb = atomicAdd(&s_res[5], (uint32_t)__popcll(bm));
+            wb = (uint32_t)__shfl((int)wb, __builtin_ctzll(bm), kWave);
+            if (take) {
+                const uint32_t c = r >> kb;
+                out[wb + lane_prefix(bm)] = make_key(c, obj0 + (kfull - (r & kfull)) - 1u);
+                cmin = min(cmin, c);
+            }
         }
     }
+    cmin = wave_min(cmin);
+    if (lane == 0 && cmin != 0xFFFFFFFFu)
+        atomicMin(&s_res[6], cmin);
     __syncthreads();
     if (tid == 0 && E >= k)
         atomicMax(&a.gthr[q], s_res[6]);
@@ -214,6 +246,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
     __shared__ uint32_t s_tab[1 << kHashLogMax];
     __shared__ Span s_span[kSpanCap];
     __shared__ uint32_t s_wsum[kScanWaves];
+    __shared__ uint32_t s_red[3 * kMaxWaves];
     __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_res[8];
     __shared__ uint32_t s_next;
@@ -308,7 +341,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
         }
         return;
     }
-    hash_select(a, s_tab, ns, kb, q, obj0, s_hist, s_res, s_wsum);
+    hash_select(a, s_tab, ns, kb, q, obj0, s_hist, s_res, s_red);
 }
 
 }  // namespace gh

@@ -27,6 +27,14 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v)
     return v;
 }
 
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = min(v, (uint32_t)__shfl_xor((int)v, off, kWave));
+    return v;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
 #pragma unroll
@@ -78,6 +86,24 @@ __device__ __forceinline__ void hist_add_aggregated(uint32_t* hist, uint32_t bin
             atomicAdd(&hist[lb], (uint32_t)__popcll(same));
         pending &= ~same;
     }
+}
+
+// Add 1 to hist[bin] for every active lane with `take`: the lanes holding
+// the first taking lane's bin add with one LDS atomic, every other lane with
+// its own (cheap when one bin dominates, and when the bins are spread).
+__device__ __forceinline__ void hist_add_leader(uint32_t* hist, uint32_t bin, bool take)
+{
+    const uint64_t act = ballot(take);
+    if (!act)
+        return;
+    const int leader = __builtin_ctzll(act);
+    const uint32_t lb = (uint32_t)__shfl((int)bin, leader, kWave);
+    const bool same = take && bin == lb;
+    const uint64_t sm = ballot(same);
+    if (lane_id() == leader)
+        atomicAdd(&hist[lb], (uint32_t)__popcll(sm));
+    else if (take && !same)
+        atomicAdd(&hist[bin], 1u);
 }
 
 } // namespace gh
