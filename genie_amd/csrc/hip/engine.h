@@ -26,6 +26,20 @@ struct CChunk {
     uint32_t k0, k1, sp, ep, w, q, kp, pad;
 };
 
+// Position -> key lookups over a non-decreasing array of key starts
+// (starts[0..nk]): buckets of 2^shift positions, each holding the index of
+// the first key that starts at or after the bucket, so a lookup searches the
+// few keys of one bucket.
+struct PosIndex {
+    uint32_t shift = 0;
+    std::vector<uint32_t> first;  // [n_buckets + 1]
+    void build(const std::vector<uint64_t>& starts);
+    // Last key whose start is <= pos (std::upper_bound - 1).
+    uint32_t last_le(const uint64_t* starts, uint64_t pos) const;
+    // First key whose start is >= pos (std::lower_bound).
+    uint32_t first_ge(const uint64_t* starts, uint64_t pos) const;
+};
+
 // Segment view of a compressed index.
 struct DevIndex {
     uint32_t n_objects = 0, n_dims = 0, n_keys = 0;
@@ -63,6 +77,7 @@ struct DevIndex {
     std::vector<uint32_t> h_seg_key;
     std::vector<uint32_t> h_key_seg;
     std::vector<uint64_t> h_key_wstart;  // [n_keys + 1] first word of each key (compressed)
+    PosIndex key_pos;                    // over h_key_start (uncompressed) or h_key_wstart (compressed)
     std::vector<uint32_t> h_seg_upos;    // [n_segments + 1] first posting of each segment (compressed)
 };
 

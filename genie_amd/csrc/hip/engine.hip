@@ -177,6 +177,9 @@ struct genie_plan {
     uint32_t* d_dkeys = nullptr;     // [max_items][2] key range of every item
     std::vector<uint32_t> h_dkeys;
     uint32_t* d_redo = nullptr;      // [groups] second count of a group
+    uint32_t* d_want = nullptr;      // [max_queries] candidate space asked for at spec / at safe
+    uint32_t* d_want_safe = nullptr;
+    unsigned long long* d_cap_used = nullptr;  // end of the assigned candidate space
     // graph replay
     bool graph_on = false, graph_valid = false;
     hipStream_t own_stream = nullptr;
@@ -243,6 +246,9 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_dlist);
     dev_free(p->d_gsafe);
     dev_free(p->d_redo);
+    dev_free(p->d_want);
+    dev_free(p->d_want_safe);
+    dev_free(p->d_cap_used);
     dev_free(p->d_dkeys);
     dev_free(p->d_hwork);
     dev_free(p->d_hq);
@@ -272,9 +278,8 @@ static int compute_bounds(const DevIndex& d, const genie_dim_t* items, uint64_t 
     }
     const uint64_t* ks = d.h_key_start.data();
     const uint32_t nk = d.n_keys;
-    auto key_of_pos = [&](uint64_t pos) -> uint32_t {
-        return (uint32_t)(std::upper_bound(ks, ks + nk + 1, pos) - ks - 1);
-    };
+    (void)nk;
+    auto key_of_pos = [&](uint64_t pos) -> uint32_t { return d.key_pos.last_le(ks, pos); };
     auto one_query = [&](uint32_t q) {
         struct E {
             int32_t order;
@@ -339,6 +344,35 @@ static int compute_bounds(const DevIndex& d, const genie_dim_t* items, uint64_t 
             x.join();
     }
     return GENIE_OK;
+}
+
+void gh::PosIndex::build(const std::vector<uint64_t>& starts)
+{
+    const uint64_t nk = starts.size() - 1, span = starts.back() + 1;
+    shift = 0;  // about two buckets per key
+    while ((span >> shift) > 2 * nk + 1)
+        ++shift;
+    const uint64_t nb = (span >> shift) + 1;
+    first.resize(nb + 1);
+    uint64_t k = 0;
+    for (uint64_t b = 0; b <= nb; ++b) {
+        while (k <= nk && starts[k] < (b << shift))
+            ++k;
+        first[b] = (uint32_t)k;
+    }
+}
+
+uint32_t gh::PosIndex::last_le(const uint64_t* starts, uint64_t pos) const
+{
+    const uint64_t b = std::min<uint64_t>(pos >> shift, first.size() - 2);
+    const uint64_t* it = std::upper_bound(starts + first[b], starts + first[b + 1], pos);
+    return (uint32_t)(it - starts) - 1;
+}
+
+uint32_t gh::PosIndex::first_ge(const uint64_t* starts, uint64_t pos) const
+{
+    const uint64_t b = std::min<uint64_t>(pos >> shift, first.size() - 2);
+    return (uint32_t)(std::lower_bound(starts + first[b], starts + first[b + 1], pos) - starts);
 }
 
 int gh::validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries)
@@ -417,6 +451,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
     d.h_key_wstart.resize(K + 1);
     for (uint32_t k = 0; k <= K; ++k)
         d.h_key_wstart[k] = cp[key_seg[k]];
+    d.key_pos.build(d.h_key_wstart);
     if (desc->postings) {
         for (uint32_t k = 0; k < K; ++k) {
             uint32_t best = 1, run = 1;
@@ -922,6 +957,7 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
     d.single_value = desc->single_value;
     d.n_dtiles = (desc->n_objects + kDirTile - 1) / kDirTile;
     d.h_key_start.assign(desc->key_start, desc->key_start + desc->n_keys + 1);
+    d.key_pos.build(d.h_key_start);
     d.h_key_dim.assign(desc->key_dim, desc->key_dim + desc->n_keys);
     d.h_key_mult.assign(desc->n_keys, 1);
     std::vector<uint32_t> ks32(desc->n_keys + 1), dir_off(desc->n_keys, kNoDir), dir_keys;
@@ -1198,7 +1234,12 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
                 return false;
             iv.push_back({(uint64_t)it.start_pos, (uint64_t)it.end_pos, i});
         }
-        std::sort(iv.begin(), iv.end(), [](const Span& x, const Span& y) { return x.sp < y.sp; });
+        // compiled batches list a query's items in position order already
+        bool ordered = true;
+        for (size_t i = 1; i < iv.size() && ordered; ++i)
+            ordered = iv[i - 1].sp <= iv[i].sp;
+        if (!ordered)
+            std::sort(iv.begin(), iv.end(), [](const Span& x, const Span& y) { return x.sp < y.sp; });
         for (size_t r = 0; r < iv.size();) {
             size_t e = r + 1;
             uint64_t end = iv[r].ep;
@@ -1207,8 +1248,8 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
             if (e < iv.size() && iv[e].sp < end)
                 return false;  // overlapping items
             const uint64_t sp = iv[r].sp;
-            const uint32_t ka = (uint32_t)(std::upper_bound(ks, ks + nk + 1, sp) - ks - 1);
-            const uint32_t kb = (uint32_t)(std::lower_bound(ks, ks + nk + 1, end) - ks);
+            const uint32_t ka = d.key_pos.last_le(ks, sp);
+            const uint32_t kb = d.key_pos.first_ge(ks, end);
             if (ks[ka] != sp || kb > nk || ks[kb] != end)
                 return false;
             p->h_dkeys[2 * iv[r].item] = ka | (q & 31u) << 24;  // keys < 2^15 in a dense index
@@ -1248,6 +1289,14 @@ static int prepare_dense(genie_plan* p)
     if (!p->d_redo)
         if (int rc = dev_alloc(&p->d_redo, (p->max_queries + 31) / 32))
             return rc;
+    if (!p->d_want) {
+        if (int rc = dev_alloc(&p->d_want, p->max_queries))
+            return rc;
+        if (int rc = dev_alloc(&p->d_want_safe, p->max_queries))
+            return rc;
+        if (int rc = dev_alloc(&p->d_cap_used, 1))
+            return rc;
+    }
     const uint32_t n_tiles = (d.n_objects + kDenseTile - 1) / kDenseTile;
     const uint32_t ns = std::min(n_tiles, std::max(8u, (n_tiles + 31) / 32));
     std::vector<uint32_t> list(ns);
@@ -1708,8 +1757,14 @@ static int run_dense(genie_plan* p, hipStream_t s)
     const uint32_t nblk = d.dense_dpad / 16;
     if (int rc = launch_dense<true>(nblk, a, s))
         return rc;
-    k_dense_thresh<<<(nq + 3) / 4, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->n_dsample, d.n_objects, p->d_gthr,
-                                                p->d_gsafe);
+    // thresholds, then every query's candidate space from its sample
+    const uint32_t mult = std::max<uint32_t>(1, p->cap_factor / initial_cap_factor(p));
+    const uint32_t ratio16 = (uint32_t)std::min<uint64_t>(
+        ((uint64_t)d.n_objects << 16) / std::max<uint32_t>(1, p->n_dsample) + 1, 0xFFFFFFFFull);
+    k_dense_thresh<<<(nq + 3) / 4, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->n_dsample, d.n_objects, ratio16, mult,
+                                                p->d_gthr, p->d_gsafe, p->d_want, p->d_want_safe);
+    k_cap_prefix<<<1, 1024, 0, s>>>(p->d_want, p->d_want_safe, mult > 1 ? 1u : 0u, nq, p->total_cap, p->d_cand_cap,
+                                    p->d_cand_base, p->d_cap_used);
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[4], s));
     a.list = nullptr;
@@ -1717,7 +1772,9 @@ static int run_dense(genie_plan* p, hipStream_t s)
     if (int rc = launch_dense<false>(nblk, a, s))
         return rc;
     // queries the speculative threshold left short: their groups again
-    k_dense_check<<<p->dn_groups, 64, 0, s>>>(p->d_kq, nq, p->d_gthr, p->d_gsafe, p->d_cand_cnt, p->d_redo);
+    k_dense_check<<<p->dn_groups, 64, 0, s>>>(p->d_kq, nq, p->d_gthr, p->d_gsafe, p->d_cand_cnt, p->d_redo,
+                                              p->d_want_safe, p->d_cand_cap, p->d_cand_base, p->d_cap_used,
+                                              p->total_cap);
     a.redo = p->d_redo;
     if (int rc = launch_dense<false>(nblk, a, s))
         return rc;

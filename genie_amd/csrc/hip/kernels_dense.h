@@ -157,10 +157,26 @@ __global__ void k_dense_holes(const uint32_t* __restrict__ bm, uint32_t n_object
 // The main pass emits from spec; a query left with fewer than k candidates
 // is counted again from safe (k_dense_check).  One wave per query, four bins
 // per lane.
+// Candidate space: the objects of the whole index expected at or above a
+// threshold are S(c) x N / n_sample; a query asks for 5/4 of that at spec
+// (want) and at safe (want_safe, for a recount), plus 2 k + 256, times
+// `mult`, at most N.  Ties at the threshold are in S(c), so a query whose
+// counts crowd at its threshold asks for what it will emit.
+// ratio16 = ceil(N * 2^16 / n_sample), from the host: no division here.
+__device__ __forceinline__ uint32_t dense_want(uint32_t s_at, uint32_t k, uint32_t ratio16, uint32_t n_objects,
+                                               uint32_t mult)
+{
+    const uint64_t e = ((uint64_t)s_at * ratio16 + 0xFFFFu) >> 16;
+    const uint64_t w = (e + e / 4 + 2ull * k + 256) * mult;
+    return (uint32_t)min<uint64_t>(w, n_objects);
+}
+
 __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ kq, uint32_t n_queries,
-                                                      uint32_t n_sample, uint32_t n_objects,
-                                                      uint32_t* __restrict__ gthr, uint32_t* __restrict__ gsafe)
+                                                      uint32_t n_sample, uint32_t n_objects, uint32_t ratio16,
+                                                      uint32_t mult,
+                                                      uint32_t* __restrict__ gthr, uint32_t* __restrict__ gsafe,
+                                                      uint32_t* __restrict__ want, uint32_t* __restrict__ want_safe)
 {
     static_assert(kDenseBins == 4 * kWave, "four bins per lane");
     const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -183,10 +199,64 @@ __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict
     };
     const uint32_t safe = max(wave_max(largest(k)), 1u);
     const uint32_t spec = max(wave_max(largest(j)), safe);
+    // S(spec) and S(safe): from the lane holding that bin
+    auto s_of = [&](uint32_t c) {
+        const uint32_t o = c - b;
+        return o == 0 ? s0 : (o == 1 ? s1 : (o == 2 ? s2 : s3));
+    };
+    const uint32_t at_spec = (uint32_t)__shfl((int)(spec >= b && spec < b + 4 ? s_of(spec) : 0u), (int)(spec / 4), kWave);
+    const uint32_t at_safe = (uint32_t)__shfl((int)(safe >= b && safe < b + 4 ? s_of(safe) : 0u), (int)(safe / 4), kWave);
     if (lane == 0) {
         gthr[q] = spec;
         gsafe[q] = safe;
+        want[q] = dense_want(at_spec, k, ratio16, n_objects, mult);
+        want_safe[q] = dense_want(at_safe, k, ratio16, n_objects, mult);
     }
+}
+
+// Per-query candidate space from the wants: bases are the exclusive prefix of
+// the wants (with `reserve_safe`, after a run that overflowed: the larger of
+// the two wants, so that a recount needs no new space); a query that does not
+// fit in `total` gets what is left (it will overflow, and the host grows the
+// total).  `used` receives the end of the assigned space (k_dense_check
+// places recounts after it).  One workgroup.
+__global__ void __launch_bounds__(1024) k_cap_prefix(const uint32_t* __restrict__ want,
+                                                    const uint32_t* __restrict__ want_safe, uint32_t reserve_safe,
+                                                    uint32_t n_queries, uint64_t total, uint32_t* __restrict__ cap,
+                                                    uint64_t* __restrict__ base, unsigned long long* __restrict__ used)
+{
+    __shared__ uint64_t s_w[16];
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+    uint64_t carry = 0;
+    for (uint32_t q0 = 0; q0 < n_queries; q0 += 1024) {
+        const uint32_t q = q0 + tid;
+        const uint32_t w = q < n_queries ? (reserve_safe ? max(want[q], want_safe[q]) : want[q]) : 0u;
+        uint64_t incl = w;  // 64-bit inclusive scan over the wave
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint64_t o = ((uint64_t)(uint32_t)__shfl_up((int)(uint32_t)(incl >> 32), off, kWave) << 32) |
+                               (uint32_t)__shfl_up((int)(uint32_t)incl, off, kWave);
+            if (lane >= off)
+                incl += o;
+        }
+        if (lane == kWave - 1)
+            s_w[wv] = incl;
+        __syncthreads();
+        uint64_t off = carry, all = carry;
+        for (int i = 0; i < 16; ++i) {
+            off += i < wv ? s_w[i] : 0;
+            all += s_w[i];
+        }
+        if (q < n_queries) {
+            const uint64_t b = off + incl - w;
+            base[q] = b;
+            cap[q] = b + w <= total ? w : (b < total ? (uint32_t)(total - b) : 0u);
+        }
+        carry = all;
+        __syncthreads();
+    }
+    if (tid == 0)
+        *used = carry;
 }
 
 // After the main pass: a query whose speculative threshold left it fewer than
@@ -194,9 +264,16 @@ __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict
 // marked for a second count; the query restarts its list from the safe
 // threshold, the other queries of the group keep theirs (threshold 255: no
 // count reaches it).  One wave per group.
+// A restarted query takes new candidate space sized for its safe threshold
+// after the assigned space (`used`, bumped atomically), when it fits in
+// `total`; otherwise it keeps its space and overflows, and the host grows the
+// total.
 __global__ void __launch_bounds__(64) k_dense_check(const uint32_t* __restrict__ kq, uint32_t n_queries,
                                                     uint32_t* __restrict__ gthr, const uint32_t* __restrict__ gsafe,
-                                                    uint32_t* __restrict__ cand_cnt, uint32_t* __restrict__ redo)
+                                                    uint32_t* __restrict__ cand_cnt, uint32_t* __restrict__ redo,
+                                                    const uint32_t* __restrict__ want_safe,
+                                                    uint32_t* __restrict__ cap, uint64_t* __restrict__ base,
+                                                    unsigned long long* __restrict__ used, uint64_t total)
 {
     const uint32_t g = blockIdx.x, lane = threadIdx.x, q = g * 32 + lane;
     const bool live = lane < 32 && q < n_queries;
@@ -208,6 +285,14 @@ __global__ void __launch_bounds__(64) k_dense_check(const uint32_t* __restrict__
         if (flagged) {
             cand_cnt[q] = 0;
             gthr[q] = gsafe[q];
+            const uint32_t w = want_safe[q];
+            if (w > cap[q]) {
+                const unsigned long long b = atomicAdd(used, (unsigned long long)w);
+                if (b + w <= total) {
+                    base[q] = b;
+                    cap[q] = w;
+                }
+            }
         } else {
             gthr[q] = 255u;
         }
