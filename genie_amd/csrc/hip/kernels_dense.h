@@ -423,9 +423,15 @@ __device__ __forceinline__ uint32_t dense_lane_count(const uint32_t (&c)[8], uin
 }
 
 // LDS bytes of the dense kernels.
+// Main pass: a tile's candidates are staged in LDS (up to kDenseStage of
+// them), so that each query reserves its candidates of the tile with one
+// global atomic.
+constexpr uint32_t kDenseStage = 2048;
+constexpr uint32_t kDenseStageWords = 2 + 32 + 32 + kDenseStage * 3;  // count, pad, per-query counts / bases, meta, keys
+
 __host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample)
 {
-    return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 8 + (sample ? 32 * (nblk * 16 + 1) : 0)) * 4u;
+    return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 8 + (sample ? 32 * (nblk * 16 + 1) : kDenseStageWords)) * 4u;
 }
 
 // Workgroup -> (query group, tiles) so that the groups of one tile run on one
@@ -441,6 +447,12 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     uint32_t* s_tab = s_lds;                             // DPAD * 256 masks
     uint32_t* s_thr = s_lds + DPAD * kDenseKeysPerDim;   // 8 threshold planes
     uint32_t* s_hist = s_thr + 8;                        // SAMPLE: 32 x (DPAD + 1)
+    // main pass: the tile's staged candidates
+    uint32_t* s_nhit = s_thr + 8;                        // staged so far
+    uint32_t* s_qcnt = s_nhit + 2;                       // per query of the group
+    uint32_t* s_qbase = s_qcnt + 32;                     // reserved base per query
+    uint32_t* s_meta = s_qbase + 32;                     // query << 16 | rank within the query
+    uint64_t* s_key = reinterpret_cast<uint64_t*>(s_meta + kDenseStage);
     constexpr uint32_t HB = NBLK * 16 + 1;
     const uint32_t tid = threadIdx.x;
     const uint32_t P = gridDim.x / 8, x = blockIdx.x % 8, s = blockIdx.x / 8;
@@ -511,6 +523,10 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 // threshold planes of the group (bit j of plane b = bit b of
                 // query j's threshold; 255 for lanes past the last query)
                 __syncthreads();
+                if (tid < 32)
+                    s_qcnt[tid] = 0;
+                if (tid == 0)
+                    *s_nhit = 0;
                 if (tid < kWave) {
                     const uint32_t q = g * 32 + tid;
                     const uint32_t th = (tid < 32 && q < a.n_queries)
@@ -558,14 +574,40 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     while (ge) {
                         const uint32_t j = (uint32_t)__builtin_ctz(ge);
                         ge &= ge - 1;
-                        const uint32_t q = g * 32 + j;
                         const uint64_t key = ((uint64_t)dense_lane_count(c, j) << 32) | (uint32_t)~o;
-                        const uint32_t slot = atomicAdd(&a.cand_cnt[q], 1u);
-                        if (slot < a.cand_cap[q])
-                            a.cand[a.cand_base[q] + slot] = key;
-                        else
-                            atomicOr(a.overflow, 1u);
+                        // staged when the stage has room (the rank is taken only
+                        // then, so a query's reserved slots have no holes)
+                        const uint32_t i = atomicAdd(s_nhit, 1u);
+                        if (i < kDenseStage) {
+                            s_meta[i] = j << 16 | atomicAdd(&s_qcnt[j], 1u);
+                            s_key[i] = key;
+                        } else {
+                            const uint32_t q = g * 32 + j;
+                            const uint32_t slot = atomicAdd(&a.cand_cnt[q], 1u);
+                            if (slot < a.cand_cap[q])
+                                a.cand[a.cand_base[q] + slot] = key;
+                            else
+                                atomicOr(a.overflow, 1u);
+                        }
                     }
+                }
+            }
+            if constexpr (!SAMPLE) {
+                // one reservation per query of the group, then the staged
+                // candidates to their slots
+                __syncthreads();
+                const uint32_t nh = min(*s_nhit, kDenseStage);
+                if (tid < 32 && s_qcnt[tid])
+                    s_qbase[tid] = atomicAdd(&a.cand_cnt[g * 32 + tid], s_qcnt[tid]);
+                __syncthreads();
+                for (uint32_t i = tid; i < nh; i += kDenseThreads) {
+                    const uint32_t m = s_meta[i], j = m >> 16;
+                    const uint32_t q = g * 32 + j;
+                    const uint32_t slot = s_qbase[j] + (m & 0xFFFFu);
+                    if (slot < a.cand_cap[q])
+                        a.cand[a.cand_base[q] + slot] = s_key[i];
+                    else
+                        atomicOr(a.overflow, 1u);
                 }
             }
         }
