@@ -106,6 +106,10 @@ int genie_server::launch(Executor& e)
     const uint32_t nq = (uint32_t)e.batch.size();
     std::vector<genie_dim_t> items;
     std::vector<uint32_t> kq(nq);
+    size_t n_items = 0;
+    for (const Request* r : e.batch)
+        n_items += r->items.size();
+    items.reserve(n_items);
     for (uint32_t q = 0; q < nq; ++q) {
         Request* r = e.batch[q];
         for (genie_dim_t it : r->items) {
@@ -148,20 +152,23 @@ void genie_server::finish(Executor& e, int rc, const std::string& err_in)
             err = genie_hip_last_error();
     }
     const auto now = Clock::now();
+    // the batch's requests are this executor's until marked done: their
+    // results are copied before the lock is taken
     uint64_t off = 0;
+    for (Request* r : e.batch) {
+        r->rc = rc;
+        r->err = err;
+        if (!rc) {
+            r->ids.assign(e.h_ids + off, e.h_ids + off + r->k);
+            r->counts.assign(e.h_counts + off, e.h_counts + off + r->k);
+        }
+        off += r->k;
+        r->t_done = now;
+    }
     {
         std::lock_guard<std::mutex> lk(mu);
-        for (Request* r : e.batch) {
-            r->rc = rc;
-            r->err = err;
-            if (!rc) {
-                r->ids.assign(e.h_ids + off, e.h_ids + off + r->k);
-                r->counts.assign(e.h_counts + off, e.h_counts + off + r->k);
-            }
-            off += r->k;
-            r->t_done = now;
+        for (Request* r : e.batch)
             r->done = true;
-        }
         n_done += e.batch.size();
         inflight -= e.batch.size();
         mem_in_use -= e.mem;
@@ -345,23 +352,28 @@ int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t 
     if (int rc = gh::validate_items(s->ix->d, r->items.data(), n_items, 1))
         return rc;
     r->t_submit = Clock::now();
+    bool wake;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         r->ticket = s->next_ticket++;
         *ticket = r->ticket;
+        // with requests already pending the scheduler needs no wake: it has
+        // filled every free executor and waits for a completion
+        wake = s->pending.empty();
         s->pending.push_back(r.get());
         s->reqs.emplace(r->ticket, std::move(r));
-        ++s->events;
+        if (wake)
+            ++s->events;
     }
-    s->cv_work.notify_one();
+    if (wake)
+        s->cv_work.notify_one();
     return GENIE_OK;
 }
 
-int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, int32_t* out_counts,
-                          double* latency_ms)
+// Waits for a ticket; returns its results and its submit and completion times.
+static int wait_ticket(genie_server* s, uint64_t ticket, int32_t* out_ids, int32_t* out_counts,
+                       Clock::time_point* t_submit, Clock::time_point* t_done)
 {
-    if (!s)
-        return fail_s(GENIE_E_INVALID, "null server");
     std::unique_ptr<Request> r;
     {
         std::unique_lock<std::mutex> lk(s->mu);
@@ -373,8 +385,8 @@ int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, in
         r = std::move(it->second);
         s->reqs.erase(it);
     }
-    if (latency_ms)
-        *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    *t_submit = r->t_submit;
+    *t_done = r->t_done;
     if (r->rc)
         return fail_s(r->rc, r->err);
     if (out_ids)
@@ -382,6 +394,18 @@ int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, in
     if (out_counts)
         std::memcpy(out_counts, r->counts.data(), r->k * 4);
     return GENIE_OK;
+}
+
+int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, int32_t* out_counts,
+                          double* latency_ms)
+{
+    if (!s)
+        return fail_s(GENIE_E_INVALID, "null server");
+    Clock::time_point ts, td;
+    const int rc = wait_ticket(s, ticket, out_ids, out_counts, &ts, &td);
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(td - ts).count();
+    return rc;
 }
 
 int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uint64_t* item_ptr,
@@ -422,8 +446,13 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
         if (submitted[r].load() < 0) {
             out_latency_ms[r] = -1;
         } else {
-            const int w = genie_hip_server_wait(s, tickets[r], out_ids ? out_ids + off : nullptr,
-                                                out_counts ? out_counts + off : nullptr, &out_latency_ms[r]);
+            // latency from the scheduled arrival: a generator running late
+            // counts against the server, not for it
+            Clock::time_point ts, td;
+            const int w = wait_ticket(s, tickets[r], out_ids ? out_ids + off : nullptr,
+                                      out_counts ? out_counts + off : nullptr, &ts, &td);
+            const auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[r]));
+            out_latency_ms[r] = std::chrono::duration<double, std::milli>(td - due).count();
             if (w && !rc)
                 rc = w;
         }

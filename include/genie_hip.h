@@ -222,8 +222,10 @@ int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, in
                           double* latency_ms);
 /* Open-loop trace replay: request r (items [item_ptr[r], item_ptr[r+1]),
  * k_per_request[r]) is submitted at arrival_s[r] seconds after the start by
- * a generator thread; returns per-request latency (ms), all results
- * (out_off[r] = sum of earlier k), and the wall time of the whole replay. */
+ * a generator thread; returns per-request latency (ms, from the scheduled
+ * arrival to completion, so a generator running late counts in it), all
+ * results (out_off[r] = sum of earlier k), and the wall time of the whole
+ * replay. */
 int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uint64_t* item_ptr,
                             const uint32_t* k_per_request, const double* arrival_s, uint32_t n_requests,
                             double* out_latency_ms, int32_t* out_ids, int32_t* out_counts, double* wall_s);
