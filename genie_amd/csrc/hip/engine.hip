@@ -1726,7 +1726,7 @@ static int run_dense(genie_plan* p, hipStream_t s)
 {
     const DevIndex& d = p->ix->d;
     const uint32_t nq = p->n_queries;
-    fill_u32(p->d_dhist, 0, (uint64_t)nq * kDenseBins, s);
+    // the sample histogram was cleared with the run's other resets
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[1], s));
     // the sample pass builds every group's mask table in LDS from the items
@@ -1831,11 +1831,19 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     const uint32_t ni = (uint32_t)p->n_items;
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[0], s));
-    fill_u32(p->d_gthr, 0, nq, s);
-    fill_u32(p->d_cand_cnt, 0, nq, s);
-    fill_u32(p->d_overflow, 0, 1, s);
-    fill_u32(p->d_hov, 0, nq, s);
-    fill_u32(p->d_hov_any, 0, 1, s);
+    // per-run resets in one launch (fill kernels, not memsets: graph-safe)
+    {
+        ZeroList z{};
+        z.p[0] = p->d_gthr, z.n[0] = nq;
+        z.p[1] = p->d_cand_cnt, z.n[1] = nq;
+        z.p[2] = p->d_overflow, z.n[2] = 1;
+        z.p[3] = p->d_hov, z.n[3] = nq;
+        z.p[4] = p->d_hov_any, z.n[4] = 1;
+        if (dense_active(p))
+            z.p[5] = p->d_dhist, z.n[5] = (uint64_t)nq * kDenseBins;
+        const uint64_t most = std::max<uint64_t>(nq, z.n[5]);
+        k_zero_many<<<(uint32_t)std::min<uint64_t>((most + 255) / 256, 1024), 256, 0, s>>>(z);
+    }
     p->timed_dense = dense_active(p);
     if (p->timed_dense)
         return run_dense(p, s);

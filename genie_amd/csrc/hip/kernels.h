@@ -14,6 +14,21 @@ namespace gh {
 // Small utility kernels
 // ---------------------------------------------------------------------------
 
+// Zeroes up to 6 arrays in one launch (the per-run resets of a plan).
+struct ZeroList {
+    uint32_t* p[6];
+    uint64_t n[6];
+};
+
+__global__ void k_zero_many(ZeroList z)
+{
+    const uint64_t i0 = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, st = (uint64_t)gridDim.x * blockDim.x;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+        for (uint64_t i = i0; i < z.n[a]; i += st)
+            z.p[a][i] = 0;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;

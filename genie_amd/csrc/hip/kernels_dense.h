@@ -151,9 +151,11 @@ __global__ void k_dense_holes(const uint32_t* __restrict__ bm, uint32_t n_object
 // objects counting >= c):
 //   safe: the largest c with S(c) >= k -- the k-th largest sample count, a
 //         lower bound of the final k-th count, or 1;
-//   spec: the largest c with S(c) >= j, j = min(k, 4 k n_sample / N + 4):
-//         where the sample holds 1/16 of the objects, some 4 k objects of the
-//         whole index are expected to reach it.
+//   spec: the largest c with S(c) >= j, j = min(k, m + 4 sqrt(m) + 4) with
+//         m = k n_sample / N, the sample objects expected among the top k:
+//         the final k-th count falls below spec only when the sample holds
+//         4 standard deviations (of a Poisson count) more of the top k than
+//         expected, or fewer than 4 objects in all.
 // The main pass emits from spec; a query left with fewer than k candidates
 // is counted again from safe (k_dense_check).  One wave per query, four bins
 // per lane.
@@ -190,8 +192,8 @@ __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict
     const uint32_t total = (uint32_t)__shfl((int)incl, kWave - 1, kWave);
     const uint32_t above = total - incl;
     const uint32_t k = kq[q];
-    const uint64_t jj = (4ull * k * n_sample + n_objects - 1) / n_objects + 4;
-    const uint32_t j = (uint32_t)min<uint64_t>(k, jj);
+    const float m = (float)k * (float)n_sample / (float)n_objects;
+    const uint32_t j = min(k, (uint32_t)ceilf(m + 4.0f * sqrtf(m) + 4.0f));
     const uint32_t s3 = above + h.w, s2 = s3 + h.z, s1 = s2 + h.y, s0 = s1 + h.x;
     const uint32_t b = 4 * lane;
     auto largest = [&](uint32_t need) {
