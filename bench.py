@@ -56,7 +56,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds of trace per offered load (0 = skip)")
-    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000])
+    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000, 200000, 400000])
+    ap.add_argument("--serve-max-requests", type=int, default=120000,
+                    help="requests per offered load at most (a load runs min(qps x seconds, this))")
     ap.add_argument("--serve-executors", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
     ap.add_argument("--serve-mode", type=int, default=2,
@@ -162,7 +164,7 @@ def main():
     serving = None
     if args.serve_seconds > 0:
         from genie_amd import Server, synth
-        nreq_max = int(max(args.serve_qps) * args.serve_seconds)
+        nreq_max = min(int(max(args.serve_qps) * args.serve_seconds), args.serve_max_requests)
         sq, _ = synth.queries(7001, w["rows"], nreq_max, noise=3, lo=0, hi=255)
         radius = np.array([0, 2, 5], np.int32)[synth.uniform_int(7002, 0, nreq_max, 0, 2)]
         sk = np.array([10, 100, 1000], np.uint32)[synth.uniform_int(7003, 1, nreq_max, 0, 2)]
@@ -176,7 +178,7 @@ def main():
                      mode=args.serve_mode)
         srv.replay(sitems[: int(sptr[64])], sptr[:65], sk[:64], np.zeros(64))  # warm
         for qps in args.serve_qps:
-            n = int(qps * args.serve_seconds)
+            n = min(int(qps * args.serve_seconds), nreq_max)
             arr = np.cumsum(synth.exponential(7004 + qps, n, float(qps)))
             lat, _, _, wall = srv.replay(sitems[: int(sptr[n])], sptr[: n + 1], sk[:n], arr)
             ent = dict(offered_qps=qps, requests=n, achieved_qps=n / wall, p50_ms=float(np.percentile(lat, 50)),
@@ -185,6 +187,11 @@ def main():
             log(f"[bench] serving @ {qps} QPS offered: achieved {ent['achieved_qps']:.0f} QPS, "
                 f"p50 {ent['p50_ms']:.2f} ms, p99 {ent['p99_ms']:.2f} ms")
         serving["stats"] = srv.stats()
+        # the highest load the server kept up with (achieved >= 97% of offered)
+        kept = [e for e in serving["loads"] if e["achieved_qps"] >= 0.97 * e["offered_qps"]]
+        serving["sustained_qps"] = max((e["achieved_qps"] for e in kept), default=None)
+        serving["sustained_vs_plain"] = (serving["sustained_qps"] / forms["plain"]["qps"]
+                                         if serving["sustained_qps"] else None)
         srv.close()
 
     # ---- CPU baseline + exactness on the sample ------------------------------

@@ -420,23 +420,31 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
         a.store(0);
     std::atomic<int> gen_rc(GENIE_OK);
     std::string gen_err;
+    std::mutex gen_mu;
     const auto t0 = Clock::now();
-    std::thread gen([&] {
-        for (uint32_t r = 0; r < n_requests; ++r) {
-            std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
-                                                   std::chrono::duration<double>(arrival_s[r])));
-            const int rc = genie_hip_server_submit(s, items + item_ptr[r], (uint32_t)(item_ptr[r + 1] - item_ptr[r]),
-                                                   k_per_request[r], &tickets[r]);
-            if (rc) {
-                gen_rc = rc;
-                gen_err = genie_hip_last_error();
-                submitted[r].store(-1);
-                // later requests are still submitted so the trace keeps its shape
-                continue;
+    // open-loop generators: thread g submits requests g, g + G, ... each at its
+    // arrival time, so that one thread's submission cost does not cap the load
+    constexpr uint32_t G = 4;
+    std::vector<std::thread> gens;
+    for (uint32_t g = 0; g < G; ++g)
+        gens.emplace_back([&, g] {
+            for (uint32_t r = g; r < n_requests; r += G) {
+                std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
+                                                       std::chrono::duration<double>(arrival_s[r])));
+                const int rc = genie_hip_server_submit(s, items + item_ptr[r],
+                                                       (uint32_t)(item_ptr[r + 1] - item_ptr[r]), k_per_request[r],
+                                                       &tickets[r]);
+                if (rc) {
+                    std::lock_guard<std::mutex> lk(gen_mu);
+                    gen_rc = rc;
+                    gen_err = genie_hip_last_error();
+                    submitted[r].store(-1);
+                    // later requests are still submitted so the trace keeps its shape
+                    continue;
+                }
+                submitted[r].store(1);
             }
-            submitted[r].store(1);
-        }
-    });
+        });
     int rc = GENIE_OK;
     uint64_t off = 0;
     for (uint32_t r = 0; r < n_requests; ++r) {
@@ -458,7 +466,8 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
         }
         off += k;
     }
-    gen.join();
+    for (std::thread& t : gens)
+        t.join();
     if (wall_s)
         *wall_s = std::chrono::duration<double>(Clock::now() - t0).count();
     if (gen_rc != GENIE_OK)

@@ -222,7 +222,7 @@ int genie_hip_server_wait(genie_server* s, uint64_t ticket, int32_t* out_ids, in
                           double* latency_ms);
 /* Open-loop trace replay: request r (items [item_ptr[r], item_ptr[r+1]),
  * k_per_request[r]) is submitted at arrival_s[r] seconds after the start by
- * a generator thread; returns per-request latency (ms, from the scheduled
+ * one of 4 generator threads (r mod 4); returns per-request latency (ms, from the scheduled
  * arrival to completion, so a generator running late counts in it), all
  * results (out_off[r] = sum of earlier k), and the wall time of the whole
  * replay. */
