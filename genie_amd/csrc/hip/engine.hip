@@ -119,6 +119,17 @@ struct genie_plan {
     uint32_t* d_dq = nullptr;
     uint32_t* d_hov = nullptr;
     uint32_t* d_hov_any = nullptr;
+    // compressed index, hash form: the batch's segments decoded once into a
+    // scratch posting array, with per-query span lists
+    bool dec_spans = false;
+    std::vector<Span> h_sspan;
+    std::vector<uint32_t> h_sq_ptr;
+    std::vector<uint2> h_djobs;
+    Span* d_sspan = nullptr;
+    uint32_t* d_sq_ptr = nullptr;
+    uint2* d_djobs = nullptr;
+    uint32_t* d_scratch = nullptr;
+    uint64_t sspan_alloc = 0, djobs_alloc = 0, scratch_alloc = 0;
     // device buffers
     genie_dim_t* d_items = nullptr;
     uint32_t* d_item_qptr = nullptr;
@@ -255,6 +266,10 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_dq);
     dev_free(p->d_hov);
     dev_free(p->d_hov_any);
+    dev_free(p->d_sspan);
+    dev_free(p->d_sq_ptr);
+    dev_free(p->d_djobs);
+    dev_free(p->d_scratch);
 }
 
 // Count bound of every query: per (query, order) group the largest weight
@@ -1364,6 +1379,72 @@ static void plain_query_stats(genie_plan* p)
     p->plain_stats = true;
 }
 
+// Compressed index, hash form: every item of a hash-form query becomes a
+// span of the batch's scratch posting array (its segments' ids, in item
+// order), decoded once per batch by k_decode_jobs; the hash kernel then
+// counts them like uncompressed postings, keeping the ids of its tile.  A
+// batch whose decoded postings pass 2^31 keeps the in-kernel decode.
+static int plan_decoded_spans(genie_plan* p)
+{
+    const DevIndex& d = p->ix->d;
+    p->dec_spans = false;
+    if (d.codec == 0 || p->h_hwork.empty())
+        return GENIE_OK;
+    const uint32_t nq = p->n_queries;
+    const uint32_t* cp = d.h_seg_cpos.data();
+    const uint32_t* up = d.h_seg_upos.data();
+    p->h_sspan.clear();
+    p->h_djobs.clear();
+    p->h_sq_ptr.assign(nq + 1, 0);
+    uint64_t off = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        p->h_sq_ptr[q] = (uint32_t)p->h_sspan.size();
+        if (!p->h_hq[q].x)
+            continue;
+        for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
+            const genie_dim_t& it = p->h_items[i];
+            if (it.end_pos <= it.start_pos || it.weight == 0.0f)
+                continue;
+            const uint32_t s0 = (uint32_t)(std::lower_bound(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) - cp);
+            const uint32_t s1 = (uint32_t)(std::lower_bound(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos) - cp);
+            const uint64_t n = up[s1] - up[s0];
+            if (off + n >= 0x80000000ull)
+                return GENIE_OK;  // too large for one batch: in-kernel decode
+            p->h_sspan.push_back(Span{(uint32_t)off, (uint32_t)(off + n), (uint32_t)it.weight << 1 | 1u});
+            for (uint32_t sg = s0; sg < s1; ++sg)
+                p->h_djobs.push_back(make_uint2(sg, (uint32_t)(off + up[sg] - up[s0])));
+            off += n;
+        }
+    }
+    p->h_sq_ptr[nq] = (uint32_t)p->h_sspan.size();
+    auto grow = [](auto** ptr, uint64_t& alloc, uint64_t need) -> int {
+        if (need <= alloc)
+            return GENIE_OK;
+        dev_free(*ptr);
+        alloc = 0;
+        if (int rc = dev_alloc(ptr, need))
+            return rc;
+        alloc = need;
+        return GENIE_OK;
+    };
+    if (int rc = grow(&p->d_sspan, p->sspan_alloc, std::max<uint64_t>(1, p->h_sspan.size())))
+        return rc;
+    if (int rc = grow(&p->d_djobs, p->djobs_alloc, std::max<uint64_t>(1, p->h_djobs.size())))
+        return rc;
+    if (int rc = grow(&p->d_scratch, p->scratch_alloc, std::max<uint64_t>(1, off)))
+        return rc;
+    if (!p->d_sq_ptr)
+        if (int rc = dev_alloc(&p->d_sq_ptr, p->max_queries + 1))
+            return rc;
+    if (!p->h_sspan.empty())
+        HIP_TRY(hipMemcpy(p->d_sspan, p->h_sspan.data(), p->h_sspan.size() * sizeof(Span), hipMemcpyHostToDevice));
+    if (!p->h_djobs.empty())
+        HIP_TRY(hipMemcpy(p->d_djobs, p->h_djobs.data(), p->h_djobs.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_sq_ptr, p->h_sq_ptr.data(), (nq + 1) * 4ull, hipMemcpyHostToDevice));
+    p->dec_spans = true;
+    return GENIE_OK;
+}
+
 // The plain form's split of the batch: a query with at most N / 16 postings
 // (every query whose counts fit, when the hash form is forced) runs the hash
 // kernel over tiles of whole directory tiles holding about kHashTarget of its
@@ -1421,6 +1502,8 @@ static int plan_split_plain(genie_plan* p)
             p->est_postings += P;
         }
     }
+    if (int rc = plan_decoded_spans(p))
+        return rc;
     if (p->h_hwork.size() > p->hwork_alloc) {
         dev_free(p->d_hwork);
         p->hwork_alloc = 0;
@@ -1670,10 +1753,24 @@ static int launch_scan(genie_plan* p, hipStream_t s)
         h.hq = p->d_hq;
         h.hov = p->d_hov;
         h.hov_any = p->d_hov_any;
-        if (p->ix->d.codec == 0)
+        h.spans = nullptr;
+        h.sq_ptr = nullptr;
+        const DevIndex& d = p->ix->d;
+        if (d.codec == 0) {
             k_scan_hash<false><<<(uint32_t)nwork, kScanThreads, 0, s>>>(h);
-        else
+        } else if (p->dec_spans) {
+            const uint32_t nj = (uint32_t)p->h_djobs.size();
+            if (nj)
+                k_decode_jobs<<<std::min<uint32_t>((nj + kDecodeWaves - 1) / kDecodeWaves, 65535),
+                                kDecodeWaves * kWave, 0, s>>>(d.codec, d.words, d.seg_cpos, p->d_djobs, nj,
+                                                              p->d_scratch);
+            h.s.post = p->d_scratch;
+            h.spans = p->d_sspan;
+            h.sq_ptr = p->d_sq_ptr;
+            k_scan_hash<false><<<(uint32_t)nwork, kScanThreads, 0, s>>>(h);
+        } else {
             k_scan_hash<true><<<(uint32_t)nwork, kScanThreads, 0, s>>>(h);
+        }
     }
     if (ndq) {
         a.qlist = ndq == p->n_queries ? nullptr : p->d_dq;

@@ -23,7 +23,37 @@ struct HashArgs {
     const uint4* hq;       // [n_queries]: directory tiles per tile, key bits, log2 slots, 0
     uint32_t* hov;         // [n_queries] set when a table of the query filled up
     uint32_t* hov_any;
+    // decoded spans of a compressed index (null otherwise): query q's spans
+    // are spans[sq_ptr[q] .. sq_ptr[q + 1]), positions in s.post (the batch's
+    // decoded postings), filter flag set
+    const Span* spans;
+    const uint32_t* sq_ptr;
 };
+
+// Decodes the segments a batch's hash-form queries name, once each per query
+// item, into the batch's scratch posting array: job = (segment, destination
+// offset); one wave per job.  Segments were validated at index creation.
+__global__ void __launch_bounds__(kDecodeWaves * kWave)
+    k_decode_jobs(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
+                  const uint2* __restrict__ jobs, uint32_t n_jobs, uint32_t* __restrict__ out)
+{
+    __shared__ DecodeScratch S[kDecodeWaves];
+    const int wv = threadIdx.x / kWave, lane = lane_id();
+    for (uint32_t j = blockIdx.x * kDecodeWaves + wv; j < n_jobs; j += gridDim.x * kDecodeWaves) {
+        const uint2 jb = jobs[j];
+        const uint32_t w0 = cpos[jb.x], nw = cpos[jb.x + 1] - w0;
+        stage_words(S[wv], words, w0, nw);
+        uint32_t v[kPerLane];
+        const uint32_t n = min(wave_decode(codec, S[wv], nw, v), (uint32_t)kSegMax);
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k) {
+            const uint32_t i = lane * kPerLane + k;
+            if (i < n)
+                out[jb.y + i] = v[k];
+        }
+        wave_sync();  // the staging buffer is reused by the next job
+    }
+}
 
 // Adds w to the count of tile-local id `loc`.  A slot word holds key = loc + 1
 // in its high kb bits and the count in the low 32 - kb bits (the host sizes
@@ -270,7 +300,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
 
     bool ok = true;
     const uint32_t c0 = a.qc_ptr[q], c1 = a.qc_ptr[q + 1];
-    const uint32_t pbase = a.qk_ptr[q], npairs = a.qk_ptr[q + 1] - pbase;
+    const uint32_t* pptr = h.spans ? h.sq_ptr : a.qk_ptr;  // decoded spans, or (chunk, key) pairs
+    const uint32_t pbase = pptr[q], npairs = pptr[q + 1] - pbase;
     for (uint32_t w0 = 0; w0 < npairs; w0 += kSpanCap) {
         const uint32_t wn = min((uint32_t)kSpanCap, npairs - w0);
         if constexpr (C) {
@@ -305,7 +336,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
                 wave_sync();  // the staging buffer is reused by the next segment
             }
         } else {
-            resolve_spans(a, c0, c1, pbase, w0, wn, dt0, dt1, s_span);
+            if (h.spans) {
+                for (uint32_t i = tid; i < wn; i += kScanThreads)
+                    s_span[i] = h.spans[pbase + w0 + i];
+            } else {
+                resolve_spans(a, c0, c1, pbase, w0, wn, dt0, dt1, s_span);
+            }
             __syncthreads();
             const uint32_t total = flat_prefix(s_span, wn, s_wsum);
             for (uint32_t g0 = 0; g0 < total; g0 += 4 * kScanThreads) {
