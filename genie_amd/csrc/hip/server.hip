@@ -64,12 +64,15 @@ struct Executor {
     std::thread th;
 };
 
-// Device scratch one request holds while in flight: its work items, the
-// candidate space (8 tiles' worth of k keys, grown on overflow) and its
-// output rows.
-uint64_t request_bytes(uint64_t n_items, uint32_t k)
+// Device scratch one request holds while in flight: its work items, its
+// candidate space as a plan first sizes it (64 k keys, at most one per
+// object, when the dense form can run: an index with forward codes and a
+// batched or automatic plan; else 8 k keys; grown on overflow) and its output
+// rows.
+uint64_t request_bytes(uint64_t n_items, uint32_t k, bool dense, uint32_t n_objects)
 {
-    return n_items * sizeof(genie_dim_t) + (uint64_t)k * 8 * 8 + (uint64_t)k * 8 + 64;
+    const uint64_t cand = dense ? std::min<uint64_t>(64ull * k, n_objects) : 8ull * k;
+    return n_items * sizeof(genie_dim_t) + cand * 8 + (uint64_t)k * 8 + 64;
 }
 
 }  // namespace
@@ -347,7 +350,7 @@ int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t 
     for (genie_dim_t& it : r->items)
         it.query = 0;
     r->k = k;
-    r->mem = request_bytes(n_items, k);
+    r->mem = request_bytes(n_items, k, s->ix->d.dense && s->cfg.mode != 0, s->ix->d.n_objects);
     // validate alone so that one bad request cannot fail a whole batch
     if (int rc = gh::validate_items(s->ix->d, r->items.data(), n_items, 1))
         return rc;
