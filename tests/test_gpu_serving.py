@@ -48,6 +48,32 @@ def test_alone_equals_in_the_mix(mode, codec):
     srv.close()
 
 
+@pytest.mark.parametrize("codec", [0, 13])
+def test_selective_requests_on_the_hash_form(codec):
+    """LSH-style exact-match requests (8,192 buckets per dim): the automatic
+    plan runs them on the hash form (compressed: segments decoded once per
+    batch); in the mix each equals its lone answer and the one-shot search
+    of the whole batch (the hash form's oracle parity is in test_gpu_hash)."""
+    rows = genie_amd.synth.rows(301, 300_000, 32, genie_amd.synth.UNIFORM, 0.0, 0.0, 0, 8191, 8192)
+    qv, _ = genie_amd.synth.queries(302, rows, 40, redraw_p=0.3, n_values=8192)
+    ranges = W.single_ranges(qv, 0)
+    t = Table.from_rows(rows, 1024, codec != 0, codec=codec)
+    w = dict(ranges=ranges, n_queries=40)
+    items, ptr = requests_of(t, w)
+    k = np.array([10, 100, 1000, 50] * 10, np.uint32)
+    srv = Server(t.device(), executors=3, max_batch=16, max_k=1000, mode=2)
+    lat, ids, cnt, wall = srv.replay(items, ptr, k, np.cumsum(genie_amd.synth.exponential(11, 40, 2000.0)))
+    pi, pc = match_topk(t.device(), items, 40, k)
+    assert np.array_equal(pi, ids) and np.array_equal(pc, cnt)
+    off = 0
+    for r in (0, 7, 22):
+        tk = srv.submit(items[ptr[r]:ptr[r + 1]], int(k[r]))
+        a, c, _ = srv.wait(tk, int(k[r]))
+        o = int(k[:r].sum())
+        assert np.array_equal(a, ids[o:o + int(k[r])]) and np.array_equal(c, cnt[o:o + int(k[r])]), r
+    srv.close()
+
+
 def test_memory_budget_admits_one_at_a_time():
     w = W.cfg1(n=50_000, d=16, q=12, k=100)
     t = Table.from_rows(w["rows"])
