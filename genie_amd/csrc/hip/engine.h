@@ -68,6 +68,12 @@ struct DevIndex {
     uint32_t dense_npad = 0;        // objects rounded up to 64 (row stride of codes)
     uint32_t* codes = nullptr;      // [dense_words(dpad)][dense_npad] packed 10-bit fields (code * 4)
     uint32_t* key_slot = nullptr;   // [n_keys] dim * 256 + code
+    // forward lists of a set-valued table (set form of the batched search):
+    // the keys of every object's occurrences, duplicates kept
+    bool sets = false;
+    uint32_t set_max_len = 0;       // most occurrences of one object
+    uint32_t* obj_ptr = nullptr;    // [n_objects + 1]
+    uint32_t* obj_key = nullptr;    // [n_postings]
     uint64_t bytes = 0;
     // host copies used for validation and count bounds
     std::vector<uint64_t> h_key_start;

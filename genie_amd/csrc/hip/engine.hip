@@ -178,6 +178,16 @@ struct genie_plan {
     // dense form of the batched search (single-value index, whole-key unit-weight batch)
     bool dense_on = true;            // use it when the batch allows
     bool dense_ok = false;           // the uploaded batch allows it
+    bool sets_ok = false;            // ... or its set form (set-valued tables)
+    std::vector<uint32_t> h_skeys;   // set form: every group's sorted keys,
+    std::vector<uint4> h_smask;      // their weight planes (two uint4 per key)
+    std::vector<uint32_t> h_sgptr;   // and the groups' entry ranges
+    std::vector<uint32_t> h_sshift;  // per query: sample histogram shift
+    uint32_t* d_skeys = nullptr;
+    uint4* d_smask = nullptr;
+    uint32_t* d_sgptr = nullptr;
+    uint32_t* d_sshift = nullptr;
+    uint64_t sent_alloc = 0;
     uint32_t dn_groups = 0;
     uint32_t* d_dtab = nullptr;      // [dn_groups][dpad * 256] query masks
     uint64_t dtab_alloc = 0;
@@ -266,6 +276,10 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_dq);
     dev_free(p->d_hov);
     dev_free(p->d_hov_any);
+    dev_free(p->d_skeys);
+    dev_free(p->d_smask);
+    dev_free(p->d_sgptr);
+    dev_free(p->d_sshift);
     dev_free(p->d_sspan);
     dev_free(p->d_sq_ptr);
     dev_free(p->d_djobs);
@@ -416,6 +430,7 @@ int gh::validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_i
 }
 
 static bool dense_shape(const genie_index_desc* desc, std::vector<uint32_t>& key_base, std::vector<uint32_t>& per_dim);
+static int build_sets(const genie_index_desc* desc, DevIndex& d);
 static int build_dense(const genie_index_desc* desc, DevIndex& d, const uint32_t* post, const uint32_t* key_start);
 
 // Compressed index: words, segment offsets, key -> segment map, per-segment
@@ -568,6 +583,8 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         }
         if (want_dense)
             rc = build_dense(desc, d, d_post, d_ks);
+        if (!rc)
+            rc = build_sets(desc, d);
     } while (0);
     dev_free(d_upos);
     dev_free(d_dir_keys);
@@ -879,6 +896,52 @@ static int build_dense(const genie_index_desc* desc, DevIndex& d, const uint32_t
     return GENIE_OK;
 }
 
+// Forward lists for the set form: for a set-valued table whose postings are
+// given and whose objects hold at most 255 occurrences, every object's keys
+// (an object listed twice in one key's list appears twice).  Built on the
+// host in one counting pass over the postings.  Any other index simply has
+// no set form; only device errors fail.
+static int build_sets(const genie_index_desc* desc, DevIndex& d)
+{
+    if (desc->single_value || !desc->postings || desc->n_postings == 0 || desc->n_keys >= (1u << 24))
+        return GENIE_OK;
+    const uint32_t N = desc->n_objects, K = desc->n_keys;
+    std::vector<uint32_t> ptr(N + 1, 0);
+    for (uint64_t i = 0; i < desc->n_postings; ++i)
+        ptr[desc->postings[i] + 1]++;
+    uint32_t mx = 0;
+    for (uint32_t o = 0; o < N; ++o) {
+        mx = std::max(mx, ptr[o + 1]);
+        ptr[o + 1] += ptr[o];
+    }
+    if (mx > 255)
+        return GENIE_OK;
+    std::vector<uint32_t> keys(desc->n_postings), fill(ptr.begin(), ptr.end() - 1);
+    for (uint32_t k = 0; k < K; ++k)
+        for (uint64_t i = desc->key_start[k]; i < desc->key_start[k + 1]; ++i)
+            keys[fill[desc->postings[i]]++] = k;
+    uint32_t *obj_ptr = nullptr, *obj_key = nullptr;
+    uint64_t bytes = 0;
+    int rc = dev_alloc(&obj_ptr, N + 1ull, &bytes);
+    if (!rc)
+        rc = dev_alloc(&obj_key, desc->n_postings, &bytes);
+    hipError_t e = hipSuccess;
+    if (!rc && ((e = hipMemcpy(obj_ptr, ptr.data(), (N + 1ull) * 4, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(obj_key, keys.data(), desc->n_postings * 4, hipMemcpyHostToDevice))))
+        rc = fail(GENIE_E_RUNTIME, std::string("forward lists: ") + hipGetErrorString(e));
+    if (rc) {
+        dev_free(obj_ptr);
+        dev_free(obj_key);
+        return rc;
+    }
+    d.sets = true;
+    d.set_max_len = mx;
+    d.obj_ptr = obj_ptr;
+    d.obj_key = obj_key;
+    d.bytes += bytes;
+    return GENIE_OK;
+}
+
 extern "C" {
 
 const char* genie_hip_last_error(void) { return g_err.c_str(); }
@@ -1029,6 +1092,8 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
     dev_free(d_dir_keys);
     if (!rc)
         rc = build_dense(desc, d, d.post, d.key_start);
+    if (!rc)
+        rc = build_sets(desc, d);
     if (rc) {
         genie_hip_index_destroy(ix);
         return rc;
@@ -1056,6 +1121,8 @@ int genie_hip_index_destroy(genie_index* ix)
     dev_free(d.sdir);
     dev_free(d.codes);
     dev_free(d.key_slot);
+    dev_free(d.obj_ptr);
+    dev_free(d.obj_key);
     delete ix;
     return GENIE_OK;
 }
@@ -1150,7 +1217,7 @@ int genie_hip_plan_destroy(genie_plan* p)
 
 // Mode 1 (batched) and mode 2 (automatic) run the dense form when the batch
 // allows it; otherwise mode 1 runs the lane form and mode 2 the plain form.
-static bool dense_active(const genie_plan* p) { return p->mode != 0 && p->dense_on && p->dense_ok; }
+static bool dense_active(const genie_plan* p) { return p->mode != 0 && p->dense_on && (p->dense_ok || p->sets_ok); }
 
 static uint32_t plan_tile_objs(const genie_plan* p)
 {
@@ -1275,6 +1342,106 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
     return true;
 }
 
+// Whether the uploaded batch can run the set form (set-valued tables with
+// forward lists): per query, the sum of its items' weights over the
+// positions (items may repeat a key, or split it into segments or chunks) is
+// constant over every key it touches, i.e. a weight per key, at most
+// kSetMaxWeight; the longest forward list times the query's largest weight is
+// below kSetNoQuery (that threshold marks "no query"); a group of 32 queries
+// holds at most kSetEntries keys.  Builds every group's entries (keys
+// ascending, bit j of plane b = bit b of query j's weight for the key) and
+// every query's sample histogram shift (its count bound >> shift < 256).
+static bool sets_batch_ok(genie_plan* p, uint32_t n_queries)
+{
+    const DevIndex& d = p->ix->d;
+    if (p->mode == 0 || !d.sets)
+        return false;
+    const uint64_t* ks = d.codec == 0 ? d.h_key_start.data() : d.h_key_wstart.data();
+    struct Ev {
+        uint64_t pos;
+        int64_t dw;
+    };
+    struct KW {
+        uint32_t key, q, w;
+    };
+    std::vector<Ev> ev;
+    std::vector<KW> gk;  // the group's (key, query in group, weight)
+    const uint32_t G = (n_queries + 31) / 32;
+    p->h_skeys.clear();
+    p->h_smask.clear();
+    p->h_sgptr.assign(G + 1, 0);
+    p->h_sshift.assign(n_queries, 0);
+    for (uint32_t g = 0; g < G; ++g) {
+        gk.clear();
+        for (uint32_t q = g * 32; q < std::min(n_queries, g * 32 + 32); ++q) {
+            // the query's weight as a step function of the position: items add
+            // their weight over their span (a key named twice adds twice)
+            ev.clear();
+            for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
+                const genie_dim_t& it = p->h_items[i];
+                if (it.end_pos <= it.start_pos || it.weight == 0.0f)
+                    continue;
+                if (it.weight > (float)kSetMaxWeight)
+                    return false;
+                ev.push_back({(uint64_t)it.start_pos, (int64_t)it.weight});
+                ev.push_back({(uint64_t)it.end_pos, -(int64_t)it.weight});
+            }
+            std::sort(ev.begin(), ev.end(), [](const Ev& x, const Ev& y) { return x.pos < y.pos; });
+            // every piece of constant nonzero weight covers whole keys
+            uint32_t wmax = 0;
+            const size_t first = gk.size();
+            int64_t w = 0;
+            uint64_t from = 0;
+            for (size_t i = 0; i < ev.size();) {
+                const uint64_t pos = ev[i].pos;
+                int64_t nw = w;
+                for (; i < ev.size() && ev[i].pos == pos; ++i)
+                    nw += ev[i].dw;
+                if (nw == w)
+                    continue;
+                if (w > 0) {
+                    if (w > (int64_t)kSetMaxWeight)
+                        return false;
+                    const uint32_t ka = d.key_pos.last_le(ks, from);
+                    const uint32_t kb = d.key_pos.first_ge(ks, pos);
+                    if (ks[ka] != from || kb > d.n_keys || ks[kb] != pos)
+                        return false;
+                    for (uint32_t k = ka; k < kb; ++k)
+                        if (ks[k + 1] > ks[k])  // keys with postings
+                            gk.push_back({k, q - g * 32, (uint32_t)w});
+                    if (gk.size() - first > kSetEntries)
+                        return false;  // more keys than one group's table holds
+                    wmax = std::max(wmax, (uint32_t)w);
+                }
+                w = nw;
+                from = pos;
+            }
+            const uint64_t bound = (uint64_t)d.set_max_len * wmax;
+            if (bound >= kSetNoQuery)
+                return false;
+            uint32_t sh = 0;
+            while ((bound >> sh) > 255)
+                ++sh;
+            p->h_sshift[q] = sh;
+        }
+        std::sort(gk.begin(), gk.end(), [](const KW& x, const KW& y) { return x.key < y.key; });
+        for (size_t i = 0; i < gk.size();) {
+            uint32_t m[8] = {};
+            const uint32_t key = gk[i].key;
+            for (; i < gk.size() && gk[i].key == key; ++i)
+                for (int b = 0; b < kSetWeightPlanes; ++b)
+                    m[b] |= ((gk[i].w >> b) & 1u) << gk[i].q;
+            p->h_skeys.push_back(key);
+            p->h_smask.push_back(make_uint4(m[0], m[1], m[2], m[3]));
+            p->h_smask.push_back(make_uint4(m[4], m[5], m[6], m[7]));
+        }
+        if (p->h_skeys.size() - p->h_sgptr[g] > kSetEntries)
+            return false;
+        p->h_sgptr[g + 1] = (uint32_t)p->h_skeys.size();
+    }
+    return true;
+}
+
 // Device buffers of the dense form for the uploaded batch: the query mask
 // tables, the sample histogram and the sample tile list (evenly spread tiles,
 // at least 16 and at least 1/16 of all tiles).
@@ -1293,11 +1460,37 @@ static int prepare_dense(genie_plan* p)
     if (!p->d_dhist)
         if (int rc = dev_alloc(&p->d_dhist, (uint64_t)p->max_queries * kDenseBins))
             return rc;
-    if (!p->d_dkeys)
-        if (int rc = dev_alloc(&p->d_dkeys, 2 * std::max<uint64_t>(1, p->max_items)))
-            return rc;
-    if (p->n_items)
-        HIP_TRY(hipMemcpy(p->d_dkeys, p->h_dkeys.data(), p->n_items * 8, hipMemcpyHostToDevice));
+    if (p->dense_ok) {
+        if (!p->d_dkeys)
+            if (int rc = dev_alloc(&p->d_dkeys, 2 * std::max<uint64_t>(1, p->max_items)))
+                return rc;
+        if (p->n_items)
+            HIP_TRY(hipMemcpy(p->d_dkeys, p->h_dkeys.data(), p->n_items * 8, hipMemcpyHostToDevice));
+    } else {  // set form: the groups' key entries
+        const uint64_t ne = std::max<uint64_t>(1, p->h_skeys.size());
+        if (ne > p->sent_alloc) {
+            dev_free(p->d_skeys);
+            dev_free(p->d_smask);
+            p->sent_alloc = 0;
+            if (int rc = dev_alloc(&p->d_skeys, ne))
+                return rc;
+            if (int rc = dev_alloc(&p->d_smask, 2 * ne))
+                return rc;
+            p->sent_alloc = ne;
+        }
+        if (!p->d_sgptr) {
+            if (int rc = dev_alloc(&p->d_sgptr, (p->max_queries + 31) / 32 + 1))
+                return rc;
+            if (int rc = dev_alloc(&p->d_sshift, p->max_queries))
+                return rc;
+        }
+        if (!p->h_skeys.empty()) {
+            HIP_TRY(hipMemcpy(p->d_skeys, p->h_skeys.data(), p->h_skeys.size() * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(p->d_smask, p->h_smask.data(), p->h_smask.size() * sizeof(uint4), hipMemcpyHostToDevice));
+        }
+        HIP_TRY(hipMemcpy(p->d_sgptr, p->h_sgptr.data(), p->h_sgptr.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(p->d_sshift, p->h_sshift.data(), p->h_sshift.size() * 4, hipMemcpyHostToDevice));
+    }
     if (!p->d_gsafe)
         if (int rc = dev_alloc(&p->d_gsafe, p->max_queries))
             return rc;
@@ -1354,7 +1547,7 @@ static void plain_query_stats(genie_plan* p)
     const uint32_t nq = p->n_queries;
     if (p->count_bound != 0) {
         p->h_qbound.assign(nq, p->count_bound);
-    } else if (p->h_qbound.size() != nq || p->dense_ok) {  // upload skips the bounds of a dense batch
+    } else if (p->h_qbound.size() != nq || p->dense_ok) {  // upload skips a dense batch's bounds
         p->h_qbound.resize(nq);
         compute_bounds(d, p->h_items.data(), p->n_items, nq, p->h_qbound.data());
     }
@@ -1573,6 +1766,7 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     p->h_qptr.swap(qptr);
     p->bytes_valid = false;
     p->dense_ok = dense_batch_ok(p, n_queries);
+    p->sets_ok = !p->dense_ok && sets_batch_ok(p, n_queries);
     p->plain_stats = false;
     p->h_force_direct.assign(n_queries, 0);
     p->h_hwork.clear();
@@ -1582,7 +1776,7 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
     // count bound of this batch
     if (p->count_bound == 0) {
         if (p->dense_ok) {
-            p->lane_bits = 8;  // unit weights, each dim once: counts <= 128
+            p->lane_bits = 8;  // dense: unit weights, each dim once, counts <= 128
         } else {
             p->h_qbound.resize(n_queries);
             compute_bounds(d, p->h_items.data(), n_items, n_queries, p->h_qbound.data());
@@ -1787,10 +1981,10 @@ static int launch_merge(genie_plan* p, hipStream_t s);
 
 // Grid of the dense kernels: every CU's share of resident workgroups, a
 // multiple of the 8 XCDs the workgroup ids are dealt to.
-template <int NBLK, bool SAMPLE>
+template <int NBLK, bool SAMPLE, bool SETS = false>
 static int launch_dense_t(const DenseArgs& a, hipStream_t s)
 {
-    auto kern = k_dense_count<NBLK, SAMPLE>;  // static LDS (dense_lds_bytes)
+    auto kern = k_dense_count<NBLK, SAMPLE, SETS>;  // static LDS
     int per_cu = 0, n_cu = 0, dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1806,6 +2000,7 @@ template <bool SAMPLE>
 static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s)
 {
     switch (nblk) {
+    case 0: return launch_dense_t<0, SAMPLE, true>(a, s);
     case 1: return launch_dense_t<1, SAMPLE>(a, s);
     case 2: return launch_dense_t<2, SAMPLE>(a, s);
     case 3: return launch_dense_t<3, SAMPLE>(a, s);
@@ -1851,7 +2046,13 @@ static int run_dense(genie_plan* p, hipStream_t s)
     a.cand_base = p->d_cand_base;
     a.cand = p->d_cand;
     a.overflow = p->d_overflow;
-    const uint32_t nblk = d.dense_dpad / 16;
+    a.obj_ptr = d.obj_ptr;
+    a.obj_key = d.obj_key;
+    a.set_keys = p->d_skeys;
+    a.set_mask = p->d_smask;
+    a.set_gptr = p->d_sgptr;
+    a.hshift = p->dense_ok ? nullptr : p->d_sshift;
+    const uint32_t nblk = p->dense_ok ? d.dense_dpad / 16 : 0;  // 0: the set form
     if (int rc = launch_dense<true>(nblk, a, s))
         return rc;
     // thresholds, then every query's candidate space from its sample
@@ -1859,7 +2060,7 @@ static int run_dense(genie_plan* p, hipStream_t s)
     const uint32_t ratio16 = (uint32_t)std::min<uint64_t>(
         ((uint64_t)d.n_objects << 16) / std::max<uint32_t>(1, p->n_dsample) + 1, 0xFFFFFFFFull);
     k_dense_thresh<<<(nq + 3) / 4, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->n_dsample, d.n_objects, ratio16, mult,
-                                                p->d_gthr, p->d_gsafe, p->d_want, p->d_want_safe);
+                                                a.hshift, p->d_gthr, p->d_gsafe, p->d_want, p->d_want_safe);
     k_cap_prefix<<<1, 1024, 0, s>>>(p->d_want, p->d_want_safe, mult > 1 ? 1u : 0u, nq, p->total_cap, p->d_cand_cap,
                                     p->d_cand_base, p->d_cap_used);
     if (p->timing)
@@ -1871,7 +2072,7 @@ static int run_dense(genie_plan* p, hipStream_t s)
     // queries the speculative threshold left short: their groups again
     k_dense_check<<<p->dn_groups, 64, 0, s>>>(p->d_kq, nq, p->d_gthr, p->d_gsafe, p->d_cand_cnt, p->d_redo,
                                               p->d_want_safe, p->d_cand_cap, p->d_cand_base, p->d_cap_used,
-                                              p->total_cap);
+                                              p->total_cap, p->dense_ok ? kDenseNoQuery : kSetNoQuery);
     a.redo = p->d_redo;
     if (int rc = launch_dense<false>(nblk, a, s))
         return rc;
@@ -2197,7 +2398,7 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
     const uint32_t blocks[] = {256, 512, 1024};
     const uint32_t tiles[] = {8192, 16384};
     const uint32_t segs[] = {16384, 65536};
-    const bool was_graph = p->graph_on, try_dense = p->dense_on && p->dense_ok;
+    const bool was_graph = p->graph_on, try_dense = p->dense_on && (p->dense_ok || p->sets_ok);
     p->graph_on = false;
     double best = 1e30;
     uint32_t bb = p->block, bt = p->tile_b, bs = p->seglen, n = 0;

@@ -98,7 +98,29 @@ struct DenseArgs {
     const uint64_t* cand_base;
     uint64_t* cand;
     uint32_t* overflow;
+    // set form (set-valued tables): forward lists and, per query group, its
+    // keys with the queries' weights as bit planes
+    const uint32_t* obj_ptr;     // [n_objects + 1]
+    const uint32_t* obj_key;     // keys of every object's occurrences
+    const uint32_t* set_keys;    // [entries] sorted keys of each group
+    const uint4* set_mask;       // [entries][2] bit j of word b = bit b of query j's weight (b < 6)
+    const uint32_t* set_gptr;    // [G + 1] entries of each group
+    const uint32_t* hshift;      // [n_queries] sample histogram bin = count >> hshift (set form)
 };
+
+// Set form: LDS hash table of a group's keys (at most kSetEntries of them,
+// at most half full).  Weights up to 63 (six planes); counts up to 2046 in
+// eleven planes, 2047 being the threshold of "no query".
+constexpr uint32_t kSetSlots = 4096;
+constexpr uint32_t kSetEntries = kSetSlots / 2;
+constexpr uint32_t kSetEmpty = 0xFFFFFFFFu;
+constexpr int kSetWeightPlanes = 6;
+constexpr int kSetCountPlanes = 11;
+constexpr uint32_t kSetMaxWeight = (1u << kSetWeightPlanes) - 1u;
+constexpr uint32_t kSetNoQuery = (1u << kSetCountPlanes) - 1u;
+constexpr uint32_t kDenseNoQuery = 255u;
+
+__device__ __forceinline__ uint32_t set_hash(uint32_t key) { return (key * 0x9E3779B1u) >> 20; }  // 12 bits
 
 // ---- index creation --------------------------------------------------------
 
@@ -173,10 +195,13 @@ __device__ __forceinline__ uint32_t dense_want(uint32_t s_at, uint32_t k, uint32
     return (uint32_t)min<uint64_t>(w, n_objects);
 }
 
+// With `hshift` (set form), bin b of query q holds the sample counts
+// [b << s, (b + 1) << s), s = hshift[q], and the thresholds are bin edges:
+// S at an edge is exact, so both keep their meaning.
 __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict__ hist,
                                                       const uint32_t* __restrict__ kq, uint32_t n_queries,
                                                       uint32_t n_sample, uint32_t n_objects, uint32_t ratio16,
-                                                      uint32_t mult,
+                                                      uint32_t mult, const uint32_t* __restrict__ hshift,
                                                       uint32_t* __restrict__ gthr, uint32_t* __restrict__ gsafe,
                                                       uint32_t* __restrict__ want, uint32_t* __restrict__ want_safe)
 {
@@ -209,8 +234,9 @@ __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict
     const uint32_t at_spec = (uint32_t)__shfl((int)(spec >= b && spec < b + 4 ? s_of(spec) : 0u), (int)(spec / 4), kWave);
     const uint32_t at_safe = (uint32_t)__shfl((int)(safe >= b && safe < b + 4 ? s_of(safe) : 0u), (int)(safe / 4), kWave);
     if (lane == 0) {
-        gthr[q] = spec;
-        gsafe[q] = safe;
+        const uint32_t sh = hshift ? hshift[q] : 0u;
+        gthr[q] = max(spec << sh, 1u);
+        gsafe[q] = max(safe << sh, 1u);
         want[q] = dense_want(at_spec, k, ratio16, n_objects, mult);
         want_safe[q] = dense_want(at_safe, k, ratio16, n_objects, mult);
     }
@@ -264,8 +290,8 @@ __global__ void __launch_bounds__(1024) k_cap_prefix(const uint32_t* __restrict_
 // After the main pass: a query whose speculative threshold left it fewer than
 // k candidates may miss objects counting between safe and spec.  Its group is
 // marked for a second count; the query restarts its list from the safe
-// threshold, the other queries of the group keep theirs (threshold 255: no
-// count reaches it).  One wave per group.
+// threshold, the other queries of the group take `none` (a threshold no
+// count reaches).  One wave per group.
 // A restarted query takes new candidate space sized for its safe threshold
 // after the assigned space (`used`, bumped atomically), when it fits in
 // `total`; otherwise it keeps its space and overflows, and the host grows the
@@ -275,7 +301,8 @@ __global__ void __launch_bounds__(64) k_dense_check(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ cand_cnt, uint32_t* __restrict__ redo,
                                                     const uint32_t* __restrict__ want_safe,
                                                     uint32_t* __restrict__ cap, uint64_t* __restrict__ base,
-                                                    unsigned long long* __restrict__ used, uint64_t total)
+                                                    unsigned long long* __restrict__ used, uint64_t total,
+                                                    uint32_t none)
 {
     const uint32_t g = blockIdx.x, lane = threadIdx.x, q = g * 32 + lane;
     const bool live = lane < 32 && q < n_queries;
@@ -296,7 +323,7 @@ __global__ void __launch_bounds__(64) k_dense_check(const uint32_t* __restrict__
                 }
             }
         } else {
-            gthr[q] = 255u;
+            gthr[q] = none;
         }
     }
 }
@@ -415,11 +442,12 @@ __device__ __forceinline__ void dense_all_counts(const uint32_t (&c)[8], uint64_
     }
 }
 
-__device__ __forceinline__ uint32_t dense_lane_count(const uint32_t (&c)[8], uint32_t j)
+template <int NP>
+__device__ __forceinline__ uint32_t dense_lane_count(const uint32_t (&c)[NP], uint32_t j)
 {
     uint32_t v = 0;
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
+    for (int b = 0; b < NP; ++b)
         v |= ((c[b] >> j) & 1u) << b;
     return v;
 }
@@ -433,29 +461,97 @@ constexpr uint32_t kDenseStageWords = 2 + 32 + 32 + kDenseStage * 3;  // count, 
 
 __host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample)
 {
-    return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 8 + (sample ? 32 * (nblk * 16 + 1) : kDenseStageWords)) * 4u;
+    return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 16 + (sample ? 32 * (nblk * 16 + 1) : kDenseStageWords)) * 4u;
+}
+
+// Set form: hash keys, weight planes 0-3 (uint4 per slot) and 4-5 (uint2 per
+// slot), then the same threshold planes, sample histogram (256 bins per
+// query) or candidate stage.
+__host__ __device__ constexpr uint32_t set_lds_bytes(bool sample)
+{
+    return (kSetSlots * 7 + 16 + (sample ? 32 * 256 : kDenseStageWords)) * 4u;
+}
+
+// c += x, bit-sliced: x is a six-plane number per query (weight planes
+// x0..x5), c an NP-plane count; full adders (xor3 and majority in one
+// v_bitop3 each) over the weight planes, then the carry ripples up.
+template <int NP>
+__device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
+{
+    const uint32_t x[kSetWeightPlanes] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
+    uint32_t carry = c[0] & x[0];
+    c[0] ^= x[0];
+#pragma unroll
+    for (int b = 1; b < kSetWeightPlanes; ++b) {
+        const uint32_t t = __builtin_amdgcn_bitop3_b32(c[b], x[b], carry, 0xE8);  // majority
+        c[b] = __builtin_amdgcn_bitop3_b32(c[b], x[b], carry, 0x96);              // xor3
+        carry = t;
+    }
+#pragma unroll
+    for (int b = kSetWeightPlanes; b < NP; ++b) {
+        const uint32_t t = c[b] & carry;
+        c[b] ^= carry;
+        carry = t;
+    }
+}
+
+// Bit-sliced counts of object o over its forward list: every occurrence whose
+// key the group's queries hold adds each query's weight (counts stay below
+// kSetNoQuery: the host admits a batch only when the longest list times each
+// query's largest weight per key does).  `wide`: some weight of the group
+// reaches 16 (uniform across the workgroup), else planes 4-5 are zero.
+template <int NP>
+__device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t o, const uint32_t* s_key,
+                                               const uint4* s_mlo, const uint2* s_mhi, bool wide,
+                                               uint32_t (&c)[NP])
+{
+#pragma unroll
+    for (int b = 0; b < NP; ++b)
+        c[b] = 0;
+    const uint32_t p0 = a.obj_ptr[o], p1 = a.obj_ptr[o + 1];
+    for (uint32_t p = p0; p < p1; ++p) {
+        const uint32_t key = a.obj_key[p];
+        uint32_t h = set_hash(key);
+        for (uint32_t probe = 0; probe < kSetSlots; ++probe) {
+            const uint32_t k = s_key[h];
+            if (k == key) {
+                set_add<NP>(c, s_mlo[h], wide ? s_mhi[h] : make_uint2(0, 0));
+                break;
+            }
+            if (k == kSetEmpty)
+                break;
+            h = (h + 1u) & (kSetSlots - 1u);
+        }
+    }
 }
 
 // Workgroup -> (query group, tiles) so that the groups of one tile run on one
 // XCD (workgroup ids are dealt to the 8 XCDs round robin) and each
 // workgroup loads one group's mask table once.
-template <int NBLK, bool SAMPLE>
+// SETS (NBLK = 0): the set form -- each workgroup builds its group's key
+// table from the host's entries and counts objects over their forward lists.
+template <int NBLK, bool SAMPLE, bool SETS = false>
 __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
 {
     // the kernel's only LDS, static so that the table sits at address 0 and
     // every table offset folds into the ds_read instruction
     constexpr int DPAD = NBLK * 16;
-    __shared__ uint32_t s_lds[dense_lds_bytes(NBLK, SAMPLE) / 4];
+    constexpr int NP = SETS ? kSetCountPlanes : 8;       // count planes
+    constexpr uint32_t NONE = SETS ? kSetNoQuery : kDenseNoQuery;
+    __shared__ uint32_t s_lds[(SETS ? set_lds_bytes(SAMPLE) : dense_lds_bytes(NBLK, SAMPLE)) / 4];
     uint32_t* s_tab = s_lds;                             // DPAD * 256 masks
-    uint32_t* s_thr = s_lds + DPAD * kDenseKeysPerDim;   // 8 threshold planes
-    uint32_t* s_hist = s_thr + 8;                        // SAMPLE: 32 x (DPAD + 1)
+    uint32_t* s_skey = s_lds;                            // SETS: kSetSlots keys, then the weight planes
+    uint4* s_mlo = reinterpret_cast<uint4*>(s_lds + kSetSlots);
+    uint2* s_mhi = reinterpret_cast<uint2*>(s_lds + kSetSlots * 5);
+    uint32_t* s_thr = s_lds + (SETS ? kSetSlots * 7 : DPAD * kDenseKeysPerDim);  // NP threshold planes
+    uint32_t* s_hist = s_thr + 16;                       // SAMPLE: 32 x (DPAD + 1), SETS 32 x 256
     // main pass: the tile's staged candidates
-    uint32_t* s_nhit = s_thr + 8;                        // staged so far
+    uint32_t* s_nhit = s_thr + 16;                       // staged so far
     uint32_t* s_qcnt = s_nhit + 2;                       // per query of the group
     uint32_t* s_qbase = s_qcnt + 32;                     // reserved base per query
     uint32_t* s_meta = s_qbase + 32;                     // query << 16 | rank within the query
     uint64_t* s_key = reinterpret_cast<uint64_t*>(s_meta + kDenseStage);
-    constexpr uint32_t HB = NBLK * 16 + 1;
+    constexpr uint32_t HB = SETS ? 256 : NBLK * 16 + 1;
     const uint32_t tid = threadIdx.x;
     const uint32_t P = gridDim.x / 8, x = blockIdx.x % 8, s = blockIdx.x / 8;
     uint32_t g_first, g_step, t_first, t_step;
@@ -481,7 +577,32 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
         if (a.redo && !a.redo[g])
             continue;  // uniform across the workgroup
         __syncthreads();
-        if constexpr (SAMPLE) {
+        bool wide = false;
+        if constexpr (SETS) {
+            // the group's key table, built from its entries (unique keys)
+            for (uint32_t i = tid; i < kSetSlots; i += kDenseThreads)
+                s_skey[i] = kSetEmpty;
+            if constexpr (SAMPLE)
+                for (uint32_t i = tid; i < 32 * HB; i += kDenseThreads)
+                    s_hist[i] = 0;
+            __syncthreads();
+            int hi_any = 0;
+            for (uint32_t e = a.set_gptr[g] + tid; e < a.set_gptr[g + 1]; e += kDenseThreads) {
+                const uint32_t key = a.set_keys[e];
+                const uint4 lo = a.set_mask[2 * e], hi = a.set_mask[2 * e + 1];
+                hi_any |= (hi.x | hi.y) != 0u;
+                uint32_t h = set_hash(key);
+                for (uint32_t probe = 0; probe < kSetSlots; ++probe) {
+                    if (atomicCAS(&s_skey[h], kSetEmpty, key) == kSetEmpty) {
+                        s_mlo[h] = lo;
+                        s_mhi[h] = make_uint2(hi.x, hi.y);
+                        break;
+                    }
+                    h = (h + 1u) & (kSetSlots - 1u);
+                }
+            }
+            wide = __syncthreads_or(hi_any) != 0;
+        } else if constexpr (SAMPLE) {
             // build the group's mask table from its items: one bit per
             // (query, covered key), at the key's slot
             uint4* dst = reinterpret_cast<uint4*>(s_tab);
@@ -518,12 +639,18 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
                 dst[i] = src[i];
         }
-        uint32_t tp[8] = {};
+        // SETS sample pass: this lane's bin shift for each query of the group
+        uint64_t shifts = 0;  // 2 bits per query
+        if constexpr (SETS && SAMPLE) {
+            for (uint32_t j = 0; j < 32 && g * 32 + j < a.n_queries; ++j)
+                shifts |= (uint64_t)min(a.hshift[g * 32 + j], 3u) << (2 * j);
+        }
+        uint32_t tp[NP] = {};
         for (uint32_t li = t_first; li < n_list; li += t_step) {
             const uint32_t t = a.list ? a.list[li] : li;
             if constexpr (!SAMPLE) {
                 // threshold planes of the group (bit j of plane b = bit b of
-                // query j's threshold; 255 for lanes past the last query)
+                // query j's threshold; NONE for lanes past the last query)
                 __syncthreads();
                 if (tid < 32)
                     s_qcnt[tid] = 0;
@@ -533,9 +660,9 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     const uint32_t q = g * 32 + tid;
                     const uint32_t th = (tid < 32 && q < a.n_queries)
                                             ? __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                            : 255u;
+                                            : NONE;
 #pragma unroll
-                    for (int b = 0; b < 8; ++b) {
+                    for (int b = 0; b < NP; ++b) {
                         const uint32_t pl = (uint32_t)ballot((th >> b) & 1u);
                         if (tid == 0)
                             s_thr[b] = pl;
@@ -545,15 +672,29 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             __syncthreads();
             if constexpr (!SAMPLE) {
 #pragma unroll
-                for (int b = 0; b < 8; ++b)
+                for (int b = 0; b < NP; ++b)
                     tp[b] = __builtin_amdgcn_readfirstlane(s_thr[b]);
             }
             const uint32_t o0 = t * kDenseTile;
             const uint32_t o1 = min(o0 + kDenseTile, a.n_objects);
             for (uint32_t o = o0 + tid; o < o1; o += kDenseThreads) {
-                uint32_t c[8];
-                dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
-                if constexpr (SAMPLE) {
+                uint32_t c[NP];
+                if constexpr (SETS)
+                    set_accumulate<NP>(a, o, s_skey, s_mlo, s_mhi, wide, c);
+                else
+                    dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
+                if constexpr (SAMPLE && SETS) {
+                    uint32_t any = 0;
+#pragma unroll
+                    for (int b = 0; b < NP; ++b)
+                        any |= c[b];
+                    while (any) {
+                        const uint32_t j = (uint32_t)__builtin_ctz(any);
+                        any &= any - 1;
+                        const uint32_t v = dense_lane_count<NP>(c, j) >> (uint32_t)((shifts >> (2 * j)) & 3u);
+                        atomicAdd(&s_hist[j * HB + min(v, HB - 1)], 1u);
+                    }
+                } else if constexpr (SAMPLE) {
                     uint64_t cnt[4];
                     dense_all_counts(c, cnt);
 #pragma unroll
@@ -568,7 +709,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     // count >= threshold, per lane, most significant plane first
                     uint32_t gt = 0, eq = 0xFFFFFFFFu;
 #pragma unroll
-                    for (int b = 7; b >= 0; --b) {
+                    for (int b = NP - 1; b >= 0; --b) {
                         gt |= eq & c[b] & ~tp[b];
                         eq &= ~(c[b] ^ tp[b]);
                     }
@@ -576,7 +717,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     while (ge) {
                         const uint32_t j = (uint32_t)__builtin_ctz(ge);
                         ge &= ge - 1;
-                        const uint64_t key = ((uint64_t)dense_lane_count(c, j) << 32) | (uint32_t)~o;
+                        const uint64_t key = ((uint64_t)dense_lane_count<NP>(c, j) << 32) | (uint32_t)~o;
                         // staged when the stage has room (the rank is taken only
                         // then, so a query's reserved slots have no holes)
                         const uint32_t i = atomicAdd(s_nhit, 1u);

@@ -11,10 +11,14 @@ def assert_topk_equal(ids, cnt, ref_ids, ref_cnt, k, n_queries, where="", strict
         ids_a, cnt_a = np.asarray(ids), np.asarray(cnt)
         if np.array_equal(ids_a, np.asarray(ref_ids)) and np.array_equal(cnt_a, np.asarray(ref_cnt)):
             return
-    ids = np.asarray(ids).reshape(n_queries, -1)
-    cnt = np.asarray(cnt).reshape(n_queries, -1)
-    ref_ids = np.asarray(ref_ids).reshape(n_queries, -1)
-    ref_cnt = np.asarray(ref_cnt).reshape(n_queries, -1)
+    if np.ndim(k):  # per-query k: results are concatenated, k[q] slots each
+        cuts = np.cumsum(np.asarray(k, np.int64))[:-1]
+        ids, cnt, ref_ids, ref_cnt = (np.split(np.asarray(x), cuts) for x in (ids, cnt, ref_ids, ref_cnt))
+    else:
+        ids = np.asarray(ids).reshape(n_queries, -1)
+        cnt = np.asarray(cnt).reshape(n_queries, -1)
+        ref_ids = np.asarray(ref_ids).reshape(n_queries, -1)
+        ref_cnt = np.asarray(ref_cnt).reshape(n_queries, -1)
     for q in range(n_queries):
         if not np.array_equal(cnt[q], ref_cnt[q]):
             bad = np.nonzero(cnt[q] != ref_cnt[q])[0][:5]

@@ -161,13 +161,16 @@ def test_dense_not_built_for_other_indexes():
     assert not p.dense_active()
     ri, rc = oracle_single(w["rows"], w["ranges"], 4, 10)
     assert_topk_equal(b[0], b[1], ri, rc, 10, 4, "130 dims")
-    # set-valued (bijectMap) table
+    # a set-valued (bijectMap) table takes the set form instead (its own tests)
     rp, vals = genie_amd.synth.sets(61, 20_000, 5, 40, 3000, 1.05)
     qp, qv = genie_amd.synth.sets(62, 6, 1, 30, 3000, 1.05)
     sr = W.set_ranges(qp, qv, np.ones(len(qv), np.float32))
     ts = Table.from_sets(rp, vals)
     (b, p) = run(ts, ts.compile(sr, 6), 6, 10)
-    assert not p.dense_active()
+    assert p.dense_active()
+    q_of = sr["query"]
+    ri, rc = oracle_topk_batch(lambda q: O.count_exhaustive_biject(rp, vals, sr[q_of == q]), 6, 10)
+    assert_topk_equal(b[0], b[1], ri, rc, 10, 6, "sets")
 
 
 def plain(table, items, nq, k):
