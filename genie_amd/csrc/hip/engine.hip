@@ -898,9 +898,14 @@ static int build_dense(const genie_index_desc* desc, DevIndex& d, const uint32_t
 
 // Forward lists for the set form: for a set-valued table whose postings are
 // given and whose objects hold at most 255 occurrences, every object's keys
-// (an object listed twice in one key's list appears twice).  Built on the
-// host in one counting pass over the postings.  Any other index simply has
-// no set form; only device errors fail.
+// (an object listed twice in one key's list appears twice).  Laid out for
+// waves: within each tile of kDenseTile objects the objects are ordered by
+// list length, longest first (ties by id; `obj_perm` maps a slot to its
+// object), and each block of 64 slots stores its lists column-major -- key i
+// of slot j at blk_off[b] + 64 i + j, kSetEmpty past the end of a list -- so
+// that a wave reads 64 consecutive words per step and loops as often as its
+// block's longest list.  Built on the host; any other index simply has no
+// set form; only device errors fail.
 static int build_sets(const genie_index_desc* desc, DevIndex& d)
 {
     if (desc->single_value || !desc->postings || desc->n_postings == 0 || desc->n_keys >= (1u << 24))
@@ -920,24 +925,62 @@ static int build_sets(const genie_index_desc* desc, DevIndex& d)
     for (uint32_t k = 0; k < K; ++k)
         for (uint64_t i = desc->key_start[k]; i < desc->key_start[k + 1]; ++i)
             keys[fill[desc->postings[i]]++] = k;
-    uint32_t *obj_ptr = nullptr, *obj_key = nullptr;
+    // slot order: per tile, by length descending, then id
+    const uint32_t NB = (N + 63) / 64, NS = NB * 64;
+    std::vector<uint32_t> perm(NS, kSetEmpty);
+    for (uint32_t t0 = 0; t0 < N; t0 += kDenseTile) {
+        const uint32_t t1 = std::min(N, t0 + kDenseTile);
+        for (uint32_t o = t0; o < t1; ++o)
+            perm[o] = o;
+        std::stable_sort(perm.begin() + t0, perm.begin() + t1, [&](uint32_t x, uint32_t y) {
+            return ptr[x + 1] - ptr[x] > ptr[y + 1] - ptr[y];
+        });
+    }
+    std::vector<uint32_t> boff(NB + 1, 0);
+    for (uint32_t b = 0; b < NB; ++b) {
+        uint32_t len = 0;
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint32_t o = perm[b * 64 + j];
+            if (o != kSetEmpty)
+                len = std::max(len, ptr[o + 1] - ptr[o]);
+        }
+        if ((uint64_t)boff[b] + 64ull * len >= 0xFFFFFFFFull)
+            return GENIE_OK;  // too large for 32-bit offsets: no set form
+        boff[b + 1] = boff[b] + 64 * len;
+    }
+    std::vector<uint32_t> bkey(std::max<uint32_t>(1, boff[NB]), kSetEmpty);
+    for (uint32_t b = 0; b < NB; ++b)
+        for (uint32_t j = 0; j < 64; ++j) {
+            const uint32_t o = perm[b * 64 + j];
+            if (o == kSetEmpty)
+                continue;
+            for (uint32_t i = 0; i < ptr[o + 1] - ptr[o]; ++i)
+                bkey[boff[b] + 64 * i + j] = keys[ptr[o] + i];
+        }
+    uint32_t *d_perm = nullptr, *d_boff = nullptr, *d_bkey = nullptr;
     uint64_t bytes = 0;
-    int rc = dev_alloc(&obj_ptr, N + 1ull, &bytes);
+    int rc = dev_alloc(&d_perm, NS, &bytes);
     if (!rc)
-        rc = dev_alloc(&obj_key, desc->n_postings, &bytes);
+        rc = dev_alloc(&d_boff, NB + 1ull, &bytes);
+    if (!rc)
+        rc = dev_alloc(&d_bkey, bkey.size(), &bytes);
     hipError_t e = hipSuccess;
-    if (!rc && ((e = hipMemcpy(obj_ptr, ptr.data(), (N + 1ull) * 4, hipMemcpyHostToDevice)) ||
-                (e = hipMemcpy(obj_key, keys.data(), desc->n_postings * 4, hipMemcpyHostToDevice))))
+    if (!rc && ((e = hipMemcpy(d_perm, perm.data(), NS * 4ull, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(d_boff, boff.data(), (NB + 1ull) * 4, hipMemcpyHostToDevice)) ||
+                (e = hipMemcpy(d_bkey, bkey.data(), bkey.size() * 4, hipMemcpyHostToDevice))))
         rc = fail(GENIE_E_RUNTIME, std::string("forward lists: ") + hipGetErrorString(e));
     if (rc) {
-        dev_free(obj_ptr);
-        dev_free(obj_key);
+        dev_free(d_perm);
+        dev_free(d_boff);
+        dev_free(d_bkey);
         return rc;
     }
     d.sets = true;
     d.set_max_len = mx;
-    d.obj_ptr = obj_ptr;
-    d.obj_key = obj_key;
+    d.set_slots = NS;
+    d.obj_perm = d_perm;
+    d.blk_off = d_boff;
+    d.blk_key = d_bkey;
     d.bytes += bytes;
     return GENIE_OK;
 }
@@ -1121,8 +1164,9 @@ int genie_hip_index_destroy(genie_index* ix)
     dev_free(d.sdir);
     dev_free(d.codes);
     dev_free(d.key_slot);
-    dev_free(d.obj_ptr);
-    dev_free(d.obj_key);
+    dev_free(d.obj_perm);
+    dev_free(d.blk_off);
+    dev_free(d.blk_key);
     delete ix;
     return GENIE_OK;
 }
@@ -2046,8 +2090,10 @@ static int run_dense(genie_plan* p, hipStream_t s)
     a.cand_base = p->d_cand_base;
     a.cand = p->d_cand;
     a.overflow = p->d_overflow;
-    a.obj_ptr = d.obj_ptr;
-    a.obj_key = d.obj_key;
+    a.obj_perm = d.obj_perm;
+    a.blk_off = d.blk_off;
+    a.blk_key = d.blk_key;
+    a.n_slots = d.set_slots;
     a.set_keys = p->d_skeys;
     a.set_mask = p->d_smask;
     a.set_gptr = p->d_sgptr;

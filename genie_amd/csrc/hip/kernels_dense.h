@@ -100,8 +100,10 @@ struct DenseArgs {
     uint32_t* overflow;
     // set form (set-valued tables): forward lists and, per query group, its
     // keys with the queries' weights as bit planes
-    const uint32_t* obj_ptr;     // [n_objects + 1]
-    const uint32_t* obj_key;     // keys of every object's occurrences
+    const uint32_t* obj_perm;    // [n_slots] object of each slot (kSetEmpty: none)
+    const uint32_t* blk_off;     // [n_slots / 64 + 1] first word of each 64-slot block
+    const uint32_t* blk_key;     // key i of slot 64 b + j at blk_off[b] + 64 i + j (kSetEmpty: none)
+    uint32_t n_slots;            // objects rounded up to 64
     const uint32_t* set_keys;    // [entries] sorted keys of each group
     const uint4* set_mask;       // [entries][2] bit j of word b = bit b of query j's weight (b < 6)
     const uint32_t* set_gptr;    // [G + 1] entries of each group
@@ -119,6 +121,7 @@ constexpr int kSetCountPlanes = 11;
 constexpr uint32_t kSetMaxWeight = (1u << kSetWeightPlanes) - 1u;
 constexpr uint32_t kSetNoQuery = (1u << kSetCountPlanes) - 1u;
 constexpr uint32_t kDenseNoQuery = 255u;
+constexpr int kSetPrefetch = 8;  // forward-list columns in flight per wave
 
 __device__ __forceinline__ uint32_t set_hash(uint32_t key) { return (key * 0x9E3779B1u) >> 20; }  // 12 bits
 
@@ -495,32 +498,46 @@ __device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
     }
 }
 
-// Bit-sliced counts of object o over its forward list: every occurrence whose
-// key the group's queries hold adds each query's weight (counts stay below
-// kSetNoQuery: the host admits a batch only when the longest list times each
-// query's largest weight per key does).  `wide`: some weight of the group
-// reaches 16 (uniform across the workgroup), else planes 4-5 are zero.
+// Bit-sliced counts of the object in slot `sl` over its forward list: every
+// occurrence whose key the group's queries hold adds each query's weight
+// (counts stay below kSetNoQuery: the host admits a batch only when the
+// longest list times each query's largest weight per key does).  The slots of
+// a wave form one block, whose lists it reads a column (64 words) per step.
+// `wide`: some weight of the group reaches 16 (uniform across the
+// workgroup), else planes 4-5 are zero.
 template <int NP>
-__device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t o, const uint32_t* s_key,
+__device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, const uint32_t* s_key,
                                                const uint4* s_mlo, const uint2* s_mhi, bool wide,
                                                uint32_t (&c)[NP])
 {
 #pragma unroll
     for (int b = 0; b < NP; ++b)
         c[b] = 0;
-    const uint32_t p0 = a.obj_ptr[o], p1 = a.obj_ptr[o + 1];
-    for (uint32_t p = p0; p < p1; ++p) {
-        const uint32_t key = a.obj_key[p];
-        uint32_t h = set_hash(key);
-        for (uint32_t probe = 0; probe < kSetSlots; ++probe) {
-            const uint32_t k = s_key[h];
-            if (k == key) {
-                set_add<NP>(c, s_mlo[h], wide ? s_mhi[h] : make_uint2(0, 0));
-                break;
+    const uint32_t blk = sl >> 6;
+    const uint32_t w1 = a.blk_off[blk + 1];
+    // kSetPrefetch columns loaded at once (the guard is uniform across the
+    // wave: a block holds whole columns), then looked up one by one
+    for (uint32_t w = a.blk_off[blk] + (sl & 63u); w < w1; w += 64 * kSetPrefetch) {
+        uint32_t kk[kSetPrefetch];
+#pragma unroll
+        for (int u = 0; u < kSetPrefetch; ++u)
+            kk[u] = w + 64u * u < w1 ? a.blk_key[w + 64u * u] : kSetEmpty;
+#pragma unroll
+        for (int u = 0; u < kSetPrefetch; ++u) {
+            const uint32_t key = kk[u];
+            if (key == kSetEmpty)
+                continue;  // past the end of this slot's list
+            uint32_t h = set_hash(key);
+            for (uint32_t probe = 0; probe < kSetSlots; ++probe) {
+                const uint32_t k = s_key[h];
+                if (k == key) {
+                    set_add<NP>(c, s_mlo[h], wide ? s_mhi[h] : make_uint2(0, 0));
+                    break;
+                }
+                if (k == kSetEmpty)
+                    break;
+                h = (h + 1u) & (kSetSlots - 1u);
             }
-            if (k == kSetEmpty)
-                break;
-            h = (h + 1u) & (kSetSlots - 1u);
         }
     }
 }
@@ -676,13 +693,16 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     tp[b] = __builtin_amdgcn_readfirstlane(s_thr[b]);
             }
             const uint32_t o0 = t * kDenseTile;
-            const uint32_t o1 = min(o0 + kDenseTile, a.n_objects);
-            for (uint32_t o = o0 + tid; o < o1; o += kDenseThreads) {
+            const uint32_t o1 = min(o0 + kDenseTile, SETS ? a.n_slots : a.n_objects);
+            for (uint32_t ol = o0 + tid; ol < o1; ol += kDenseThreads) {
                 uint32_t c[NP];
-                if constexpr (SETS)
-                    set_accumulate<NP>(a, o, s_skey, s_mlo, s_mhi, wide, c);
-                else
+                uint32_t o = ol;
+                if constexpr (SETS) {
+                    set_accumulate<NP>(a, ol, s_skey, s_mlo, s_mhi, wide, c);
+                    o = a.obj_perm[ol];  // kSetEmpty past the last object: counts 0, nothing emitted
+                } else {
                     dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
+                }
                 if constexpr (SAMPLE && SETS) {
                     uint32_t any = 0;
 #pragma unroll
