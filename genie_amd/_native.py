@@ -98,6 +98,7 @@ _sig(_hip, "genie_hip_server_submit", C.c_int, _p, _p, _u32, _u32, C.POINTER(_u6
 _sig(_hip, "genie_hip_server_wait", C.c_int, _p, _u64, _p, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_server_replay", C.c_int, _p, _p, _p, _p, _p, _u32, _p, _p, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_server_stats", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64))
+_sig(_hip, "genie_hip_server_phase_seconds", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_decode", C.c_int, C.c_int, _p, _p, _u32, _u32, _p, _p)
 _sig(_hip, "genie_hip_last_error", C.c_char_p)
 _sig(_hip, "genie_hip_stream_synchronize", C.c_int, _p)
@@ -376,7 +377,10 @@ class Server:
     def stats(self):
         a, b, c = _u64(), _u64(), _u64()
         _chk_hip(_hip.genie_hip_server_stats(self.handle, C.byref(a), C.byref(b), C.byref(c)))
-        return dict(batches=a.value, requests=b.value, max_inflight=c.value)
+        ph = (C.c_double * 4)()
+        _chk_hip(_hip.genie_hip_server_phase_seconds(self.handle, ph))
+        return dict(batches=a.value, requests=b.value, max_inflight=c.value,
+                    executor_seconds=dict(prepare_upload=ph[0], launch=ph[1], wait=ph[2], hand_back=ph[3]))
 
     def close(self):
         if self.handle:

@@ -201,6 +201,20 @@ struct genie_plan {
     uint32_t* d_want = nullptr;      // [max_queries] candidate space asked for at spec / at safe
     uint32_t* d_want_safe = nullptr;
     unsigned long long* d_cap_used = nullptr;  // end of the assigned candidate space
+    // uploads: what genie_hip_plan_upload copies to the device is gathered in
+    // one pinned buffer and copied asynchronously by the next run, on its
+    // stream; ev_up marks the end of those copies.  A staged copy names the
+    // plan's pointer field, read when the copies are issued (a buffer grown
+    // meanwhile is copied to where it lives)
+    struct Staged {
+        void** dst;
+        uint64_t off, bytes;
+    };
+    uint8_t* h_stage = nullptr;
+    uint64_t stage_cap = 0, stage_used = 0;
+    std::vector<Staged> staged;
+    bool staging = false, up_pending = false;
+    hipEvent_t ev_up = nullptr;
     // graph replay
     bool graph_on = false, graph_valid = false;
     hipStream_t own_stream = nullptr;
@@ -1225,7 +1239,8 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         if ((rc = dev_alloc(&p->d_dq, max_queries))) break;
         if ((rc = dev_alloc(&p->d_hov, max_queries))) break;
         if ((rc = dev_alloc(&p->d_hov_any, 1))) break;
-        if (hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        if (hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->ev_up, hipEventDisableTiming) != hipSuccess) {
             rc = fail(GENIE_E_RUNTIME, "hipStreamCreate");
             break;
         }
@@ -1247,17 +1262,92 @@ int genie_hip_plan_destroy(genie_plan* p)
 {
     if (!p)
         return GENIE_OK;
+    if (p->up_pending)
+        (void)hipEventSynchronize(p->ev_up);  // issued copies land before their buffers go
     plan_free_buffers(p);
     for (auto& e : p->ev)
         if (e)
             (void)hipEventDestroy(e);
+    if (p->ev_up)
+        (void)hipEventDestroy(p->ev_up);
     if (p->own_stream)
         (void)hipStreamDestroy(p->own_stream);
+    if (p->h_stage)
+        (void)hipHostFree(p->h_stage);
     delete p;
     return GENIE_OK;
 }
 
 }  // extern "C"
+
+// ---- uploads ---------------------------------------------------------------
+
+// Opens the staging of an upload: the copies of the previous one (issued by a
+// run) have left the pinned buffer before it is written again.
+static int stage_begin(genie_plan* p)
+{
+    if (p->up_pending) {
+        HIP_TRY(hipEventSynchronize(p->ev_up));
+        p->up_pending = false;
+    }
+    p->stage_used = 0;
+    p->staged.clear();
+    p->staging = true;
+    return GENIE_OK;
+}
+
+// Issues the staged copies on stream s, in order (the next run on s reads
+// them); ev_up marks their end.
+static int stage_issue(genie_plan* p, hipStream_t s)
+{
+    if (p->staged.empty())
+        return GENIE_OK;
+    for (const genie_plan::Staged& c : p->staged)
+        HIP_TRY(hipMemcpyAsync(*c.dst, p->h_stage + c.off, c.bytes, hipMemcpyHostToDevice, s));
+    p->staged.clear();
+    HIP_TRY(hipEventRecord(p->ev_up, s));
+    p->up_pending = true;
+    return GENIE_OK;
+}
+
+// Host-to-device copy of `bytes` from `src` to the buffer *dst: staged while
+// an upload is open (src may go away at once; the copies go out with the next
+// run, on its stream), else synchronous after the staged ones.
+template <class T>
+static int h2d(genie_plan* p, T** dst, const void* src, uint64_t bytes)
+{
+    if (bytes == 0)
+        return GENIE_OK;
+    if (!p->staging) {
+        if (!p->staged.empty()) {
+            if (int rc = stage_issue(p, p->own_stream))
+                return rc;
+        }
+        if (p->up_pending) {
+            HIP_TRY(hipEventSynchronize(p->ev_up));
+            p->up_pending = false;
+        }
+        HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return GENIE_OK;
+    }
+    const uint64_t off = (p->stage_used + 255) & ~255ull;
+    if (off + bytes > p->stage_cap) {
+        const uint64_t cap = std::max<uint64_t>({off + bytes, 2 * p->stage_cap, 1ull << 20});
+        uint8_t* nb = nullptr;
+        if (hipHostMalloc((void**)&nb, cap, hipHostMallocDefault) != hipSuccess)
+            return fail(GENIE_E_OOM, "pinned upload buffer of " + std::to_string(cap) + " bytes");
+        if (p->stage_used)
+            std::memcpy(nb, p->h_stage, p->stage_used);
+        if (p->h_stage)
+            (void)hipHostFree(p->h_stage);
+        p->h_stage = nb;
+        p->stage_cap = cap;
+    }
+    std::memcpy(p->h_stage + off, src, bytes);
+    p->staged.push_back({reinterpret_cast<void**>(dst), off, bytes});
+    p->stage_used = off + bytes;
+    return GENIE_OK;
+}
 
 // Mode 1 (batched) and mode 2 (automatic) run the dense form when the batch
 // allows it; otherwise mode 1 runs the lane form and mode 2 the plain form.
@@ -1321,9 +1411,9 @@ static int plan_size_candidates(genie_plan* p)
         p->cand_alloc = tot;
     }
     p->total_cap = tot;
-    HIP_TRY(hipMemcpy(p->d_cand_cap, cap.data(), cap.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_cand_base, base.data(), base.size() * 8, hipMemcpyHostToDevice));
-    return GENIE_OK;
+    if (int rc = h2d(p, &p->d_cand_cap, cap.data(), cap.size() * 4))
+        return rc;
+    return h2d(p, &p->d_cand_base, base.data(), base.size() * 8);
 }
 
 // Whether the uploaded batch (kept on the host by mode-1 plans) can run the
@@ -1509,7 +1599,8 @@ static int prepare_dense(genie_plan* p)
             if (int rc = dev_alloc(&p->d_dkeys, 2 * std::max<uint64_t>(1, p->max_items)))
                 return rc;
         if (p->n_items)
-            HIP_TRY(hipMemcpy(p->d_dkeys, p->h_dkeys.data(), p->n_items * 8, hipMemcpyHostToDevice));
+            if (int rc = h2d(p, &p->d_dkeys, p->h_dkeys.data(), p->n_items * 8))
+                return rc;
     } else {  // set form: the groups' key entries
         const uint64_t ne = std::max<uint64_t>(1, p->h_skeys.size());
         if (ne > p->sent_alloc) {
@@ -1529,11 +1620,15 @@ static int prepare_dense(genie_plan* p)
                 return rc;
         }
         if (!p->h_skeys.empty()) {
-            HIP_TRY(hipMemcpy(p->d_skeys, p->h_skeys.data(), p->h_skeys.size() * 4, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(p->d_smask, p->h_smask.data(), p->h_smask.size() * sizeof(uint4), hipMemcpyHostToDevice));
+            if (int rc = h2d(p, &p->d_skeys, p->h_skeys.data(), p->h_skeys.size() * 4))
+                return rc;
+            if (int rc = h2d(p, &p->d_smask, p->h_smask.data(), p->h_smask.size() * sizeof(uint4)))
+                return rc;
         }
-        HIP_TRY(hipMemcpy(p->d_sgptr, p->h_sgptr.data(), p->h_sgptr.size() * 4, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(p->d_sshift, p->h_sshift.data(), p->h_sshift.size() * 4, hipMemcpyHostToDevice));
+        if (int rc = h2d(p, &p->d_sgptr, p->h_sgptr.data(), p->h_sgptr.size() * 4))
+            return rc;
+        if (int rc = h2d(p, &p->d_sshift, p->h_sshift.data(), p->h_sshift.size() * 4))
+            return rc;
     }
     if (!p->d_gsafe)
         if (int rc = dev_alloc(&p->d_gsafe, p->max_queries))
@@ -1565,7 +1660,8 @@ static int prepare_dense(genie_plan* p)
             return rc;
         p->dlist_alloc = ns;
     }
-    HIP_TRY(hipMemcpy(p->d_dlist, list.data(), ns * 4ull, hipMemcpyHostToDevice));
+    if (int rc = h2d(p, &p->d_dlist, list.data(), ns * 4ull))
+        return rc;
     p->n_dlist = ns;
     return GENIE_OK;
 }
@@ -1674,10 +1770,13 @@ static int plan_decoded_spans(genie_plan* p)
         if (int rc = dev_alloc(&p->d_sq_ptr, p->max_queries + 1))
             return rc;
     if (!p->h_sspan.empty())
-        HIP_TRY(hipMemcpy(p->d_sspan, p->h_sspan.data(), p->h_sspan.size() * sizeof(Span), hipMemcpyHostToDevice));
+        if (int rc = h2d(p, &p->d_sspan, p->h_sspan.data(), p->h_sspan.size() * sizeof(Span)))
+            return rc;
     if (!p->h_djobs.empty())
-        HIP_TRY(hipMemcpy(p->d_djobs, p->h_djobs.data(), p->h_djobs.size() * sizeof(uint2), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_sq_ptr, p->h_sq_ptr.data(), (nq + 1) * 4ull, hipMemcpyHostToDevice));
+        if (int rc = h2d(p, &p->d_djobs, p->h_djobs.data(), p->h_djobs.size() * sizeof(uint2)))
+            return rc;
+    if (int rc = h2d(p, &p->d_sq_ptr, p->h_sq_ptr.data(), (nq + 1) * 4ull))
+        return rc;
     p->dec_spans = true;
     return GENIE_OK;
 }
@@ -1749,10 +1848,13 @@ static int plan_split_plain(genie_plan* p)
         p->hwork_alloc = p->h_hwork.size();
     }
     if (!p->h_hwork.empty())
-        HIP_TRY(hipMemcpy(p->d_hwork, p->h_hwork.data(), p->h_hwork.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_hq, p->h_hq.data(), nq * sizeof(uint4), hipMemcpyHostToDevice));
+        if (int rc = h2d(p, &p->d_hwork, p->h_hwork.data(), p->h_hwork.size() * 4))
+            return rc;
+    if (int rc = h2d(p, &p->d_hq, p->h_hq.data(), nq * sizeof(uint4)))
+        return rc;
     if (!p->h_dq.empty())
-        HIP_TRY(hipMemcpy(p->d_dq, p->h_dq.data(), p->h_dq.size() * 4, hipMemcpyHostToDevice));
+        if (int rc = h2d(p, &p->d_dq, p->h_dq.data(), p->h_dq.size() * 4))
+            return rc;
     return GENIE_OK;
 }
 
@@ -1777,13 +1879,9 @@ static int plan_settle_form(genie_plan* p)
     return plan_size_candidates(p);
 }
 
-extern "C" {
-
-int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
-                          const uint32_t* k_per_query)
+static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                       const uint32_t* k_per_query)
 {
-    if (!p || (!items && n_items) || !k_per_query)
-        return fail(GENIE_E_INVALID, "null argument");
     if (n_queries == 0 || n_queries > p->max_queries || n_items > p->max_items)
         return fail(GENIE_E_INVALID, "batch larger than the plan");
     const DevIndex& d = p->ix->d;
@@ -1888,10 +1986,14 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
         p->out_alloc = p->total_k;
     }
     if (n_items)
-        HIP_TRY(hipMemcpy(p->d_items, p->h_items.data(), n_items * sizeof(genie_dim_t), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_item_qptr, p->h_qptr.data(), p->h_qptr.size() * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_kq, k_per_query, n_queries * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(p->d_out_off, p->h_out_off.data(), p->h_out_off.size() * 8, hipMemcpyHostToDevice));
+        if (int rc = h2d(p, &p->d_items, p->h_items.data(), n_items * sizeof(genie_dim_t)))
+            return rc;
+    if (int rc = h2d(p, &p->d_item_qptr, p->h_qptr.data(), p->h_qptr.size() * 4))
+        return rc;
+    if (int rc = h2d(p, &p->d_kq, k_per_query, n_queries * 4ull))
+        return rc;
+    if (int rc = h2d(p, &p->d_out_off, p->h_out_off.data(), p->h_out_off.size() * 8))
+        return rc;
     plan_drop_graph(p);
     if (p->mode != 0) {
         p->lanes_built = false;
@@ -1901,6 +2003,22 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
         return rc;
     p->cap_factor = initial_cap_factor(p);
     return plan_size_candidates(p);
+}
+
+extern "C" {
+
+int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                          const uint32_t* k_per_query)
+{
+    if (!p || (!items && n_items) || !k_per_query)
+        return fail(GENIE_E_INVALID, "null argument");
+    // the batch's device copies are staged in pinned memory and go out with
+    // the next run, on its stream
+    if (int rc = stage_begin(p))
+        return rc;
+    const int rc = plan_upload(p, items, n_items, n_queries, k_per_query);
+    p->staging = false;
+    return rc;
 }
 
 }  // extern "C"
@@ -2278,6 +2396,8 @@ int genie_hip_plan_run(genie_plan* p, void* stream)
     if (p->n_queries == 0)
         return fail(GENIE_E_NOT_BUILT, "no batch uploaded");
     hipStream_t s = (hipStream_t)stream;
+    if (int rc = stage_issue(p, s))
+        return rc;
     if (!p->graph_on || p->timing)
         return plan_run_once(p, s);
     if (!p->graph_valid) {
@@ -2444,6 +2564,8 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
     const uint32_t blocks[] = {256, 512, 1024};
     const uint32_t tiles[] = {8192, 16384};
     const uint32_t segs[] = {16384, 65536};
+    if (int rc = stage_issue(p, p->own_stream))
+        return rc;
     const bool was_graph = p->graph_on, try_dense = p->dense_on && (p->dense_ok || p->sets_ok);
     p->graph_on = false;
     double best = 1e30;

@@ -477,10 +477,13 @@ __host__ __device__ constexpr uint32_t set_lds_bytes(bool sample)
 
 // c += x, bit-sliced: x is a six-plane number per query (weight planes
 // x0..x5), c an NP-plane count; full adders (xor3 and majority in one
-// v_bitop3 each) over the weight planes, then the carry ripples up.
+// v_bitop3 each) over the weight planes, then the carry ripples up.  Past
+// plane 7 (counts of 256 and more, rare) only when some lane of the wave
+// carries there (a uniform branch), so the common step stops at plane 7.
 template <int NP>
 __device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
 {
+    static_assert(NP > 8, "planes 0-7 always, the rest on demand");
     const uint32_t x[kSetWeightPlanes] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
     uint32_t carry = c[0] & x[0];
     c[0] ^= x[0];
@@ -491,10 +494,18 @@ __device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
         carry = t;
     }
 #pragma unroll
-    for (int b = kSetWeightPlanes; b < NP; ++b) {
+    for (int b = kSetWeightPlanes; b < 8; ++b) {
         const uint32_t t = c[b] & carry;
         c[b] ^= carry;
         carry = t;
+    }
+    if (ballot(carry != 0u) != 0) {
+#pragma unroll
+        for (int b = 8; b < NP; ++b) {
+            const uint32_t t = c[b] & carry;
+            c[b] ^= carry;
+            carry = t;
+        }
     }
 }
 
@@ -524,20 +535,27 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
             kk[u] = w + 64u * u < w1 ? a.blk_key[w + 64u * u] : kSetEmpty;
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u) {
+            // probe to the key or an empty slot (the table is at most half
+            // full, so every probe sequence ends), then add the slot's weights
+            // where the key was found and zero elsewhere: one path for the
+            // whole wave, skipped when no lane found its key
             const uint32_t key = kk[u];
-            if (key == kSetEmpty)
-                continue;  // past the end of this slot's list
             uint32_t h = set_hash(key);
-            for (uint32_t probe = 0; probe < kSetSlots; ++probe) {
-                const uint32_t k = s_key[h];
-                if (k == key) {
-                    set_add<NP>(c, s_mlo[h], wide ? s_mhi[h] : make_uint2(0, 0));
-                    break;
-                }
-                if (k == kSetEmpty)
-                    break;
+            uint32_t k = s_key[h];
+            for (uint32_t probe = 1; k != key && k != kSetEmpty && probe < kSetSlots; ++probe) {
                 h = (h + 1u) & (kSetSlots - 1u);
+                k = s_key[h];
             }
+            const bool found = k == key && key != kSetEmpty;  // kSetEmpty: past the end of the list
+            if (ballot(found) == 0)
+                continue;
+            uint4 lo = s_mlo[h];
+            uint2 hi = wide ? s_mhi[h] : make_uint2(0, 0);
+            if (!found) {
+                lo = make_uint4(0, 0, 0, 0);
+                hi = make_uint2(0, 0);
+            }
+            set_add<NP>(c, lo, hi);
         }
     }
 }

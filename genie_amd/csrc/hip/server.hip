@@ -62,7 +62,14 @@ struct Executor {
     bool assigned = false;  // a batch waits for the executor's thread
     std::condition_variable cv;
     std::thread th;
+    // time spent per phase of a batch (ns, this executor's thread): prepare
+    // and upload, launch, wait for the results, hand them back
+    std::atomic<uint64_t> phase_ns[4] = {};
 };
+
+uint64_t ns_since(Clock::time_point t) {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t).count();
+}
 
 // Device scratch one request holds while in flight: its work items, its
 // candidate space as a plan first sizes it (64 k keys, at most one per
@@ -106,6 +113,7 @@ static int fail_s(int code, const std::string& m)
 
 int genie_server::launch(Executor& e)
 {
+    const auto t0 = Clock::now();
     const uint32_t nq = (uint32_t)e.batch.size();
     std::vector<genie_dim_t> items;
     std::vector<uint32_t> kq(nq);
@@ -137,12 +145,15 @@ int genie_server::launch(Executor& e)
     }
     if (int rc = genie_hip_plan_upload(e.plan, items.data(), items.size(), nq, kq.data()))
         return rc;
+    e.phase_ns[0].fetch_add(ns_since(t0), std::memory_order_relaxed);
+    const auto t1 = Clock::now();
     if (int rc = genie_hip_plan_run(e.plan, e.stream))
         return rc;
     if (int rc = genie_hip_plan_download_async(e.plan, e.stream, e.h_ids, e.h_counts, e.h_ovf))
         return rc;
     if (hipEventRecord(e.ev, e.stream) != hipSuccess)
         return fail_s(GENIE_E_RUNTIME, "hipEventRecord");
+    e.phase_ns[1].fetch_add(ns_since(t1), std::memory_order_relaxed);
     return GENIE_OK;
 }
 
@@ -198,6 +209,7 @@ void genie_server::exec_loop(Executor& e)
         lk.unlock();
         int rc = launch(e);
         std::string err = rc ? genie_hip_last_error() : "";
+        const auto tw = Clock::now();
         if (!rc) {
             hipError_t q;
             while ((q = hipEventQuery(e.ev)) == hipErrorNotReady)
@@ -208,7 +220,10 @@ void genie_server::exec_loop(Executor& e)
                 err = hipGetErrorString(q);
             }
         }
+        e.phase_ns[2].fetch_add(ns_since(tw), std::memory_order_relaxed);
+        const auto tf = Clock::now();
         finish(e, rc, err);
+        e.phase_ns[3].fetch_add(ns_since(tf), std::memory_order_relaxed);
         lk.lock();
     }
 }
@@ -489,6 +504,19 @@ int genie_hip_server_stats(genie_server* s, uint64_t* batches, uint64_t* request
         *requests = s->n_done;
     if (max_inflight)
         *max_inflight = s->max_inflight;
+    return GENIE_OK;
+}
+
+int genie_hip_server_phase_seconds(genie_server* s, double* out)
+{
+    if (!s || !out)
+        return fail_s(GENIE_E_INVALID, "null argument");
+    for (int i = 0; i < 4; ++i) {
+        uint64_t ns = 0;
+        for (const Executor& e : s->ex)
+            ns += e.phase_ns[i].load(std::memory_order_relaxed);
+        out[i] = (double)ns * 1e-9;
+    }
     return GENIE_OK;
 }
 

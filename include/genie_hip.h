@@ -231,6 +231,12 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
                             double* out_latency_ms, int32_t* out_ids, int32_t* out_counts, double* wall_s);
 /* Batches run and requests completed so far. */
 int genie_hip_server_stats(genie_server* s, uint64_t* batches, uint64_t* requests, uint64_t* max_inflight);
+/* Seconds the executors spent so far, summed over executors, per phase of a
+ * batch: out[0] gathering the batch and uploading it (host work and copies),
+ * out[1] launching its kernels and result copies, out[2] waiting for them,
+ * out[3] handing the results back.  Measurement only (no counterpart in the
+ * reference, whose dist_genie scheduler has no timing). */
+int genie_hip_server_phase_seconds(genie_server* s, double* out);
 
 /* ---- codec decode (tests and measurement) ------------------------------ */
 /* Decode n_arrays compressed arrays ([cpos[i], cpos[i+1]) words of `words`)
