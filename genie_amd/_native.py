@@ -283,7 +283,9 @@ class Plan:
 
     def set_dense(self, on=True):
         """Batched plans: count whole-key unit-weight batches of a single-value
-        index from per-object key codes (bit-sliced, 32 queries per lookup)."""
+        index from per-object key codes (bit-sliced, 32 queries per lookup), and
+        weighted batches of a set-valued index from per-object token lists
+        against an LDS table of each 32-query group's tokens (the set form)."""
         _chk_hip(_hip.genie_hip_plan_set_dense(self.handle, 1 if on else 0))
 
     def dense_active(self):

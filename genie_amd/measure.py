@@ -21,14 +21,15 @@ import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-# form name -> (plan mode, hash form, sparse select)
+# form name -> (plan mode, hash form, sparse select, dense/set form)
 FORMS = {
-    "plain-hash": (0, 1, -1),      # every query whose counts fit on the hash form
-    "plain-auto": (0, -1, -1),     # the plain form's own per-query choice
-    "plain-direct": (0, 0, -1),    # direct-mapped kernel, its own select choice
-    "plain-direct-dense": (0, 0, 0),
-    "batched": (1, -1, -1),        # dense form when the batch allows, else the lane form
-    "automatic": (2, -1, -1),
+    "plain-hash": (0, 1, -1, True),      # every query whose counts fit on the hash form
+    "plain-auto": (0, -1, -1, True),     # the plain form's own per-query choice
+    "plain-direct": (0, 0, -1, True),    # direct-mapped kernel, its own select choice
+    "plain-direct-dense": (0, 0, 0, True),
+    "batched": (1, -1, -1, True),        # dense (or set) form when the batch allows, else the lane form
+    "batched-lanes": (1, -1, -1, False),  # the lane form
+    "automatic": (2, -1, -1, True),
 }
 
 
@@ -67,10 +68,12 @@ def main():
     kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
     ref = None
     for f in args.forms:
-        mode, hm, sp = FORMS[f]
+        mode, hm, sp, dn = FORMS[f]
         p = Plan(t.device(), nq, len(items), int(kq.max()), 0, mode=mode)
         p.set_hash(hm)
         p.set_sparse(sp)
+        if mode:
+            p.set_dense(dn)
         p.upload(items, nq, kq)
         p.run()
         got = p.download()

@@ -113,7 +113,9 @@ typedef struct genie_plan genie_plan;
 int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_items,
                           uint32_t max_k, uint32_t count_bound, int mode, genie_plan** out);
 int genie_hip_plan_destroy(genie_plan* p);
-/* Copy a batch (host arrays) into the plan's device buffers. */
+/* Take a batch (host arrays; they may be reused on return): its device
+ * copies are staged in the plan's pinned host buffer and issued by the next
+ * genie_hip_plan_run on that run's stream, ahead of its kernels. */
 int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items,
                           uint32_t n_queries, const uint32_t* k_per_query);
 /* Run the search on the uploaded batch on `stream` (a hipStream_t, NULL =
@@ -150,10 +152,17 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
  * codes built at index creation (a compressed table is decoded once on the
  * device for this), 32 queries per LDS lookup with bit-sliced counters,
  * instead of from the posting lists; results are identical.
+ * The same switch covers the set form of set-valued (bijectMap) tables: when
+ * each query's weight per key is an integer up to 63 (items summed over the
+ * positions they cover, constant over each key), the longest forward list
+ * times the query's largest weight is below 2047, and each group of 32
+ * queries names at most 2048 keys, objects are counted over their forward
+ * lists (built at index creation) against an LDS hash table of the group's
+ * keys, with 11 bit-sliced count planes.
  * No counterpart in the reference (its matching always walks the lists,
  * match.cu:161-220).  on = 0 forces the lane form. */
 int genie_hip_plan_set_dense(genie_plan* p, int on);
-/* Whether the last uploaded batch runs the dense form. */
+/* Whether the last uploaded batch runs the dense form (or the set form). */
 int genie_hip_plan_dense_active(genie_plan* p, int* active);
 /* Sparse select of the plain form: counting marks the counter words a
  * (query, tile) touches and the per-tile top-k visits only those.  mode -1
