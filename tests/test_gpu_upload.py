@@ -1,0 +1,89 @@
+"""Staged uploads: a plan's upload gathers its device copies in pinned host
+memory and the next run issues them on its own stream.  Every way of reaching
+the device after an upload must see that batch: a second upload before any
+run, a form switch (synchronous copies after the staged ones), the autotuner,
+a run on a user stream, and a batch that grows the pinned buffer."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import assert_topk_equal, oracle_topk_batch
+from oracle import oracle as O
+
+from genie_amd import Plan, Table, match_topk, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def cfg(n, d, q, seed, radius=2):
+    w = W.cfg1(seed=seed, n=n, d=d, q=q, k=50, radius=radius)
+    return w["rows"], w["ranges"]
+
+
+def expect(rows, ranges, nq, k):
+    q_of = ranges["query"]
+    return oracle_topk_batch(lambda q: O.count_exhaustive_single(rows, ranges[q_of == q]), nq, k)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_second_upload_before_run_wins(mode):
+    rows, ra = cfg(60_000, 24, 40, 901)
+    _, rb = cfg(60_000, 24, 40, 902)
+    t = Table.from_rows(rows)
+    ia, ib = t.compile(ra, 40), t.compile(rb, 40)
+    p = Plan(t.device(), 40, max(len(ia), len(ib)), 50, 0, mode=mode)
+    p.upload(ia, 40, np.full(40, 50, np.uint32))
+    p.upload(ib, 40, np.full(40, 50, np.uint32))
+    p.run()
+    got = p.download()
+    ri, rc = expect(rows, rb, 40, 50)
+    assert_topk_equal(got[0], got[1], ri, rc, 50, 40, "second upload")
+
+
+def test_form_switch_and_autotune_after_upload():
+    rows, r = cfg(80_000, 32, 33, 911)
+    t = Table.from_rows(rows)
+    items = t.compile(r, 33)
+    ri, rc = expect(rows, r, 33, 50)
+    p = Plan(t.device(), 33, len(items), 50, 0, mode=1)
+    p.upload(items, 33, np.full(33, 50, np.uint32))
+    p.set_dense(False)  # settles the lane form with synchronous copies after the staged ones
+    assert not p.dense_active()
+    p.run()
+    got = p.download()
+    assert_topk_equal(got[0], got[1], ri, rc, 50, 33, "lane form after switch")
+    p.set_dense(True)
+    assert p.dense_active()
+    p.run()
+    got = p.download()
+    assert_topk_equal(got[0], got[1], ri, rc, 50, 33, "dense form after switch")
+    # a fresh upload straight into the autotuner
+    p.upload(items, 33, np.full(33, 50, np.uint32))
+    p.autotune(reps=1)
+    p.run()
+    got = p.download()
+    assert_topk_equal(got[0], got[1], ri, rc, 50, 33, "after autotune")
+
+
+def test_user_stream_growing_batches_and_graph():
+    """Batches of growing size on a user stream (the pinned buffer grows),
+    each checked; then graph replays of the last with eager work between."""
+    rows, _ = cfg(100_000, 48, 8, 921)
+    t = Table.from_rows(rows)
+    s = torch.cuda.Stream()
+    p = Plan(t.device(), 256, 256 * 48 * 3, 100, 0, mode=2)
+    for nq, radius, seed in [(8, 2, 922), (64, 5, 923), (256, 5, 924), (16, 2, 925)]:
+        _, r = cfg(100_000, 48, nq, seed, radius)
+        items = t.compile(r, nq)
+        kq = np.full(nq, 100, np.uint32)
+        p.upload(items, nq, kq)
+        p.run(s.cuda_stream)
+        got = p.download(s.cuda_stream)
+        ri, rc = expect(rows, r, nq, 100)
+        assert_topk_equal(got[0], got[1], ri, rc, 100, nq, f"{nq} queries, range {radius}")
+    p.set_graph(True)
+    for _ in range(3):
+        p.run(s.cuda_stream)
+        again = p.download(s.cuda_stream)
+        assert np.array_equal(again[0], got[0]) and np.array_equal(again[1], got[1])
+        match_topk(t.device(), items[:200], nq, 5)
