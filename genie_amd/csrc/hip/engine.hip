@@ -2151,8 +2151,21 @@ static int launch_dense_t(const DenseArgs& a, hipStream_t s)
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDenseThreads, 0));
-    if (per_cu < 1)
-        return fail(GENIE_E_INVALID, "dense kernel does not fit a CU");
+    if (per_cu < 1) {
+        // never launched when the occupancy query says no workgroup fits: a
+        // workgroup that cannot be placed would never run
+        hipFuncAttributes fa{};
+        hipDeviceProp_t prop{};
+        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        const std::string why = std::to_string(fa.sharedSizeBytes) + " B of LDS, " + std::to_string(fa.numRegs) + " VGPRs, " +
+              std::to_string(fa.localSizeBytes) + " B of scratch per lane; device " + prop.gcnArchName +
+              ": " + std::to_string(prop.sharedMemPerBlock) + " B of LDS per workgroup, " +
+              std::to_string(prop.maxSharedMemoryPerMultiProcessor) + " per CU, " +
+              std::to_string(prop.regsPerBlock) + " registers per workgroup, " +
+              std::to_string(prop.maxThreadsPerMultiProcessor) + " threads per CU";
+        return fail(GENIE_E_INVALID, "dense kernel does not fit a CU: " + why);
+    }
     const uint32_t grid = std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8);
     kern<<<grid, kDenseThreads, 0, s>>>(a);
     return GENIE_OK;

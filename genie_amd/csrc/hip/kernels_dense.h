@@ -608,7 +608,12 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     const uint32_t tab_words = NBLK * 16 * kDenseKeysPerDim;
     const __amdgpu_buffer_rsrc_t rs = codes_rsrc(a.codes);
     const uint32_t row_bytes = a.n_pad * 4u;
-    for (uint32_t g = g_first; g < a.G; g += g_step) {
+    // sample pass: a workgroup without a sample tile builds no table, unless
+    // it is the one that writes its group's table out (few groups, many
+    // workgroups each: most have none of the few sample tiles)
+    const bool writer = !SETS && x == 0 && (a.G > P || s < a.G);
+    const bool idle = SAMPLE && t_first >= n_list && !writer;
+    for (uint32_t g = g_first; g < a.G && !idle; g += g_step) {
         if (a.redo && !a.redo[g])
             continue;  // uniform across the workgroup
         __syncthreads();
@@ -663,7 +668,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 }
             }
             __syncthreads();
-            if (x == 0 && (a.G > P || s < a.G)) {  // one workgroup per group keeps it for the main pass
+            if (writer) {  // one workgroup per group keeps it for the main pass
                 uint4* out = reinterpret_cast<uint4*>(a.tab + (uint64_t)g * tab_words);
                 for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
                     out[i] = dst[i];
