@@ -76,6 +76,7 @@ _sig(_hip, "genie_hip_plan_get_config", C.c_int, _p, C.POINTER(_u32), C.POINTER(
      C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_autotune", C.c_int, _p, C.c_int, _p, C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_set_dense", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_set_dense_threads", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_dense_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_set_sparse", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_sparse_active", C.c_int, _p, C.POINTER(C.c_int))
@@ -287,6 +288,10 @@ class Plan:
         weighted batches of a set-valued index from per-object token lists
         against an LDS table of each 32-query group's tokens (the set form)."""
         _chk_hip(_hip.genie_hip_plan_set_dense(self.handle, 1 if on else 0))
+
+    def set_dense_threads(self, threads=0):
+        """Threads per workgroup of the dense/set kernels: 0 automatic, 512, 1024."""
+        _chk_hip(_hip.genie_hip_plan_set_dense_threads(self.handle, threads))
 
     def dense_active(self):
         v = C.c_int()

@@ -177,6 +177,7 @@ struct genie_plan {
     bool lanes_built = false;        // work queue of the lane form built for this batch
     // dense form of the batched search (single-value index, whole-key unit-weight batch)
     bool dense_on = true;            // use it when the batch allows
+    int dense_nt = 0;                // threads per workgroup of its kernels (0: 1,024 when placed, else 512)
     bool dense_ok = false;           // the uploaded batch allows it
     bool sets_ok = false;            // ... or its set form (set-valued tables)
     std::vector<uint32_t> h_skeys;   // set form: every group's sorted keys,
@@ -2142,48 +2143,55 @@ static int launch_scan(genie_plan* p, hipStream_t s)
 static int launch_merge(genie_plan* p, hipStream_t s);
 
 // Grid of the dense kernels: every CU's share of resident workgroups, a
-// multiple of the 8 XCDs the workgroup ids are dealt to.
+// multiple of the 8 XCDs the workgroup ids are dealt to.  The 1,024-thread
+// kernel when the occupancy query places it, else the 512-thread one (some
+// runtimes place no 1,024-thread workgroup of more than 64 VGPRs); a kernel
+// the query does not place is never launched, since it would never run.
 template <int NBLK, bool SAMPLE, bool SETS = false>
-static int launch_dense_t(const DenseArgs& a, hipStream_t s)
+static int launch_dense_t(const DenseArgs& a, hipStream_t s, int nt)
 {
-    auto kern = k_dense_count<NBLK, SAMPLE, SETS>;  // static LDS
+    auto k1 = k_dense_count<NBLK, SAMPLE, SETS, kDenseThreads>;  // static LDS
+    auto k2 = k_dense_count<NBLK, SAMPLE, SETS, kDenseThreads / 2>;
     int per_cu = 0, n_cu = 0, dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kDenseThreads, 0));
-    if (per_cu < 1) {
-        // never launched when the occupancy query says no workgroup fits: a
-        // workgroup that cannot be placed would never run
-        hipFuncAttributes fa{};
-        hipDeviceProp_t prop{};
-        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(kern)));
-        HIP_TRY(hipGetDeviceProperties(&prop, dev));
-        const std::string why = std::to_string(fa.sharedSizeBytes) + " B of LDS, " + std::to_string(fa.numRegs) + " VGPRs, " +
-              std::to_string(fa.localSizeBytes) + " B of scratch per lane; device " + prop.gcnArchName +
-              ": " + std::to_string(prop.sharedMemPerBlock) + " B of LDS per workgroup, " +
-              std::to_string(prop.maxSharedMemoryPerMultiProcessor) + " per CU, " +
-              std::to_string(prop.regsPerBlock) + " registers per workgroup, " +
-              std::to_string(prop.maxThreadsPerMultiProcessor) + " threads per CU";
-        return fail(GENIE_E_INVALID, "dense kernel does not fit a CU: " + why);
+    if (nt != kDenseThreads / 2)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k1, kDenseThreads, 0));
+    if (per_cu >= 1) {
+        k1<<<std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8), kDenseThreads, 0, s>>>(a);
+        return GENIE_OK;
     }
-    const uint32_t grid = std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8);
-    kern<<<grid, kDenseThreads, 0, s>>>(a);
-    return GENIE_OK;
+    if (nt != kDenseThreads)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k2, kDenseThreads / 2, 0));
+    if (per_cu >= 1) {
+        k2<<<std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8), kDenseThreads / 2, 0, s>>>(a);
+        return GENIE_OK;
+    }
+    hipFuncAttributes fa{};
+    hipDeviceProp_t prop{};
+    HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k2)));
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    return fail(GENIE_E_INVALID, "dense kernel does not fit a CU at 1024 or 512 threads: " +
+                                     std::to_string(fa.sharedSizeBytes) + " B of LDS, " +
+                                     std::to_string(fa.numRegs) + " VGPRs at 512; device " + prop.gcnArchName +
+                                     ": " + std::to_string(prop.maxSharedMemoryPerMultiProcessor) +
+                                     " B of LDS per CU, " + std::to_string(prop.maxThreadsPerMultiProcessor) +
+                                     " threads per CU");
 }
 
 template <bool SAMPLE>
-static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s)
+static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s, int nt)
 {
     switch (nblk) {
-    case 0: return launch_dense_t<0, SAMPLE, true>(a, s);
-    case 1: return launch_dense_t<1, SAMPLE>(a, s);
-    case 2: return launch_dense_t<2, SAMPLE>(a, s);
-    case 3: return launch_dense_t<3, SAMPLE>(a, s);
-    case 4: return launch_dense_t<4, SAMPLE>(a, s);
-    case 5: return launch_dense_t<5, SAMPLE>(a, s);
-    case 6: return launch_dense_t<6, SAMPLE>(a, s);
-    case 7: return launch_dense_t<7, SAMPLE>(a, s);
-    case 8: return launch_dense_t<8, SAMPLE>(a, s);
+    case 0: return launch_dense_t<0, SAMPLE, true>(a, s, nt);
+    case 1: return launch_dense_t<1, SAMPLE>(a, s, nt);
+    case 2: return launch_dense_t<2, SAMPLE>(a, s, nt);
+    case 3: return launch_dense_t<3, SAMPLE>(a, s, nt);
+    case 4: return launch_dense_t<4, SAMPLE>(a, s, nt);
+    case 5: return launch_dense_t<5, SAMPLE>(a, s, nt);
+    case 6: return launch_dense_t<6, SAMPLE>(a, s, nt);
+    case 7: return launch_dense_t<7, SAMPLE>(a, s, nt);
+    case 8: return launch_dense_t<8, SAMPLE>(a, s, nt);
     }
     return fail(GENIE_E_INVALID, "dense form supports at most 128 dims");
 }
@@ -2230,7 +2238,7 @@ static int run_dense(genie_plan* p, hipStream_t s)
     a.set_gptr = p->d_sgptr;
     a.hshift = p->dense_ok ? nullptr : p->d_sshift;
     const uint32_t nblk = p->dense_ok ? d.dense_dpad / 16 : 0;  // 0: the set form
-    if (int rc = launch_dense<true>(nblk, a, s))
+    if (int rc = launch_dense<true>(nblk, a, s, p->dense_nt))
         return rc;
     // thresholds, then every query's candidate space from its sample
     const uint32_t mult = std::max<uint32_t>(1, p->cap_factor / initial_cap_factor(p));
@@ -2244,14 +2252,14 @@ static int run_dense(genie_plan* p, hipStream_t s)
         HIP_TRY(hipEventRecord(p->ev[4], s));
     a.list = nullptr;
     a.n_list = 0;
-    if (int rc = launch_dense<false>(nblk, a, s))
+    if (int rc = launch_dense<false>(nblk, a, s, p->dense_nt))
         return rc;
     // queries the speculative threshold left short: their groups again
     k_dense_check<<<p->dn_groups, 64, 0, s>>>(p->d_kq, nq, p->d_gthr, p->d_gsafe, p->d_cand_cnt, p->d_redo,
                                               p->d_want_safe, p->d_cand_cap, p->d_cand_base, p->d_cap_used,
                                               p->total_cap, p->dense_ok ? kDenseNoQuery : kSetNoQuery);
     a.redo = p->d_redo;
-    if (int rc = launch_dense<false>(nblk, a, s))
+    if (int rc = launch_dense<false>(nblk, a, s, p->dense_nt))
         return rc;
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[2], s));
@@ -2479,6 +2487,17 @@ int genie_hip_plan_set_dense(genie_plan* p, int on)
         return GENIE_OK;
     p->dense_on = on != 0;
     return plan_settle_form(p);
+}
+
+int genie_hip_plan_set_dense_threads(genie_plan* p, int threads)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (threads != 0 && threads != kDenseThreads && threads != kDenseThreads / 2)
+        return fail(GENIE_E_INVALID, "dense threads must be 0, 512 or 1024");
+    p->dense_nt = threads;
+    plan_drop_graph(p);
+    return GENIE_OK;
 }
 
 int genie_hip_plan_set_sparse(genie_plan* p, int mode)

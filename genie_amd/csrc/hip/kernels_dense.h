@@ -565,7 +565,11 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 // workgroup loads one group's mask table once.
 // SETS (NBLK = 0): the set form -- each workgroup builds its group's key
 // table from the host's entries and counts objects over their forward lists.
-template <int NBLK, bool SAMPLE, bool SETS = false>
+// NT: threads per workgroup, kDenseThreads (16 waves over the CU's table)
+// or half of it for runtimes whose occupancy rule places no 1,024-thread
+// workgroup of more than 64 VGPRs.  Both are compiled for 1,024 threads, so
+// both stay within 128 VGPRs (two such waves per SIMD fit any such rule).
+template <int NBLK, bool SAMPLE, bool SETS = false, int NT = kDenseThreads>
 __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
 {
     // the kernel's only LDS, static so that the table sits at address 0 and
@@ -620,14 +624,14 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
         bool wide = false;
         if constexpr (SETS) {
             // the group's key table, built from its entries (unique keys)
-            for (uint32_t i = tid; i < kSetSlots; i += kDenseThreads)
+            for (uint32_t i = tid; i < kSetSlots; i += NT)
                 s_skey[i] = kSetEmpty;
             if constexpr (SAMPLE)
-                for (uint32_t i = tid; i < 32 * HB; i += kDenseThreads)
+                for (uint32_t i = tid; i < 32 * HB; i += NT)
                     s_hist[i] = 0;
             __syncthreads();
             int hi_any = 0;
-            for (uint32_t e = a.set_gptr[g] + tid; e < a.set_gptr[g + 1]; e += kDenseThreads) {
+            for (uint32_t e = a.set_gptr[g] + tid; e < a.set_gptr[g + 1]; e += NT) {
                 const uint32_t key = a.set_keys[e];
                 const uint4 lo = a.set_mask[2 * e], hi = a.set_mask[2 * e + 1];
                 hi_any |= (hi.x | hi.y) != 0u;
@@ -646,13 +650,13 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             // build the group's mask table from its items: one bit per
             // (query, covered key), at the key's slot
             uint4* dst = reinterpret_cast<uint4*>(s_tab);
-            for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
+            for (uint32_t i = tid; i < tab_words / 4; i += NT)
                 dst[i] = make_uint4(0, 0, 0, 0);
-            for (uint32_t i = tid; i < 32 * HB; i += kDenseThreads)
+            for (uint32_t i = tid; i < 32 * HB; i += NT)
                 s_hist[i] = 0;
             __syncthreads();
             const uint32_t i0 = a.item_qptr[g * 32], i1 = a.item_qptr[min(g * 32 + 32, a.n_queries)];
-            for (uint32_t i = i0 + tid; i < i1; i += kDenseThreads) {
+            for (uint32_t i = i0 + tid; i < i1; i += NT) {
                 const uint32_t k0 = a.dkeys[2 * i], k1 = a.dkeys[2 * i + 1];
                 const uint32_t bit = 1u << (k0 >> 24);
                 // up to 16 slot loads in flight per item, then the LDS ors
@@ -670,13 +674,13 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             __syncthreads();
             if (writer) {  // one workgroup per group keeps it for the main pass
                 uint4* out = reinterpret_cast<uint4*>(a.tab + (uint64_t)g * tab_words);
-                for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
+                for (uint32_t i = tid; i < tab_words / 4; i += NT)
                     out[i] = dst[i];
             }
         } else {
             const uint4* src = reinterpret_cast<const uint4*>(a.tab + (uint64_t)g * tab_words);
             uint4* dst = reinterpret_cast<uint4*>(s_tab);
-            for (uint32_t i = tid; i < tab_words / 4; i += kDenseThreads)
+            for (uint32_t i = tid; i < tab_words / 4; i += NT)
                 dst[i] = src[i];
         }
         // SETS sample pass: this lane's bin shift for each query of the group
@@ -717,7 +721,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             }
             const uint32_t o0 = t * kDenseTile;
             const uint32_t o1 = min(o0 + kDenseTile, SETS ? a.n_slots : a.n_objects);
-            for (uint32_t ol = o0 + tid; ol < o1; ol += kDenseThreads) {
+            for (uint32_t ol = o0 + tid; ol < o1; ol += NT) {
                 uint32_t c[NP];
                 uint32_t o = ol;
                 if constexpr (SETS) {
@@ -786,7 +790,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 if (tid < 32 && s_qcnt[tid])
                     s_qbase[tid] = atomicAdd(&a.cand_cnt[g * 32 + tid], s_qcnt[tid]);
                 __syncthreads();
-                for (uint32_t i = tid; i < nh; i += kDenseThreads) {
+                for (uint32_t i = tid; i < nh; i += NT) {
                     const uint32_t m = s_meta[i], j = m >> 16;
                     const uint32_t q = g * 32 + j;
                     const uint32_t slot = s_qbase[j] + (m & 0xFFFFu);
@@ -799,7 +803,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
         }
         if constexpr (SAMPLE) {
             __syncthreads();
-            for (uint32_t i = tid; i < 32 * HB; i += kDenseThreads) {
+            for (uint32_t i = tid; i < 32 * HB; i += NT) {
                 const uint32_t j = i / HB, bin = i % HB;
                 const uint32_t q = g * 32 + j;
                 const uint32_t v = s_hist[i];

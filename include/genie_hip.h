@@ -162,6 +162,12 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
  * No counterpart in the reference (its matching always walks the lists,
  * match.cu:161-220).  on = 0 forces the lane form. */
 int genie_hip_plan_set_dense(genie_plan* p, int on);
+/* Threads per workgroup of the dense and set forms' kernels: 0 (default)
+ * runs 1,024 when the runtime's occupancy query places such a workgroup and
+ * 512 otherwise (some runtimes place no 1,024-thread workgroup of more than
+ * 64 VGPRs); 512 or 1024 forces one (1024 then fails where it is not
+ * placed).  Results are identical.  No counterpart in the reference. */
+int genie_hip_plan_set_dense_threads(genie_plan* p, int threads);
 /* Whether the last uploaded batch runs the dense form (or the set form). */
 int genie_hip_plan_dense_active(genie_plan* p, int* active);
 /* Sparse select of the plain form: counting marks the counter words a

@@ -159,3 +159,33 @@ def test_set_form_not_used_where_it_cannot_be_exact():
     assert not p.dense_active()
     ri, rc = oracle_sets(rp2, vals2, r3, 1, 2)
     assert_topk_equal(b[0], b[1], ri, rc, 2, 1, "long lists")
+
+
+def test_half_size_workgroups_give_the_same_results():
+    """The dense and set kernels at 512 threads per workgroup (taken where a
+    runtime places no 1,024-thread workgroup of their size) against the
+    oracle and the automatic choice, graph replays included."""
+    from helpers import oracle_topk_batch as otb
+    w = W.cfg1(seed=471, n=200_000, d=128, q=40, k=100)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(w["ranges"], 40)
+    q_of = w["ranges"]["query"]
+    ri, rc = otb(lambda q: O.count_exhaustive_single(w["rows"], w["ranges"][q_of == q]), 40, 100)
+    rp, vals, ranges = sets_case(472, 80_000, 36, 12_000)
+    ts = Table.from_sets(rp, vals)
+    sitems = ts.compile(ranges, 36)
+    si, sc = oracle_sets(rp, vals, ranges, 36, 50)
+    for table, its, nq, k, ref in [(t, items, 40, 100, (ri, rc)), (ts, sitems, 36, 50, (si, sc))]:
+        got = {}
+        for nt in (0, 512):
+            p = Plan(table.device(), nq, len(its), k, 0, mode=1)
+            p.set_dense_threads(nt)
+            p.upload(its, nq, np.full(nq, k, np.uint32))
+            assert p.dense_active()
+            p.set_graph(True)
+            for _ in range(3):
+                p.run()
+                got[nt] = p.download()
+                match_topk(table.device(), its[:50], nq, 5)
+            assert_topk_equal(got[nt][0], got[nt][1], ref[0], ref[1], k, nq, f"{nt} threads")
+        assert np.array_equal(got[0][0], got[512][0]) and np.array_equal(got[0][1], got[512][1])

@@ -3,9 +3,10 @@ memory and the next run issues them on its own stream.  Every way of reaching
 the device after an upload must see that batch: a second upload before any
 run, a form switch (synchronous copies after the staged ones), the autotuner,
 a run on a user stream, and a batch that grows the pinned buffer."""
+import ctypes as C
+
 import numpy as np
 import pytest
-import torch
 
 from helpers import assert_topk_equal, oracle_topk_batch
 from oracle import oracle as O
@@ -70,20 +71,26 @@ def test_user_stream_growing_batches_and_graph():
     each checked; then graph replays of the last with eager work between."""
     rows, _ = cfg(100_000, 48, 8, 921)
     t = Table.from_rows(rows)
-    s = torch.cuda.Stream()
+    # a caller's stream, from the HIP runtime the library itself uses
+    hip = C.CDLL("libamdhip64.so.7")
+    sh = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(sh), 1) == 0  # non-blocking, default priority
+    stream = sh.value
     p = Plan(t.device(), 256, 256 * 48 * 3, 100, 0, mode=2)
     for nq, radius, seed in [(8, 2, 922), (64, 5, 923), (256, 5, 924), (16, 2, 925)]:
         _, r = cfg(100_000, 48, nq, seed, radius)
         items = t.compile(r, nq)
         kq = np.full(nq, 100, np.uint32)
         p.upload(items, nq, kq)
-        p.run(s.cuda_stream)
-        got = p.download(s.cuda_stream)
+        p.run(stream)
+        got = p.download(stream)
         ri, rc = expect(rows, r, nq, 100)
         assert_topk_equal(got[0], got[1], ri, rc, 100, nq, f"{nq} queries, range {radius}")
     p.set_graph(True)
     for _ in range(3):
-        p.run(s.cuda_stream)
-        again = p.download(s.cuda_stream)
+        p.run(stream)
+        again = p.download(stream)
         assert np.array_equal(again[0], got[0]) and np.array_equal(again[1], got[1])
         match_topk(t.device(), items[:200], nq, 5)
+    p.close()
+    assert hip.hipStreamDestroy(C.c_void_p(stream)) == 0
