@@ -21,15 +21,16 @@ import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-# form name -> (plan mode, hash form, sparse select, dense/set form)
+# form name -> (plan mode, hash form, sparse select, dense/set form, its threads per workgroup)
 FORMS = {
-    "plain-hash": (0, 1, -1, True),      # every query whose counts fit on the hash form
-    "plain-auto": (0, -1, -1, True),     # the plain form's own per-query choice
-    "plain-direct": (0, 0, -1, True),    # direct-mapped kernel, its own select choice
-    "plain-direct-dense": (0, 0, 0, True),
-    "batched": (1, -1, -1, True),        # dense (or set) form when the batch allows, else the lane form
-    "batched-lanes": (1, -1, -1, False),  # the lane form
-    "automatic": (2, -1, -1, True),
+    "plain-hash": (0, 1, -1, True, 0),      # every query whose counts fit on the hash form
+    "plain-auto": (0, -1, -1, True, 0),     # the plain form's own per-query choice
+    "plain-direct": (0, 0, -1, True, 0),    # direct-mapped kernel, its own select choice
+    "plain-direct-dense": (0, 0, 0, True, 0),
+    "batched": (1, -1, -1, True, 0),        # dense (or set) form when the batch allows, else the lane form
+    "batched-512": (1, -1, -1, True, 512),  # the same with 512-thread workgroups
+    "batched-lanes": (1, -1, -1, False, 0),  # the lane form
+    "automatic": (2, -1, -1, True, 0),
 }
 
 
@@ -68,12 +69,13 @@ def main():
     kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
     ref = None
     for f in args.forms:
-        mode, hm, sp, dn = FORMS[f]
+        mode, hm, sp, dn, nt = FORMS[f]
         p = Plan(t.device(), nq, len(items), int(kq.max()), 0, mode=mode)
         p.set_hash(hm)
         p.set_sparse(sp)
         if mode:
             p.set_dense(dn)
+            p.set_dense_threads(nt)
         p.upload(items, nq, kq)
         p.run()
         got = p.download()
