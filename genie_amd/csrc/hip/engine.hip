@@ -740,9 +740,12 @@ static int build_batched(genie_plan* p)
     return GENIE_OK;
 }
 
+// *placed = false (nothing launched) when the occupancy query places no
+// workgroup of NT threads.
 template <int LB, bool CMP, int NT>
-static int launch_batched_t(genie_plan* p, const BatchArgs& b, hipStream_t s)
+static int launch_batched_t(genie_plan* p, const BatchArgs& b, hipStream_t s, bool* placed)
 {
+    *placed = false;
     const size_t lds = (size_t)(p->tile_b + kWave) * 8;
     auto kern = k_scan_batched<LB, CMP, NT>;
     hipFuncAttributes fa;
@@ -756,21 +759,39 @@ static int launch_batched_t(genie_plan* p, const BatchArgs& b, hipStream_t s)
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, lds));
     if (per_cu < 1)
-        return fail(GENIE_E_INVALID, "batched kernel does not fit a CU with tile " + std::to_string(p->tile_b));
+        return GENIE_OK;
     const uint32_t grid = std::max(1u, std::min<uint32_t>(b.n_work, (uint32_t)(per_cu * n_cu)));
     kern<<<grid, NT, lds, s>>>(b);
+    *placed = true;
     return GENIE_OK;
 }
 
+// The plan's block size, or the next smaller one where the occupancy query
+// places none (some runtimes place no 1,024-thread workgroup of more than 64
+// VGPRs); every block size gives the same results.
 template <int LB>
 static int launch_batched_lb(genie_plan* p, const BatchArgs& b, hipStream_t s)
 {
     const bool c = p->ix->d.codec != 0;
-    if (p->block == 256)
-        return c ? launch_batched_t<LB, true, 256>(p, b, s) : launch_batched_t<LB, false, 256>(p, b, s);
-    if (p->block == 1024)
-        return c ? launch_batched_t<LB, true, 1024>(p, b, s) : launch_batched_t<LB, false, 1024>(p, b, s);
-    return c ? launch_batched_t<LB, true, 512>(p, b, s) : launch_batched_t<LB, false, 512>(p, b, s);
+    bool placed = false;
+    if (p->block == 1024) {
+        if (int rc = c ? launch_batched_t<LB, true, 1024>(p, b, s, &placed)
+                       : launch_batched_t<LB, false, 1024>(p, b, s, &placed))
+            return rc;
+    }
+    if (!placed && p->block >= 512) {
+        if (int rc = c ? launch_batched_t<LB, true, 512>(p, b, s, &placed)
+                       : launch_batched_t<LB, false, 512>(p, b, s, &placed))
+            return rc;
+    }
+    if (!placed) {
+        if (int rc = c ? launch_batched_t<LB, true, 256>(p, b, s, &placed)
+                       : launch_batched_t<LB, false, 256>(p, b, s, &placed))
+            return rc;
+    }
+    if (!placed)
+        return fail(GENIE_E_INVALID, "batched kernel does not fit a CU with tile " + std::to_string(p->tile_b));
+    return GENIE_OK;
 }
 
 static int launch_batched(genie_plan* p, hipStream_t s)
