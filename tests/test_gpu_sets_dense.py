@@ -189,3 +189,21 @@ def test_half_size_workgroups_give_the_same_results():
                 match_topk(table.device(), its[:50], nq, 5)
             assert_topk_equal(got[nt][0], got[nt][1], ref[0], ref[1], k, nq, f"{nt} threads")
         assert np.array_equal(got[0][0], got[512][0]) and np.array_equal(got[0][1], got[512][1])
+
+
+def test_empty_documents_queries_and_groups():
+    """Documents without tokens, queries without items or with zero weights
+    only, a whole group of 32 queries naming no key, and k above the matches."""
+    rp, vals = genie_amd.synth.sets(481, 40_000, 0, 12, 3_000, 1.05)
+    assert (np.diff(rp) == 0).any()
+    qp, qv = genie_amd.synth.sets(482, 70, 0, 20, 3_000, 1.05)
+    wts = genie_amd.synth.uniform_int(483, 0, len(qv), 1, 5).astype(np.float32)
+    r = W.set_ranges(qp, qv, wts)
+    r["weight"][r["query"] == 5] = 0          # a query whose items all weigh 0
+    r = r[(r["query"] < 32) | (r["query"] >= 64)]  # queries 32-63: no items at all
+    t = Table.from_sets(rp, vals)
+    kq = np.array([5, 3000, 40] * 23 + [7], np.uint32)
+    (b, p) = run(t, t.compile(r, 70), 70, kq)
+    assert p.dense_active()
+    ri, rc = oracle_sets(rp, vals, r, 70, kq)
+    assert_topk_equal(b[0], b[1], ri, rc, kq, 70, "empty cases")
