@@ -2040,6 +2040,8 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
         return rc;
     const int rc = plan_upload(p, items, n_items, n_queries, k_per_query);
     p->staging = false;
+    if (rc)
+        p->staged.clear();  // a failed upload sends nothing
     return rc;
 }
 
