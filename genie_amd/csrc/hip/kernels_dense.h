@@ -122,6 +122,7 @@ constexpr uint32_t kSetMaxWeight = (1u << kSetWeightPlanes) - 1u;
 constexpr uint32_t kSetNoQuery = (1u << kSetCountPlanes) - 1u;
 constexpr uint32_t kDenseNoQuery = 255u;
 constexpr int kSetPrefetch = 8;  // forward-list columns in flight per wave
+constexpr uint32_t kSetWindowTiles = 8;  // tiles per XCD counted together (main pass)
 
 __device__ __forceinline__ uint32_t set_hash(uint32_t key) { return (key * 0x9E3779B1u) >> 20; }  // 12 bits
 
@@ -617,6 +618,13 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     // workgroups each: most have none of the few sample tiles)
     const bool writer = !SETS && x == 0 && (a.G > P || s < a.G);
     const bool idle = SAMPLE && t_first >= n_list && !writer;
+    // set form, main pass: tiles in windows of kSetWindowTiles per XCD, each
+    // window counted for all of the workgroup's groups before the next, so
+    // that the 32 workgroups of an XCD read one window's lists (about 3 MB)
+    // from its L2 (a group's table is built again per window); elsewhere one
+    // window holds every tile
+    const uint32_t win = (SETS && !SAMPLE) ? t_step * kSetWindowTiles : n_list;
+    for (uint32_t w0 = 0; w0 < n_list; w0 += win)
     for (uint32_t g = g_first; g < a.G && !idle; g += g_step) {
         if (a.redo && !a.redo[g])
             continue;  // uniform across the workgroup
@@ -690,7 +698,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 shifts |= (uint64_t)min(a.hshift[g * 32 + j], 3u) << (2 * j);
         }
         uint32_t tp[NP] = {};
-        for (uint32_t li = t_first; li < n_list; li += t_step) {
+        for (uint32_t li = t_first + w0; li < min(n_list, w0 + win); li += t_step) {
             const uint32_t t = a.list ? a.list[li] : li;
             if constexpr (!SAMPLE) {
                 // threshold planes of the group (bit j of plane b = bit b of
