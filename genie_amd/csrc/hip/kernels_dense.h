@@ -478,13 +478,12 @@ __host__ __device__ constexpr uint32_t set_lds_bytes(bool sample)
 
 // c += x, bit-sliced: x is a six-plane number per query (weight planes
 // x0..x5), c an NP-plane count; full adders (xor3 and majority in one
-// v_bitop3 each) over the weight planes, then the carry ripples up.  Past
-// plane 7 (counts of 256 and more, rare) only when some lane of the wave
-// carries there (a uniform branch), so the common step stops at plane 7.
+// v_bitop3 each) over the weight planes, then the carry ripples up (two
+// ops a plane, no branch: a branch on the rare upper carry costs more in
+// register moves at its join than the three planes it skips).
 template <int NP>
 __device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
 {
-    static_assert(NP > 8, "planes 0-7 always, the rest on demand");
     const uint32_t x[kSetWeightPlanes] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
     uint32_t carry = c[0] & x[0];
     c[0] ^= x[0];
@@ -495,18 +494,10 @@ __device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
         carry = t;
     }
 #pragma unroll
-    for (int b = kSetWeightPlanes; b < 8; ++b) {
+    for (int b = kSetWeightPlanes; b < NP; ++b) {
         const uint32_t t = c[b] & carry;
         c[b] ^= carry;
         carry = t;
-    }
-    if (ballot(carry != 0u) != 0) {
-#pragma unroll
-        for (int b = 8; b < NP; ++b) {
-            const uint32_t t = c[b] & carry;
-            c[b] ^= carry;
-            carry = t;
-        }
     }
 }
 
@@ -515,12 +506,13 @@ __device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
 // (counts stay below kSetNoQuery: the host admits a batch only when the
 // longest list times each query's largest weight per key does).  The slots of
 // a wave form one block, whose lists it reads a column (64 words) per step.
-// `wide`: some weight of the group reaches 16 (uniform across the
-// workgroup), else planes 4-5 are zero.
-template <int NP>
+// WIDE: some weight of the group reaches 16 (planes 4-5 are read), chosen once
+// per group.  No branch per occurrence but the rare extra probes: a branch
+// whose paths rejoin with new count planes costs more in register moves than
+// the work it could skip.
+template <int NP, bool WIDE>
 __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, const uint32_t* s_key,
-                                               const uint4* s_mlo, const uint2* s_mhi, bool wide,
-                                               uint32_t (&c)[NP])
+                                               const uint4* s_mlo, const uint2* s_mhi, uint32_t (&c)[NP])
 {
 #pragma unroll
     for (int b = 0; b < NP; ++b)
@@ -537,9 +529,9 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u) {
             // probe to the key or an empty slot (the table is at most half
-            // full, so every probe sequence ends), then add the slot's weights
-            // where the key was found and zero elsewhere: one path for the
-            // whole wave, skipped when no lane found its key
+            // full, so every probe sequence ends there), then add the slot's
+            // weights: an empty slot's are zero (a key the group does not
+            // hold, or kSetEmpty past the end of the list)
             const uint32_t key = kk[u];
             uint32_t h = set_hash(key);
             uint32_t k = s_key[h];
@@ -547,16 +539,7 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
                 h = (h + 1u) & (kSetSlots - 1u);
                 k = s_key[h];
             }
-            const bool found = k == key && key != kSetEmpty;  // kSetEmpty: past the end of the list
-            if (ballot(found) == 0)
-                continue;
-            uint4 lo = s_mlo[h];
-            uint2 hi = wide ? s_mhi[h] : make_uint2(0, 0);
-            if (!found) {
-                lo = make_uint4(0, 0, 0, 0);
-                hi = make_uint2(0, 0);
-            }
-            set_add<NP>(c, lo, hi);
+            set_add<NP>(c, s_mlo[h], WIDE ? s_mhi[h] : make_uint2(0, 0));
         }
     }
 }
@@ -631,9 +614,13 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
         __syncthreads();
         bool wide = false;
         if constexpr (SETS) {
-            // the group's key table, built from its entries (unique keys)
-            for (uint32_t i = tid; i < kSetSlots; i += NT)
+            // the group's key table, built from its entries (unique keys);
+            // empty slots hold zero weights
+            for (uint32_t i = tid; i < kSetSlots; i += NT) {
                 s_skey[i] = kSetEmpty;
+                s_mlo[i] = make_uint4(0, 0, 0, 0);
+                s_mhi[i] = make_uint2(0, 0);
+            }
             if constexpr (SAMPLE)
                 for (uint32_t i = tid; i < 32 * HB; i += NT)
                     s_hist[i] = 0;
@@ -733,7 +720,10 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 uint32_t c[NP];
                 uint32_t o = ol;
                 if constexpr (SETS) {
-                    set_accumulate<NP>(a, ol, s_skey, s_mlo, s_mhi, wide, c);
+                    if (wide)  // uniform across the workgroup
+                        set_accumulate<NP, true>(a, ol, s_skey, s_mlo, s_mhi, c);
+                    else
+                        set_accumulate<NP, false>(a, ol, s_skey, s_mlo, s_mhi, c);
                     o = a.obj_perm[ol];  // kSetEmpty past the last object: counts 0, nothing emitted
                 } else {
                     dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
