@@ -56,11 +56,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds of trace per offered load (0 = skip)")
-    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000, 200000, 400000])
+    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000, 200000, 300000, 350000, 400000])
     ap.add_argument("--serve-max-requests", type=int, default=120000,
                     help="requests per offered load at most (a load runs min(qps x seconds, this))")
     ap.add_argument("--serve-executors", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=256)
+    ap.add_argument("--serve-codec", type=int, default=11,
+                    help="serve from the index compressed with this codec (segments of <= 1,024; default "
+                         "SERIAL_DELTA_BP32; 0 = the uncompressed index)")
     ap.add_argument("--serve-mode", type=int, default=2,
                     help="plan mode of the server (0 plain, 1 batched, 2 automatic: dense when it applies, else plain)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -169,12 +172,46 @@ def main():
         radius = np.array([0, 2, 5], np.int32)[synth.uniform_int(7002, 0, nreq_max, 0, 2)]
         sk = np.array([10, 100, 1000], np.uint32)[synth.uniform_int(7003, 1, nreq_max, 0, 2)]
         sranges = W.single_ranges(sq, radius)
-        sitems = table.compile(sranges, nreq_max)
+        stable, sindex, plain_ref = table, index, forms["plain"]["qps"]
+        index_desc = "the 1M x 128 index above, resident"
+        if args.serve_codec:
+            # the same rows compressed (segments of <= 1,024 ids), resident; the
+            # plain form timed on it for the comparison
+            t0 = time.time()
+            stable = Table.from_rows(w["rows"], 1024, True, codec=args.serve_codec)
+            sindex = stable.device()
+            citems = stable.compile(w["ranges"], w["n_queries"], merge_runs=True)
+            cplan = Plan(sindex, w["n_queries"], len(citems), int(np.max(kq)), 0, mode=0)
+            cplan.upload(citems, w["n_queries"], kq)
+            cplan.run()
+            cres = cplan.download()
+            genie_amd.device_synchronize()
+            c0 = time.perf_counter()
+            for _ in range(3):
+                cplan.run()
+            genie_amd.device_synchronize()
+            plain_ref = w["n_queries"] / ((time.perf_counter() - c0) / 3)
+            same_c = bool(np.array_equal(cres[0], results["plain"][0]) and np.array_equal(cres[1], results["plain"][1]))
+            index_desc = (f"the same rows compressed with codec {args.serve_codec} (segments of <= 1,024 ids), "
+                          f"resident; requests as merged segment runs")
+            log(f"[bench] serving index compressed (codec {args.serve_codec}) in {time.time() - t0:.1f}s; plain form "
+                f"on it: {plain_ref:.0f} queries/s, identical to uncompressed: {same_c}")
+        # requests compiled in chunks (a compressed table yields one item per
+        # segment before merging)
+        parts, step = [], 4096
+        for a0 in range(0, nreq_max, step):
+            a1 = min(nreq_max, a0 + step)
+            rr = sranges[(sranges["query"] >= a0) & (sranges["query"] < a1)].copy()
+            rr["query"] -= a0
+            it = stable.compile(rr, a1 - a0, merge_runs=bool(args.serve_codec))
+            it["query"] += a0
+            parts.append(it)
+        sitems = np.concatenate(parts)
         sptr = np.searchsorted(sitems["query"], np.arange(nreq_max + 1)).astype(np.uint64)
-        serving = {"index": "the 1M x 128 index above, resident", "requests": "ranges {0,+-2,+-5}, k {10,100,1000}",
+        serving = {"index": index_desc, "requests": "ranges {0,+-2,+-5}, k {10,100,1000}",
                    "mode": ["plain", "batched", "automatic"][args.serve_mode], "executors": args.serve_executors,
-                   "max_batch": args.serve_batch, "loads": []}
-        srv = Server(index, executors=args.serve_executors, max_batch=args.serve_batch, max_k=1000,
+                   "max_batch": args.serve_batch, "plain_qps_on_this_index": plain_ref, "loads": []}
+        srv = Server(sindex, executors=args.serve_executors, max_batch=args.serve_batch, max_k=1000,
                      mode=args.serve_mode)
         srv.replay(sitems[: int(sptr[64])], sptr[:65], sk[:64], np.zeros(64))  # warm
         for qps in args.serve_qps:
@@ -190,8 +227,10 @@ def main():
         # the highest load the server kept up with (achieved >= 97% of offered)
         kept = [e for e in serving["loads"] if e["achieved_qps"] >= 0.97 * e["offered_qps"]]
         serving["sustained_qps"] = max((e["achieved_qps"] for e in kept), default=None)
-        serving["sustained_vs_plain"] = (serving["sustained_qps"] / forms["plain"]["qps"]
+        serving["sustained_vs_plain"] = (serving["sustained_qps"] / plain_ref
                                          if serving["sustained_qps"] else None)
+        serving["sustained_vs_plain_uncompressed"] = (serving["sustained_qps"] / forms["plain"]["qps"]
+                                                      if serving["sustained_qps"] else None)
         srv.close()
 
     # ---- CPU baseline + exactness on the sample ------------------------------

@@ -401,6 +401,22 @@ class Server:
             pass
 
 
+def merge_item_runs(items):
+    """Joins items of one query, range (order) and weight whose spans follow
+    one another without a gap into one item over the whole run."""
+    items = np.ascontiguousarray(items, DIM_DTYPE)
+    if len(items) < 2:
+        return items.copy()
+    q, o, w = items["query"], items["order"], items["weight"]
+    sp, ep = items["start_pos"], items["end_pos"]
+    cont = (q[1:] == q[:-1]) & (o[1:] == o[:-1]) & (w[1:] == w[:-1]) & (sp[1:] == ep[:-1]) & (ep[:-1] > sp[:-1])
+    first = np.flatnonzero(np.concatenate([[True], ~cont]))
+    last = np.concatenate([first[1:], [len(items)]]) - 1
+    out = items[first].copy()
+    out["end_pos"] = ep[last]
+    return out
+
+
 class Table:
     """A GENIE table built by the C++ host library (inv_table / inv_compr_table)."""
 
@@ -456,7 +472,16 @@ class Table:
                                            _ptr(cpos) if len(cpos) else None, _ptr(lo), _ptr(hi), _ptr(ks)))
         return dict(inv=inv, inv_pos=pos, words=words, cpos=cpos, lo=lo, hi=hi, keyseg=ks, **s)
 
-    def compile(self, ranges, n_queries, load_balance=False, max_load=0, cap=None):
+    def compile(self, ranges, n_queries, load_balance=False, max_load=0, cap=None, merge_runs=False):
+        """Work items of a batch (Query::dim layout, grouped by query).  A
+        compressed table gives one item per segment (query.cc:464-533);
+        merge_runs joins the consecutive segments of one range into one item
+        spanning them, which every form of the search takes as the same work
+        (items may span whole segments), with far fewer items."""
+        out = self._compile(ranges, n_queries, load_balance, max_load, cap)
+        return merge_item_runs(out) if merge_runs else out
+
+    def _compile(self, ranges, n_queries, load_balance, max_load, cap):
         ranges = _c(ranges, RANGE_DTYPE)
         s = self.sizes()
         if cap is None:

@@ -87,6 +87,8 @@ struct DevIndex {
     std::vector<uint64_t> h_key_wstart;  // [n_keys + 1] first word of each key (compressed)
     PosIndex key_pos;                    // over h_key_start (uncompressed) or h_key_wstart (compressed)
     std::vector<uint32_t> h_seg_upos;    // [n_segments + 1] first posting of each segment (compressed)
+    std::vector<uint64_t> h_seg_cpos64;  // h_seg_cpos widened, and its position index (compressed)
+    PosIndex seg_pos;
 };
 
 void set_error(const std::string& msg);

@@ -434,9 +434,11 @@ int gh::validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_i
                                              ": weight must be a non-negative integer <= 65535");
         if (d.codec != 0 && it.end_pos > it.start_pos) {
             // compressed items cover whole segments
-            const uint32_t* cp = d.h_seg_cpos.data();
-            if (!std::binary_search(cp, cp + d.n_segments + 1, (uint32_t)it.start_pos) ||
-                !std::binary_search(cp, cp + d.n_segments + 1, (uint32_t)it.end_pos))
+            const uint64_t* cp = d.h_seg_cpos64.data();
+            const uint32_t a = d.seg_pos.first_ge(cp, (uint64_t)it.start_pos);
+            const uint32_t b = d.seg_pos.first_ge(cp, (uint64_t)it.end_pos);
+            if (a > d.n_segments || cp[a] != (uint64_t)it.start_pos || b > d.n_segments ||
+                cp[b] != (uint64_t)it.end_pos)
                 return fail(GENIE_E_INVALID, "item " + std::to_string(i) +
                                                  ": compressed span does not align with segments");
         }
@@ -497,6 +499,8 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
     for (uint32_t k = 0; k <= K; ++k)
         d.h_key_wstart[k] = cp[key_seg[k]];
     d.key_pos.build(d.h_key_wstart);
+    d.h_seg_cpos64.assign(cp, cp + S + 1);
+    d.seg_pos.build(d.h_seg_cpos64);
     if (desc->postings) {
         for (uint32_t k = 0; k < K; ++k) {
             uint32_t best = 1, run = 1;

@@ -105,3 +105,44 @@ def _key_dim(e):
 
 def _ks_off(e, d):
     return int(sum(e["hi"][j] - e["lo"][j] + 2 for j in range(d)))
+
+
+@pytest.mark.parametrize("table_kind", ["rows", "sets"])
+def test_merged_segment_items_give_the_same_results(table_kind):
+    """Items spanning several whole segments (a range's segments merged into
+    one item) against one item per segment, in every form and against the
+    oracle: plain (automatic, direct-mapped, hash), batched (dense or set
+    form, lane form) and automatic."""
+    from genie_amd import Plan
+    from helpers import oracle_topk_batch
+    if table_kind == "rows":
+        w = W.cfg1(seed=931, n=150_000, d=24, q=40, k=50, radius=3)
+        r = w["ranges"].copy()
+        r["weight"][::5] = 2
+        t = Table.from_rows(w["rows"], 256, True, codec=11)
+        q_of = r["query"]
+        ri, rc = oracle_topk_batch(lambda q: O.count_exhaustive_single(w["rows"], r[q_of == q]), 40, 50)
+    else:
+        rp, vals = genie_amd.synth.sets(932, 80_000, 5, 40, 4_000, 1.05)
+        qp, qv = genie_amd.synth.sets(933, 40, 1, 30, 4_000, 1.05)
+        r = W.set_ranges(qp, qv, genie_amd.synth.uniform_int(934, 0, len(qv), 1, 4))
+        t = Table.from_sets(rp, vals, 256, True, codec=6)
+        q_of = r["query"]
+        ri, rc = oracle_topk_batch(lambda q: O.count_exhaustive_biject(rp, vals, r[q_of == q]), 40, 50)
+    per_seg = t.compile(r, 40)
+    merged = t.compile(r, 40, merge_runs=True)
+    assert len(merged) < len(per_seg)
+    kq = np.full(40, 50, np.uint32)
+    for mode, hm, dense in [(0, -1, True), (0, 0, True), (0, 1, True), (1, -1, True), (1, -1, False), (2, -1, True)]:
+        res = []
+        for items in (per_seg, merged):
+            p = Plan(t.device(), 40, len(per_seg), 50, 0, mode=mode)
+            p.set_hash(hm)
+            if mode:
+                p.set_dense(dense)
+            p.upload(items, 40, kq)
+            p.run()
+            res.append(p.download())
+        where = f"{table_kind} mode {mode} hash {hm} dense {dense}"
+        assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), where
+        assert_topk_equal(res[1][0], res[1][1], ri, rc, 50, 40, where)

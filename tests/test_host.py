@@ -78,6 +78,31 @@ def test_compile_matches_oracle(lb, max_load):
     assert np.array_equal(got.view(np.uint8), want.view(np.uint8))
 
 
+def test_merged_runs_cover_the_same_segments():
+    """Merging a range's consecutive segment items keeps exactly the covered
+    positions, per (query, order, weight), and never joins different ranges."""
+    w = W.cfg0(n=3000, d=8, q=7)
+    r = w["ranges"].copy()
+    r["low"] -= 3
+    r["up"] += 3
+    r["weight"][::3] = 2
+    t = Table.from_rows(w["rows"], 50, True, codec=13)
+    per_seg = t.compile(r, w["n_queries"])
+    merged = t.compile(r, w["n_queries"], merge_runs=True)
+    assert len(merged) < len(per_seg)
+
+    def cover(items):
+        out = set()
+        for it in items:
+            for p in range(int(it["start_pos"]), int(it["end_pos"])):
+                out.add((int(it["query"]), int(it["order"]), float(it["weight"]), p))
+        return out
+    assert cover(per_seg) == cover(merged)
+    # one item per (query, range) that covers any segment
+    keys = {(int(it["query"]), int(it["order"])) for it in per_seg if it["end_pos"] > it["start_pos"]}
+    assert len([it for it in merged if it["end_pos"] > it["start_pos"]]) == len(keys)
+
+
 def test_compile_compressed_matches_oracle():
     w = W.cfg0(n=900, d=8, q=5)
     t = Table.from_rows(w["rows"], 50, True, codec=13)
