@@ -45,7 +45,7 @@ struct Request {
     std::vector<int32_t> ids, counts;
     int rc = GENIE_OK;
     std::string err;
-    bool done = false;
+    std::atomic<bool> done{false};  // set (release) after the results and t_done are written
 };
 
 struct Executor {
@@ -182,7 +182,7 @@ void genie_server::finish(Executor& e, int rc, const std::string& err_in)
     {
         std::lock_guard<std::mutex> lk(mu);
         for (Request* r : e.batch)
-            r->done = true;
+            r->done.store(true, std::memory_order_release);
         n_done += e.batch.size();
         inflight -= e.batch.size();
         mem_in_use -= e.mem;
@@ -351,11 +351,11 @@ int genie_hip_server_destroy(genie_server* s)
     return GENIE_OK;
 }
 
-int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t n_items, uint32_t k,
-                            uint64_t* ticket)
+// Submits one request; *out (optional) receives it, which stays valid until
+// its ticket is waited for or erased.
+static int submit_request(genie_server* s, const genie_dim_t* items, uint32_t n_items, uint32_t k,
+                          uint64_t* ticket, Request** out)
 {
-    if (!s || (!items && n_items) || !ticket)
-        return fail_s(GENIE_E_INVALID, "null argument");
     if (k == 0 || k > s->cfg.max_k)
         return fail_s(GENIE_E_INVALID, "k outside [1, max_k]");
     if (n_items > s->cfg.max_batch_items)
@@ -379,6 +379,8 @@ int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t 
         // filled every free executor and waits for a completion
         wake = s->pending.empty();
         s->pending.push_back(r.get());
+        if (out)
+            *out = r.get();
         s->reqs.emplace(r->ticket, std::move(r));
         if (wake)
             ++s->events;
@@ -386,6 +388,14 @@ int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t 
     if (wake)
         s->cv_work.notify_one();
     return GENIE_OK;
+}
+
+int genie_hip_server_submit(genie_server* s, const genie_dim_t* items, uint32_t n_items, uint32_t k,
+                            uint64_t* ticket)
+{
+    if (!s || (!items && n_items) || !ticket)
+        return fail_s(GENIE_E_INVALID, "null argument");
+    return submit_request(s, items, n_items, k, ticket, nullptr);
 }
 
 // Waits for a ticket; returns its results and its submit and completion times.
@@ -399,7 +409,7 @@ static int wait_ticket(genie_server* s, uint64_t ticket, int32_t* out_ids, int32
         if (it == s->reqs.end())
             return fail_s(GENIE_E_INVALID, "unknown ticket");
         Request* rp = it->second.get();
-        s->cv_done.wait(lk, [&] { return rp->done; });
+        s->cv_done.wait(lk, [&] { return rp->done.load(std::memory_order_acquire); });
         r = std::move(it->second);
         s->reqs.erase(it);
     }
@@ -433,6 +443,7 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
     if (!s || !item_ptr || !k_per_request || !arrival_s || !out_latency_ms)
         return fail_s(GENIE_E_INVALID, "null argument");
     std::vector<uint64_t> tickets(n_requests, 0);
+    std::vector<Request*> reqp(n_requests, nullptr);
     std::vector<std::atomic<int>> submitted(n_requests);
     for (auto& a : submitted)
         a.store(0);
@@ -449,9 +460,10 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
             for (uint32_t r = g; r < n_requests; r += G) {
                 std::this_thread::sleep_until(t0 + std::chrono::duration_cast<Clock::duration>(
                                                        std::chrono::duration<double>(arrival_s[r])));
-                const int rc = genie_hip_server_submit(s, items + item_ptr[r],
-                                                       (uint32_t)(item_ptr[r + 1] - item_ptr[r]), k_per_request[r],
-                                                       &tickets[r]);
+                const int rc = items || item_ptr[r + 1] == item_ptr[r]
+                                   ? submit_request(s, items + item_ptr[r], (uint32_t)(item_ptr[r + 1] - item_ptr[r]),
+                                                    k_per_request[r], &tickets[r], &reqp[r])
+                                   : fail_s(GENIE_E_INVALID, "null items");
                 if (rc) {
                     std::lock_guard<std::mutex> lk(gen_mu);
                     gen_rc = rc;
@@ -463,8 +475,18 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
                 submitted[r].store(1);
             }
         });
+    // results in request order: each request's completion flag is watched
+    // without the server's lock, and finished requests are erased in batches
+    // (one lock for many), so that collecting does not cap the load
     int rc = GENIE_OK;
     uint64_t off = 0;
+    std::vector<uint64_t> finished;
+    auto erase_finished = [&] {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (uint64_t t : finished)
+            s->reqs.erase(t);
+        finished.clear();
+    };
     for (uint32_t r = 0; r < n_requests; ++r) {
         while (submitted[r].load() == 0)  // the generator has not reached request r yet
             std::this_thread::sleep_for(std::chrono::microseconds(50));
@@ -472,18 +494,29 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
         if (submitted[r].load() < 0) {
             out_latency_ms[r] = -1;
         } else {
+            Request* rp = reqp[r];
+            while (!rp->done.load(std::memory_order_acquire))
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
             // latency from the scheduled arrival: a generator running late
             // counts against the server, not for it
-            Clock::time_point ts, td;
-            const int w = wait_ticket(s, tickets[r], out_ids ? out_ids + off : nullptr,
-                                      out_counts ? out_counts + off : nullptr, &ts, &td);
             const auto due = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[r]));
-            out_latency_ms[r] = std::chrono::duration<double, std::milli>(td - due).count();
-            if (w && !rc)
-                rc = w;
+            out_latency_ms[r] = std::chrono::duration<double, std::milli>(rp->t_done - due).count();
+            if (rp->rc) {
+                if (!rc)
+                    rc = fail_s(rp->rc, rp->err);
+            } else {
+                if (out_ids)
+                    std::memcpy(out_ids + off, rp->ids.data(), k * 4ull);
+                if (out_counts)
+                    std::memcpy(out_counts + off, rp->counts.data(), k * 4ull);
+            }
+            finished.push_back(tickets[r]);
+            if (finished.size() >= 256)
+                erase_finished();
         }
         off += k;
     }
+    erase_finished();
     for (std::thread& t : gens)
         t.join();
     if (wall_s)
