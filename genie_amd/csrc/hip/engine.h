@@ -97,6 +97,12 @@ int fail(int code, const std::string& msg);
 // posting (or compressed word) array, integral weights in [0, 65535],
 // compressed spans on segment boundaries.
 int validate_items(const DevIndex& d, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries);
+// genie_hip_plan_upload for items already validated one request at a time
+// (the server's batches): the same, without validating the items again.
+int plan_upload_validated(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                          const uint32_t* k_per_query);
+int plan_upload_staged(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                       const uint32_t* k_per_query, bool validate);
 
 }  // namespace gh
 

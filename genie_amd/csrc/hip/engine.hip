@@ -1906,13 +1906,14 @@ static int plan_settle_form(genie_plan* p)
 }
 
 static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
-                       const uint32_t* k_per_query)
+                       const uint32_t* k_per_query, bool validate)
 {
     if (n_queries == 0 || n_queries > p->max_queries || n_items > p->max_items)
         return fail(GENIE_E_INVALID, "batch larger than the plan");
     const DevIndex& d = p->ix->d;
-    if (int rc = validate_items(d, items, n_items, n_queries))
-        return rc;
+    if (validate)
+        if (int rc = validate_items(d, items, n_items, n_queries))
+            return rc;
     for (uint32_t q = 0; q < n_queries; ++q)
         if (k_per_query[q] == 0 || k_per_query[q] > p->max_k)
             return fail(GENIE_E_INVALID, "k of query " + std::to_string(q) + " outside [1, max_k]");
@@ -2031,6 +2032,28 @@ static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items
     return plan_size_candidates(p);
 }
 
+// The batch's device copies are staged in pinned memory and go out with the
+// next run, on its stream.
+int gh::plan_upload_staged(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                           const uint32_t* k_per_query, bool validate)
+{
+    if (int rc = stage_begin(p))
+        return rc;
+    const int rc = plan_upload(p, items, n_items, n_queries, k_per_query, validate);
+    p->staging = false;
+    if (rc)
+        p->staged.clear();  // a failed upload sends nothing
+    return rc;
+}
+
+int gh::plan_upload_validated(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                              const uint32_t* k_per_query)
+{
+    if (!p || (!items && n_items) || !k_per_query)
+        return fail(GENIE_E_INVALID, "null argument");
+    return plan_upload_staged(p, items, n_items, n_queries, k_per_query, false);
+}
+
 extern "C" {
 
 int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
@@ -2038,15 +2061,7 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
 {
     if (!p || (!items && n_items) || !k_per_query)
         return fail(GENIE_E_INVALID, "null argument");
-    // the batch's device copies are staged in pinned memory and go out with
-    // the next run, on its stream
-    if (int rc = stage_begin(p))
-        return rc;
-    const int rc = plan_upload(p, items, n_items, n_queries, k_per_query);
-    p->staging = false;
-    if (rc)
-        p->staged.clear();  // a failed upload sends nothing
-    return rc;
+    return gh::plan_upload_staged(p, items, n_items, n_queries, k_per_query, true);
 }
 
 }  // extern "C"

@@ -143,7 +143,8 @@ int genie_server::launch(Executor& e)
             return rc;
         e.host_cap = tot;
     }
-    if (int rc = genie_hip_plan_upload(e.plan, items.data(), items.size(), nq, kq.data()))
+    // every request was validated when it was submitted
+    if (int rc = gh::plan_upload_validated(e.plan, items.data(), items.size(), nq, kq.data()))
         return rc;
     e.phase_ns[0].fetch_add(ns_since(t0), std::memory_order_relaxed);
     const auto t1 = Clock::now();
