@@ -10,18 +10,21 @@ from genie_amd import GenieError, Server, Table, match_topk, workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def requests_of(table, w):
-    items = table.compile(w["ranges"], w["n_queries"])
+def requests_of(table, w, merge_runs=False):
+    items = table.compile(w["ranges"], w["n_queries"], merge_runs=merge_runs)
     q = items["query"]
     ptr = np.searchsorted(q, np.arange(w["n_queries"] + 1)).astype(np.uint64)
     return items, ptr
 
 
-@pytest.mark.parametrize("mode,codec", [(0, 0), (1, 0), (2, 0), (0, 11)])
-def test_alone_equals_in_the_mix(mode, codec):
+@pytest.mark.parametrize("mode,codec,merged", [(0, 0, False), (1, 0, False), (2, 0, False), (0, 11, False),
+                                               (2, 11, True), (1, 11, True)])
+def test_alone_equals_in_the_mix(mode, codec, merged):
+    """merged: a compressed index's requests carry one item per range (its
+    segments merged), as the serving benchmark sends them."""
     w = W.cfg4(n=200_000, d=32, q=60)
     t = Table.from_rows(w["rows"], 1024, codec != 0, codec=codec)
-    items, ptr = requests_of(t, w)
+    items, ptr = requests_of(t, w, merged)
     k = w["k"]
     srv = Server(t.device(), executors=3, max_batch=16, max_k=1000, mode=mode)
     alone = []
