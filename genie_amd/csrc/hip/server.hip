@@ -43,6 +43,8 @@ struct Request {
     uint64_t mem = 0;
     Clock::time_point t_submit, t_done;
     std::vector<int32_t> ids, counts;
+    int32_t* dst_ids = nullptr;     // when set (a replay): the results go straight here
+    int32_t* dst_counts = nullptr;
     int rc = GENIE_OK;
     std::string err;
     std::atomic<bool> done{false};  // set (release) after the results and t_done are written
@@ -173,7 +175,10 @@ void genie_server::finish(Executor& e, int rc, const std::string& err_in)
     for (Request* r : e.batch) {
         r->rc = rc;
         r->err = err;
-        if (!rc) {
+        if (!rc && r->dst_ids) {
+            std::memcpy(r->dst_ids, e.h_ids + off, r->k * 4ull);
+            std::memcpy(r->dst_counts, e.h_counts + off, r->k * 4ull);
+        } else if (!rc) {
             r->ids.assign(e.h_ids + off, e.h_ids + off + r->k);
             r->counts.assign(e.h_counts + off, e.h_counts + off + r->k);
         }
@@ -355,7 +360,8 @@ int genie_hip_server_destroy(genie_server* s)
 // Submits one request; *out (optional) receives it, which stays valid until
 // its ticket is waited for or erased.
 static int submit_request(genie_server* s, const genie_dim_t* items, uint32_t n_items, uint32_t k,
-                          uint64_t* ticket, Request** out)
+                          uint64_t* ticket, Request** out, int32_t* dst_ids = nullptr,
+                          int32_t* dst_counts = nullptr)
 {
     if (k == 0 || k > s->cfg.max_k)
         return fail_s(GENIE_E_INVALID, "k outside [1, max_k]");
@@ -366,6 +372,8 @@ static int submit_request(genie_server* s, const genie_dim_t* items, uint32_t n_
     for (genie_dim_t& it : r->items)
         it.query = 0;
     r->k = k;
+    r->dst_ids = dst_ids;
+    r->dst_counts = dst_counts;
     r->mem = request_bytes(n_items, k, s->ix->d.dense && s->cfg.mode != 0, s->ix->d.n_objects);
     // validate alone so that one bad request cannot fail a whole batch
     if (int rc = gh::validate_items(s->ix->d, r->items.data(), n_items, 1))
@@ -445,6 +453,11 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
         return fail_s(GENIE_E_INVALID, "null argument");
     std::vector<uint64_t> tickets(n_requests, 0);
     std::vector<Request*> reqp(n_requests, nullptr);
+    // each request's results go straight to its rows of the caller's arrays
+    const bool direct = out_ids && out_counts;
+    std::vector<uint64_t> offs(n_requests + 1, 0);
+    for (uint32_t r = 0; r < n_requests; ++r)
+        offs[r + 1] = offs[r] + k_per_request[r];
     std::vector<std::atomic<int>> submitted(n_requests);
     for (auto& a : submitted)
         a.store(0);
@@ -463,7 +476,9 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
                                                        std::chrono::duration<double>(arrival_s[r])));
                 const int rc = items || item_ptr[r + 1] == item_ptr[r]
                                    ? submit_request(s, items + item_ptr[r], (uint32_t)(item_ptr[r + 1] - item_ptr[r]),
-                                                    k_per_request[r], &tickets[r], &reqp[r])
+                                                    k_per_request[r], &tickets[r], &reqp[r],
+                                                    direct ? out_ids + offs[r] : nullptr,
+                                                    direct ? out_counts + offs[r] : nullptr)
                                    : fail_s(GENIE_E_INVALID, "null items");
                 if (rc) {
                     std::lock_guard<std::mutex> lk(gen_mu);
@@ -505,7 +520,7 @@ int genie_hip_server_replay(genie_server* s, const genie_dim_t* items, const uin
             if (rp->rc) {
                 if (!rc)
                     rc = fail_s(rp->rc, rp->err);
-            } else {
+            } else if (!direct) {
                 if (out_ids)
                     std::memcpy(out_ids + off, rp->ids.data(), k * 4ull);
                 if (out_counts)
