@@ -384,9 +384,9 @@ class Server:
     def stats(self):
         a, b, c = _u64(), _u64(), _u64()
         _chk_hip(_hip.genie_hip_server_stats(self.handle, C.byref(a), C.byref(b), C.byref(c)))
-        ph = (C.c_double * 4)()
+        ph = (C.c_double * 5)()
         _chk_hip(_hip.genie_hip_server_phase_seconds(self.handle, ph))
-        return dict(batches=a.value, requests=b.value, max_inflight=c.value,
+        return dict(batches=a.value, requests=b.value, max_inflight=c.value, reruns=int(ph[4]),
                     executor_seconds=dict(prepare_upload=ph[0], launch=ph[1], wait=ph[2], hand_back=ph[3]))
 
     def close(self):

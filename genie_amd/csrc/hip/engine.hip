@@ -1405,6 +1405,7 @@ static uint32_t max_cap_factor(const genie_plan* p)
 }
 
 static uint32_t initial_cap_factor(const genie_plan* p) { return dense_active(p) ? 64u : 8u; }
+constexpr uint64_t kDenseCapFloor = 16384;  // candidates per query the dense form's total holds at least
 
 // Candidate capacity per query: cap_factor x k, at most what the form can emit.
 static int plan_size_candidates(genie_plan* p)
@@ -1425,7 +1426,11 @@ static int plan_size_candidates(genie_plan* p)
             tiles = (p->ix->d.n_objects + tobj - 1) / tobj;
         }
         const uint64_t limit = dn ? (uint64_t)p->ix->d.n_objects : tiles * p->h_kq[q];
-        cap[q] = (uint32_t)std::min<uint64_t>((uint64_t)f * p->h_kq[q], limit);
+        // the dense form's device sizing lays each query's ask out in this
+        // total; a query whose counts crowd at its threshold asks for its
+        // ties, so small k gets a floor of kDenseCapFloor (memory only)
+        const uint64_t want = (uint64_t)f * p->h_kq[q];
+        cap[q] = (uint32_t)std::min<uint64_t>(dn ? std::max<uint64_t>(want, kDenseCapFloor) : want, limit);
         base[q] = tot;
         tot += cap[q];
     }

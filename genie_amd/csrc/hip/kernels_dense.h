@@ -186,16 +186,17 @@ __global__ void k_dense_holes(const uint32_t* __restrict__ bm, uint32_t n_object
 // is counted again from safe (k_dense_check).  One wave per query, four bins
 // per lane.
 // Candidate space: the objects of the whole index expected at or above a
-// threshold are S(c) x N / n_sample; a query asks for 5/4 of that at spec
-// (want) and at safe (want_safe, for a recount), plus 2 k + 256, times
-// `mult`, at most N.  Ties at the threshold are in S(c), so a query whose
+// threshold are S(c) x N / n_sample; a query asks for 3/2 of that at spec
+// (want) and at safe (want_safe, for a recount), plus 4 k + 1024, times
+// `mult`, at most N (the slack is address space only: a query writes what
+// it emits).  Ties at the threshold are in S(c), so a query whose
 // counts crowd at its threshold asks for what it will emit.
 // ratio16 = ceil(N * 2^16 / n_sample), from the host: no division here.
 __device__ __forceinline__ uint32_t dense_want(uint32_t s_at, uint32_t k, uint32_t ratio16, uint32_t n_objects,
                                                uint32_t mult)
 {
     const uint64_t e = ((uint64_t)s_at * ratio16 + 0xFFFFu) >> 16;
-    const uint64_t w = (e + e / 4 + 2ull * k + 256) * mult;
+    const uint64_t w = (e + e / 2 + 4ull * k + 1024) * mult;
     return (uint32_t)min<uint64_t>(w, n_objects);
 }
 

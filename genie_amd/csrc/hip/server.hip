@@ -67,6 +67,7 @@ struct Executor {
     // time spent per phase of a batch (ns, this executor's thread): prepare
     // and upload, launch, wait for the results, hand them back
     std::atomic<uint64_t> phase_ns[4] = {};
+    std::atomic<uint64_t> reruns{0};  // batches run again after a candidate overflow
 };
 
 uint64_t ns_since(Clock::time_point t) {
@@ -164,6 +165,7 @@ void genie_server::finish(Executor& e, int rc, const std::string& err_in)
 {
     std::string err = err_in;
     if (!rc && *e.h_ovf) {  // candidate space too small: grow, rerun, copy synchronously
+        e.reruns.fetch_add(1, std::memory_order_relaxed);
         rc = genie_hip_plan_download(e.plan, e.stream, e.h_ids, e.h_counts);
         if (rc)
             err = genie_hip_last_error();
@@ -566,6 +568,10 @@ int genie_hip_server_phase_seconds(genie_server* s, double* out)
             ns += e.phase_ns[i].load(std::memory_order_relaxed);
         out[i] = (double)ns * 1e-9;
     }
+    uint64_t reruns = 0;
+    for (const Executor& e : s->ex)
+        reruns += e.reruns.load(std::memory_order_relaxed);
+    out[4] = (double)reruns;
     return GENIE_OK;
 }
 

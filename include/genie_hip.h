@@ -249,8 +249,10 @@ int genie_hip_server_stats(genie_server* s, uint64_t* batches, uint64_t* request
 /* Seconds the executors spent so far, summed over executors, per phase of a
  * batch: out[0] gathering the batch and uploading it (host work and copies),
  * out[1] launching its kernels and result copies, out[2] waiting for them,
- * out[3] handing the results back.  Measurement only (no counterpart in the
- * reference, whose dist_genie scheduler has no timing). */
+ * out[3] handing the results back; out[4] the number of batches run again
+ * after their candidates overflowed the space first given them (out holds 5
+ * values).  Measurement only (no counterpart in the reference, whose
+ * dist_genie scheduler has no timing). */
 int genie_hip_server_phase_seconds(genie_server* s, double* out);
 
 /* ---- codec decode (tests and measurement) ------------------------------ */
