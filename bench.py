@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline wall time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds of trace per offered load (0 = skip)")
-    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000, 200000, 300000, 400000, 450000, 500000])
+    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000, 200000, 300000, 400000, 500000, 600000, 700000])
     ap.add_argument("--serve-max-requests", type=int, default=120000,
                     help="requests per offered load at most (a load runs min(qps x seconds, this))")
     ap.add_argument("--serve-executors", type=int, default=4)
