@@ -169,7 +169,7 @@ struct genie_plan {
     uint32_t* d_gp_ptr = nullptr;
     int32_t* d_gq = nullptr;
     uint32_t* d_queue = nullptr;
-    uint64_t gq_alloc = 0;
+    uint64_t gq_alloc = 0, gp_alloc = 0;
     std::vector<genie_dim_t> h_items;  // the uploaded batch sorted by query (work queue, byte figures)
     bool bytes_valid = false;
     std::vector<uint32_t> h_qptr;
@@ -188,7 +188,7 @@ struct genie_plan {
     uint4* d_smask = nullptr;
     uint32_t* d_sgptr = nullptr;
     uint32_t* d_sshift = nullptr;
-    uint64_t sent_alloc = 0;
+    uint64_t sent_alloc = 0, smask_alloc = 0;
     uint32_t dn_groups = 0;
     uint32_t* d_dtab = nullptr;      // [dn_groups][dpad * 256] query masks
     uint64_t dtab_alloc = 0;
@@ -621,6 +621,42 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
 static uint32_t plan_n_tiles(const genie_plan* p);
 static ScanArgs scan_args(genie_plan* p);
 
+// ---- per-batch buffers that grow ------------------------------------------
+
+constexpr uint64_t kGrowSlackBytes = 256ull << 20;  // most a growing buffer takes beyond its need
+
+// Grows *buf (holding *alloc elements) to at least `need` elements; the
+// contents are not kept.  Freeing the old buffer waits for the whole device
+// (every stream, every executor of a server), so a buffer grows to room for a
+// full batch at this batch's size per query, at least twice its old size, at
+// most kGrowSlackBytes beyond the need (the exact need when that fails): a
+// serving loop whose batches grow step by step then stalls a few times at
+// most instead of at every new largest batch.
+template <class T>
+static int grow_buf(const genie_plan* p, T** buf, uint64_t* alloc, uint64_t need)
+{
+    if (need <= *alloc)
+        return GENIE_OK;
+    const uint64_t old = *alloc;
+    dev_free(*buf);
+    *alloc = 0;
+    const uint64_t nq = std::max<uint32_t>(1, p->n_queries);
+    const uint64_t full = (need + nq - 1) / nq * std::max<uint64_t>(nq, p->max_queries);
+    const uint64_t n = std::max(need, std::min(std::max(full, 2 * old), need + kGrowSlackBytes / sizeof(T)));
+    if (n > need) {
+        if (hipMalloc((void**)buf, n * sizeof(T)) == hipSuccess) {
+            *alloc = n;
+            return GENIE_OK;
+        }
+        *buf = nullptr;
+        (void)hipGetLastError();  // the exact size is tried next
+    }
+    if (int rc = dev_alloc(buf, need))
+        return rc;
+    *alloc = need;
+    return GENIE_OK;
+}
+
 // ---- batched form: host-side work queue ---------------------------------
 
 // Group consecutive queries into lanes of one u64 counter word, resolve every
@@ -716,23 +752,13 @@ static int build_batched(genie_plan* p)
     std::vector<GPair> all(np);
     for (uint32_t g = 0; g < G; ++g)
         std::copy(per[g].begin(), per[g].end(), all.begin() + gp[g]);
-    if (np > p->pairs_alloc) {
-        dev_free(p->d_pairs);
-        p->pairs_alloc = 0;
-        if (int rc = dev_alloc(&p->d_pairs, np))
-            return rc;
-        p->pairs_alloc = np;
-    }
-    if ((uint64_t)G * lanes > p->gq_alloc) {
-        dev_free(p->d_gp_ptr);
-        dev_free(p->d_gq);
-        p->gq_alloc = 0;
-        if (int rc = dev_alloc(&p->d_gp_ptr, G + 1))
-            return rc;
-        if (int rc = dev_alloc(&p->d_gq, (uint64_t)G * lanes))
-            return rc;
-        p->gq_alloc = (uint64_t)G * lanes;
-    }
+    // sized apart: a batch of a wider lane width has more groups in the same G x lanes
+    if (int rc = grow_buf(p, &p->d_pairs, &p->pairs_alloc, np))
+        return rc;
+    if (int rc = grow_buf(p, &p->d_gp_ptr, &p->gp_alloc, G + 1ull))
+        return rc;
+    if (int rc = grow_buf(p, &p->d_gq, &p->gq_alloc, (uint64_t)G * lanes))
+        return rc;
     if (np)
         HIP_TRY(hipMemcpy(p->d_pairs, all.data(), np * sizeof(GPair), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_gp_ptr, gp.data(), gp.size() * 4, hipMemcpyHostToDevice));
@@ -1434,13 +1460,8 @@ static int plan_size_candidates(genie_plan* p)
         base[q] = tot;
         tot += cap[q];
     }
-    if (tot > p->cand_alloc) {
-        dev_free(p->d_cand);
-        p->cand_alloc = 0;
-        if (int rc = dev_alloc(&p->d_cand, tot))
-            return rc;
-        p->cand_alloc = tot;
-    }
+    if (int rc = grow_buf(p, &p->d_cand, &p->cand_alloc, tot))
+        return rc;
     p->total_cap = tot;
     if (int rc = h2d(p, &p->d_cand_cap, cap.data(), cap.size() * 4))
         return rc;
@@ -1615,13 +1636,8 @@ static int prepare_dense(genie_plan* p)
     const DevIndex& d = p->ix->d;
     p->dn_groups = (p->n_queries + 31) / 32;
     const uint64_t tw = (uint64_t)d.dense_dpad * kDenseKeysPerDim;
-    if (p->dn_groups * tw > p->dtab_alloc) {
-        dev_free(p->d_dtab);
-        p->dtab_alloc = 0;
-        if (int rc = dev_alloc(&p->d_dtab, p->dn_groups * tw))
-            return rc;
-        p->dtab_alloc = p->dn_groups * tw;
-    }
+    if (int rc = grow_buf(p, &p->d_dtab, &p->dtab_alloc, p->dn_groups * tw))
+        return rc;
     if (!p->d_dhist)
         if (int rc = dev_alloc(&p->d_dhist, (uint64_t)p->max_queries * kDenseBins))
             return rc;
@@ -1634,16 +1650,10 @@ static int prepare_dense(genie_plan* p)
                 return rc;
     } else {  // set form: the groups' key entries
         const uint64_t ne = std::max<uint64_t>(1, p->h_skeys.size());
-        if (ne > p->sent_alloc) {
-            dev_free(p->d_skeys);
-            dev_free(p->d_smask);
-            p->sent_alloc = 0;
-            if (int rc = dev_alloc(&p->d_skeys, ne))
-                return rc;
-            if (int rc = dev_alloc(&p->d_smask, 2 * ne))
-                return rc;
-            p->sent_alloc = ne;
-        }
+        if (int rc = grow_buf(p, &p->d_skeys, &p->sent_alloc, ne))
+            return rc;
+        if (int rc = grow_buf(p, &p->d_smask, &p->smask_alloc, 2 * ne))
+            return rc;
         if (!p->d_sgptr) {
             if (int rc = dev_alloc(&p->d_sgptr, (p->max_queries + 31) / 32 + 1))
                 return rc;

@@ -76,6 +76,27 @@ def test_batched_lane_widths(wmax, lb):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+def test_lane_widths_change_between_uploads():
+    """One plan, batches of 64 queries at lane widths 8, 16, 32 and 8 again:
+    4, 8 and 16 queries share a counter word, so the later batches hold more
+    query groups (16, then 32) than the first (8) in the same group table."""
+    w = W.cfg1(n=40_000, d=24, q=64, k=20)
+    t = Table.from_rows(w["rows"])
+    p = Plan(t.device(), 64, 64 * 24, 20, 0, mode=1)
+    p.set_dense(False)
+    for wmax, lb in [(1, 8), (300, 16), (5000, 32), (1, 8)]:
+        r = w["ranges"].copy()
+        r["weight"] = np.random.default_rng(wmax).integers(1, wmax + 1, len(r))
+        items = t.compile(r, 64)
+        p.upload(items, 64, np.full(64, 20, np.uint32))
+        p.run()
+        b = p.download()
+        assert p.config()["lane_bits"] == lb
+        a = plain(t, items, 64, 20)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), f"lane width {lb}"
+    p.close()
+
+
 def test_batched_compressed():
     w = W.cfg4(n=120_000, d=16, q=20)
     plain_t = Table.from_rows(w["rows"])
