@@ -215,16 +215,25 @@ def main():
                    "max_batch": args.serve_batch, "plain_qps_on_this_index": plain_ref, "loads": []}
         srv = Server(sindex, executors=args.serve_executors, max_batch=args.serve_batch, max_k=1000,
                      mode=args.serve_mode)
-        srv.replay(sitems[: int(sptr[64])], sptr[:65], sk[:64], np.zeros(64))  # warm
+        # untimed warm-up: the first batches of each shape pay one-time costs
+        # (first launches, buffers sized): full batches (all requests at
+        # once), then small ones (20 K QPS)
+        nw = min(nreq_max, 4 * args.serve_batch * args.serve_executors)
+        srv.replay(sitems[: int(sptr[nw])], sptr[: nw + 1], sk[:nw], np.zeros(nw))
+        nw = min(nreq_max, 2000)
+        srv.replay(sitems[: int(sptr[nw])], sptr[: nw + 1], sk[:nw], np.arange(nw) / 20000.0)
+        srv.slowest(reset=True)
         for qps in args.serve_qps:
             n = min(int(qps * args.serve_seconds), nreq_max)
             arr = np.cumsum(synth.exponential(7004 + qps, n, float(qps)))
             lat, _, _, wall = srv.replay(sitems[: int(sptr[n])], sptr[: n + 1], sk[:n], arr)
             ent = dict(offered_qps=qps, requests=n, achieved_qps=n / wall, p50_ms=float(np.percentile(lat, 50)),
-                       p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()))
+                       p99_ms=float(np.percentile(lat, 99)), mean_ms=float(lat.mean()),
+                       slowest_ms={k: v * 1e3 for k, v in srv.slowest(reset=True).items()})
             serving["loads"].append(ent)
             log(f"[bench] serving @ {qps} QPS offered: achieved {ent['achieved_qps']:.0f} QPS, "
-                f"p50 {ent['p50_ms']:.2f} ms, p99 {ent['p99_ms']:.2f} ms")
+                f"p50 {ent['p50_ms']:.2f} ms, p99 {ent['p99_ms']:.2f} ms; slowest batch (ms): "
+                + ", ".join(f"{k} {v:.2f}" for k, v in ent["slowest_ms"].items()))
         serving["stats"] = srv.stats()
         # the highest load the server kept up with (achieved >= 97% of offered)
         kept = [e for e in serving["loads"] if e["achieved_qps"] >= 0.97 * e["offered_qps"]]

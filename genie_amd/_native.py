@@ -100,6 +100,7 @@ _sig(_hip, "genie_hip_server_wait", C.c_int, _p, _u64, _p, _p, C.POINTER(C.c_dou
 _sig(_hip, "genie_hip_server_replay", C.c_int, _p, _p, _p, _p, _p, _u32, _p, _p, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_server_stats", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64))
 _sig(_hip, "genie_hip_server_phase_seconds", C.c_int, _p, C.POINTER(C.c_double))
+_sig(_hip, "genie_hip_server_phase_max_seconds", C.c_int, _p, C.POINTER(C.c_double), C.c_int)
 _sig(_hip, "genie_hip_decode", C.c_int, C.c_int, _p, _p, _u32, _u32, _p, _p)
 _sig(_hip, "genie_hip_last_error", C.c_char_p)
 _sig(_hip, "genie_hip_stream_synchronize", C.c_int, _p)
@@ -388,6 +389,13 @@ class Server:
         _chk_hip(_hip.genie_hip_server_phase_seconds(self.handle, ph))
         return dict(batches=a.value, requests=b.value, max_inflight=c.value, reruns=int(ph[4]),
                     executor_seconds=dict(prepare_upload=ph[0], launch=ph[1], wait=ph[2], hand_back=ph[3]))
+
+    def slowest(self, reset=True):
+        """The longest single batch per phase and the longest queue wait (s)
+        since the last reset."""
+        ph = (C.c_double * 5)()
+        _chk_hip(_hip.genie_hip_server_phase_max_seconds(self.handle, ph, int(bool(reset))))
+        return dict(prepare_upload=ph[0], launch=ph[1], wait=ph[2], hand_back=ph[3], queue=ph[4])
 
     def close(self):
         if self.handle:

@@ -156,7 +156,7 @@ struct genie_plan {
     uint64_t* d_out_off = nullptr;
     int32_t* d_out_ids = nullptr;
     int32_t* d_out_counts = nullptr;
-    uint64_t out_alloc = 0;
+    uint64_t out_alloc = 0, outc_alloc = 0;
     std::vector<uint32_t> h_kq;
     std::vector<uint64_t> h_out_off;
     // batched form (mode 1)
@@ -631,9 +631,11 @@ constexpr uint64_t kGrowSlackBytes = 256ull << 20;  // most a growing buffer tak
 // full batch at this batch's size per query, at least twice its old size, at
 // most kGrowSlackBytes beyond the need (the exact need when that fails): a
 // serving loop whose batches grow step by step then stalls a few times at
-// most instead of at every new largest batch.
+// most instead of at every new largest batch.  `worst` (elements, optional)
+// is what the plan's largest batch can need; it is taken within the same
+// limit.
 template <class T>
-static int grow_buf(const genie_plan* p, T** buf, uint64_t* alloc, uint64_t need)
+static int grow_buf(const genie_plan* p, T** buf, uint64_t* alloc, uint64_t need, uint64_t worst = 0)
 {
     if (need <= *alloc)
         return GENIE_OK;
@@ -642,7 +644,7 @@ static int grow_buf(const genie_plan* p, T** buf, uint64_t* alloc, uint64_t need
     *alloc = 0;
     const uint64_t nq = std::max<uint32_t>(1, p->n_queries);
     const uint64_t full = (need + nq - 1) / nq * std::max<uint64_t>(nq, p->max_queries);
-    const uint64_t n = std::max(need, std::min(std::max(full, 2 * old), need + kGrowSlackBytes / sizeof(T)));
+    const uint64_t n = std::max(need, std::min(std::max({full, 2 * old, worst}), need + kGrowSlackBytes / sizeof(T)));
     if (n > need) {
         if (hipMalloc((void**)buf, n * sizeof(T)) == hipSuccess) {
             *alloc = n;
@@ -1384,7 +1386,11 @@ static int h2d(genie_plan* p, T** dst, const void* src, uint64_t bytes)
     }
     const uint64_t off = (p->stage_used + 255) & ~255ull;
     if (off + bytes > p->stage_cap) {
-        const uint64_t cap = std::max<uint64_t>({off + bytes, 2 * p->stage_cap, 1ull << 20});
+        // first time: room for a batch of max_items items (their 20 B, key
+        // ranges and per-query words), so that a serving loop seldom grows it
+        // (freeing pinned memory waits for the whole device)
+        const uint64_t est = p->stage_cap ? 0 : std::min<uint64_t>(64ull << 20, p->max_items * 40 + p->max_queries * 64ull);
+        const uint64_t cap = std::max<uint64_t>({off + bytes, 2 * p->stage_cap, 1ull << 20, est});
         uint8_t* nb = nullptr;
         if (hipHostMalloc((void**)&nb, cap, hipHostMallocDefault) != hipSuccess)
             return fail(GENIE_E_OOM, "pinned upload buffer of " + std::to_string(cap) + " bytes");
@@ -1460,7 +1466,11 @@ static int plan_size_candidates(genie_plan* p)
         base[q] = tot;
         tot += cap[q];
     }
-    if (int rc = grow_buf(p, &p->d_cand, &p->cand_alloc, tot))
+    // the largest batch: max_queries queries of max_k, sized as above
+    const uint64_t kw = p->max_k;
+    const uint64_t qworst = std::min<uint64_t>(dn ? std::max<uint64_t>((uint64_t)f * kw, kDenseCapFloor) : (uint64_t)f * kw,
+                                               dn ? (uint64_t)p->ix->d.n_objects : (uint64_t)n_tiles * kw);
+    if (int rc = grow_buf(p, &p->d_cand, &p->cand_alloc, tot, qworst * p->max_queries))
         return rc;
     p->total_cap = tot;
     if (int rc = h2d(p, &p->d_cand_cap, cap.data(), cap.size() * 4))
@@ -2017,16 +2027,11 @@ static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items
     for (uint32_t q = 0; q < n_queries; ++q)
         p->h_out_off[q + 1] = p->h_out_off[q] + k_per_query[q];
     p->total_k = p->h_out_off[n_queries];
-    if (p->total_k > p->out_alloc) {
-        dev_free(p->d_out_ids);
-        dev_free(p->d_out_counts);
-        p->out_alloc = 0;
-        if (int rc = dev_alloc(&p->d_out_ids, p->total_k))
-            return rc;
-        if (int rc = dev_alloc(&p->d_out_counts, p->total_k))
-            return rc;
-        p->out_alloc = p->total_k;
-    }
+    const uint64_t out_worst = (uint64_t)p->max_queries * p->max_k;
+    if (int rc = grow_buf(p, &p->d_out_ids, &p->out_alloc, p->total_k, out_worst))
+        return rc;
+    if (int rc = grow_buf(p, &p->d_out_counts, &p->outc_alloc, p->total_k, out_worst))
+        return rc;
     if (n_items)
         if (int rc = h2d(p, &p->d_items, p->h_items.data(), n_items * sizeof(genie_dim_t)))
             return rc;

@@ -67,11 +67,27 @@ struct Executor {
     // time spent per phase of a batch (ns, this executor's thread): prepare
     // and upload, launch, wait for the results, hand them back
     std::atomic<uint64_t> phase_ns[4] = {};
+    std::atomic<uint64_t> phase_max_ns[4] = {};  // the longest single batch per phase
     std::atomic<uint64_t> reruns{0};  // batches run again after a candidate overflow
 };
 
 uint64_t ns_since(Clock::time_point t) {
     return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t).count();
+}
+
+void note_max(std::atomic<uint64_t>& m, uint64_t d)
+{
+    uint64_t v = m.load(std::memory_order_relaxed);
+    while (d > v && !m.compare_exchange_weak(v, d, std::memory_order_relaxed))
+        ;
+}
+
+// Adds the time since t to phase i of executor e and to its longest batch.
+void note_phase(Executor& e, int i, Clock::time_point t)
+{
+    const uint64_t d = ns_since(t);
+    e.phase_ns[i].fetch_add(d, std::memory_order_relaxed);
+    note_max(e.phase_max_ns[i], d);
 }
 
 // Device scratch one request holds while in flight: its work items, its
@@ -98,6 +114,7 @@ struct genie_server {
     uint64_t next_ticket = 1, mem_in_use = 0;
     uint64_t n_batches = 0, n_done = 0, max_inflight = 0, inflight = 0;
     uint64_t events = 0;  // submissions and completions: what the scheduler waits for
+    std::atomic<uint64_t> queue_max_ns{0};  // longest wait from submission to a batch
     bool stop = false, stop_exec = false;
     std::thread sched;
     int device = 0;
@@ -136,20 +153,22 @@ int genie_server::launch(Executor& e)
     for (uint32_t k : kq)
         tot += k;
     if (tot > e.host_cap) {
+        // once for the largest batch (pinned memory: freeing it waits for the device)
+        const uint64_t cap = std::max<uint64_t>(tot, (uint64_t)cfg.max_batch * cfg.max_k);
         genie_hip_host_free(e.h_ids);
         genie_hip_host_free(e.h_counts);
         e.h_ids = e.h_counts = nullptr;
         e.host_cap = 0;
-        if (int rc = genie_hip_host_alloc((void**)&e.h_ids, tot * 4))
+        if (int rc = genie_hip_host_alloc((void**)&e.h_ids, cap * 4))
             return rc;
-        if (int rc = genie_hip_host_alloc((void**)&e.h_counts, tot * 4))
+        if (int rc = genie_hip_host_alloc((void**)&e.h_counts, cap * 4))
             return rc;
-        e.host_cap = tot;
+        e.host_cap = cap;
     }
     // every request was validated when it was submitted
     if (int rc = gh::plan_upload_validated(e.plan, items.data(), items.size(), nq, kq.data()))
         return rc;
-    e.phase_ns[0].fetch_add(ns_since(t0), std::memory_order_relaxed);
+    note_phase(e, 0, t0);
     const auto t1 = Clock::now();
     if (int rc = genie_hip_plan_run(e.plan, e.stream))
         return rc;
@@ -157,7 +176,7 @@ int genie_server::launch(Executor& e)
         return rc;
     if (hipEventRecord(e.ev, e.stream) != hipSuccess)
         return fail_s(GENIE_E_RUNTIME, "hipEventRecord");
-    e.phase_ns[1].fetch_add(ns_since(t1), std::memory_order_relaxed);
+    note_phase(e, 1, t1);
     return GENIE_OK;
 }
 
@@ -228,10 +247,10 @@ void genie_server::exec_loop(Executor& e)
                 err = hipGetErrorString(q);
             }
         }
-        e.phase_ns[2].fetch_add(ns_since(tw), std::memory_order_relaxed);
+        note_phase(e, 2, tw);
         const auto tf = Clock::now();
         finish(e, rc, err);
-        e.phase_ns[3].fetch_add(ns_since(tf), std::memory_order_relaxed);
+        note_phase(e, 3, tf);
         lk.lock();
     }
 }
@@ -261,6 +280,7 @@ void genie_server::run()
             }
             if (e.batch.empty())
                 continue;
+            note_max(queue_max_ns, ns_since(e.batch.front()->t_submit));  // FIFO: the oldest
             e.mem = mem;
             e.busy = true;
             e.assigned = true;
@@ -572,6 +592,21 @@ int genie_hip_server_phase_seconds(genie_server* s, double* out)
     for (const Executor& e : s->ex)
         reruns += e.reruns.load(std::memory_order_relaxed);
     out[4] = (double)reruns;
+    return GENIE_OK;
+}
+
+int genie_hip_server_phase_max_seconds(genie_server* s, double* out, int reset)
+{
+    if (!s || !out)
+        return fail_s(GENIE_E_INVALID, "null argument");
+    for (int i = 0; i < 4; ++i) {
+        uint64_t ns = 0;
+        for (Executor& e : s->ex)
+            ns = std::max(ns, reset ? e.phase_max_ns[i].exchange(0, std::memory_order_relaxed)
+                                    : e.phase_max_ns[i].load(std::memory_order_relaxed));
+        out[i] = (double)ns * 1e-9;
+    }
+    out[4] = (double)(reset ? s->queue_max_ns.exchange(0) : s->queue_max_ns.load()) * 1e-9;
     return GENIE_OK;
 }
 

@@ -254,6 +254,12 @@ int genie_hip_server_stats(genie_server* s, uint64_t* batches, uint64_t* request
  * values).  Measurement only (no counterpart in the reference, whose
  * dist_genie scheduler has no timing). */
 int genie_hip_server_phase_seconds(genie_server* s, double* out);
+/* The longest single batch per phase (out[0..3], as above, over executors)
+ * and the longest time a request waited between its submission and its batch
+ * going to an executor (out[4]); out holds 5 values.  reset != 0 clears them
+ * after reading, so that each call covers the span since the last one.
+ * Measurement only (no counterpart in the reference). */
+int genie_hip_server_phase_max_seconds(genie_server* s, double* out, int reset);
 
 /* ---- codec decode (tests and measurement) ------------------------------ */
 /* Decode n_arrays compressed arrays ([cpos[i], cpos[i+1]) words of `words`)

@@ -45,6 +45,12 @@ def test_alone_equals_in_the_mix(mode, codec, merged):
     st = srv.stats()
     assert st["requests"] == 3 * w["n_queries"] and st["batches"] < st["requests"]
     assert st["max_inflight"] > 1
+    # the slowest batch per phase is within the phase's total, and a reset clears it
+    sl = srv.slowest(reset=True)
+    tot = st["executor_seconds"]
+    assert 0 < sl["wait"] <= tot["wait"] and 0 < sl["prepare_upload"] <= tot["prepare_upload"]
+    assert sl["queue"] > 0
+    assert all(v == 0 for v in srv.slowest(reset=False).values())
     # and the same as one plain batch of everything
     pi, pc = match_topk(t.device(), items, w["n_queries"], k)
     assert np.array_equal(pi, ids) and np.array_equal(pc, cnt)
