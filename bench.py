@@ -70,8 +70,8 @@ def main():
                     help="also run one device copy of this size (scales FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} refused: this build measures one GPU "
-              "(sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus} refused: the search runs on one GPU here; "
+              "an index sharded by object would run one such process per GPU", file=sys.stderr)
         sys.exit(2)
 
     import genie_amd
