@@ -1473,6 +1473,11 @@ static int plan_size_candidates(genie_plan* p)
     if (int rc = grow_buf(p, &p->d_cand, &p->cand_alloc, tot, qworst * p->max_queries))
         return rc;
     p->total_cap = tot;
+    // the dense and set forms lay every query's space out in this total on
+    // the device (k_cap_prefix, each run, before anything reads it); a switch
+    // to another form sizes the candidates again and uploads its own
+    if (dn)
+        return GENIE_OK;
     if (int rc = h2d(p, &p->d_cand_cap, cap.data(), cap.size() * 4))
         return rc;
     return h2d(p, &p->d_cand_base, base.data(), base.size() * 8);
