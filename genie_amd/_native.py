@@ -134,7 +134,7 @@ _sig(_host, "genie_table_destroy", None, _p)
 _sig(_host, "genie_table_sizes", C.c_int, _p, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64),
      C.POINTER(_u32), C.POINTER(_u32), C.POINTER(_u64))
 _sig(_host, "genie_table_export", C.c_int, _p, _p, _p, _p, _p, _p, _p, _p)
-_sig(_host, "genie_table_compile", C.c_int, _p, _p, _u64, _u32, C.c_int, _i32, _p, _u64, C.POINTER(_u64))
+_sig(_host, "genie_table_compile", C.c_int, _p, _p, _u64, _u32, C.c_int, _i32, C.c_int, _p, _u64, C.POINTER(_u64))
 _sig(_host, "genie_table_device", C.c_int, _p, C.POINTER(_p))
 _sig(_host, "genie_table_save", C.c_int, _p, C.c_char_p)
 _sig(_host, "genie_wire_parse_query", C.c_int, C.c_char_p, C.POINTER(_i32), C.POINTER(_u32))
@@ -409,22 +409,6 @@ class Server:
             pass
 
 
-def merge_item_runs(items):
-    """Joins items of one query, range (order) and weight whose spans follow
-    one another without a gap into one item over the whole run."""
-    items = np.ascontiguousarray(items, DIM_DTYPE)
-    if len(items) < 2:
-        return items.copy()
-    q, o, w = items["query"], items["order"], items["weight"]
-    sp, ep = items["start_pos"], items["end_pos"]
-    cont = (q[1:] == q[:-1]) & (o[1:] == o[:-1]) & (w[1:] == w[:-1]) & (sp[1:] == ep[:-1]) & (ep[:-1] > sp[:-1])
-    first = np.flatnonzero(np.concatenate([[True], ~cont]))
-    last = np.concatenate([first[1:], [len(items)]]) - 1
-    out = items[first].copy()
-    out["end_pos"] = ep[last]
-    return out
-
-
 class Table:
     """A GENIE table built by the C++ host library (inv_table / inv_compr_table)."""
 
@@ -484,25 +468,24 @@ class Table:
         """Work items of a batch (Query::dim layout, grouped by query).  A
         compressed table gives one item per segment (query.cc:464-533);
         merge_runs joins the consecutive segments of one range into one item
-        spanning them, which every form of the search takes as the same work
-        (items may span whole segments), with far fewer items."""
-        out = self._compile(ranges, n_queries, load_balance, max_load, cap)
-        return merge_item_runs(out) if merge_runs else out
-
-    def _compile(self, ranges, n_queries, load_balance, max_load, cap):
+        spanning them (Query::build_compressed in the host library), which
+        every form of the search takes as the same work (items may span whole
+        segments), with far fewer items."""
         ranges = _c(ranges, RANGE_DTYPE)
         s = self.sizes()
-        if cap is None:
+        if cap is None and merge_runs and not load_balance:
+            cap = len(ranges) + 1024  # at most one item per range
+        elif cap is None:
             cap = max(1024, 4 * len(ranges) + s["n_seg"] * 2 + (s["n_post"] // max(1, max_load) + 1 if load_balance else 0))
         out = np.zeros(cap, DIM_DTYPE)
         n = _u64()
         rc = _host.genie_table_compile(self.handle, _ptr(ranges), len(ranges), n_queries, int(load_balance),
-                                       max_load, _ptr(out), cap, C.byref(n))
+                                       max_load, int(merge_runs), _ptr(out), cap, C.byref(n))
         if rc != 0 and n.value > cap:  # capacity too small: the call reported the size needed
             cap = n.value
             out = np.zeros(cap, DIM_DTYPE)
             rc = _host.genie_table_compile(self.handle, _ptr(ranges), len(ranges), n_queries, int(load_balance),
-                                           max_load, _ptr(out), cap, C.byref(n))
+                                           max_load, int(merge_runs), _ptr(out), cap, C.byref(n))
         _chk_host(rc)
         return out[: n.value].copy()
 

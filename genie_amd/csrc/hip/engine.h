@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -90,6 +92,25 @@ struct DevIndex {
     std::vector<uint64_t> h_seg_cpos64;  // h_seg_cpos widened, and its position index (compressed)
     PosIndex seg_pos;
 };
+
+// Candidates per query the dense and set forms' host total holds at least
+// (a query whose counts crowd at its threshold asks for its ties).
+constexpr uint64_t kDenseCapFloor = 16384;
+// Initial factor of k that a plan gives each query's candidate space: the
+// dense and set forms, then the tile forms (plain and lane).
+constexpr uint32_t kDenseCapFactor = 64;
+constexpr uint32_t kTileCapFactor = 8;
+
+// Candidate keys a new plan first gives one query with this k (the sizing of
+// plan_size_candidates before any overflow): the dense and set forms hold
+// max(64 k, kDenseCapFloor), at most one per object; the tile forms 8 k.
+inline uint64_t initial_candidates(uint32_t k, bool dense, uint32_t n_objects)
+{
+    if (!dense)
+        return (uint64_t)kTileCapFactor * k;
+    const uint64_t c = (uint64_t)kDenseCapFactor * k;
+    return std::min<uint64_t>(c > kDenseCapFloor ? c : kDenseCapFloor, n_objects);
+}
 
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);

@@ -98,6 +98,10 @@ struct genie_plan {
     uint32_t n_queries = 0;
     uint64_t n_items = 0, n_chunks = 0, total_k = 0, total_cap = 0;
     uint32_t cap_factor = 8;
+    // dense and set forms: multiplier of the device-side candidate asks,
+    // raised after a run whose asks were too small (a sample that
+    // underestimated a query's count distribution)
+    uint32_t dense_mult = 1;
     uint64_t logical_bytes = 0, compulsory_bytes = 0;
     // plain form: per query, the hash form (selective queries) or the
     // direct-mapped kernels; the direct queries' postings choose the sparse
@@ -153,6 +157,7 @@ struct genie_plan {
     uint64_t* d_cand = nullptr;
     uint64_t cand_alloc = 0;
     uint32_t* d_overflow = nullptr;
+    uint32_t* d_flag_any = nullptr;  // d_overflow | d_hov_any, written by the merge
     uint64_t* d_out_off = nullptr;
     int32_t* d_out_ids = nullptr;
     int32_t* d_out_counts = nullptr;
@@ -274,6 +279,7 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_cand_base);
     dev_free(p->d_cand);
     dev_free(p->d_overflow);
+    dev_free(p->d_flag_any);
     dev_free(p->d_out_off);
     dev_free(p->d_out_ids);
     dev_free(p->d_out_counts);
@@ -1287,6 +1293,7 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         if ((rc = dev_alloc(&p->d_cand_cap, max_queries))) break;
         if ((rc = dev_alloc(&p->d_cand_base, max_queries))) break;
         if ((rc = dev_alloc(&p->d_overflow, 1))) break;
+        if ((rc = dev_alloc(&p->d_flag_any, 1))) break;
         if ((rc = dev_alloc(&p->d_out_off, max_queries + 1))) break;
         if ((rc = dev_alloc(&p->d_queue, 1))) break;
         if ((rc = dev_alloc(&p->d_hq, max_queries))) break;
@@ -1436,8 +1443,12 @@ static uint32_t max_cap_factor(const genie_plan* p)
     return (p->ix->d.n_objects + kmin - 1) / kmin;
 }
 
-static uint32_t initial_cap_factor(const genie_plan* p) { return dense_active(p) ? 64u : 8u; }
-constexpr uint64_t kDenseCapFloor = 16384;  // candidates per query the dense form's total holds at least
+static uint32_t initial_cap_factor(const genie_plan* p) { return dense_active(p) ? kDenseCapFactor : kTileCapFactor; }
+
+// Smallest dense_mult at which every device-side ask is the whole index: an
+// ask is at least 1,024 x dense_mult before it is capped at n_objects
+// (dense_want).
+static uint32_t dense_mult_max(const genie_plan* p) { return (p->ix->d.n_objects + 1023) / 1024; }
 
 // Candidate capacity per query: cap_factor x k, at most what the form can emit.
 static int plan_size_candidates(genie_plan* p)
@@ -1932,6 +1943,7 @@ static int plan_settle_form(genie_plan* p)
             return rc;
     }
     p->cap_factor = initial_cap_factor(p);
+    p->dense_mult = 1;
     return plan_size_candidates(p);
 }
 
@@ -2058,7 +2070,9 @@ static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items
 }
 
 // The batch's device copies are staged in pinned memory and go out with the
-// next run, on its stream.
+// next run, on its stream.  A failed upload leaves the plan without a batch:
+// it may have replaced the host copy of the previous batch or some of its
+// device buffers, so run and download refuse until the next upload succeeds.
 int gh::plan_upload_staged(genie_plan* p, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
                            const uint32_t* k_per_query, bool validate)
 {
@@ -2066,8 +2080,15 @@ int gh::plan_upload_staged(genie_plan* p, const genie_dim_t* items, uint64_t n_i
         return rc;
     const int rc = plan_upload(p, items, n_items, n_queries, k_per_query, validate);
     p->staging = false;
-    if (rc)
+    if (rc) {
         p->staged.clear();  // a failed upload sends nothing
+        p->n_queries = 0;
+        p->n_items = 0;
+        p->bytes_valid = false;
+        p->ran = false;
+        p->lanes_built = false;
+        plan_drop_graph(p);
+    }
     return rc;
 }
 
@@ -2308,7 +2329,7 @@ static int run_dense(genie_plan* p, hipStream_t s)
     if (int rc = launch_dense<true>(nblk, a, s, p->dense_nt))
         return rc;
     // thresholds, then every query's candidate space from its sample
-    const uint32_t mult = std::max<uint32_t>(1, p->cap_factor / initial_cap_factor(p));
+    const uint32_t mult = p->dense_mult;
     const uint32_t ratio16 = (uint32_t)std::min<uint64_t>(
         ((uint64_t)d.n_objects << 16) / std::max<uint32_t>(1, p->n_dsample) + 1, 0xFFFFFFFFull);
     k_dense_thresh<<<(nq + 3) / 4, 256, 0, s>>>(p->d_dhist, p->d_kq, nq, p->n_dsample, d.n_objects, ratio16, mult,
@@ -2362,8 +2383,17 @@ static int settle_overflow(genie_plan* p, hipStream_t s)
         }
         if (ov) {
             const uint32_t fmax = max_cap_factor(p);
-            if (p->cap_factor >= fmax)
+            if (dense_active(p)) {
+                // two limits: the host total (cap_factor) and each query's
+                // ask on the device (dense_mult); each grows until it gives
+                // every query room for every object
+                const uint32_t mmax = dense_mult_max(p);
+                if (p->cap_factor >= fmax && p->dense_mult >= mmax)
+                    return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
+                p->dense_mult = (uint32_t)std::min<uint64_t>((uint64_t)p->dense_mult * 4, mmax);
+            } else if (p->cap_factor >= fmax) {
                 return fail(GENIE_E_RUNTIME, "candidate overflow at full capacity");
+            }
             p->cap_factor = (uint32_t)std::min<uint64_t>((uint64_t)p->cap_factor * 4, fmax);
         }
         if (int rc = plan_size_candidates(p))
@@ -2467,6 +2497,9 @@ static int launch_merge(genie_plan* p, hipStream_t s)
     m.out_ids = p->d_out_ids;
     m.out_counts = p->d_out_counts;
     m.key_bits = 32 + p->lane_bits;
+    m.flag_a = p->d_overflow;
+    m.flag_b = p->d_hov_any;
+    m.flag_any = p->d_flag_any;
     k_merge<<<nq, kMergeThreads, 0, s>>>(m);
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[3], s));
@@ -2790,7 +2823,9 @@ int genie_hip_plan_download_async(genie_plan* p, void* stream, int32_t* out_ids,
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipMemcpyAsync(out_ids, p->d_out_ids, p->total_k * 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(out_counts, p->d_out_counts, p->total_k * 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(overflow, p->d_overflow, 4, hipMemcpyDeviceToHost, s));
+    // nonzero when the candidate space overflowed or a hash-form table filled
+    // up: either way genie_hip_plan_download settles the batch
+    HIP_TRY(hipMemcpyAsync(overflow, p->d_flag_any, 4, hipMemcpyDeviceToHost, s));
     return GENIE_OK;
 }
 

@@ -1302,6 +1302,12 @@ struct MergeArgs {
     int32_t* out_ids;
     int32_t* out_counts;
     uint32_t key_bits;  // significant bits of a key (32 + bits of the count bound)
+    // the run's two rerun flags (candidate overflow, a hash table filled up)
+    // folded into one word for an asynchronous download; every kernel that
+    // sets them has finished when the merge starts
+    const uint32_t* flag_a;
+    const uint32_t* flag_b;
+    uint32_t* flag_any;
 };
 
 // Per query: select the k largest candidate keys (radix select over the key
@@ -1318,6 +1324,8 @@ __global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
     __shared__ uint32_t s_res[4];
     const uint32_t q = blockIdx.x;
     const int tid = threadIdx.x;
+    if (q == 0 && tid == 0)
+        *a.flag_any = *a.flag_a | *a.flag_b;
     const uint32_t k = a.kq[q];
     const uint32_t M = min(a.cand_cnt[q], a.cand_cap[q]);
     const uint64_t* keys = a.cand + a.cand_base[q];

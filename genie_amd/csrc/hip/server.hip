@@ -91,13 +91,16 @@ void note_phase(Executor& e, int i, Clock::time_point t)
 }
 
 // Device scratch one request holds while in flight: its work items, its
-// candidate space as a plan first sizes it (64 k keys, at most one per
-// object, when the dense form can run: an index with forward codes and a
-// batched or automatic plan; else 8 k keys; grown on overflow) and its output
-// rows.
+// candidate space as a plan first sizes it (gh::initial_candidates: the dense
+// and set forms when the index has forward codes or lists and the plan is
+// batched or automatic, else the tile forms) and its output rows.  The
+// budget covers these per-request items only: each executor's plan also
+// holds buffers sized once for its largest batch (candidate space and
+// output rows for max_batch requests of max_k, grown after an overflow),
+// which stay allocated while the server lives.
 uint64_t request_bytes(uint64_t n_items, uint32_t k, bool dense, uint32_t n_objects)
 {
-    const uint64_t cand = dense ? std::min<uint64_t>(64ull * k, n_objects) : 8ull * k;
+    const uint64_t cand = gh::initial_candidates(k, dense, n_objects);
     return n_items * sizeof(genie_dim_t) + cand * 8 + (uint64_t)k * 8 + 64;
 }
 
@@ -396,7 +399,7 @@ static int submit_request(genie_server* s, const genie_dim_t* items, uint32_t n_
     r->k = k;
     r->dst_ids = dst_ids;
     r->dst_counts = dst_counts;
-    r->mem = request_bytes(n_items, k, s->ix->d.dense && s->cfg.mode != 0, s->ix->d.n_objects);
+    r->mem = request_bytes(n_items, k, (s->ix->d.dense || s->ix->d.sets) && s->cfg.mode != 0, s->ix->d.n_objects);
     // validate alone so that one bad request cannot fail a whole batch
     if (int rc = gh::validate_items(s->ix->d, r->items.data(), n_items, 1))
         return rc;

@@ -173,12 +173,13 @@ int genie_table_export(genie_table* tt, int32_t* inv, int32_t* inv_pos, uint32_t
 }
 
 int genie_table_compile(genie_table* tt, const genie_range_t* ranges, uint64_t n_ranges, uint32_t n_queries,
-                        int use_load_balance, int32_t max_load, genie_dim_t* out, uint64_t cap, uint64_t* n_out)
+                        int use_load_balance, int32_t max_load, int merge_runs, genie_dim_t* out, uint64_t cap,
+                        uint64_t* n_out)
 {
     return guard([&] {
         std::vector<genie::query::Query> qs = make_queries(*tt->t, ranges, n_ranges, n_queries, use_load_balance, 1);
         std::vector<genie::query::Query::dim> dims;
-        genie::matching::build_queries(qs, *tt->t, dims, max_load);
+        genie::matching::build_queries(qs, *tt->t, dims, max_load, merge_runs != 0);
         *n_out = dims.size();
         if (dims.size() > cap)
             throw genie::exception::cpu_runtime_error("output capacity too small");

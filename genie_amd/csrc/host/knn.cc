@@ -16,7 +16,7 @@ namespace matching {
 using genie::query::Query;
 
 int build_queries(std::vector<Query>& queries, genie::table::inv_table& table, std::vector<Query::dim>& dims,
-                  int max_load)
+                  int max_load, bool merge_runs)
 {
     if (table.build_status() != genie::table::inv_table::builded)
         throw genie::exception::cpu_runtime_error("table not built!");
@@ -28,7 +28,7 @@ int build_queries(std::vector<Query>& queries, genie::table::inv_table& table, s
                 throw genie::exception::cpu_runtime_error(
                     "Can't build queries. Queries constructed for different table!");
             if (compressed)
-                q.build_compressed(max_load);
+                q.build_compressed(max_load, merge_runs);
             else if (q.use_load_balance)
                 q.build_and_apply_load_balance(max_load);
             else

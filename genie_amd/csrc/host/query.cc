@@ -166,7 +166,7 @@ void Query::build_and_apply_load_balance(int max_load)
     is_load_balanced = true;
 }
 
-void Query::build_compressed(int max_load)
+void Query::build_compressed(int max_load, bool merge_runs)
 {
     (void)max_load;
     auto* table = dynamic_cast<genie::table::inv_compr_table*>(_ref_table);
@@ -182,6 +182,11 @@ void Query::build_compressed(int max_load)
             int sp, ep, s0, s1;
             if (!span_of(*table, index, r.low, r.up, &sp, &ep, &s0, &s1))
                 continue;
+            if (merge_runs) {  // the range's segments follow one another: one item
+                if (s1 > s0)
+                    out.emplace_back(_index, r.order, cpos[(size_t)s0], cpos[(size_t)s1], r.weight);
+                continue;
+            }
             // one work item per compressed segment of the range
             for (int s = s0; s < s1; ++s)
                 out.emplace_back(_index, r.order, cpos[(size_t)s], cpos[(size_t)s + 1], r.weight);

@@ -26,9 +26,11 @@ void knn_MT(std::vector<genie::table::inv_table*>& table, std::vector<std::vecto
             std::vector<int>& hash_table_size, std::vector<int>& max_load, int bitmap_bits);
 
 // Compile the queries into work items (match.cu:161-210 build_queries);
-// returns the largest number of items of one query.
+// returns the largest number of items of one query.  merge_runs (compressed
+// tables): one item per range over its consecutive segments
+// (Query::build_compressed).
 int build_queries(std::vector<genie::query::Query>& queries, genie::table::inv_table& table,
-                  std::vector<genie::query::Query::dim>& dims, int max_load);
+                  std::vector<genie::query::Query::dim>& dims, int max_load, bool merge_runs = false);
 
 }  // namespace matching
 }  // namespace genie

@@ -60,7 +60,10 @@ public:
     void build();
     void build(std::vector<dim>& out);
     void build_and_apply_load_balance(int max_load);
-    void build_compressed(int max_load);
+    // One work item per compressed segment of each range (query.cc:464-533);
+    // with merge_runs, one item per range spanning all of its segments, which
+    // the engine takes as the same work with far fewer items.
+    void build_compressed(int max_load, bool merge_runs = false);
 
     void topk(int k) { _topk = k; }
     int topk() const { return _topk; }

@@ -201,9 +201,11 @@ int genie_hip_plan_host_ms(genie_plan* p, double* ms);
 int genie_hip_plan_bytes(genie_plan* p, uint64_t* logical, uint64_t* compulsory);
 
 /* Asynchronous result copy into caller-owned pinned host memory
- * (genie_hip_host_alloc): ids, counts and the candidate-overflow flag.  When
- * *overflow reads nonzero after the stream completes, call
- * genie_hip_plan_download, which grows the candidate space and reruns. */
+ * (genie_hip_host_alloc): ids, counts and the rerun flag.  The flag reads
+ * nonzero after the stream completes when the candidate space overflowed or a
+ * hash-form query's table filled up; the rows are then incomplete: call
+ * genie_hip_plan_download, which grows the candidate space or moves those
+ * queries to the direct-mapped kernel, reruns and copies the final rows. */
 int genie_hip_plan_download_async(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts,
                                   uint32_t* overflow);
 int genie_hip_host_alloc(void** ptr, uint64_t bytes);

@@ -50,10 +50,14 @@ int genie_table_export(genie_table* t, int32_t* inv, int32_t* inv_pos, uint32_t*
 
 /* Compile ranges of n_queries queries (range.query in [0, n_queries)) with
  * Query::attr then build / build_and_apply_load_balance / build_compressed
- * and dump, query by query (match.cu:161-210).  *n_out receives the item
- * count; when it exceeds cap nothing is copied and GENIE_E_INVALID returns. */
+ * and dump, query by query (match.cu:161-210).  merge_runs (compressed
+ * tables): one item per range spanning its consecutive segments instead of
+ * one per segment; every form of the search takes it as the same work.
+ * *n_out receives the item count; when it exceeds cap nothing is copied and
+ * GENIE_E_INVALID returns. */
 int genie_table_compile(genie_table* t, const genie_range_t* ranges, uint64_t n_ranges, uint32_t n_queries,
-                        int use_load_balance, int32_t max_load, genie_dim_t* out, uint64_t cap, uint64_t* n_out);
+                        int use_load_balance, int32_t max_load, int merge_runs, genie_dim_t* out, uint64_t cap,
+                        uint64_t* n_out);
 
 /* Engine handle of the table (cpy_data_to_gpu); owned by the table. */
 int genie_table_device(genie_table* t, genie_index** out);

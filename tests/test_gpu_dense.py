@@ -100,6 +100,42 @@ def test_dense_candidate_regrowth():
     assert_topk_equal(b[0], b[1], ri, rc, 10, nq, "dense regrowth")
 
 
+def test_dense_asks_grow_when_the_sample_misses():
+    """Objects of the dense sample tiles never match, so every query's sample
+    histogram is empty and its device-side candidate ask (4k + 1,024) is far
+    below the ~167 K objects that count 1.  With k >= 1,000 the index holds
+    at most 200 k, so the host total soon gives each query the whole index:
+    the asks themselves have to grow, run after run."""
+    n, tile = 200_000, 4096
+    n_tiles = (n + tile - 1) // tile
+    ns = min(n_tiles, max(8, (n_tiles + 31) // 32))
+    sample = {i * n_tiles // ns for i in range(ns)}
+    obj_tile = np.arange(n) // tile
+    rows = np.zeros((n, 2), np.int32)
+    rows[:, 0] = [0 if t in sample else 1 for t in obj_tile]
+    rows[:, 1] = np.arange(n) % 7
+    nq = 6
+    r = np.zeros(nq, RANGE_DTYPE)
+    r["query"] = np.arange(nq)
+    r["dim"] = 0
+    r["low"] = 1
+    r["up"] = 1
+    r["weight"] = 1
+    r["order"] = 0
+    kq = np.array([2000, 2000, 1000, 2000, 1500, 2000], np.uint32)
+    t = Table.from_rows(rows)
+    items = t.compile(r, nq)
+    p = Plan(t.device(), nq, len(items), 2000, 0, mode=1)
+    p.upload(items, nq, kq)
+    assert p.dense_active()
+    p.run()
+    got = p.download()
+    ri, rc = oracle_single(rows, r, nq, kq)
+    assert_topk_equal(got[0], got[1], ri, rc, kq, nq, "dense asks regrowth")
+    # the smallest ids outside the sample tiles, all with count 1
+    assert got[0][0] == tile and np.all(got[1][:2000] == 1)
+
+
 def test_dense_graph_replays_match_eager():
     w = W.cfg1(n=100_000, d=16, q=40, k=32)
     t = Table.from_rows(w["rows"])

@@ -1,11 +1,17 @@
 """Serving form: continuous batching over a resident index.  Every request's
-result is the same alone and in the mix (bit-exact), the memory budget
-admission holds, and bad requests are refused without failing others."""
+result equals the exhaustive oracle's and is the same alone and in the mix
+(bit-exact), the memory budget admission holds, a hash-form table that fills
+up is settled before its results are handed back, and bad requests are
+refused without failing others."""
 import numpy as np
 import pytest
 
+from helpers import assert_topk_equal, oracle_topk_batch
+from oracle import oracle as O
+
 import genie_amd
-from genie_amd import GenieError, Server, Table, match_topk, workloads as W
+from genie_amd import GenieError, Server, Table, workloads as W
+from genie_amd._native import RANGE_DTYPE
 
 pytestmark = pytest.mark.gpu
 
@@ -17,6 +23,12 @@ def requests_of(table, w, merge_runs=False):
     return items, ptr
 
 
+def oracle_of(rows, ranges, nq, k):
+    """Exhaustive closed-form counts from the raw rows, top k per request."""
+    q_of = ranges["query"]
+    return oracle_topk_batch(lambda q: O.count_exhaustive_single(rows, ranges[q_of == q]), nq, k)
+
+
 @pytest.mark.parametrize("mode,codec,merged", [(0, 0, False), (1, 0, False), (2, 0, False), (0, 11, False),
                                                (2, 11, True), (1, 11, True)])
 def test_alone_equals_in_the_mix(mode, codec, merged):
@@ -26,6 +38,7 @@ def test_alone_equals_in_the_mix(mode, codec, merged):
     t = Table.from_rows(w["rows"], 1024, codec != 0, codec=codec)
     items, ptr = requests_of(t, w, merged)
     k = w["k"]
+    ri, rc = oracle_of(w["rows"], w["ranges"], w["n_queries"], k)
     srv = Server(t.device(), executors=3, max_batch=16, max_k=1000, mode=mode)
     alone = []
     for r in range(w["n_queries"]):
@@ -33,6 +46,8 @@ def test_alone_equals_in_the_mix(mode, codec, merged):
         ids, cnt, lat = srv.wait(tk, int(k[r]))
         alone.append((ids, cnt))
         assert lat >= 0
+    assert_topk_equal(np.concatenate([a[0] for a in alone]), np.concatenate([a[1] for a in alone]), ri, rc, k,
+                      w["n_queries"], f"served alone, mode {mode} codec {codec}")
     # all at once (t = 0) and as a Poisson trace: batches form and overlap
     for arrivals in (np.zeros(w["n_queries"]), np.cumsum(genie_amd.synth.exponential(9, w["n_queries"], 4000.0))):
         lat, ids, cnt, wall = srv.replay(items, ptr, k, arrivals)
@@ -51,9 +66,8 @@ def test_alone_equals_in_the_mix(mode, codec, merged):
     assert 0 < sl["wait"] <= tot["wait"] and 0 < sl["prepare_upload"] <= tot["prepare_upload"]
     assert sl["queue"] > 0
     assert all(v == 0 for v in srv.slowest(reset=False).values())
-    # and the same as one plain batch of everything
-    pi, pc = match_topk(t.device(), items, w["n_queries"], k)
-    assert np.array_equal(pi, ids) and np.array_equal(pc, cnt)
+    # and the oracle's answer for the whole mix
+    assert_topk_equal(ids, cnt, ri, rc, k, w["n_queries"], f"served in the mix, mode {mode} codec {codec}")
     srv.close()
 
 
@@ -61,8 +75,7 @@ def test_alone_equals_in_the_mix(mode, codec, merged):
 def test_selective_requests_on_the_hash_form(codec):
     """LSH-style exact-match requests (8,192 buckets per dim): the automatic
     plan runs them on the hash form (compressed: segments decoded once per
-    batch); in the mix each equals its lone answer and the one-shot search
-    of the whole batch (the hash form's oracle parity is in test_gpu_hash)."""
+    batch); in the mix each equals the oracle's answer and its lone one."""
     rows = genie_amd.synth.rows(301, 300_000, 32, genie_amd.synth.UNIFORM, 0.0, 0.0, 0, 8191, 8192)
     qv, _ = genie_amd.synth.queries(302, rows, 40, redraw_p=0.3, n_values=8192)
     ranges = W.single_ranges(qv, 0)
@@ -72,8 +85,8 @@ def test_selective_requests_on_the_hash_form(codec):
     k = np.array([10, 100, 1000, 50] * 10, np.uint32)
     srv = Server(t.device(), executors=3, max_batch=16, max_k=1000, mode=2)
     lat, ids, cnt, wall = srv.replay(items, ptr, k, np.cumsum(genie_amd.synth.exponential(11, 40, 2000.0)))
-    pi, pc = match_topk(t.device(), items, 40, k)
-    assert np.array_equal(pi, ids) and np.array_equal(pc, cnt)
+    ri, rc = oracle_of(rows, ranges, 40, k)
+    assert_topk_equal(ids, cnt, ri, rc, k, 40, f"selective served, codec {codec}")
     off = 0
     for r in (0, 7, 22):
         tk = srv.submit(items[ptr[r]:ptr[r + 1]], int(k[r]))
@@ -89,8 +102,8 @@ def test_memory_budget_admits_one_at_a_time():
     items, ptr = requests_of(t, w)
     srv = Server(t.device(), executors=4, max_batch=8, max_k=100, memory_budget=1)
     lat, ids, cnt, wall = srv.replay(items, ptr, 100, np.zeros(12))
-    pi, pc = match_topk(t.device(), items, 12, 100)
-    assert np.array_equal(pi, ids) and np.array_equal(pc, cnt)
+    ri, rc = oracle_of(w["rows"], w["ranges"], 12, 100)
+    assert_topk_equal(ids, cnt, ri, rc, 100, 12, "one request at a time")
     assert srv.stats()["max_inflight"] == 1
     srv.close()
 
@@ -108,6 +121,44 @@ def test_bad_request_is_refused():
         srv.submit(items[:3], 51)
     tk = srv.submit(items[items["query"] == 0], 10)
     ids, cnt, _ = srv.wait(tk, 10)
-    ref = match_topk(t.device(), items[items["query"] == 0], 1, 10)
-    assert np.array_equal(ids, ref[0]) and np.array_equal(cnt, ref[1])
+    r0 = w["ranges"][w["ranges"]["query"] == 0]
+    ri, rc = O.topk(O.count_exhaustive_single(w["rows"], r0), 10)
+    assert np.array_equal(ids, ri) and np.array_equal(cnt, rc)
+    srv.close()
+
+
+@pytest.mark.parametrize("executors", [1, 3])
+def test_hash_table_full_is_settled_before_hand_back(executors):
+    """Request 0 names one key of 20,000 objects, all in its first tile: its
+    hash-form table fills up, and the run leaves its rows incomplete until
+    the batch is settled on the direct-mapped kernel.  Served alone and in a
+    mix of selective requests, every result equals the oracle's."""
+    n = 400_000
+    rows = genie_amd.synth.rows(221, n, 4, genie_amd.synth.UNIFORM, 0.0, 0.0, 2, 4097, 4096)
+    rows[:20_000, 0] = 1
+    nq = 24
+    r = np.zeros(nq * 4, RANGE_DTYPE)
+    r["query"] = np.repeat(np.arange(nq), 4)
+    r["dim"] = np.tile(np.arange(4), nq)
+    r["low"] = r["up"] = rows[50_000:50_000 + nq].reshape(-1)
+    r["weight"] = 1
+    r["order"] = r["dim"]
+    full = [0, 9, 17]  # requests that name only the crowded key
+    keep = ~np.isin(r["query"], full) | (r["dim"] == 0)
+    r = r[keep].copy()
+    r["low"][np.isin(r["query"], full)] = 1
+    r["up"][np.isin(r["query"], full)] = 1
+    t = Table.from_rows(rows)
+    items = t.compile(r, nq)
+    ptr = np.searchsorted(items["query"], np.arange(nq + 1)).astype(np.uint64)
+    k = np.full(nq, 100, np.uint32)
+    ri, rc = oracle_of(rows, r, nq, k)
+    srv = Server(t.device(), executors=executors, max_batch=8, max_k=100, mode=2)
+    tk = srv.submit(items[ptr[0]:ptr[1]], 100)
+    ids, cnt, _ = srv.wait(tk, 100)
+    assert list(ids) == list(range(100)) and np.all(cnt == 1)
+    for arrivals in (np.zeros(nq), np.cumsum(genie_amd.synth.exponential(13, nq, 3000.0))):
+        lat, ids, cnt, wall = srv.replay(items, ptr, k, arrivals)
+        assert_topk_equal(ids, cnt, ri, rc, k, nq, f"table full served, {executors} executors")
+    assert srv.stats()["reruns"] >= 1
     srv.close()

@@ -94,3 +94,31 @@ def test_user_stream_growing_batches_and_graph():
         match_topk(t.device(), items[:200], nq, 5)
     p.close()
     assert hip.hipStreamDestroy(C.c_void_p(stream)) == 0
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_failed_upload_leaves_no_batch(mode):
+    """An upload that fails after a good batch has run leaves the plan without
+    a batch: run and download refuse until the next good upload."""
+    from genie_amd import GenieError
+    rows, r = cfg(40_000, 16, 12, 931)
+    t = Table.from_rows(rows)
+    items = t.compile(r, 12)
+    p = Plan(t.device(), 12, len(items), 50, 0, mode=mode)
+    p.upload(items, 12, np.full(12, 50, np.uint32))
+    p.run()
+    first = p.download()
+    bad = np.full(12, 50, np.uint32)
+    bad[3] = 51  # above the plan's max_k
+    with pytest.raises(GenieError):
+        p.upload(items, 12, bad)
+    with pytest.raises(GenieError):
+        p.run()
+    with pytest.raises(GenieError):
+        p.download()
+    p.upload(items, 12, np.full(12, 50, np.uint32))
+    p.run()
+    again = p.download()
+    assert np.array_equal(first[0], again[0]) and np.array_equal(first[1], again[1])
+    ri, rc = expect(rows, r, 12, 50)
+    assert_topk_equal(again[0], again[1], ri, rc, 50, 12, "after a failed upload")
