@@ -8,7 +8,7 @@ has no compute path of its own.  Importing it raises when the libraries are
 missing: run ``python genie_amd/build.py`` (or ``__graft_entry__.build()``).
 """
 from ._native import (DIM_DTYPE, RANGE_DTYPE, GenieError, Index, Plan, Server, Table, device_synchronize,
-                      lib_paths, match_topk, synth, wire)
+                      knn_mt_admit, lib_paths, match_topk, mem_info, synth, tables_knn_mt, wire)
 
-__all__ = ["DIM_DTYPE", "RANGE_DTYPE", "GenieError", "Index", "Plan", "Server", "Table", "device_synchronize", "lib_paths",
-           "match_topk", "synth", "wire"]
+__all__ = ["DIM_DTYPE", "RANGE_DTYPE", "GenieError", "Index", "Plan", "Server", "Table", "device_synchronize",
+           "knn_mt_admit", "lib_paths", "match_topk", "mem_info", "synth", "tables_knn_mt", "wire"]

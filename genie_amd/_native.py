@@ -82,6 +82,8 @@ _sig(_hip, "genie_hip_plan_set_sparse", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_sparse_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_set_hash", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_hash_queries", C.c_int, _p, C.POINTER(_u32))
+_sig(_hip, "genie_hip_plan_set_decode", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_decode_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_plan_download_async", C.c_int, _p, _p, _p, _p, _p)
 _sig(_hip, "genie_hip_host_alloc", C.c_int, C.POINTER(_p), _u64)
@@ -106,6 +108,8 @@ _sig(_hip, "genie_hip_last_error", C.c_char_p)
 _sig(_hip, "genie_hip_stream_synchronize", C.c_int, _p)
 _sig(_hip, "genie_hip_device_synchronize", C.c_int)
 _sig(_hip, "genie_hip_copy_probe", C.c_int, _u64)
+_sig(_hip, "genie_hip_plan_estimate", C.c_int, _p, _u32, _u64, _u32, C.c_int, C.POINTER(_u64))
+_sig(_hip, "genie_hip_mem_info", C.c_int, C.POINTER(_u64), C.POINTER(_u64))
 
 
 def decode(codec, words, cpos, stride=1024):
@@ -145,6 +149,8 @@ _sig(_host, "genie_wire_search", C.c_int, C.POINTER(_p), _u32, C.c_char_p, C.c_c
 _sig(_host, "genie_table_load", C.c_int, C.c_char_p, C.POINTER(_p))
 _sig(_host, "genie_table_knn", C.c_int, _p, _p, _u64, _u32, _u32, C.c_int, _i32, _p, _p)
 _sig(_host, "genie_table_last_error", C.c_char_p)
+_sig(_host, "genie_knn_mt_admit", C.c_int, _p, _u64, _u64, _u64, C.POINTER(_u64))
+_sig(_host, "genie_tables_knn_mt", C.c_int, _p, _u32, _p, _p, _p, _u32, _u64, _p, _p)
 
 _sig(_synth, "synth_rows", C.c_int, _u64, _u32, _u32, C.c_int, C.c_double, C.c_double, _i32, _i32, _u32, _p)
 _sig(_synth, "synth_queries", C.c_int, _u64, _p, _u32, _u32, _u32, _i32, _i32, _i32, C.c_double, _u32, _p, _p)
@@ -201,6 +207,13 @@ class Index:
     def bytes(self):
         return int(_hip.genie_hip_index_bytes(self.handle))
 
+    def plan_estimate(self, n_queries, n_items, max_k, mode=2):
+        """Device bytes a plan of this shape holds for one batch
+        (genie_hip_plan_estimate, knn_MT's admission estimate)."""
+        v = _u64()
+        _chk_hip(_hip.genie_hip_plan_estimate(self.handle, n_queries, n_items, max_k, mode, C.byref(v)))
+        return int(v.value)
+
     def count_bounds(self, items, n_queries):
         items = _c(items, DIM_DTYPE)
         out = np.zeros(n_queries, np.uint32)
@@ -217,6 +230,47 @@ class Index:
             self.close()
         except Exception:
             pass
+
+
+def mem_info():
+    """(free, total) bytes of device memory."""
+    f, t = _u64(), _u64()
+    _chk_hip(_hip.genie_hip_mem_info(C.byref(f), C.byref(t)))
+    return int(f.value), int(t.value)
+
+
+def knn_mt_admit(estimate, start, free_bytes):
+    """knn_MT's admission round (host only): the end of the round of tables
+    admitted from `start` against free_bytes; GenieError (GENIE_E_OOM,
+    "MEMORY NOT ENOUGH") when table `start` does not fit alone."""
+    est = _c(estimate, np.uint64)
+    fin = _u64()
+    _chk_host(_host.genie_knn_mt_admit(_ptr(est), len(est), start, free_bytes, C.byref(fin)))
+    return int(fin.value)
+
+
+def tables_knn_mt(tables, ranges, n_queries, k, budget=0):
+    """knn_search_MT over several tables (GENIE's multi-table search):
+    ranges[i] / n_queries[i] are table i's queries; returns per-table
+    (ids, counts), n_queries[i] x k slots each.  budget caps the device
+    memory knn_MT admits against (0: the device's free memory)."""
+    n = len(tables)
+    handles = (_p * max(1, n))(*[t.handle for t in tables])
+    rr = [_c(r, RANGE_DTYPE) for r in ranges]
+    allr = np.concatenate(rr) if rr else np.zeros(0, RANGE_DTYPE)
+    rptr = np.zeros(n + 1, np.uint64)
+    rptr[1:] = np.cumsum([len(r) for r in rr])
+    nq = _c(n_queries, np.uint32)
+    tot = int(nq.sum()) * k
+    ids = np.zeros(max(1, tot), np.int32)
+    cnt = np.zeros(max(1, tot), np.int32)
+    _chk_host(_host.genie_tables_knn_mt(handles, n, _ptr(allr), _ptr(rptr), _ptr(nq), k, budget, _ptr(ids),
+                                        _ptr(cnt)))
+    out, off = [], 0
+    for q in nq:
+        out.append((ids[off:off + int(q) * k].copy(), cnt[off:off + int(q) * k].copy()))
+        off += int(q) * k
+    return out
 
 
 def match_topk(index, items, n_queries, k, count_bound=0):
@@ -318,6 +372,18 @@ class Plan:
         v = _u32()
         _chk_hip(_hip.genie_hip_plan_hash_queries(self.handle, C.byref(v)))
         return int(v.value)
+
+    def set_decode(self, mode=-1):
+        """Plain form on a compressed index: -1 decodes the batch's segments
+        once each into a full-layout scratch and counts it like an
+        uncompressed index, 0 leaves the decode to the kernels (results are
+        identical)."""
+        _chk_hip(_hip.genie_hip_plan_set_decode(self.handle, int(mode)))
+
+    def decode_active(self):
+        v = C.c_int()
+        _chk_hip(_hip.genie_hip_plan_decode_active(self.handle, C.byref(v)))
+        return bool(v.value)
 
     def autotune(self, reps=2):
         ms = np.full(16, -1.0)

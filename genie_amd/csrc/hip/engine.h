@@ -53,6 +53,9 @@ struct DevIndex {
     uint32_t n_dtiles = 0;
     // device
     uint32_t* post = nullptr;       // [n_postings]   (uncompressed)
+    // key starts and tile directory in posting positions; a compressed index
+    // has them too (built from its one decode at creation), for batches whose
+    // segments are decoded once into a scratch of the uncompressed layout
     uint32_t* key_start = nullptr;  // [n_keys + 1]   positions (u32)
     uint32_t* dir_off = nullptr;    // [n_keys]
     uint32_t* dir = nullptr;        // [n_dir_keys * (n_dtiles + 1)]
@@ -63,7 +66,8 @@ struct DevIndex {
     uint32_t* seg_first = nullptr;  // [n_segments] first id of each segment
     uint32_t* seg_last = nullptr;   // [n_segments] last id
     uint32_t* key_seg = nullptr;    // [n_keys + 1] first segment of each key
-    uint32_t* sdir = nullptr;       // [n_dir_keys * (n_dtiles + 1)] segment directory (compressed)
+    uint32_t* sdir_off = nullptr;   // [n_keys] row of each key with more than 4 segments (compressed)
+    uint32_t* sdir = nullptr;       // [n_sdir_keys * (n_dtiles + 1)] segment directory (compressed)
     // forward codes of a single-value table (dense form of the batched search)
     bool dense = false;
     uint32_t dense_dpad = 0;        // dims rounded up to 16

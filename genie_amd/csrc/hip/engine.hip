@@ -97,6 +97,12 @@ struct genie_plan {
     // batch shape of the last upload
     uint32_t n_queries = 0;
     uint64_t n_items = 0, n_chunks = 0, total_k = 0, total_cap = 0;
+    // a compressed index's batch compiles into runs of contiguous segments,
+    // or (decoded once into the full-layout scratch) like an uncompressed
+    // batch: chunks of at most kChunk postings per item.  n_chunks is the
+    // count of the compile path the next run takes; the chunk buffers hold
+    // the larger of the two
+    uint64_t n_chunks_runs = 0, n_chunks_full = 0;
     uint32_t cap_factor = 8;
     // dense and set forms: multiplier of the device-side candidate asks,
     // raised after a run whose asks were too small (a sample that
@@ -134,6 +140,18 @@ struct genie_plan {
     uint2* d_djobs = nullptr;
     uint32_t* d_scratch = nullptr;
     uint64_t sspan_alloc = 0, djobs_alloc = 0, scratch_alloc = 0;
+    // compressed index, plain form: the batch's distinct segments decoded
+    // once into a scratch of the uncompressed layout (each segment at its
+    // posting positions), the items translated to posting positions, and the
+    // batch then counted like one on an uncompressed index
+    int decode_mode = -1;                 // -1 automatic (when the scratch can be allocated), 0 off
+    bool dec_full = false;
+    std::vector<uint2> h_fjobs;           // (segment, first posting position)
+    std::vector<genie_dim_t> h_uitems;    // items in posting positions
+    uint2* d_fjobs = nullptr;
+    uint64_t fjobs_alloc = 0;
+    genie_dim_t* d_uitems = nullptr;      // [max_items]
+    uint32_t* d_full = nullptr;           // [n_postings]
     // device buffers
     genie_dim_t* d_items = nullptr;
     uint32_t* d_item_qptr = nullptr;
@@ -305,6 +323,9 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_sq_ptr);
     dev_free(p->d_djobs);
     dev_free(p->d_scratch);
+    dev_free(p->d_fjobs);
+    dev_free(p->d_uitems);
+    dev_free(p->d_full);
 }
 
 // Count bound of every query: per (query, order) group the largest weight
@@ -523,30 +544,38 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
             sdir_off[k] = (uint32_t)(dir_keys.size() * (uint64_t)(d.n_dtiles + 1));
             dir_keys.push_back(k);
         }
-    if ((uint64_t)dir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull) {
+    // tile directory of the posting positions (keys longer than kShortList),
+    // as an uncompressed index has it
+    std::vector<uint32_t> tdir_off(K, kNoDir), tdir_keys;
+    for (uint32_t k = 0; k < K; ++k)
+        if (desc->key_start[k + 1] - desc->key_start[k] > kShortList) {
+            tdir_off[k] = (uint32_t)(tdir_keys.size() * (uint64_t)(d.n_dtiles + 1));
+            tdir_keys.push_back(k);
+        }
+    if ((uint64_t)dir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull ||
+        (uint64_t)tdir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull) {
         delete ix;
-        return fail(GENIE_E_INVALID, "segment directory too large");
+        return fail(GENIE_E_INVALID, "segment or tile directory too large");
     }
-    // a table that can have the dense form is also decoded once into a
-    // temporary posting array, from which its forward codes are built
+    // the whole table is decoded once into a temporary posting array: the
+    // tile directory and, when the table can have the dense form, its
+    // forward codes are built from it
     std::vector<uint32_t> kbase, kper;
     const bool want_dense = dense_shape(desc, kbase, kper);
-    std::vector<uint32_t> ks32;
-    if (want_dense) {
-        ks32.resize(K + 1);
-        for (uint32_t k = 0; k <= K; ++k)
-            ks32[k] = (uint32_t)desc->key_start[k];
-    }
+    std::vector<uint32_t> ks32(K + 1);
+    for (uint32_t k = 0; k <= K; ++k)
+        ks32[k] = (uint32_t)desc->key_start[k];
     int rc = GENIE_OK;
-    uint32_t *d_upos = nullptr, *d_dir_keys = nullptr, *d_err = nullptr, *d_post = nullptr, *d_ks = nullptr;
+    uint32_t *d_upos = nullptr, *d_dir_keys = nullptr, *d_err = nullptr, *d_post = nullptr, *d_tdir_keys = nullptr;
     do {
-        if (want_dense) {
-            if ((rc = dev_alloc(&d_post, d.n_postings))) break;
-            if ((rc = dev_alloc(&d_ks, K + 1))) break;
-            if (hipError_t e = hipMemcpy(d_ks, ks32.data(), (K + 1) * 4ull, hipMemcpyHostToDevice)) {
-                rc = fail(GENIE_E_RUNTIME, std::string("index upload: ") + hipGetErrorString(e));
-                break;
-            }
+        if ((rc = dev_alloc(&d_post, d.n_postings))) break;
+        if ((rc = dev_alloc(&d.key_start, K + 1, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.dir_off, K, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.dir, tdir_keys.size() * (uint64_t)(d.n_dtiles + 1), &d.bytes))) break;
+        if ((rc = dev_alloc(&d_tdir_keys, tdir_keys.size()))) break;
+        if (hipError_t e = hipMemcpy(d.key_start, ks32.data(), (K + 1) * 4ull, hipMemcpyHostToDevice)) {
+            rc = fail(GENIE_E_RUNTIME, std::string("index upload: ") + hipGetErrorString(e));
+            break;
         }
         if ((rc = dev_alloc(&d.words, d.n_words, &d.bytes))) break;
         if ((rc = dev_alloc(&d.seg_cpos, S + 1, &d.bytes))) break;
@@ -554,7 +583,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         if ((rc = dev_alloc(&d.seg_first, S, &d.bytes))) break;
         if ((rc = dev_alloc(&d.seg_last, S, &d.bytes))) break;
         if ((rc = dev_alloc(&d.key_seg, K + 1, &d.bytes))) break;
-        if ((rc = dev_alloc(&d.dir_off, K, &d.bytes))) break;  // segment directory offsets
+        if ((rc = dev_alloc(&d.sdir_off, K, &d.bytes))) break;
         if ((rc = dev_alloc(&d.sdir, dir_keys.size() * (uint64_t)(d.n_dtiles + 1), &d.bytes))) break;
         if ((rc = dev_alloc(&d_upos, S + 1))) break;
         if ((rc = dev_alloc(&d_dir_keys, dir_keys.size()))) break;
@@ -566,8 +595,10 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
             (e = hipMemcpy(d_upos, up, (S + 1) * 4, hipMemcpyHostToDevice)) ||
             (e = hipMemcpy(d.seg_key, seg_key.data(), S * 4, hipMemcpyHostToDevice)) ||
             (e = hipMemcpy(d.key_seg, key_seg.data(), (K + 1) * 4, hipMemcpyHostToDevice)) ||
-            (e = hipMemcpy(d.dir_off, sdir_off.data(), K * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.sdir_off, sdir_off.data(), K * 4, hipMemcpyHostToDevice)) ||
             (e = hipMemcpy(d_dir_keys, dir_keys.data(), dir_keys.size() * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d.dir_off, tdir_off.data(), K * 4, hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(d_tdir_keys, tdir_keys.data(), tdir_keys.size() * 4, hipMemcpyHostToDevice)) ||
             (e = hipMemcpy(d_err, &zero, 4, hipMemcpyHostToDevice))) {
             rc = fail(GENIE_E_RUNTIME, std::string("index upload: ") + hipGetErrorString(e));
             break;
@@ -606,16 +637,26 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
                 break;
             }
         }
+        if (!tdir_keys.empty()) {
+            const uint64_t total = tdir_keys.size() * (uint64_t)(d.n_dtiles + 1);
+            const uint32_t g = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+            k_build_dir<<<g, 256>>>(d_post, d.key_start, d_tdir_keys, (uint32_t)tdir_keys.size(), d.n_dtiles,
+                                     d.dir);
+            if ((e = hipGetLastError()) || (e = hipDeviceSynchronize())) {
+                rc = fail(GENIE_E_RUNTIME, std::string("tile directory: ") + hipGetErrorString(e));
+                break;
+            }
+        }
         if (want_dense)
-            rc = build_dense(desc, d, d_post, d_ks);
+            rc = build_dense(desc, d, d_post, d.key_start);
         if (!rc)
             rc = build_sets(desc, d);
     } while (0);
     dev_free(d_upos);
     dev_free(d_dir_keys);
+    dev_free(d_tdir_keys);
     dev_free(d_err);
     dev_free(d_post);
-    dev_free(d_ks);
     if (rc) {
         genie_hip_index_destroy(ix);
         return rc;
@@ -1069,6 +1110,34 @@ int genie_hip_stream_synchronize(void* stream)
     return GENIE_OK;
 }
 
+int genie_hip_stream_create(void** stream)
+{
+    if (!stream)
+        return fail(GENIE_E_INVALID, "null argument");
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return GENIE_OK;
+}
+
+int genie_hip_stream_destroy(void* stream)
+{
+    if (stream)
+        HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return GENIE_OK;
+}
+
+int genie_hip_mem_info(uint64_t* free_bytes, uint64_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes)
+        return fail(GENIE_E_INVALID, "null argument");
+    size_t f = 0, t = 0;
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return GENIE_OK;
+}
+
 int genie_hip_copy_probe(uint64_t bytes)
 {
     uint8_t *a = nullptr, *b = nullptr;
@@ -1235,6 +1304,7 @@ int genie_hip_index_destroy(genie_index* ix)
     dev_free(d.seg_first);
     dev_free(d.seg_last);
     dev_free(d.key_seg);
+    dev_free(d.sdir_off);
     dev_free(d.sdir);
     dev_free(d.codes);
     dev_free(d.key_slot);
@@ -1316,6 +1386,31 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         return rc;
     }
     *out = p;
+    return GENIE_OK;
+}
+
+int genie_hip_plan_estimate(const genie_index* ix, uint32_t n_queries, uint64_t n_items, uint32_t max_k, int mode,
+                            uint64_t* bytes)
+{
+    if (!ix || !bytes)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (mode != 0 && mode != 1 && mode != 2)
+        return fail(GENIE_E_INVALID, "unknown plan mode");
+    const DevIndex& d = ix->d;
+    const uint64_t nq = n_queries, ni = n_items;
+    uint64_t b = 1ull << 20;                      // small fixed buffers, events, rounding
+    b += ni * (sizeof(genie_dim_t) + 4 + 4 + 8);  // items, chunk counts and offsets, dense key ranges
+    b += ni * (sizeof(CChunk) + 8);               // compiled chunks (one per item; spans past kChunk add more)
+    b += nq * 96;                                 // per-query thresholds, counters, offsets
+    const bool dense = (d.dense || d.sets) && mode != 0;
+    b += nq * initial_candidates(max_k, dense, d.n_objects) * 8;  // candidate keys
+    b += nq * (uint64_t)max_k * 8;                                // output rows
+    if (dense)  // query-mask tables per group of 32 and the sample histograms
+        b += (nq + 31) / 32 * (uint64_t)std::max<uint32_t>(d.dense_dpad, 1) * kDenseKeysPerDim * 4 +
+             nq * kDenseBins * 4ull;
+    if (d.codec != 0 && mode != 1)  // full-layout decode scratch of the plain form
+        b += d.n_postings * 4;
+    *bytes = b;
     return GENIE_OK;
 }
 
@@ -1788,6 +1883,7 @@ static int plan_decoded_spans(genie_plan* p)
 {
     const DevIndex& d = p->ix->d;
     p->dec_spans = false;
+    p->h_djobs.clear();
     if (d.codec == 0 || p->h_hwork.empty())
         return GENIE_OK;
     const uint32_t nq = p->n_queries;
@@ -1848,6 +1944,74 @@ static int plan_decoded_spans(genie_plan* p)
     return GENIE_OK;
 }
 
+// Compressed index, plain form: the segments any item of the batch names,
+// each once, become decode jobs into a scratch array of the uncompressed
+// layout, and every item is translated from word to posting positions (items
+// start and end on segment boundaries).  The batch then runs the
+// uncompressed kernels over the scratch: a segment shared by many queries and
+// tiles is decoded once per batch instead of once per (query, tile).  Off
+// (the in-kernel decode stays) when the scratch cannot be allocated.
+static int plan_full_decode(genie_plan* p)
+{
+    const DevIndex& d = p->ix->d;
+    p->dec_full = false;
+    p->n_chunks = p->n_chunks_runs;
+    if (d.codec == 0 || p->decode_mode == 0 || !d.key_start || p->n_items == 0)
+        return GENIE_OK;
+    if (!p->d_full) {
+        if (dev_alloc(&p->d_full, std::max<uint64_t>(1, d.n_postings)) != GENIE_OK)
+            return GENIE_OK;  // no room for the scratch: in-kernel decode
+    }
+    if (!p->d_uitems)
+        if (int rc = dev_alloc(&p->d_uitems, std::max<uint64_t>(1, p->max_items)))
+            return rc;
+    const uint64_t* cp = d.h_seg_cpos64.data();
+    const uint32_t* up = d.h_seg_upos.data();
+    std::vector<uint2> runs;
+    runs.reserve(p->n_items);
+    p->h_uitems.resize(p->n_items);
+    for (uint64_t i = 0; i < p->n_items; ++i) {
+        genie_dim_t u = p->h_items[i];
+        if (u.end_pos > u.start_pos) {
+            const uint32_t s0 = d.seg_pos.first_ge(cp, (uint64_t)u.start_pos);
+            const uint32_t s1 = d.seg_pos.first_ge(cp, (uint64_t)u.end_pos);
+            u.start_pos = (int32_t)up[s0];
+            u.end_pos = (int32_t)up[s1];
+            runs.push_back(make_uint2(s0, s1));
+        } else {
+            u.start_pos = u.end_pos = 0;
+        }
+        p->h_uitems[i] = u;
+    }
+    std::sort(runs.begin(), runs.end(), [](const uint2& a, const uint2& b) { return a.x < b.x; });
+    p->h_fjobs.clear();
+    uint32_t next = 0;  // runs are sorted by start: segments below `next` are jobs already
+    for (const uint2& r : runs) {
+        for (uint32_t sg = std::max(r.x, next); sg < r.y; ++sg)
+            p->h_fjobs.push_back(make_uint2(sg, up[sg]));
+        next = std::max(next, r.y);
+    }
+    if (p->h_fjobs.size() > p->fjobs_alloc) {
+        dev_free(p->d_fjobs);
+        p->fjobs_alloc = 0;
+        if (int rc = dev_alloc(&p->d_fjobs, p->h_fjobs.size()))
+            return rc;
+        p->fjobs_alloc = p->h_fjobs.size();
+    }
+    if (!p->h_fjobs.empty())
+        if (int rc = h2d(p, &p->d_fjobs, p->h_fjobs.data(), p->h_fjobs.size() * sizeof(uint2)))
+            return rc;
+    if (int rc = h2d(p, &p->d_uitems, p->h_uitems.data(), p->n_items * sizeof(genie_dim_t)))
+        return rc;
+    // the run compiles the translated items like an uncompressed batch; the
+    // chunk buffers were sized for it at upload
+    if (p->n_chunks_full > p->chunks_cap)
+        return fail(GENIE_E_RUNTIME, "chunk buffers smaller than the decoded batch needs");
+    p->n_chunks = p->n_chunks_full;
+    p->dec_full = true;
+    return GENIE_OK;
+}
+
 // The plain form's split of the batch: a query with at most N / 16 postings
 // (every query whose counts fit, when the hash form is forced) runs the hash
 // kernel over tiles of whole directory tiles holding about kHashTarget of its
@@ -1905,7 +2069,11 @@ static int plan_split_plain(genie_plan* p)
             p->est_postings += P;
         }
     }
-    if (int rc = plan_decoded_spans(p))
+    if (int rc = plan_full_decode(p))
+        return rc;
+    if (p->dec_full)
+        p->dec_spans = false;
+    else if (int rc = plan_decoded_spans(p))
         return rc;
     if (p->h_hwork.size() > p->hwork_alloc) {
         dev_free(p->d_hwork);
@@ -1998,7 +2166,7 @@ static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items
         }
     }
     // chunks of the plain and lane forms
-    uint64_t nch = 0, span_total = 0;
+    uint64_t nch = 0, nch_full = 0, span_total = 0;
     bool prev_ok = false;
     for (uint64_t i = 0; i < n_items; ++i) {
         const genie_dim_t& it = p->h_items[i];
@@ -2013,29 +2181,40 @@ static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items
                 const bool starts = i == 0 || !prev_ok || pv.query != it.query || pv.weight != it.weight ||
                                     pv.end_pos != it.start_pos;
                 nch += starts ? 1 : 0;
+                // the item's postings, chunked as k_chunk_count does after
+                // the full-layout decode translates it (plan_full_decode)
+                const uint64_t* cp = d.h_seg_cpos64.data();
+                const uint32_t s0 = d.seg_pos.first_ge(cp, (uint64_t)it.start_pos);
+                const uint32_t s1 = d.seg_pos.first_ge(cp, (uint64_t)it.end_pos);
+                const uint64_t plen = d.h_seg_upos[s1] - d.h_seg_upos[s0];
+                nch_full += (plen + kChunk - 1) / kChunk;
             }
         }
         prev_ok = it.end_pos > it.start_pos;
     }
-    if (nch > p->chunks_cap) {
+    const uint64_t nch_cap = std::max(nch, nch_full);
+    if (nch_cap > p->chunks_cap) {
         dev_free(p->d_chunks);
         dev_free(p->d_nk);
         dev_free(p->d_kptr);
         dev_free(p->d_chunk_sums);
         p->chunks_cap = 0;
-        if (int rc = dev_alloc(&p->d_chunks, nch))
+        if (int rc = dev_alloc(&p->d_chunks, nch_cap))
             return rc;
-        if (int rc = dev_alloc(&p->d_nk, nch))
+        if (int rc = dev_alloc(&p->d_nk, nch_cap))
             return rc;
-        if (int rc = dev_alloc(&p->d_kptr, nch))
+        if (int rc = dev_alloc(&p->d_kptr, nch_cap))
             return rc;
-        if (int rc = dev_alloc(&p->d_chunk_sums, (nch + 1023) / 1024 + 1))
+        if (int rc = dev_alloc(&p->d_chunk_sums, (nch_cap + 1023) / 1024 + 1))
             return rc;
-        p->chunks_cap = nch;
+        p->chunks_cap = nch_cap;
     }
     p->n_queries = n_queries;
     p->n_items = n_items;
     p->n_chunks = nch;
+    p->n_chunks_runs = nch;
+    p->n_chunks_full = d.codec == 0 ? nch : nch_full;
+    p->dec_full = false;
     p->est_postings = d.codec == 0 || d.n_words == 0
                           ? span_total
                           : (uint64_t)((double)span_total * (double)d.n_postings / (double)d.n_words);
@@ -2122,7 +2301,7 @@ static ScanArgs scan_args(genie_plan* p)
     a.seg_first = d.seg_first;
     a.seg_last = d.seg_last;
     a.key_seg = d.key_seg;
-    a.sdir_off = d.dir_off;
+    a.sdir_off = d.sdir_off;
     a.sdir = d.sdir;
     a.post = d.post;
     a.key_start = d.key_start;
@@ -2166,7 +2345,7 @@ static bool plain_sparse(const genie_plan* p, uint32_t n_tiles)
 template <bool SPARSE>
 static void launch_scan_t(const genie_plan* p, const ScanArgs& a, uint32_t blocks, hipStream_t s)
 {
-    if (p->ix->d.codec == 0) {
+    if (a.codec == 0) {
         switch (p->lane_bits) {
         case 8: k_scan_tile<8, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
         case 16: k_scan_tile<16, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
@@ -2191,6 +2370,17 @@ static int launch_scan(genie_plan* p, hipStream_t s)
     const uint64_t blocks = (uint64_t)a.n_tiles * ndq;
     if (blocks >= 0x7FFFFFFFull || nwork >= 0x7FFFFFFFull)
         return fail(GENIE_E_INVALID, "batch too large for one launch");
+    if (p->dec_full) {
+        // the batch's segments, each once, into the full-layout scratch; the
+        // kernels below then read it as an uncompressed posting array
+        const DevIndex& d = p->ix->d;
+        const uint32_t nj = (uint32_t)p->h_fjobs.size();
+        if (nj)
+            k_decode_jobs<<<std::min<uint32_t>((nj + kDecodeWaves - 1) / kDecodeWaves, 65535), kDecodeWaves * kWave,
+                            0, s>>>(d.codec, d.words, d.seg_cpos, p->d_fjobs, nj, p->d_full);
+        a.codec = 0;
+        a.post = p->d_full;
+    }
     if (nwork) {
         HashArgs h;
         h.s = a;
@@ -2201,7 +2391,7 @@ static int launch_scan(genie_plan* p, hipStream_t s)
         h.spans = nullptr;
         h.sq_ptr = nullptr;
         const DevIndex& d = p->ix->d;
-        if (d.codec == 0) {
+        if (a.codec == 0) {
             k_scan_hash<false><<<(uint32_t)nwork, kScanThreads, 0, s>>>(h);
         } else if (p->dec_spans) {
             const uint32_t nj = (uint32_t)p->h_djobs.size();
@@ -2438,14 +2628,19 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
             HIP_TRY(hipEventRecord(p->ev[2], s));
         return launch_merge(p, s);
     }
-    // compile items into chunks (uncompressed) or segment runs (compressed)
-    if (ni && d.codec == 0) {
-        k_chunk_count<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, p->d_nchunks);
+    // compile items into chunks (uncompressed, or a compressed batch decoded
+    // into the full-layout scratch: its items in posting positions) or
+    // segment runs (compressed); the kernels below write n_chunks chunks
+    if (p->n_chunks > p->chunks_cap)
+        return fail(GENIE_E_RUNTIME, "batch compiles into more chunks than the plan holds");
+    if (ni && (d.codec == 0 || p->dec_full)) {
+        const genie_dim_t* items = p->dec_full ? p->d_uitems : p->d_items;
+        k_chunk_count<<<(ni + 255) / 256, 256, 0, s>>>(items, ni, p->d_nchunks);
         const uint32_t nb = (ni + 1023) / 1024;
         k_scan_blocks<<<nb, 256, 0, s>>>(p->d_nchunks, p->d_chunk_ptr, ni, p->d_block_sums);
         k_scan_sums<<<1, 256, 0, s>>>(p->d_block_sums, nb, p->d_total);
         k_scan_add<<<nb, 256, 0, s>>>(p->d_chunk_ptr, ni, p->d_block_sums);
-        k_chunk_write<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, p->d_chunk_ptr, d.key_start,
+        k_chunk_write<<<(ni + 255) / 256, 256, 0, s>>>(items, ni, p->d_chunk_ptr, d.key_start,
                                                       d.n_keys, p->d_chunks);
     } else if (ni) {
         k_crun_flags<<<(ni + 255) / 256, 256, 0, s>>>(p->d_items, ni, d.seg_cpos, d.n_segments, p->d_nchunks);
@@ -2625,6 +2820,30 @@ int genie_hip_plan_set_hash(genie_plan* p, int mode)
             return rc;
         return plan_size_candidates(p);
     }
+    return GENIE_OK;
+}
+
+int genie_hip_plan_set_decode(genie_plan* p, int mode)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (mode != -1 && mode != 0)
+        return fail(GENIE_E_INVALID, "decode mode must be -1 or 0");
+    plan_drop_graph(p);
+    p->decode_mode = mode;
+    if (p->n_queries && p->mode != 1 && !dense_active(p)) {
+        if (int rc = plan_split_plain(p))
+            return rc;
+        return plan_size_candidates(p);
+    }
+    return GENIE_OK;
+}
+
+int genie_hip_plan_decode_active(genie_plan* p, int* active)
+{
+    if (!p || !active)
+        return fail(GENIE_E_INVALID, "null argument");
+    *active = p->n_queries && p->mode != 1 && !dense_active(p) && p->dec_full ? 1 : 0;
     return GENIE_OK;
 }
 

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(NT) k_scan_batched(BatchArgs b)
                 sp.inc = pr.inc;
                 sp.pad = 0;
                 uint32_t lo = pr.lo, hi = pr.hi;
-                const uint32_t doff = a.dir_off[pr.key];
+                const uint32_t doff = CMP ? a.sdir_off[pr.key] : a.dir_off[pr.key];
                 if constexpr (CMP) {
                     const uint32_t ks = a.key_seg[pr.key], ke = a.key_seg[pr.key + 1];
                     if (doff != kNoDir) {

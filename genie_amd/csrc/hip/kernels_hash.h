@@ -30,9 +30,12 @@ struct HashArgs {
     const uint32_t* sq_ptr;
 };
 
-// Decodes the segments a batch's hash-form queries name, once each per query
-// item, into the batch's scratch posting array: job = (segment, destination
-// offset); one wave per job.  Segments were validated at index creation.
+// Decodes segments of a compressed index into a scratch posting array: job =
+// (segment, destination offset); one wave per job.  Two uses: the segments
+// of a batch's hash-form queries, once per query item, packed one after the
+// other; or every segment a batch names, once, at its own posting positions
+// (a scratch of the uncompressed layout).  Segments were validated at index
+// creation.
 __global__ void __launch_bounds__(kDecodeWaves * kWave)
     k_decode_jobs(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
                   const uint2* __restrict__ jobs, uint32_t n_jobs, uint32_t* __restrict__ out)

@@ -304,4 +304,50 @@ int genie_table_knn(genie_table* tt, const genie_range_t* ranges, uint64_t n_ran
     });
 }
 
+int genie_knn_mt_admit(const uint64_t* estimate, uint64_t n_tables, uint64_t start, uint64_t free_bytes,
+                       uint64_t* finish)
+{
+    if ((!estimate && n_tables) || !finish)
+        return GENIE_E_INVALID;
+    return guard([&] {
+        const std::vector<uint64_t> est(estimate, estimate + n_tables);
+        *finish = genie::matching::admit_tables(est, (size_t)start, free_bytes);
+    });
+}
+
+int genie_tables_knn_mt(genie_table* const* tables, uint32_t n_tables, const genie_range_t* ranges,
+                        const uint64_t* range_ptr, const uint32_t* n_queries, uint32_t k, uint64_t budget,
+                        int32_t* out_ids, int32_t* out_counts)
+{
+    if ((!tables || !range_ptr || !n_queries) && n_tables)
+        return GENIE_E_INVALID;
+    return guard([&] {
+        std::vector<genie::table::inv_table*> raw(n_tables);
+        std::vector<std::vector<genie::query::Query> > qs(n_tables);
+        std::vector<genie::original::GPUGenie_Config> configs(std::max<uint32_t>(1, n_tables));
+        for (uint32_t i = 0; i < n_tables; ++i) {
+            if (!tables[i] || !tables[i]->t)
+                throw genie::exception::cpu_runtime_error("null table");
+            raw[i] = tables[i]->t.get();
+            qs[i] = make_queries(*raw[i], ranges + range_ptr[i], range_ptr[i + 1] - range_ptr[i], n_queries[i], 0, k);
+            configs[i].num_of_topk = (int)k;
+        }
+        std::vector<std::vector<int> > ids, counts;
+        genie::matching::set_knn_mt_memory_budget(budget);
+        try {
+            genie::original::knn_search_MT(raw, qs, ids, counts, configs);
+        } catch (...) {
+            genie::matching::set_knn_mt_memory_budget(0);
+            throw;
+        }
+        genie::matching::set_knn_mt_memory_budget(0);
+        uint64_t off = 0;
+        for (uint32_t i = 0; i < n_tables; ++i) {
+            std::copy(ids[i].begin(), ids[i].end(), out_ids + off);
+            std::copy(counts[i].begin(), counts[i].end(), out_counts + off);
+            off += (uint64_t)n_queries[i] * k;
+        }
+    });
+}
+
 }  // extern "C"

@@ -113,6 +113,13 @@ typedef struct genie_plan genie_plan;
 int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_items,
                           uint32_t max_k, uint32_t count_bound, int mode, genie_plan** out);
 int genie_hip_plan_destroy(genie_plan* p);
+/* Device memory a plan of this shape holds for one batch of n_queries queries
+ * and n_items items with k <= max_k (its candidate space as first sized, the
+ * dense form's tables, the full-layout decode scratch of a compressed
+ * index's plain form): the admission estimate of knn_MT (the reference's
+ * estimate is knn.cu:218-227). */
+int genie_hip_plan_estimate(const genie_index* ix, uint32_t n_queries, uint64_t n_items, uint32_t max_k, int mode,
+                            uint64_t* bytes);
 /* Take a batch (host arrays; they may be reused on return): its device
  * copies are staged in the plan's pinned host buffer and issued by the next
  * genie_hip_plan_run on that run's stream, ahead of its kernels. */
@@ -191,6 +198,20 @@ int genie_hip_plan_sparse_active(genie_plan* p, int* active);
  * table (match_device_utils.h:161-419), which evicts entries and can lose
  * the k-th count; these tables keep every touched id. */
 int genie_hip_plan_set_hash(genie_plan* p, int mode);
+/* Decode of a compressed index in the plain form (modes 0 and 2).  mode -1
+ * (default): the segments the uploaded batch names are decoded once each per
+ * run into a plan-owned scratch of the uncompressed layout (4 B per posting
+ * of the index, allocated at first use), and the batch is counted from it by
+ * the uncompressed kernels; where that scratch cannot be allocated, or with
+ * mode 0, the kernels decode segments themselves (the direct-mapped kernel
+ * once per (query, tile) it overlaps; the hash form once per query item into
+ * a packed scratch).  Results are identical.  The reference decodes each
+ * (query, segment) work item in shared memory inside its match kernel
+ * (match_integrated.cu:169-325). */
+int genie_hip_plan_set_decode(genie_plan* p, int mode);
+/* Whether the last uploaded batch decodes its segments once into the
+ * full-layout scratch. */
+int genie_hip_plan_decode_active(genie_plan* p, int* active);
 /* Number of queries of the last uploaded batch on the hash form. */
 int genie_hip_plan_hash_queries(genie_plan* p, uint32_t* n);
 /* Host time of the last batched work-queue build (upload), ms. */
@@ -276,6 +297,13 @@ int genie_hip_copy_probe(uint64_t bytes);
 
 /* Wait for all work on `stream` (NULL = default stream). */
 int genie_hip_stream_synchronize(void* stream);
+/* A non-blocking stream of the default priority (a hipStream_t), and its
+ * release. */
+int genie_hip_stream_create(void** stream);
+int genie_hip_stream_destroy(void* stream);
+/* Free and total device memory of the current device (cudaMemGetInfo, which
+ * knn_MT's admission reads, knn.cu:210). */
+int genie_hip_mem_info(uint64_t* free_bytes, uint64_t* total_bytes);
 /* Wait for all work on the current device. */
 int genie_hip_device_synchronize(void);
 

@@ -59,6 +59,21 @@ int genie_table_compile(genie_table* t, const genie_range_t* ranges, uint64_t n_
                         int use_load_balance, int32_t max_load, int merge_runs, genie_dim_t* out, uint64_t cap,
                         uint64_t* n_out);
 
+/* knn_MT's admission round (genie::matching::admit_tables): tables from
+ * `start` admitted against free_bytes; GENIE_E_OOM ("MEMORY NOT ENOUGH")
+ * when table `start` does not fit alone.  Host-only. */
+int genie_knn_mt_admit(const uint64_t* estimate, uint64_t n_tables, uint64_t start, uint64_t free_bytes,
+                       uint64_t* finish);
+
+/* knn_search_MT over n_tables tables (interface.cc:859-909): table i's
+ * ranges are ranges[range_ptr[i] .. range_ptr[i + 1]) for n_queries[i]
+ * queries, k results each; results are concatenated by table (table i:
+ * n_queries[i] x k slots).  budget: device memory knn_MT admits against
+ * (0 = the device's free memory). */
+int genie_tables_knn_mt(genie_table* const* tables, uint32_t n_tables, const genie_range_t* ranges,
+                        const uint64_t* range_ptr, const uint32_t* n_queries, uint32_t k, uint64_t budget,
+                        int32_t* out_ids, int32_t* out_counts);
+
 /* Engine handle of the table (cpy_data_to_gpu); owned by the table. */
 int genie_table_device(genie_table* t, genie_index** out);
 

@@ -35,25 +35,43 @@ def test_device_decoders_match_oracle(codec):
         assert y is not None and np.array_equal(x, y), (codec, len(x))
 
 
-def _compare(rows, ranges, nq, k, codec, max_length=1024, label=""):
+def _oracle(rows, ranges, nq, k):
+    from helpers import oracle_topk_batch
+    q_of = ranges["query"]
+    return oracle_topk_batch(lambda q: O.count_exhaustive_single(rows, ranges[q_of == q]), nq, k)
+
+
+def _compare(rows, ranges, nq, k, codec, max_length=1024, label="", oracle=True):
+    """Compressed against uncompressed (bit-exact, every query and slot), and
+    both against the exhaustive oracle; the compressed index in the plain
+    form with its segments decoded once per batch (the default) and decoded
+    by the kernels themselves."""
+    from genie_amd import Plan
     plain = Table.from_rows(rows)
     comp = Table.from_rows(rows, max_length, True, codec=codec)
     ids_u, cnt_u = match_topk(plain.device(), plain.compile(ranges, nq), nq, k)
-    ids_c, cnt_c = match_topk(comp.device(), comp.compile(ranges, nq), nq, k)
+    items = comp.compile(ranges, nq)
+    ids_c, cnt_c = match_topk(comp.device(), items, nq, k)
     assert np.array_equal(cnt_u, cnt_c) and np.array_equal(ids_u, ids_c), f"{label} codec {codec}"
+    kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
+    for dm in (-1, 0):
+        p = Plan(comp.device(), nq, len(items), int(kq.max()), 0, mode=0)
+        p.set_decode(dm)
+        p.upload(items, nq, kq)
+        assert p.decode_active() == (dm == -1)
+        p.run()
+        got = p.download()
+        assert np.array_equal(got[0], ids_u) and np.array_equal(got[1], cnt_u), f"{label} codec {codec} decode {dm}"
+    if oracle:
+        ri, rc = _oracle(rows, ranges, nq, k)
+        assert_topk_equal(ids_c, cnt_c, ri, rc, k, nq, f"{label} codec {codec} vs oracle")
     return ids_c, cnt_c
 
 
 @pytest.mark.parametrize("codec", range(1, 14))
 def test_compressed_search_every_codec(codec):
     w = W.cfg1(n=60_000, d=16, q=6, k=40)
-    ids, cnt = _compare(w["rows"], w["ranges"], w["n_queries"], w["k"], codec, 512, "zipf")
-    res_i, res_c = [], []
-    for q in range(w["n_queries"]):
-        i, c = O.topk(O.count_exhaustive_single(w["rows"], w["ranges"][w["ranges"]["query"] == q]), w["k"])
-        res_i.append(i)
-        res_c.append(c)
-    assert_topk_equal(ids, cnt, np.concatenate(res_i), np.concatenate(res_c), w["k"], w["n_queries"], "oracle")
+    _compare(w["rows"], w["ranges"], w["n_queries"], w["k"], codec, 512, "zipf")
 
 
 def test_lsh_heavyweight_many_tiles():

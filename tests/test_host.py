@@ -154,3 +154,27 @@ def test_adaptive_query_range_known_answers():
     r = subprocess.run([exe, GOLDEN, "adaptive_range"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=120)
     assert r.returncode == 0 and "[  OK  ] adaptive_range" in r.stdout, r.stdout
+
+
+def test_knn_mt_admission_rounds():
+    """knn_MT admits tables in order while more than estimate + 50 MiB stays
+    free (knn.cu:207-243); a table that does not fit alone is refused with
+    gpu_bad_alloc("MEMORY NOT ENOUGH")."""
+    from genie_amd import GenieError, knn_mt_admit
+    MiB = 1 << 20
+    est = [100 * MiB, 100 * MiB, 100 * MiB, 10 * MiB]
+    # 300 MiB free: 0 (200 left), 1 (100 left); 2 needs 150
+    assert knn_mt_admit(est, 0, 300 * MiB) == 2
+    # the next round starts at 2 with the memory of that moment
+    assert knn_mt_admit(est, 2, 300 * MiB) == 4
+    assert knn_mt_admit(est, 2, 160 * MiB) == 3
+    # exactly estimate + 50 MiB is not enough (strictly more is needed)
+    assert knn_mt_admit(est, 3, 60 * MiB + 1) == 4
+    with pytest.raises(GenieError) as e:
+        knn_mt_admit(est, 3, 60 * MiB)
+    assert e.value.code == 2 and "MEMORY NOT ENOUGH" in str(e.value)
+    with pytest.raises(GenieError):
+        knn_mt_admit(est, 0, 150 * MiB)
+    # empty query sets cost nothing; the end of the list is a complete round
+    assert knn_mt_admit([0, 0], 0, 51 * MiB) == 2
+    assert knn_mt_admit(est, 4, 0) == 4
