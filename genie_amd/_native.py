@@ -83,6 +83,8 @@ _sig(_hip, "genie_hip_plan_sparse_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_set_hash", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_hash_queries", C.c_int, _p, C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_set_decode", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_set_form", C.c_int, _p, C.c_int)
+_sig(_hip, "genie_hip_plan_form", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_decode_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_plan_download_async", C.c_int, _p, _p, _p, _p, _p)
@@ -335,7 +337,20 @@ class Plan:
         v = [_u32() for _ in range(4)]
         _chk_hip(_hip.genie_hip_plan_get_config(self.handle, *[C.byref(x) for x in v]))
         return dict(block=v[0].value, tile_objs=v[1].value, seglen=v[2].value, lane_bits=v[3].value,
-                    dense=self.dense_active())
+                    dense=self.dense_active(), form=self.form())
+
+    FORMS = ("plain", "lane", "dense", "set")
+
+    def form(self):
+        """The form the uploaded batch runs: plain, lane, dense or set."""
+        v = C.c_int()
+        _chk_hip(_hip.genie_hip_plan_form(self.handle, C.byref(v)))
+        return self.FORMS[v.value]
+
+    def set_form(self, form):
+        """Batched plans: 'plain' or 'lane' for batches the dense and set forms
+        do not take."""
+        _chk_hip(_hip.genie_hip_plan_set_form(self.handle, {"plain": 0, "lane": 1}[form]))
 
     def set_dense(self, on=True):
         """Batched plans: count whole-key unit-weight batches of a single-value

@@ -198,6 +198,7 @@ struct genie_plan {
     std::vector<uint32_t> h_qptr;
     double host_build_ms = 0;
     bool lanes_built = false;        // work queue of the lane form built for this batch
+    bool batched_plain = false;      // batched plan: the plain form where the dense and set forms do not apply
     // dense form of the batched search (single-value index, whole-key unit-weight batch)
     bool dense_on = true;            // use it when the batch allows
     int dense_nt = 0;                // threads per workgroup of its kernels (0: 1,024 when placed, else 512)
@@ -1513,11 +1514,16 @@ static int h2d(genie_plan* p, T** dst, const void* src, uint64_t bytes)
 // allows it; otherwise mode 1 runs the lane form and mode 2 the plain form.
 static bool dense_active(const genie_plan* p) { return p->mode != 0 && p->dense_on && (p->dense_ok || p->sets_ok); }
 
+// Whether the plan's batches that the dense and set forms do not take run the
+// lane form (batched plans, unless the autotuner or the caller chose the
+// plain form for them); otherwise they run the plain form.
+static bool lane_form(const genie_plan* p) { return p->mode == 1 && !p->batched_plain; }
+
 static uint32_t plan_tile_objs(const genie_plan* p)
 {
     if (dense_active(p))
         return kDenseTile;
-    return p->mode == 1 ? p->tile_b : tile_objs_for(p->lane_bits);
+    return lane_form(p) ? p->tile_b : tile_objs_for(p->lane_bits);
 }
 
 static uint32_t plan_n_tiles(const genie_plan* p)
@@ -1531,7 +1537,7 @@ static uint32_t plan_n_tiles(const genie_plan* p)
 static uint32_t max_cap_factor(const genie_plan* p)
 {
     if (!dense_active(p))
-        return std::max(plan_n_tiles(p), p->mode == 1 ? 0u : p->max_hash_tiles);
+        return std::max(plan_n_tiles(p), lane_form(p) ? 0u : p->max_hash_tiles);
     uint32_t kmin = 0xFFFFFFFFu;
     for (uint32_t q = 0; q < p->n_queries; ++q)
         kmin = std::min(kmin, p->h_kq[q]);
@@ -1554,7 +1560,7 @@ static int plan_size_candidates(genie_plan* p)
     std::vector<uint32_t> cap(p->n_queries);
     std::vector<uint64_t> base(p->n_queries);
     uint64_t tot = 0;
-    const bool plain = !dn && p->mode != 1;
+    const bool plain = !dn && !lane_form(p);
     for (uint32_t q = 0; q < p->n_queries; ++q) {
         // the dense form emits each object at most once; the tile forms at
         // most k per tile (a hash-form query: per tile of its own)
@@ -2103,7 +2109,7 @@ static int plan_settle_form(genie_plan* p)
     if (dense_active(p)) {
         if (int rc = prepare_dense(p))
             return rc;
-    } else if (p->mode == 1) {
+    } else if (lane_form(p)) {
         if (int rc = ensure_lanes(p))
             return rc;
     } else {
@@ -2617,8 +2623,8 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
     p->timed_dense = dense_active(p);
     if (p->timed_dense)
         return run_dense(p, s);
-    if (p->mode == 1) {
-        // batched form: the work queue was built at upload
+    if (lane_form(p)) {
+        // lane form: the work queue was built at upload
         fill_u32(p->d_queue, 0, 1, s);
         if (p->timing)
             HIP_TRY(hipEventRecord(p->ev[1], s));
@@ -2767,7 +2773,7 @@ int genie_hip_plan_set_config(genie_plan* p, uint32_t block, uint32_t tile_objs,
     plan_drop_graph(p);
     if (rebuild)
         p->lanes_built = false;
-    if (p->mode == 1 && p->n_queries && rebuild && !dense_active(p))
+    if (lane_form(p) && p->n_queries && rebuild && !dense_active(p))
         return plan_settle_form(p);
     return GENIE_OK;
 }
@@ -2814,7 +2820,7 @@ int genie_hip_plan_set_hash(genie_plan* p, int mode)
         return fail(GENIE_E_INVALID, "hash form mode must be -1, 0 or 1");
     plan_drop_graph(p);
     p->hash_mode = mode;
-    if (p->n_queries && p->mode != 1 && !dense_active(p)) {
+    if (p->n_queries && !lane_form(p) && !dense_active(p)) {
         std::fill(p->h_force_direct.begin(), p->h_force_direct.end(), 0);
         if (int rc = plan_split_plain(p))
             return rc;
@@ -2831,7 +2837,7 @@ int genie_hip_plan_set_decode(genie_plan* p, int mode)
         return fail(GENIE_E_INVALID, "decode mode must be -1 or 0");
     plan_drop_graph(p);
     p->decode_mode = mode;
-    if (p->n_queries && p->mode != 1 && !dense_active(p)) {
+    if (p->n_queries && !lane_form(p) && !dense_active(p)) {
         if (int rc = plan_split_plain(p))
             return rc;
         return plan_size_candidates(p);
@@ -2843,7 +2849,7 @@ int genie_hip_plan_decode_active(genie_plan* p, int* active)
 {
     if (!p || !active)
         return fail(GENIE_E_INVALID, "null argument");
-    *active = p->n_queries && p->mode != 1 && !dense_active(p) && p->dec_full ? 1 : 0;
+    *active = p->n_queries && !lane_form(p) && !dense_active(p) && p->dec_full ? 1 : 0;
     return GENIE_OK;
 }
 
@@ -2852,7 +2858,7 @@ int genie_hip_plan_hash_queries(genie_plan* p, uint32_t* n)
     if (!p || !n)
         return fail(GENIE_E_INVALID, "null argument");
     uint32_t c = 0;
-    if (p->mode != 1 && !dense_active(p))
+    if (!lane_form(p) && !dense_active(p))
         for (const uint4& h : p->h_hq)
             c += h.x != 0;
     *n = c;
@@ -2887,7 +2893,7 @@ int genie_hip_plan_get_config(genie_plan* p, uint32_t* block, uint32_t* tile_obj
     if (tile_objs)
         *tile_objs = plan_tile_objs(p);
     if (seglen)
-        *seglen = p->mode == 1 ? p->seglen : kChunk;
+        *seglen = lane_form(p) ? p->seglen : kChunk;
     if (lane_bits)
         *lane_bits = p->lane_bits;
     return GENIE_OK;
@@ -2921,7 +2927,7 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
     p->graph_on = false;
     double best = 1e30;
     uint32_t bb = p->block, bt = p->tile_b, bs = p->seglen, n = 0;
-    bool best_dense = false;
+    bool best_dense = false, best_plain = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -2945,10 +2951,10 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
         return GENIE_OK;
     };
     int rc = GENIE_OK;
-    if (try_dense) {
+    p->batched_plain = false;
+    if (try_dense)
         p->dense_on = false;
-        rc = plan_settle_form(p);
-    }
+    rc = plan_settle_form(p);
     for (uint32_t blk : blocks)
         for (uint32_t tl : tiles)
             for (uint32_t sg : segs) {
@@ -2972,7 +2978,7 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
                 }
             }
     if (!rc && try_dense) {
-        // the dense form is the last candidate
+        // then the dense (or set) form
         double ms = -1;
         p->dense_on = true;
         if ((rc = plan_settle_form(p)) == GENIE_OK)
@@ -2987,6 +2993,24 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
             }
         }
     }
+    if (!rc) {
+        // and the plain form last: a batched plan never runs slower than it
+        double ms = -1;
+        p->dense_on = false;
+        p->batched_plain = true;
+        if ((rc = plan_settle_form(p)) == GENIE_OK)
+            rc = time_it(&ms);
+        if (!rc) {
+            if (out_ms)
+                out_ms[n] = ms;
+            ++n;
+            if (ms >= 0 && ms < best) {
+                best = ms;
+                best_plain = true;
+                best_dense = false;
+            }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (n_out)
@@ -2994,12 +3018,38 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
     p->graph_on = was_graph;
     if (rc)
         return rc;
-    if ((rc = genie_hip_plan_set_config(p, bb, bt, bs)))
-        return rc;
-    if (!try_dense)
+    p->batched_plain = best_plain;
+    p->dense_on = best_dense;
+    p->block = bb;
+    p->tile_b = bt;
+    p->seglen = bs;
+    p->lanes_built = false;  // the lane form's queue is rebuilt for the chosen configuration
+    return plan_settle_form(p);
+}
+
+int genie_hip_plan_set_form(genie_plan* p, int form)
+{
+    if (!p)
+        return fail(GENIE_E_INVALID, "null plan");
+    if (p->mode != 1)
+        return fail(GENIE_E_INVALID, "the form of a batched plan only");
+    if (form != 0 && form != 1)
+        return fail(GENIE_E_INVALID, "form must be 0 (plain) or 1 (lane)");
+    if (p->batched_plain == (form == 0))
         return GENIE_OK;
-    p->dense_on = !best_dense;  // force a settle below
-    return genie_hip_plan_set_dense(p, best_dense ? 1 : 0);
+    p->batched_plain = form == 0;
+    return plan_settle_form(p);
+}
+
+int genie_hip_plan_form(genie_plan* p, int* form)
+{
+    if (!p || !form)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (dense_active(p))
+        *form = p->dense_ok ? 2 : 3;
+    else
+        *form = lane_form(p) ? 1 : 0;
+    return GENIE_OK;
 }
 
 int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts)

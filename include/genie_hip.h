@@ -146,12 +146,19 @@ int genie_hip_plan_set_graph(genie_plan* p, int on);
 int genie_hip_plan_set_config(genie_plan* p, uint32_t block, uint32_t tile_objs, uint32_t seglen);
 int genie_hip_plan_get_config(genie_plan* p, uint32_t* block, uint32_t* tile_objs, uint32_t* seglen,
                               uint32_t* lane_bits);
-/* Autotuner of the batched form over block x tile x segment length of the
- * lane form, plus the dense form as a 13th candidate when the batch allows it
- * and it is switched on; keeps the fastest (form and configuration).  out_ms
- * (optional, >= 13 entries) receives every candidate's time per run (-1 if it
- * could not run); *n_out the number of candidates. */
+/* Autotuner of a batched plan on its uploaded batch: the lane form over block
+ * x tile x segment length (12 candidates), then the dense or set form when
+ * the batch allows it and it is switched on, then the plain form, so that a
+ * batched plan never runs slower than the plain one; keeps the fastest (form
+ * and configuration).  out_ms (optional, >= 14 entries) receives every
+ * candidate's time per run in that order (-1 if it could not run); *n_out the
+ * number of candidates. */
 int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n_out);
+/* Batched plans: the form of batches the dense and set forms do not take:
+ * 0 the plain form, 1 the lane form (the default until an autotune). */
+int genie_hip_plan_set_form(genie_plan* p, int form);
+/* The form the uploaded batch runs: 0 plain, 1 lane, 2 dense, 3 set. */
+int genie_hip_plan_form(genie_plan* p, int* form);
 /* Dense form of the batched search (modes 1 and 2; on by default).  When the
  * index is a single-value table of at most 128 dims with at most 256 keys per
  * dim, and every item of the uploaded batch covers whole keys with weight 1

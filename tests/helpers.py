@@ -54,3 +54,42 @@ def oracle_topk_batch(counts_fn, n_queries, k):
         ids.append(i)
         cnt.append(c)
     return np.concatenate(ids), np.concatenate(cnt)
+
+
+def posting_items(export, items):
+    """Items of a compressed table (word positions on segment boundaries)
+    translated to positions of its uncompressed posting array (the table's
+    inv), so that the oracle can count them with codec 0."""
+    cpos = np.asarray(export["cpos"], np.int64)
+    upos = np.asarray(export["inv_pos"], np.int64)
+    out = items.copy()
+    out["start_pos"] = upos[np.searchsorted(cpos, items["start_pos"].astype(np.int64))]
+    out["end_pos"] = upos[np.searchsorted(cpos, items["end_pos"].astype(np.int64))]
+    return out
+
+
+def oracle_search_sample(words, items, n_objects, k, qs):
+    """The oracle's multithreaded index-based search (the CPU-Idx restatement,
+    test_compression_query.cu:256-265) over the queries qs of a batch;
+    returns their rows concatenated in the order of qs."""
+    from oracle import oracle as O
+    qs = np.asarray(qs, np.int64)
+    n = len(k) if np.ndim(k) else max(int(items["query"].max()) + 1 if len(items) else 1, int(qs.max()) + 1)
+    kq = np.broadcast_to(np.asarray(k, np.int64), (n,))
+    sel = np.isin(items["query"], qs)
+    sub = items[sel].copy()
+    remap = np.full(len(kq), -1, np.int64)
+    remap[qs] = np.arange(len(qs))
+    sub["query"] = remap[sub["query"]]
+    sub = sub[np.argsort(sub["query"], kind="stable")]
+    ids, cnt, _ = O.search_mt(np.asarray(words, np.uint32), sub, n_objects, len(qs), kq[qs].astype(np.int32))
+    return ids, cnt
+
+
+def rows_of(ids, cnt, k, n_queries, qs):
+    """Rows of queries qs out of a batch result (k an int or per query)."""
+    kq = np.broadcast_to(np.asarray(k, np.int64), (n_queries,))
+    off = np.zeros(n_queries + 1, np.int64)
+    off[1:] = np.cumsum(kq)
+    return (np.concatenate([ids[off[q]:off[q + 1]] for q in qs]),
+            np.concatenate([cnt[off[q]:off[q + 1]] for q in qs]))

@@ -1,11 +1,12 @@
 """Batched form (lane form: persistent kernel over a (query group, tile) work
 queue, queries packed into count lanes, merged spans; dense form: bit-sliced
-counting over per-object key codes; hipGraph replay, autotuner) against the
-plain form and the oracle: identical results."""
+counting over per-object key codes; the plain form where the autotuner finds
+it faster; hipGraph replay, autotuner) against the exhaustive oracle:
+identical results."""
 import numpy as np
 import pytest
 
-from helpers import assert_topk_equal
+from helpers import assert_topk_equal, oracle_topk_batch
 from oracle import oracle as O
 
 import genie_amd
@@ -16,6 +17,11 @@ pytestmark = pytest.mark.gpu
 
 def plain(table, items, nq, k):
     return match_topk(table.device(), items, nq, k)
+
+
+def oracle_rows(rows, ranges, nq, k):
+    q_of = ranges["query"]
+    return oracle_topk_batch(lambda q: O.count_exhaustive_single(rows, ranges[q_of == q]), nq, k)
 
 
 def batched(table, items, nq, k, config=None, graph=False, runs=1, dense=False):
@@ -42,9 +48,9 @@ def test_batched_equals_plain_zipf(config):
     w = W.cfg1(n=150_000, d=32, q=37, k=50)  # 37 queries: a partly filled last group
     t = Table.from_rows(w["rows"])
     items = t.compile(w["ranges"], w["n_queries"])
-    a = plain(t, items, w["n_queries"], w["k"])
+    ri, rc = oracle_rows(w["rows"], w["ranges"], w["n_queries"], w["k"])
     (b, _) = batched(t, items, w["n_queries"], w["k"], config)
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert_topk_equal(b[0], b[1], ri, rc, w["k"], w["n_queries"], f"lane form {config}")
 
 
 def test_batched_mixed_k_weights_bijectmap():
@@ -70,10 +76,10 @@ def test_batched_lane_widths(wmax, lb):
     r["weight"] = np.random.default_rng(wmax).integers(1, wmax + 1, len(r))
     t = Table.from_rows(w["rows"])
     items = t.compile(r, 9)
-    a = plain(t, items, 9, 20)
+    ri, rc = oracle_rows(w["rows"], r, 9, 20)
     (b, p) = batched(t, items, 9, 20)
     assert p.config()["lane_bits"] == lb
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert_topk_equal(b[0], b[1], ri, rc, 20, 9, f"lane width {lb}")
 
 
 def test_lane_widths_change_between_uploads():
@@ -99,11 +105,10 @@ def test_lane_widths_change_between_uploads():
 
 def test_batched_compressed():
     w = W.cfg4(n=120_000, d=16, q=20)
-    plain_t = Table.from_rows(w["rows"])
     comp_t = Table.from_rows(w["rows"], 1024, True, codec=11)
-    a = plain(plain_t, plain_t.compile(w["ranges"], 20), 20, w["k"])
+    ri, rc = oracle_rows(w["rows"], w["ranges"], 20, w["k"])
     (b, _) = batched(comp_t, comp_t.compile(w["ranges"], 20), 20, w["k"])
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert_topk_equal(b[0], b[1], ri, rc, w["k"], 20, "lane form, compressed")
 
 
 def test_graph_replays_match_eager():
@@ -133,9 +138,40 @@ def test_autotune_picks_a_valid_config():
     p.set_dense(False)
     p.upload(items, 32, 100)
     ms = p.autotune(reps=1)
-    assert len(ms) == 12 and np.any(ms > 0)
+    assert len(ms) == 13 and np.all(ms > 0)  # 12 lane configurations, then the plain form
     cfg = p.config()
-    assert cfg["block"] in (256, 512, 1024) and cfg["tile_objs"] in (8192, 16384)
+    assert cfg["form"] == (["lane"] * 12 + ["plain"])[int(np.argmin(ms))]
+    if cfg["form"] == "lane":
+        assert cfg["block"] in (256, 512, 1024) and cfg["tile_objs"] in (8192, 16384)
     p.run()
     b = p.download()
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    ri, rc = oracle_rows(w["rows"], w["ranges"], 32, 100)
+    assert_topk_equal(b[0], b[1], ri, rc, 100, 32, "after autotune")
+
+
+@pytest.mark.parametrize("form", ["plain", "lane"])
+def test_batched_plan_forms_on_weighted_batches(form):
+    """Weights 1-4 (the dense form refuses them): a batched plan runs the lane
+    or the plain form, switched explicitly or chosen by the autotuner, with
+    the oracle's results either way; a graph replay of the chosen form too."""
+    w = W.cfg1(seed=955, n=120_000, d=32, q=40, k=64)
+    r = w["ranges"].copy()
+    r["weight"] = genie_amd.synth.uniform_int(956, 0, len(r), 1, 4)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(r, 40)
+    ri, rc = oracle_rows(w["rows"], r, 40, 64)
+    p = Plan(t.device(), 40, len(items), 64, 0, mode=1)
+    p.upload(items, 40, np.full(40, 64, np.uint32))
+    p.set_form(form)
+    assert p.form() == form
+    p.run()
+    b = p.download()
+    assert_topk_equal(b[0], b[1], ri, rc, 64, 40, f"weighted, {form} form")
+    ms = p.autotune(reps=1)
+    assert len(ms) == 13 and p.form() in ("plain", "lane")
+    p.set_graph(True)
+    for _ in range(2):
+        p.run()
+        b = p.download()
+        assert_topk_equal(b[0], b[1], ri, rc, 64, 40, f"weighted, autotuned {p.form()} form")

@@ -161,7 +161,8 @@ def test_dense_autotune_includes_dense():
     p = Plan(t.device(), 32, len(items), 100, 0, mode=1)
     p.upload(items, 32, np.full(32, 100, np.uint32))
     ms = p.autotune(reps=1)
-    assert len(ms) == 13 and ms[12] > 0
+    assert len(ms) == 14 and ms[12] > 0 and ms[13] > 0  # 12 lane configurations, dense, plain
+    assert p.form() == (["lane"] * 12 + ["dense", "plain"])[int(np.argmin(ms))]
     p.run()
     b = p.download()
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
