@@ -85,6 +85,7 @@ _sig(_hip, "genie_hip_plan_hash_queries", C.c_int, _p, C.POINTER(_u32))
 _sig(_hip, "genie_hip_plan_set_decode", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_set_form", C.c_int, _p, C.c_int)
 _sig(_hip, "genie_hip_plan_form", C.c_int, _p, C.POINTER(C.c_int))
+_sig(_hip, "genie_hip_plan_thresholds", C.c_int, _p, _p, _p)
 _sig(_hip, "genie_hip_plan_decode_active", C.c_int, _p, C.POINTER(C.c_int))
 _sig(_hip, "genie_hip_plan_host_ms", C.c_int, _p, C.POINTER(C.c_double))
 _sig(_hip, "genie_hip_plan_download_async", C.c_int, _p, _p, _p, _p, _p)
@@ -340,6 +341,12 @@ class Plan:
                     dense=self.dense_active(), form=self.form())
 
     FORMS = ("plain", "lane", "dense", "set")
+
+    def thresholds(self, n_queries, stream=None):
+        """Each query's adaptive threshold after the last run."""
+        out = np.zeros(n_queries, np.uint32)
+        _chk_hip(_hip.genie_hip_plan_thresholds(self.handle, stream, _ptr(out)))
+        return out
 
     def form(self):
         """The form the uploaded batch runs: plain, lane, dense or set."""

@@ -70,8 +70,12 @@ def build(verbose=False):
           _obj("synth/synth.cc")])
     test_src = os.path.join(ROOT, "tests", "cpp", "test_api.cc")
     if os.path.exists(test_src):
-        _run([CXX, *CXX_FLAGS, "-o", os.path.join(LIB, "test_api"), test_src,
-              f"-L{LIB}", "-lgenie", "-lgenie_hip", "-Wl,-rpath,$ORIGIN"])
+        # the C++ API test driver; it links the CPU oracle as its checker
+        odir = os.path.join(ROOT, "oracle")
+        _run(["make", "-C", odir, "-s", "liboracle.so"])
+        _run([CXX, *CXX_FLAGS, f"-I{odir}", "-o", os.path.join(LIB, "test_api"), test_src,
+              f"-L{LIB}", "-lgenie", "-lgenie_hip", f"-L{odir}", "-loracle",
+              "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/../../oracle"])
     return LIB
 
 

@@ -3027,6 +3027,17 @@ int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n
     return plan_settle_form(p);
 }
 
+int genie_hip_plan_thresholds(genie_plan* p, void* stream, uint32_t* out)
+{
+    if (!p || !out)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (!p->ran)
+        return fail(GENIE_E_NOT_BUILT, "plan has not run");
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    HIP_TRY(hipMemcpy(out, p->d_gthr, p->n_queries * 4ull, hipMemcpyDeviceToHost));
+    return GENIE_OK;
+}
+
 int genie_hip_plan_set_form(genie_plan* p, int form)
 {
     if (!p)

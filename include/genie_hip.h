@@ -154,6 +154,14 @@ int genie_hip_plan_get_config(genie_plan* p, uint32_t* block, uint32_t* tile_obj
  * candidate's time per run in that order (-1 if it could not run); *n_out the
  * number of candidates. */
 int genie_hip_plan_autotune(genie_plan* p, int reps, double* out_ms, uint32_t* n_out);
+/* Each query's adaptive threshold after the last run (synchronises stream;
+ * out[n_queries]), the role of updateThreshold's d_threshold
+ * (match_device_utils.h:449-473).  Tile forms: the largest k-th count of any
+ * tile that holds at least k objects counting at least 1 (0 if none); with
+ * one tile over the whole index that is the k-th largest count c_(k), the
+ * reference's closed form thr - 1 = c_(k).  Dense and set forms: the
+ * threshold their main pass emitted from.  Measurement and tests. */
+int genie_hip_plan_thresholds(genie_plan* p, void* stream, uint32_t* out);
 /* Batched plans: the form of batches the dense and set forms do not take:
  * 0 the plain form, 1 the lane form (the default until an autotune). */
 int genie_hip_plan_set_form(genie_plan* p, int form);

@@ -2,9 +2,13 @@
 // (test/test_csv_input_sift_data.cc, test_refactoring_new_interface.cc,
 // test_csv_input_tweets.cc, test_multiple_search.cc, test_hashtable_overflow.cc,
 // test_csv_input_multiload.cc, test_csv_multiload_tweets.cc,
-// test_serialization.cc).  Counts are
-// the exact ones (the reference's 16s on tweets_20 are wrapped 4-bit
-// counters).  Usage: test_api <golden dir> [test name]
+// test_serialization.cc, example/example_sift.cc).  Counts are the exact ones
+// (the reference's 16s on tweets_20 are wrapped 4-bit counters).  Where the
+// reference asserts nothing (test_hashtable_overflow.cc:37, example_sift.cc),
+// every result slot is compared with the CPU oracle (oracle/oracle.h, linked
+// into this test binary as the checker).  Usage: test_api <golden dir> [test]
+#include "oracle.h"
+
 #include <genie/genie.h>
 #include <genie/original/interface.h>
 
@@ -174,6 +178,58 @@ static void test_multiple_search()
     delete[] table;
 }
 
+// The oracle's top-k of one query from its final ranges (Query::dump): the
+// exhaustive closed-form count over the raw rows (orc_count_exhaustive_single
+// / _biject), then the exact top-k under (count desc, id asc).
+static void oracle_topk_of(genie::query::Query& q, const vector<vector<int> >& rows, bool biject, int k,
+                           vector<int>& ids, vector<int>& counts)
+{
+    vector<genie::query::Query::range> rs;
+    q.dump(rs);
+    vector<orc_range> ors;
+    for (const auto& r : rs)
+        ors.push_back(orc_range{0, r.dim, r.low, r.up, r.weight, r.order});
+    const int32_t n = (int32_t)rows.size();
+    vector<uint32_t> c((size_t)n, 0);
+    if (biject) {
+        vector<int64_t> ptr(1, 0);
+        vector<int32_t> vals;
+        for (const auto& row : rows) {
+            vals.insert(vals.end(), row.begin(), row.end());
+            ptr.push_back((int64_t)vals.size());
+        }
+        orc_count_exhaustive_biject(ptr.data(), vals.data(), n, ors.data(), (int64_t)ors.size(), c.data());
+    } else {
+        const int32_t d = (int32_t)rows[0].size();
+        vector<int32_t> flat;
+        for (const auto& row : rows)
+            flat.insert(flat.end(), row.begin(), row.end());
+        orc_count_exhaustive_single(flat.data(), n, d, ors.data(), (int64_t)ors.size(), c.data());
+    }
+    ids.assign((size_t)k, 0);
+    counts.assign((size_t)k, 0);
+    orc_topk(c.data(), n, k, ids.data(), counts.data());
+}
+
+// Every slot of a knn result against the oracle, query by query (queries in
+// the order the search numbered them).
+static void check_against_oracle(vector<genie::query::Query>& qs, const vector<vector<int> >& rows, bool biject,
+                                 int k, const vector<int>& result, const vector<int>& count)
+{
+    CHECK(result.size() == qs.size() * (size_t)k && count.size() == result.size());
+    int bad = 0;
+    for (size_t q = 0; q < qs.size() && result.size() == qs.size() * (size_t)k; ++q) {
+        vector<int> ri, rc;
+        oracle_topk_of(qs[q], rows, biject, k, ri, rc);
+        for (int j = 0; j < k; ++j)
+            if (result[q * k + j] != ri[(size_t)j] || count[q * k + j] != rc[(size_t)j])
+                ++bad;
+    }
+    if (bad)
+        std::printf("  %d of %zu slots differ from the oracle\n", bad, result.size());
+    CHECK(bad == 0);
+}
+
 static void test_hashtable_overflow()
 {
     vector<vector<int> > data, queries;
@@ -192,6 +248,80 @@ static void test_hashtable_overflow()
     // every row matches itself with its full length
     for (int q = 0; q < 20; ++q)
         CHECK(count[(size_t)q * 10] >= (int)data[(size_t)q].size());
+    // and all 200 slots are the oracle's
+    vector<genie::query::Query> qs;
+    load_query(table[0], qs, config);
+    check_against_oracle(qs, data, true, 10, result, count);
+    delete[] table;
+}
+
+// example/example_sift.cc: sift_1k.csv (1,000 x 128) and the multirange query
+// file sift_1k_query.csv (qid, dim, value, selectivity 0.04, weight), adaptive
+// query range (query.cc:196-261) from radius 1, load balance with posting
+// lists of 256, k = 10.  The search's results against the oracle's counts of
+// the adapted ranges.
+static void test_sift_1k_multirange()
+{
+    vector<vector<int> > data, queries;
+    vector<genie::utility::attr_t> mq;
+    GPUGenie_Config config = base_config(data, queries, 128, 10);
+    config.num_of_queries = 500;
+    config.count_threshold = -128;
+    config.hashtable_size = 128 * 10 * 1.5f;
+    config.query_radius = 1;
+    config.use_adaptive_range = true;
+    config.selectivity = 0.2f;
+    config.multiplier = 1.5f;
+    config.posting_list_max_length = 256;
+    config.use_load_balance = true;
+    config.use_multirange = true;
+    genie::utility::read_file(data, (G + "/sift_1k.csv").c_str(), -1);
+    genie::utility::read_query(mq, (G + "/sift_1k_query.csv").c_str(), -1);
+    config.multirange_query_points = &mq;
+    CHECK(data.size() == 1000 && mq.size() == 12800);
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    vector<genie::query::Query> qs;
+    load_query(table[0], qs, config);
+    CHECK(qs.size() == 100);
+    // the adaptive range widened the ranges: some query covers more than one value somewhere
+    bool widened = false;
+    for (auto& q : qs) {
+        vector<genie::query::Query::range> rs;
+        q.dump(rs);
+        for (const auto& r : rs)
+            widened = widened || r.up - r.low > 2;
+    }
+    CHECK(widened);
+    check_against_oracle(qs, data, false, 10, result, count);
+    delete[] table;
+}
+
+// tweets_4k.csv (4,096 token lists) as a bijectMap table: 64 of its rows as
+// queries, k = 25, every slot against the oracle's multiset counts.
+static void test_tweets_4k_bijectmap()
+{
+    vector<vector<int> > data, queries;
+    GPUGenie_Config config = base_config(data, queries, 14, 25);
+    config.search_type = 1;
+    config.num_of_queries = 64;
+    genie::utility::read_file(data, (G + "/tweets_4k.csv").c_str(), -1);
+    genie::utility::read_file(queries, (G + "/tweets_4k.csv").c_str(), config.num_of_queries);
+    CHECK(data.size() == 4096 && queries.size() == 64);
+    init_genie(config);
+    inv_table* table = nullptr;
+    preprocess_for_knn_csv(config, table);
+    vector<int> result, count;
+    knn_search_after_preprocess(config, table, result, count);
+    vector<genie::query::Query> qs;
+    load_query(table[0], qs, config);
+    check_against_oracle(qs, data, true, 25, result, count);
+    // each query row finds itself with its full length at the top
+    for (int q = 0; q < 64; ++q)
+        CHECK(count[(size_t)q * 25] >= (int)queries[(size_t)q].size());
     delete[] table;
 }
 
@@ -514,6 +644,8 @@ int main(int argc, char** argv)
         {"wire", test_wire},
         {"adaptive_range", test_adaptive_range},
         {"multirange", test_multirange},
+        {"sift_1k_multirange", test_sift_1k_multirange},
+        {"tweets_4k_bijectmap", test_tweets_4k_bijectmap},
     };
     for (auto& kv : tests) {
         if (argc > 2 && kv.first != argv[2])

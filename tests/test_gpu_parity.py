@@ -232,3 +232,41 @@ def test_loaded_tables_answer_the_same(tmp_path):
         a = match_topk(t.device(), t.compile(ranges, nq), nq, 30)
         b = match_topk(u.device(), u.compile(ranges, nq), nq, 30)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), i
+
+
+@pytest.mark.parametrize("n,sparse", [(50_000, 0), (50_000, 1), (300_000, 0), (300_000, 1)])
+def test_adaptive_threshold_closed_form(n, sparse):
+    """updateThreshold's closed form on device output (match_device_utils.h:
+    449-473): the final threshold of a query is the largest k-th count of any
+    of its tiles that holds k objects counting at least 1.  With the whole
+    index in one tile (50,000 objects, tiles of 65,536) that is the query's
+    k-th largest count c_(k) -- the reference's thr - 1 = c_(k) -- and 0 for
+    a query with fewer than k matching objects."""
+    from genie_amd import Plan
+    w = W.cfg1(seed=977, n=n, d=20, q=24, k=60)
+    r = w["ranges"].copy()
+    r = r[~((r["query"] == 5) & (r["dim"] > 0))].copy()  # query 5: one exact value -> few matches
+    r["low"][r["query"] == 5] = r["up"][r["query"] == 5]
+    t = Table.from_rows(w["rows"])
+    items = t.compile(r, 24)
+    p = Plan(t.device(), 24, len(items), 60, 0, mode=0)
+    p.set_hash(0)
+    p.set_sparse(sparse)
+    p.upload(items, 24, np.full(24, 60, np.uint32))
+    p.run()
+    ids, cnt = p.download()
+    thr = p.thresholds(24)
+    tile = 65536  # direct-mapped tiles at 8-bit counters
+    for q in range(24):
+        c = O.count_exhaustive_single(w["rows"], r[r["query"] == q]).astype(np.int64)
+        want = 0
+        for t0 in range(0, n, tile):
+            ct = np.sort(c[t0:t0 + tile])[::-1]
+            if len(ct) >= 60 and ct[59] >= 1:
+                want = max(want, int(ct[59]))
+        assert thr[q] == want, (q, thr[q], want)
+        if n <= tile:
+            ck = int(np.sort(c)[::-1][59])
+            assert thr[q] == (ck if ck >= 1 else 0)
+            assert thr[q] == (cnt[q * 60 + 59] if np.count_nonzero(c) >= 60 else 0)
+    assert thr[5] == 0 or np.count_nonzero(O.count_exhaustive_single(w["rows"], r[r["query"] == 5])) >= 60
