@@ -6,7 +6,11 @@ one (bench.py measures 1M x 128, k = 100), with the batch already in HBM:
   lsh-compressed   the same index with SERIAL_DELTA_COMP_BP32_VARINT segments
   sets             2M Zipf(1.05) token sets over a 1M vocabulary, 16,384 weighted queries, mixed k
 
-    python genie_amd/measure.py lsh [--reps 5] [--forms plain-hash plain-direct ...]
+    python genie_amd/measure.py lsh [--reps 5] [--forms plain-hash plain-direct ...] [--merge-runs]
+
+Compressed workloads: the plain forms decode the batch's segments once into a
+full-layout scratch (the default); the *-kdecode forms leave the decode to
+the kernels.  --merge-runs compiles one item per range (its segments joined).
 
 Every form's results are compared with the first form's.  One line per form:
 ms per batch (run + download), queries/s, per-stage kernel times (HIP
@@ -21,20 +25,23 @@ import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-# form name -> (plan mode, hash form, sparse select, dense/set form, its threads per workgroup)
+# form name -> (plan mode, hash form, sparse select, dense/set form, its threads per workgroup,
+#               decode of a compressed index: -1 once per batch, 0 in the kernels)
 FORMS = {
-    "plain-hash": (0, 1, -1, True, 0),      # every query whose counts fit on the hash form
-    "plain-auto": (0, -1, -1, True, 0),     # the plain form's own per-query choice
-    "plain-direct": (0, 0, -1, True, 0),    # direct-mapped kernel, its own select choice
-    "plain-direct-dense": (0, 0, 0, True, 0),
-    "batched": (1, -1, -1, True, 0),        # dense (or set) form when the batch allows, else the lane form
-    "batched-512": (1, -1, -1, True, 512),  # the same with 512-thread workgroups
-    "batched-lanes": (1, -1, -1, False, 0),  # the lane form
-    "automatic": (2, -1, -1, True, 0),
+    "plain-hash": (0, 1, -1, True, 0, -1),      # every query whose counts fit on the hash form
+    "plain-auto": (0, -1, -1, True, 0, -1),     # the plain form's own per-query choice
+    "plain-direct": (0, 0, -1, True, 0, -1),    # direct-mapped kernel, its own select choice
+    "plain-direct-dense": (0, 0, 0, True, 0, -1),
+    "plain-auto-kdecode": (0, -1, -1, True, 0, 0),
+    "plain-direct-kdecode": (0, 0, -1, True, 0, 0),
+    "batched": (1, -1, -1, True, 0, -1),        # dense (or set) form when the batch allows, else the lane form
+    "batched-512": (1, -1, -1, True, 512, -1),  # the same with 512-thread workgroups
+    "batched-lanes": (1, -1, -1, False, 0, -1),  # the lane form
+    "automatic": (2, -1, -1, True, 0, -1),
 }
 
 
-def workload(name):
+def workload(name, merge_runs=False):
     from genie_amd import Table, workloads as W
     if name in ("zipf-compressed",):
         w = W.cfg1()
@@ -47,7 +54,7 @@ def workload(name):
         t = Table.from_sets(w["row_ptr"], w["vals"])
     else:
         raise SystemExit(f"unknown workload {name}")
-    items = t.compile(w["ranges"], w["n_queries"])
+    items = t.compile(w["ranges"], w["n_queries"], merge_runs=merge_runs)
     return t, items, w["n_queries"], w["k"]
 
 
@@ -56,6 +63,7 @@ def main():
     ap.add_argument("workload", choices=["zipf-compressed", "lsh", "lsh-compressed", "sets"])
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--forms", nargs="+", default=["plain-auto", "plain-direct", "automatic"], choices=list(FORMS))
+    ap.add_argument("--merge-runs", action="store_true", help="one item per range over a compressed index")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device copy of this size (scales FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -64,15 +72,16 @@ def main():
         from genie_amd._native import copy_probe
         copy_probe(args.copy_probe_mib << 20)
     t0 = time.time()
-    t, items, nq, k = workload(args.workload)
+    t, items, nq, k = workload(args.workload, args.merge_runs)
     print(f"{args.workload}: {nq} queries, {len(items)} items, built in {time.time() - t0:.1f} s", flush=True)
     kq = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.uint32), (nq,)))
     ref = None
     for f in args.forms:
-        mode, hm, sp, dn, nt = FORMS[f]
+        mode, hm, sp, dn, nt, dec = FORMS[f]
         p = Plan(t.device(), nq, len(items), int(kq.max()), 0, mode=mode)
         p.set_hash(hm)
         p.set_sparse(sp)
+        p.set_decode(dec)
         if mode:
             p.set_dense(dn)
             p.set_dense_threads(nt)
@@ -92,7 +101,8 @@ def main():
                 stages[s].append(p.kernel_ms(s))
         dt = (time.perf_counter() - t1) / args.reps
         ks = {s: round(float(np.mean(v)), 3) for s, v in stages.items()}
-        extra = f"dense={p.dense_active()}" if mode else f"hash_queries={p.hash_queries()} sparse={p.sparse_active()}"
+        extra = (f"form={p.form()}" if mode else
+                 f"hash_queries={p.hash_queries()} sparse={p.sparse_active()} decode_once={p.decode_active()}")
         print(f"{args.workload} {f}: {dt * 1e3:.2f} ms/batch, {nq / dt:.0f} queries/s, {extra}, kernels {ks}, "
               f"identical={same}", flush=True)
 
