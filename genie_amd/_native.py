@@ -113,6 +113,8 @@ _sig(_hip, "genie_hip_device_synchronize", C.c_int)
 _sig(_hip, "genie_hip_copy_probe", C.c_int, _u64)
 _sig(_hip, "genie_hip_plan_estimate", C.c_int, _p, _u32, _u64, _u32, C.c_int, C.POINTER(_u64))
 _sig(_hip, "genie_hip_mem_info", C.c_int, C.POINTER(_u64), C.POINTER(_u64))
+_sig(_hip, "genie_hip_match_topk_static", C.c_int, _p, _p, _u64, _u32, _p, _u32, _p, _p, _p)
+_sig(_hip, "genie_hip_static_scratch_bytes", C.c_int, C.POINTER(_u64))
 
 
 def decode(codec, words, cpos, stride=1024):
@@ -274,6 +276,26 @@ def tables_knn_mt(tables, ranges, n_queries, k, budget=0):
         out.append((ids[off:off + int(q) * k].copy(), cnt[off:off + int(q) * k].copy()))
         off += int(q) * k
     return out
+
+
+def match_topk_static(index, items, n_queries, k, count_bound=0, stream=None):
+    """match_topk without device allocation (genie_hip_match_topk_static):
+    every device buffer lies in the library's static scratch; a batch that
+    does not fit it raises GenieError."""
+    items = _c(items, DIM_DTYPE)
+    kq = _c(np.broadcast_to(np.asarray(k, dtype=np.uint32), (n_queries,)), np.uint32)
+    tot = int(kq.sum())
+    ids = np.zeros(max(1, tot), np.int32)
+    cnt = np.zeros(max(1, tot), np.int32)
+    _chk_hip(_hip.genie_hip_match_topk_static(index.handle, _ptr(items), len(items), n_queries, _ptr(kq),
+                                              count_bound, _ptr(ids), _ptr(cnt), stream))
+    return ids[:tot], cnt[:tot]
+
+
+def static_scratch_bytes():
+    v = _u64()
+    _chk_hip(_hip.genie_hip_static_scratch_bytes(C.byref(v)))
+    return int(v.value)
 
 
 def match_topk(index, items, n_queries, k, count_bound=0):

@@ -22,6 +22,7 @@
 #include <thread>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -1101,6 +1102,8 @@ static int build_sets(const genie_index_desc* desc, DevIndex& d)
     return GENIE_OK;
 }
 
+static int static_scratch_init();
+
 extern "C" {
 
 const char* genie_hip_last_error(void) { return g_err.c_str(); }
@@ -1175,7 +1178,8 @@ int genie_hip_init(int device)
     if (device < 0 || device >= n)
         device = 0;  // Init falls back to device 0 (init.cu:21-28)
     HIP_TRY(hipSetDevice(device));
-    return GENIE_OK;
+    // set-up of the device's static scratch (genie_hip_match_topk_static)
+    return static_scratch_init();
 }
 
 int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
@@ -2349,16 +2353,16 @@ static bool plain_sparse(const genie_plan* p, uint32_t n_tiles)
 }
 
 template <bool SPARSE>
-static void launch_scan_t(const genie_plan* p, const ScanArgs& a, uint32_t blocks, hipStream_t s)
+static void launch_scan_t(uint32_t lane_bits, const ScanArgs& a, uint32_t blocks, hipStream_t s)
 {
     if (a.codec == 0) {
-        switch (p->lane_bits) {
+        switch (lane_bits) {
         case 8: k_scan_tile<8, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
         case 16: k_scan_tile<16, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
         default: k_scan_tile<32, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
         }
     } else {
-        switch (p->lane_bits) {
+        switch (lane_bits) {
         case 8: k_scan_tile_c<8, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
         case 16: k_scan_tile_c<16, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
         default: k_scan_tile_c<32, SPARSE><<<blocks, kScanThreads, 0, s>>>(a); break;
@@ -2417,9 +2421,9 @@ static int launch_scan(genie_plan* p, hipStream_t s)
         a.qlist = ndq == p->n_queries ? nullptr : p->d_dq;
         a.n_qlist = (uint32_t)ndq;
         if (plain_sparse(p, a.n_tiles))
-            launch_scan_t<true>(p, a, (uint32_t)blocks, s);
+            launch_scan_t<true>(p->lane_bits, a, (uint32_t)blocks, s);
         else
-            launch_scan_t<false>(p, a, (uint32_t)blocks, s);
+            launch_scan_t<false>(p->lane_bits, a, (uint32_t)blocks, s);
     }
     return GENIE_OK;
 }
@@ -3248,6 +3252,329 @@ int genie_hip_decode(int codec, const uint32_t* words, const uint32_t* cpos, uin
     dev_free(d_o);
     dev_free(d_l);
     return rc;
+}
+
+}  // extern "C"
+
+// ---- no-allocation entry point ----------------------------------------------
+// genie_hip_match_topk_static: the plain per-batch search (direct-mapped
+// kernel, no hash form) with every device buffer carved out of one
+// file-scope array instead of hipMalloc, for callers that cannot allocate.
+// Each GPU the library runs on holds its own copy of the array until the
+// process ends.
+
+constexpr uint64_t kStaticScratchBytes = 32ull << 20;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+
+struct StaticScratch {
+    std::mutex mu;          // calls on one device run one at a time
+    uint8_t* ptr = nullptr; // this device's copy of `scratch`
+};
+std::mutex g_static_mu;
+std::vector<std::unique_ptr<StaticScratch> > g_static;  // by device id
+
+// The device buffers of one call, all inside the array.
+struct StaticBufs {
+    genie_dim_t* d_items;
+    uint32_t *d_item_qptr, *d_nchunks, *d_chunk_ptr, *d_block_sums, *d_total;
+    CChunk* d_chunks;
+    uint32_t *d_nk, *d_kptr, *d_chunk_sums, *d_qc_ptr, *d_qk_ptr, *d_total_pairs;
+    uint32_t *d_kq, *d_gthr, *d_cand_cnt, *d_cand_cap;
+    uint64_t *d_cand_base, *d_cand;
+    uint32_t *d_overflow, *d_hov_any, *d_flag_any;
+    uint64_t* d_out_off;
+    int32_t *d_out_ids, *d_out_counts;
+};
+
+// Bump allocation inside the array (256-byte aligned pieces); with a null
+// base it only measures.
+struct Carve {
+    uint8_t* base = nullptr;
+    uint64_t off = 0;
+    template <class T>
+    T* take(uint64_t n)
+    {
+        off = (off + 255) & ~255ull;
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += std::max<uint64_t>(n, 1) * sizeof(T);
+        return p;
+    }
+};
+
+}  // namespace
+
+// This device's address of `scratch`, looked up once per device (with the
+// device current); host code uses it, never the array's name.
+static int static_scratch(StaticScratch** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if ((size_t)dev >= g_static.size())
+        g_static.resize((size_t)dev + 1);
+    if (!g_static[(size_t)dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GENIE_E_RUNTIME, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        auto s = std::make_unique<StaticScratch>();
+        s->ptr = static_cast<uint8_t*>(p);
+        g_static[(size_t)dev] = std::move(s);
+    }
+    *out = g_static[(size_t)dev].get();
+    return GENIE_OK;
+}
+
+static int static_scratch_init()
+{
+    StaticScratch* ss = nullptr;
+    return static_scratch(&ss);
+}
+
+extern "C" {
+
+int genie_hip_static_scratch_bytes(uint64_t* bytes)
+{
+    if (!bytes)
+        return fail(GENIE_E_INVALID, "null argument");
+    *bytes = kStaticScratchBytes;
+    return GENIE_OK;
+}
+
+int genie_hip_match_topk_static(genie_index* ix, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                                const uint32_t* k_per_query, uint32_t count_bound, int32_t* out_ids,
+                                int32_t* out_counts, void* stream)
+{
+    if (!ix || (!items && n_items) || !k_per_query || !out_ids || !out_counts)
+        return fail(GENIE_E_INVALID, "null argument");
+    if (n_queries == 0)
+        return GENIE_OK;
+    const DevIndex& d = ix->d;
+    for (uint32_t q = 0; q < n_queries; ++q)
+        if (k_per_query[q] == 0 || k_per_query[q] > GENIE_MAX_K)
+            return fail(GENIE_E_INVALID, "k of query " + std::to_string(q) + " outside [1, GENIE_MAX_K]");
+    if (int rc = validate_items(d, items, n_items, n_queries))
+        return rc;
+    if (n_items >= 0xFFFFFFFFull)
+        return fail(GENIE_E_INVALID, "too many work items");
+    // items grouped by query (stable), CSR pointer
+    const uint32_t nq = n_queries;
+    std::vector<uint32_t> qptr(nq + 1, 0);
+    for (uint64_t i = 0; i < n_items; ++i)
+        qptr[items[i].query + 1]++;
+    for (uint32_t q = 0; q < nq; ++q)
+        qptr[q + 1] += qptr[q];
+    std::vector<genie_dim_t> sorted(n_items);
+    {
+        std::vector<uint32_t> fillp(qptr.begin(), qptr.end() - 1);
+        for (uint64_t i = 0; i < n_items; ++i)
+            sorted[fillp[items[i].query]++] = items[i];
+    }
+    uint32_t bound = count_bound;
+    if (bound == 0) {
+        std::vector<uint32_t> qb(nq);
+        compute_bounds(d, sorted.data(), n_items, nq, qb.data());
+        for (uint32_t v : qb)
+            bound = std::max(bound, v);
+    }
+    const uint32_t lane_bits = bits_for(bound);
+    // chunks, as the compile kernels will write them
+    uint64_t nch = 0, span_total = 0;
+    for (uint64_t i = 0; i < n_items; ++i) {
+        const genie_dim_t& it = sorted[i];
+        if (it.end_pos <= it.start_pos)
+            continue;
+        const uint64_t len = (uint64_t)(it.end_pos - it.start_pos);
+        span_total += len;
+        if (d.codec == 0) {
+            nch += (len + kChunk - 1) / kChunk;
+        } else {
+            const genie_dim_t* pv = i ? &sorted[i - 1] : nullptr;
+            const bool starts = !pv || pv->end_pos <= pv->start_pos || pv->query != it.query ||
+                                pv->weight != it.weight || pv->end_pos != it.start_pos;
+            nch += starts ? 1 : 0;
+        }
+    }
+    // candidate space: the direct-mapped select emits at most k per tile and
+    // query, so this space never overflows
+    const uint32_t tile_objs = tile_objs_for(lane_bits);
+    const uint64_t n_tiles = (d.n_objects + tile_objs - 1) / tile_objs;
+    if (n_tiles * nq >= 0x7FFFFFFFull)
+        return fail(GENIE_E_INVALID, "batch too large for one launch");
+    std::vector<uint32_t> cap(nq);
+    std::vector<uint64_t> cbase(nq), out_off(nq + 1, 0);
+    uint64_t cand_total = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint64_t c = n_tiles * k_per_query[q];
+        if (c >= 0xFFFFFFFFull)
+            return fail(GENIE_E_INVALID, "candidate space of a query too large");
+        cap[q] = (uint32_t)c;
+        cbase[q] = cand_total;
+        cand_total += c;
+        out_off[q + 1] = out_off[q] + k_per_query[q];
+    }
+    const uint64_t total_k = out_off[nq];
+    const uint64_t nb = (n_items + 1023) / 1024 + 1, ncb = (nch + 1023) / 1024 + 1;
+    // the layout, measured first, then carved out of the array
+    auto layout = [&](Carve& c, StaticBufs& v) {
+        v.d_items = c.take<genie_dim_t>(n_items);
+        v.d_item_qptr = c.take<uint32_t>(nq + 1);
+        v.d_nchunks = c.take<uint32_t>(n_items);
+        v.d_chunk_ptr = c.take<uint32_t>(n_items);
+        v.d_block_sums = c.take<uint32_t>(nb);
+        v.d_total = c.take<uint32_t>(1);
+        v.d_chunks = c.take<CChunk>(nch);
+        v.d_nk = c.take<uint32_t>(nch);
+        v.d_kptr = c.take<uint32_t>(nch);
+        v.d_chunk_sums = c.take<uint32_t>(ncb);
+        v.d_qc_ptr = c.take<uint32_t>(nq + 1);
+        v.d_qk_ptr = c.take<uint32_t>(nq + 1);
+        v.d_total_pairs = c.take<uint32_t>(1);
+        v.d_kq = c.take<uint32_t>(nq);
+        v.d_gthr = c.take<uint32_t>(nq);
+        v.d_cand_cnt = c.take<uint32_t>(nq);
+        v.d_cand_cap = c.take<uint32_t>(nq);
+        v.d_cand_base = c.take<uint64_t>(nq);
+        v.d_cand = c.take<uint64_t>(cand_total);
+        v.d_overflow = c.take<uint32_t>(1);
+        v.d_hov_any = c.take<uint32_t>(1);
+        v.d_flag_any = c.take<uint32_t>(1);
+        v.d_out_off = c.take<uint64_t>(nq + 1);
+        v.d_out_ids = c.take<int32_t>(total_k);
+        v.d_out_counts = c.take<int32_t>(total_k);
+    };
+    StaticBufs v{};
+    Carve probe;
+    layout(probe, v);
+    if (probe.off > kStaticScratchBytes)
+        return fail(GENIE_E_INVALID, "batch needs " + std::to_string(probe.off) + " bytes of device scratch; the static "
+                                     "scratch holds " + std::to_string(kStaticScratchBytes) +
+                                     " (genie_hip_match_topk allocates what a batch needs)");
+    StaticScratch* ss = nullptr;
+    if (int rc = static_scratch(&ss))
+        return rc;
+    std::lock_guard<std::mutex> lk(ss->mu);
+    Carve c;
+    c.base = ss->ptr;
+    layout(c, v);
+    hipStream_t s = (hipStream_t)stream;
+    if (n_items)
+        HIP_TRY(hipMemcpyAsync(v.d_items, sorted.data(), n_items * sizeof(genie_dim_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.d_item_qptr, qptr.data(), (nq + 1) * 4ull, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.d_kq, k_per_query, nq * 4ull, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.d_cand_cap, cap.data(), nq * 4ull, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.d_cand_base, cbase.data(), nq * 8ull, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(v.d_out_off, out_off.data(), (nq + 1) * 8ull, hipMemcpyHostToDevice, s));
+    // what must start at zero: the array keeps the previous call's contents
+    {
+        ZeroList z{};
+        z.p[0] = v.d_gthr, z.n[0] = nq;
+        z.p[1] = v.d_cand_cnt, z.n[1] = nq;
+        z.p[2] = v.d_overflow, z.n[2] = 1;
+        z.p[3] = v.d_hov_any, z.n[3] = 1;
+        k_zero_many<<<(uint32_t)std::min<uint64_t>((nq + 255) / 256, 1024), 256, 0, s>>>(z);
+    }
+    const uint32_t ni = (uint32_t)n_items;
+    if (ni) {
+        const uint32_t nbk = (ni + 1023) / 1024;
+        if (d.codec == 0)
+            k_chunk_count<<<(ni + 255) / 256, 256, 0, s>>>(v.d_items, ni, v.d_nchunks);
+        else
+            k_crun_flags<<<(ni + 255) / 256, 256, 0, s>>>(v.d_items, ni, d.seg_cpos, d.n_segments, v.d_nchunks);
+        k_scan_blocks<<<nbk, 256, 0, s>>>(v.d_nchunks, v.d_chunk_ptr, ni, v.d_block_sums);
+        k_scan_sums<<<1, 256, 0, s>>>(v.d_block_sums, nbk, v.d_total);
+        k_scan_add<<<nbk, 256, 0, s>>>(v.d_chunk_ptr, ni, v.d_block_sums);
+        if (d.codec == 0)
+            k_chunk_write<<<(ni + 255) / 256, 256, 0, s>>>(v.d_items, ni, v.d_chunk_ptr, d.key_start, d.n_keys,
+                                                          v.d_chunks);
+        else
+            k_crun_write<<<(ni + 255) / 256, 256, 0, s>>>(v.d_items, ni, d.seg_cpos, d.n_segments, d.seg_key,
+                                                         v.d_chunk_ptr, v.d_chunks);
+    } else {
+        fill_u32(v.d_total, 0, 1, s);
+    }
+    k_query_chunk_ptr<<<(nq + 1 + 255) / 256, 256, 0, s>>>(v.d_item_qptr, nq, v.d_chunk_ptr, ni, v.d_total,
+                                                          v.d_qc_ptr);
+    if (nch) {
+        const uint32_t ncu = (uint32_t)nch;
+        const uint32_t g = std::min<uint32_t>((ncu + 255) / 256, 4096);
+        k_chunk_nkeys<<<g, 256, 0, s>>>(v.d_chunks, v.d_total, v.d_nk);
+        const uint32_t nbk = (ncu + 1023) / 1024;
+        k_scan_blocks<<<nbk, 256, 0, s>>>(v.d_nk, v.d_kptr, ncu, v.d_chunk_sums);
+        k_scan_sums<<<1, 256, 0, s>>>(v.d_chunk_sums, nbk, v.d_total_pairs);
+        k_scan_add<<<nbk, 256, 0, s>>>(v.d_kptr, ncu, v.d_chunk_sums);
+        k_chunk_set_kp<<<g, 256, 0, s>>>(v.d_chunks, v.d_total, v.d_kptr);
+    } else {
+        fill_u32(v.d_total_pairs, 0, 1, s);
+    }
+    k_query_pair_ptr<<<(nq + 1 + 255) / 256, 256, 0, s>>>(v.d_qc_ptr, nq, v.d_kptr, v.d_total, v.d_total_pairs,
+                                                         v.d_qk_ptr);
+    // the direct-mapped kernel over every query
+    ScanArgs a{};
+    a.codec = d.codec;
+    a.words = d.words;
+    a.seg_cpos = d.seg_cpos;
+    a.seg_first = d.seg_first;
+    a.seg_last = d.seg_last;
+    a.key_seg = d.key_seg;
+    a.sdir_off = d.sdir_off;
+    a.sdir = d.sdir;
+    a.post = d.post;
+    a.key_start = d.key_start;
+    a.dir_off = d.dir_off;
+    a.dir = d.dir;
+    a.chunks = v.d_chunks;
+    a.qc_ptr = v.d_qc_ptr;
+    a.qk_ptr = v.d_qk_ptr;
+    a.kq = v.d_kq;
+    a.qlist = nullptr;
+    a.n_qlist = nq;
+    a.n_objects = d.n_objects;
+    a.n_queries = nq;
+    a.tile_objs = tile_objs;
+    a.n_tiles = (uint32_t)n_tiles;
+    a.dt_per_tile = tile_objs / kDirTile;
+    a.n_dtiles = d.n_dtiles;
+    a.gthr = v.d_gthr;
+    a.cand_cnt = v.d_cand_cnt;
+    a.cand_cap = v.d_cand_cap;
+    a.cand_base = v.d_cand_base;
+    a.cand = v.d_cand;
+    a.overflow = v.d_overflow;
+    const uint64_t est = d.codec == 0 || d.n_words == 0
+                             ? span_total
+                             : (uint64_t)((double)span_total * (double)d.n_postings / (double)d.n_words);
+    const bool sparse = (double)est / ((double)nq * (double)n_tiles) <= (double)kCntWords / 4;
+    const uint32_t blocks = (uint32_t)(n_tiles * nq);
+    if (sparse)
+        launch_scan_t<true>(lane_bits, a, blocks, s);
+    else
+        launch_scan_t<false>(lane_bits, a, blocks, s);
+    MergeArgs m;
+    m.cand = v.d_cand;
+    m.cand_base = v.d_cand_base;
+    m.cand_cnt = v.d_cand_cnt;
+    m.cand_cap = v.d_cand_cap;
+    m.kq = v.d_kq;
+    m.out_off = v.d_out_off;
+    m.out_ids = v.d_out_ids;
+    m.out_counts = v.d_out_counts;
+    m.key_bits = 32 + lane_bits;
+    m.flag_a = v.d_overflow;
+    m.flag_b = v.d_hov_any;
+    m.flag_any = v.d_flag_any;
+    k_merge<<<nq, kMergeThreads, 0, s>>>(m);
+    HIP_TRY(hipGetLastError());
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpyAsync(out_ids, v.d_out_ids, total_k * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(out_counts, v.d_out_counts, total_k * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&ovf, v.d_flag_any, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (ovf)
+        return fail(GENIE_E_RUNTIME, "candidate overflow in the static scratch");
+    return GENIE_OK;
 }
 
 }  // extern "C"

@@ -299,6 +299,28 @@ int genie_hip_server_phase_seconds(genie_server* s, double* out);
  * Measurement only (no counterpart in the reference). */
 int genie_hip_server_phase_max_seconds(genie_server* s, double* out, int reset);
 
+/* ---- no-allocation entry point ------------------------------------------ */
+/* genie_hip_match_topk without device allocation, for callers that cannot
+ * allocate (code that must not call hipMalloc: tight serving loops,
+ * harnesses that forbid it).  No counterpart in the reference.  Same
+ * arguments and results as genie_hip_match_topk (host arrays in and out,
+ * returns when the results are copied), run on `stream` (a hipStream_t,
+ * NULL = default stream) as the plain form's direct-mapped kernel over every
+ * query (no hash form, compressed segments decoded by the kernel).  Every
+ * device buffer is carved out of one static array in the library's code
+ * object, genie_hip_static_scratch_bytes() bytes per GPU (32 MiB, held by
+ * each GPU the library runs on until the process ends); a batch that needs
+ * more is refused with GENIE_E_INVALID.  Calls on one GPU share that array:
+ * the library runs them one at a time within a process, and the array keeps
+ * the last call's contents; calls from several processes on one GPU each
+ * have their own copy.  The device address of the array is looked up at
+ * genie_hip_init (or at the first call on a device). */
+int genie_hip_match_topk_static(genie_index* ix, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                                const uint32_t* k_per_query, uint32_t count_bound, int32_t* out_ids,
+                                int32_t* out_counts, void* stream);
+/* Size of the static scratch of genie_hip_match_topk_static, bytes. */
+int genie_hip_static_scratch_bytes(uint64_t* bytes);
+
 /* ---- codec decode (tests and measurement) ------------------------------ */
 /* Decode n_arrays compressed arrays ([cpos[i], cpos[i+1]) words of `words`)
  * on the device with the engine's wave decoders; output i goes to
