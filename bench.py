@@ -247,6 +247,24 @@ def main():
             f"{ent['identical_to_uncompressed']} ({time.time() - t0:.1f}s)")
         del cidx, ctab, citems
 
+    if not args.no_workloads:
+        # ---- the same batch with weights 1-4 (the dense form takes unit
+        # weights only): plain and batched forms
+        t0 = time.time()
+        wr = w["ranges"].copy()
+        wr["weight"] = genie_amd.synth.uniform_int(1011, 0, len(wr), 1, 4)
+        witems = table.compile(wr, nq)
+        fw, rwp, rwb = plain_and_batched(index, witems, nq, kq, max(5, args.steps // 2), 2, "1M x 128 weighted")
+        qsw = sample_of(nq, 64)
+        owi, owc = oracle_sample(table.export()["inv"].view(np.uint32), witems, w["rows"].shape[0], kq, qsw)
+        fw["bit_exact_vs_oracle_sample"] = bool(all(np.array_equal(a, b) for a, b in
+                                                    zip(rows_of(rwb, kq, qsw) + rows_of(rwp, kq, qsw),
+                                                        (owi, owc, owi, owc))))
+        workloads["cfg1_weighted"] = dict(workload="the headline batch with weights uniform in {1,2,3,4}",
+                                          sample=len(qsw), **fw)
+        log(f"[bench] weighted: oracle sample exact {fw['bit_exact_vs_oracle_sample']} ({time.time() - t0:.1f}s)")
+        del witems
+
     words1 = table.export()["inv"].view(np.uint32)
     qs1 = sample_of(nq, args.sample)
     oi, oc = oracle_sample(words1, items[np.argsort(items["query"], kind="stable")], w["rows"].shape[0], kq, qs1)

@@ -114,6 +114,7 @@ _sig(_hip, "genie_hip_copy_probe", C.c_int, _u64)
 _sig(_hip, "genie_hip_plan_estimate", C.c_int, _p, _u32, _u64, _u32, C.c_int, C.POINTER(_u64))
 _sig(_hip, "genie_hip_mem_info", C.c_int, C.POINTER(_u64), C.POINTER(_u64))
 _sig(_hip, "genie_hip_match_topk_static", C.c_int, _p, _p, _u64, _u32, _p, _u32, _p, _p, _p)
+_sig(_hip, "genie_hip_match_topk_stream", C.c_int, _p, _p, _u64, _u32, _p, _u32, _p, _p, _p)
 _sig(_hip, "genie_hip_static_scratch_bytes", C.c_int, C.POINTER(_u64))
 
 
@@ -298,16 +299,22 @@ def static_scratch_bytes():
     return int(v.value)
 
 
-def match_topk(index, items, n_queries, k, count_bound=0):
+def match_topk(index, items, n_queries, k, count_bound=0, stream=None):
     """Plain per-batch search through the C ABI; k is an int or per-query array.
-    Returns (ids, counts) as flat int32 arrays, query-major."""
+    Returns (ids, counts) as flat int32 arrays, query-major.  stream (a
+    hipStream_t handle): the search is ordered on it
+    (genie_hip_match_topk_stream)."""
     items = _c(items, DIM_DTYPE)
     kq = _c(np.broadcast_to(np.asarray(k, dtype=np.uint32), (n_queries,)), np.uint32)
     tot = int(kq.sum())
     ids = np.zeros(tot, np.int32)
     cnt = np.zeros(tot, np.int32)
-    _chk_hip(_hip.genie_hip_match_topk(index.handle, _ptr(items), len(items), n_queries, _ptr(kq),
-                                       count_bound, _ptr(ids), _ptr(cnt)))
+    if stream is None:
+        _chk_hip(_hip.genie_hip_match_topk(index.handle, _ptr(items), len(items), n_queries, _ptr(kq),
+                                           count_bound, _ptr(ids), _ptr(cnt)))
+    else:
+        _chk_hip(_hip.genie_hip_match_topk_stream(index.handle, _ptr(items), len(items), n_queries, _ptr(kq),
+                                                  count_bound, _ptr(ids), _ptr(cnt), stream))
     return ids, cnt
 
 

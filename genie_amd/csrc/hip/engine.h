@@ -100,18 +100,20 @@ struct DevIndex {
 // Candidates per query the dense and set forms' host total holds at least
 // (a query whose counts crowd at its threshold asks for its ties).
 constexpr uint64_t kDenseCapFloor = 16384;
-// Initial factor of k that a plan gives each query's candidate space: the
-// dense and set forms, then the tile forms (plain and lane).
+// Initial factor of k that a plan gives each query's candidate space in the
+// dense and set forms; the tile forms (plain and lane) take k per tile.
 constexpr uint32_t kDenseCapFactor = 64;
-constexpr uint32_t kTileCapFactor = 8;
+// Fewest objects per tile of the tile forms (32-bit counters).
+constexpr uint32_t kMinTileObjs = 16384;
 
 // Candidate keys a new plan first gives one query with this k (the sizing of
 // plan_size_candidates before any overflow): the dense and set forms hold
-// max(64 k, kDenseCapFloor), at most one per object; the tile forms 8 k.
+// max(64 k, kDenseCapFloor), at most one per object; the tile forms k per
+// tile, counted here at the smallest tile size.
 inline uint64_t initial_candidates(uint32_t k, bool dense, uint32_t n_objects)
 {
     if (!dense)
-        return (uint64_t)kTileCapFactor * k;
+        return (uint64_t)((n_objects + kMinTileObjs - 1) / kMinTileObjs) * k;
     const uint64_t c = (uint64_t)kDenseCapFactor * k;
     return std::min<uint64_t>(c > kDenseCapFloor ? c : kDenseCapFloor, n_objects);
 }

@@ -1548,7 +1548,14 @@ static uint32_t max_cap_factor(const genie_plan* p)
     return (p->ix->d.n_objects + kmin - 1) / kmin;
 }
 
-static uint32_t initial_cap_factor(const genie_plan* p) { return dense_active(p) ? kDenseCapFactor : kTileCapFactor; }
+// The tile forms start with the most the batch can emit (k per tile and
+// query): on a large index a batch of a few queries runs every tile at once,
+// and each emits its k before any threshold is known, so a smaller space
+// overflowed on almost every batch and ran it again.
+static uint32_t initial_cap_factor(const genie_plan* p)
+{
+    return dense_active(p) ? kDenseCapFactor : max_cap_factor(p);
+}
 
 // Smallest dense_mult at which every device-side ask is the whole index: an
 // ask is at least 1,024 x dense_mult before it is capped at n_objects
@@ -3092,8 +3099,10 @@ int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32
         p->ms_scan = b;
         p->ms_merge = c;
     }
-    HIP_TRY(hipMemcpy(out_ids, p->d_out_ids, p->total_k * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(out_counts, p->d_out_counts, p->total_k * 4, hipMemcpyDeviceToHost));
+    // on the run's stream; the buffers may be host or device memory
+    HIP_TRY(hipMemcpyAsync(out_ids, p->d_out_ids, p->total_k * 4, hipMemcpyDefault, s));
+    HIP_TRY(hipMemcpyAsync(out_counts, p->d_out_counts, p->total_k * 4, hipMemcpyDefault, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return GENIE_OK;
 }
 
@@ -3191,6 +3200,14 @@ int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_i
                          const uint32_t* k_per_query, uint32_t count_bound, int32_t* out_ids,
                          int32_t* out_counts)
 {
+    return genie_hip_match_topk_stream(ix, items, n_items, n_queries, k_per_query, count_bound, out_ids, out_counts,
+                                       nullptr);
+}
+
+int genie_hip_match_topk_stream(genie_index* ix, const genie_dim_t* items, uint64_t n_items, uint32_t n_queries,
+                                const uint32_t* k_per_query, uint32_t count_bound, int32_t* out_ids,
+                                int32_t* out_counts, void* stream)
+{
     if (!ix || (!items && n_items) || !k_per_query || !out_ids || !out_counts)
         return fail(GENIE_E_INVALID, "null argument");
     if (n_queries == 0)
@@ -3205,8 +3222,8 @@ int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_i
     if (rc)
         return rc;
     if (!(rc = genie_hip_plan_upload(p, items, n_items, n_queries, k_per_query)) &&
-        !(rc = genie_hip_plan_run(p, nullptr)))
-        rc = genie_hip_plan_download(p, nullptr, out_ids, out_counts);
+        !(rc = genie_hip_plan_run(p, stream)))
+        rc = genie_hip_plan_download(p, stream, out_ids, out_counts);
     genie_hip_plan_destroy(p);
     return rc;
 }

@@ -101,6 +101,16 @@ int genie_hip_count_bounds(const genie_index* ix, const genie_dim_t* items, uint
 int genie_hip_match_topk(genie_index* ix, const genie_dim_t* items, uint64_t n_items,
                          uint32_t n_queries, const uint32_t* k_per_query, uint32_t count_bound,
                          int32_t* out_ids, int32_t* out_counts);
+/* The same search ordered on the caller's stream (a hipStream_t, NULL = the
+ * default stream): its copies and kernels are issued on `stream` after the
+ * caller's earlier work there, and the call returns once the results are in
+ * out_ids / out_counts (host or device memory), waiting for `stream` only.
+ * genie_hip_match_topk is this call on the default stream.  For a call that
+ * returns before the results are ready, use a plan: genie_hip_plan_run and
+ * genie_hip_plan_download_async on the caller's stream. */
+int genie_hip_match_topk_stream(genie_index* ix, const genie_dim_t* items, uint64_t n_items,
+                                uint32_t n_queries, const uint32_t* k_per_query, uint32_t count_bound,
+                                int32_t* out_ids, int32_t* out_counts, void* stream);
 
 /* ---- device-resident form (inputs already in HBM) ---------------------- */
 /* A plan holds device scratch for batches up to the given shape, so the hot
@@ -128,7 +138,8 @@ int genie_hip_plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_it
 /* Run the search on the uploaded batch on `stream` (a hipStream_t, NULL =
  * default stream); asynchronous. */
 int genie_hip_plan_run(genie_plan* p, void* stream);
-/* Copy the results of the last run to host buffers (synchronises stream). */
+/* Copy the results of the last run to the caller's buffers (host or device
+ * memory), on `stream`, and wait for them (synchronises stream). */
 int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32_t* out_counts);
 /* Per-stage timing of the last run when timing is on (HIP events recorded
  * on the run's stream): "compile" (item -> chunk kernels; query masks for

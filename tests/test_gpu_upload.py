@@ -122,3 +122,31 @@ def test_failed_upload_leaves_no_batch(mode):
     assert np.array_equal(first[0], again[0]) and np.array_equal(first[1], again[1])
     ri, rc = expect(rows, r, 12, 50)
     assert_topk_equal(again[0], again[1], ri, rc, 50, 12, "after a failed upload")
+
+
+def test_match_topk_on_a_user_stream_into_device_buffers():
+    """genie_hip_match_topk_stream: ordered on the caller's stream, results
+    into host buffers and into device buffers (copied back here), equal to
+    the oracle's."""
+    from genie_amd._native import _hip as H, _chk_hip, DIM_DTYPE
+    rows, r = cfg(80_000, 24, 20, 941)
+    t = Table.from_rows(rows)
+    items = np.ascontiguousarray(t.compile(r, 20), DIM_DTYPE)
+    ri, rc = expect(rows, r, 20, 50)
+    hip = C.CDLL("libamdhip64.so.7")
+    sh = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(sh), 1) == 0
+    ids, cnt = match_topk(t.device(), items, 20, 50, stream=sh.value)
+    assert_topk_equal(ids, cnt, ri, rc, 50, 20, "user stream, host buffers")
+    d_ids, d_cnt = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d_ids), C.c_size_t(20 * 50 * 4)) == 0
+    assert hip.hipMalloc(C.byref(d_cnt), C.c_size_t(20 * 50 * 4)) == 0
+    kq = np.full(20, 50, np.uint32)
+    _chk_hip(H.genie_hip_match_topk_stream(t.device().handle, items.ctypes.data, len(items), 20, kq.ctypes.data, 0,
+                                           d_ids.value, d_cnt.value, sh.value))
+    gi, gc = np.zeros(1000, np.int32), np.zeros(1000, np.int32)
+    assert hip.hipMemcpy(C.c_void_p(gi.ctypes.data), d_ids, C.c_size_t(4000), 2) == 0  # device to host
+    assert hip.hipMemcpy(C.c_void_p(gc.ctypes.data), d_cnt, C.c_size_t(4000), 2) == 0
+    assert_topk_equal(gi, gc, ri, rc, 50, 20, "user stream, device buffers")
+    assert hip.hipFree(d_ids) == 0 and hip.hipFree(d_cnt) == 0
+    assert hip.hipStreamDestroy(sh) == 0
