@@ -686,35 +686,35 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 shifts |= (uint64_t)min(a.hshift[g * 32 + j], 3u) << (2 * j);
         }
         uint32_t tp[NP] = {};
-        for (uint32_t li = t_first + w0; li < min(n_list, w0 + win); li += t_step) {
-            const uint32_t t = a.list ? a.list[li] : li;
-            if constexpr (!SAMPLE) {
-                // threshold planes of the group (bit j of plane b = bit b of
-                // query j's threshold; NONE for lanes past the last query)
-                __syncthreads();
-                if (tid < 32)
-                    s_qcnt[tid] = 0;
-                if (tid == 0)
-                    *s_nhit = 0;
-                if (tid < kWave) {
-                    const uint32_t q = g * 32 + tid;
-                    const uint32_t th = (tid < 32 && q < a.n_queries)
-                                            ? __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                            : NONE;
+        if constexpr (!SAMPLE) {
+            // threshold planes of the group (bit j of plane b = bit b of
+            // query j's threshold; NONE for lanes past the last query): no
+            // kernel changes a threshold while a pass runs, so once per group;
+            // the stage counters start at zero
+            __syncthreads();
+            if (tid < 32)
+                s_qcnt[tid] = 0;
+            if (tid == 0)
+                *s_nhit = 0;
+            if (tid < kWave) {
+                const uint32_t q = g * 32 + tid;
+                const uint32_t th = (tid < 32 && q < a.n_queries) ? a.gthr[q] : NONE;
 #pragma unroll
-                    for (int b = 0; b < NP; ++b) {
-                        const uint32_t pl = (uint32_t)ballot((th >> b) & 1u);
-                        if (tid == 0)
-                            s_thr[b] = pl;
-                    }
+                for (int b = 0; b < NP; ++b) {
+                    const uint32_t pl = (uint32_t)ballot((th >> b) & 1u);
+                    if (tid == 0)
+                        s_thr[b] = pl;
                 }
             }
             __syncthreads();
-            if constexpr (!SAMPLE) {
 #pragma unroll
-                for (int b = 0; b < NP; ++b)
-                    tp[b] = __builtin_amdgcn_readfirstlane(s_thr[b]);
-            }
+            for (int b = 0; b < NP; ++b)
+                tp[b] = __builtin_amdgcn_readfirstlane(s_thr[b]);
+        }
+        for (uint32_t li = t_first + w0; li < min(n_list, w0 + win); li += t_step) {
+            const uint32_t t = a.list ? a.list[li] : li;
+            if constexpr (SAMPLE)
+                __syncthreads();
             const uint32_t o0 = t * kDenseTile;
             const uint32_t o1 = min(o0 + kDenseTile, SETS ? a.n_slots : a.n_objects);
             for (uint32_t ol = o0 + tid; ol < o1; ol += NT) {
@@ -783,7 +783,8 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             }
             if constexpr (!SAMPLE) {
                 // one reservation per query of the group, then the staged
-                // candidates to their slots
+                // candidates to their slots; the stage counters are reset
+                // for the next tile once every thread has read them
                 __syncthreads();
                 const uint32_t nh = min(*s_nhit, kDenseStage);
                 if (tid < 32 && s_qcnt[tid])
@@ -798,6 +799,11 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     else
                         atomicOr(a.overflow, 1u);
                 }
+                if (tid < 32)
+                    s_qcnt[tid] = 0;
+                if (tid == 0)
+                    *s_nhit = 0;
+                __syncthreads();  // the stage is free before the next tile's hits
             }
         }
         if constexpr (SAMPLE) {

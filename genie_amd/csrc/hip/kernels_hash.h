@@ -41,20 +41,30 @@ __global__ void __launch_bounds__(kDecodeWaves * kWave)
                   const uint2* __restrict__ jobs, uint32_t n_jobs, uint32_t* __restrict__ out)
 {
     __shared__ DecodeScratch S[kDecodeWaves];
+    static_assert(sizeof(DecodeScratch) >= kWave * (kPerLane + 1) * 4, "transpose fits the scratch");
     const int wv = threadIdx.x / kWave, lane = lane_id();
+    uint32_t* t = reinterpret_cast<uint32_t*>(&S[wv]);  // the transpose, once the decode is done
     for (uint32_t j = blockIdx.x * kDecodeWaves + wv; j < n_jobs; j += gridDim.x * kDecodeWaves) {
         const uint2 jb = jobs[j];
         const uint32_t w0 = cpos[jb.x], nw = cpos[jb.x + 1] - w0;
         stage_words(S[wv], words, w0, nw);
         uint32_t v[kPerLane];
         const uint32_t n = min(wave_decode(codec, S[wv], nw, v), (uint32_t)kSegMax);
+        wave_sync();  // every lane is done with the staged words
+        // lane l holds values 16 l .. 16 l + 15: through LDS (rows of 17
+        // words, so the writes of one k fall in distinct banks) to stores of
+        // 64 consecutive values per instruction
+#pragma unroll
+        for (int k = 0; k < kPerLane; ++k)
+            t[lane * (kPerLane + 1) + k] = v[k];
+        wave_sync();
 #pragma unroll
         for (int k = 0; k < kPerLane; ++k) {
-            const uint32_t i = lane * kPerLane + k;
+            const uint32_t i = (uint32_t)k * kWave + lane;
             if (i < n)
-                out[jb.y + i] = v[k];
+                out[jb.y + i] = t[(i / kPerLane) * (kPerLane + 1) + i % kPerLane];
         }
-        wave_sync();  // the staging buffer is reused by the next job
+        wave_sync();  // the scratch is reused by the next job
     }
 }
 

@@ -164,3 +164,40 @@ def test_merged_segment_items_give_the_same_results(table_kind):
         where = f"{table_kind} mode {mode} hash {hm} dense {dense}"
         assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1]), where
         assert_topk_equal(res[1][0], res[1][1], ri, rc, 50, 40, where)
+
+
+def test_decode_once_edge_batches():
+    """The plain form's decode-once path on batches with zero-weight items,
+    empty items, queries without items, load-balanced per-segment items and
+    mixed k, one plan across several batches (the scratch keeps segments of
+    earlier batches that later ones do not name): every batch equals the
+    oracle's results and the in-kernel decode's."""
+    from genie_amd import Plan
+    from helpers import oracle_topk_batch
+    w = W.cfg4(seed=961, n=90_000, d=20, q=30)
+    rows = w["rows"]
+    t = Table.from_rows(rows, 256, True, codec=11)
+    p = Plan(t.device(), 40, 40 * 20 * 200, 1000, 0, mode=0)
+    q_of0 = w["ranges"]["query"]
+    for b, (nq, lb, zero_every) in enumerate([(30, False, 0), (12, True, 4), (40, False, 3), (5, False, 0)]):
+        r = w["ranges"][q_of0 < min(nq, 30)].copy()
+        if zero_every:
+            r["weight"][::zero_every] = 0
+        if nq > 30:  # queries 30.. have no ranges
+            pass
+        items = t.compile(r, nq, load_balance=lb, max_load=300 if lb else 0)
+        empty = items[:3].copy()
+        empty["end_pos"] = empty["start_pos"]  # empty spans
+        items = np.concatenate([items, empty])
+        kq = np.array([(10, 100, 1000, 1)[(i + b) % 4] for i in range(nq)], np.uint32)
+        q_of = r["query"]
+        ri, rc = oracle_topk_batch(lambda q: O.count_exhaustive_single(rows, r[q_of == q]), nq, kq)
+        got = []
+        for dm in (-1, 0):
+            p.set_decode(dm)
+            p.upload(items, nq, kq)
+            assert p.decode_active() == (dm == -1)
+            p.run()
+            got.append(p.download())
+        assert np.array_equal(got[0][0], got[1][0]) and np.array_equal(got[0][1], got[1][1]), b
+        assert_topk_equal(got[0][0], got[0][1], ri, rc, kq, nq, f"decode-once batch {b}")
