@@ -546,11 +546,15 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
             sdir_off[k] = (uint32_t)(dir_keys.size() * (uint64_t)(d.n_dtiles + 1));
             dir_keys.push_back(k);
         }
-    // tile directory of the posting positions (keys longer than kShortList),
-    // as an uncompressed index has it
+    // tile directory of the posting positions, for the keys whose row (4 B
+    // per directory tile) costs at most a quarter of their postings; the
+    // others are read whole and filtered, as short keys are.  (On an index of
+    // many short lists, such as LSH buckets, rows for every key would hold
+    // more bytes than the compressed words.)
     std::vector<uint32_t> tdir_off(K, kNoDir), tdir_keys;
+    const uint64_t min_len = std::max<uint64_t>(kShortList, 4ull * (d.n_dtiles + 1));
     for (uint32_t k = 0; k < K; ++k)
-        if (desc->key_start[k + 1] - desc->key_start[k] > kShortList) {
+        if (desc->key_start[k + 1] - desc->key_start[k] > min_len) {
             tdir_off[k] = (uint32_t)(tdir_keys.size() * (uint64_t)(d.n_dtiles + 1));
             tdir_keys.push_back(k);
         }
