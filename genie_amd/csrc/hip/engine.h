@@ -21,6 +21,55 @@ constexpr uint32_t kShortList = 256;
 constexpr uint32_t kChunk = 32768;
 constexpr uint32_t kNoDir = 0xFFFFFFFFu;
 
+// Directory rows are graded: key k's row has an entry for every 2^s-th
+// directory tile boundary (entry j describes boundary (j << s) * kDirTile,
+// j = 0 .. ceil(n_dtiles / 2^s)), s chosen per key so that the row stays
+// small next to the list it indexes.  dir_off[k] is the row's offset in
+// words, a multiple of kDirAlign, with s in its low bits.
+constexpr uint32_t kDirAlign = 16;
+constexpr uint32_t kDirShiftMask = kDirAlign - 1;
+// Tile-directory rows cost at most an eighth of their key's posting bytes;
+// segment-directory rows at most 16 entries per segment of their key.
+constexpr uint32_t kDirPostingsPerEntry = 8;
+constexpr uint32_t kSdirEntriesPerSegment = 16;
+
+__host__ __device__ inline uint32_t dir_row_len(uint32_t n_dtiles, uint32_t s)
+{
+    return ((n_dtiles + (1u << s) - 1) >> s) + 1;
+}
+
+// Finest grade whose row has at most max_entries entries; -1 when none has.
+inline int dir_grade(uint32_t n_dtiles, uint64_t max_entries)
+{
+    for (uint32_t s = 0; s <= kDirShiftMask; ++s)
+        if (dir_row_len(n_dtiles, s) <= max_entries)
+            return (int)s;
+    return -1;
+}
+
+// Lays out the rows of the keys for which grade_of(k) >= 0: off[k] gets the
+// packed offset (kNoDir for the others), keys the row keys in order.
+// Returns the directory's words, or 0 past the 32-bit offset range.
+template <class GradeOf>
+inline uint64_t dir_layout(uint32_t n_keys, uint32_t n_dtiles, GradeOf grade_of, std::vector<uint32_t>& off,
+                           std::vector<uint32_t>& keys)
+{
+    off.assign(n_keys, kNoDir);
+    keys.clear();
+    uint64_t words = 0;
+    for (uint32_t k = 0; k < n_keys; ++k) {
+        const int s = grade_of(k);
+        if (s < 0)
+            continue;
+        off[k] = (uint32_t)words | (uint32_t)s;
+        keys.push_back(k);
+        words += (dir_row_len(n_dtiles, (uint32_t)s) + kDirShiftMask) & ~(uint64_t)kDirShiftMask;
+        if (words >= 0xFFFFFFF0ull)
+            return 0;
+    }
+    return words ? words : kDirAlign;
+}
+
 // One compiled chunk of a work item: postings [sp, ep) covering keys k0..k1.
 // kp is the position of the chunk's first (chunk, key) pair among all pairs
 // of the batch (exclusive prefix of k1 - k0 + 1 over chunks).
@@ -57,8 +106,8 @@ struct DevIndex {
     // has them too (built from its one decode at creation), for batches whose
     // segments are decoded once into a scratch of the uncompressed layout
     uint32_t* key_start = nullptr;  // [n_keys + 1]   positions (u32)
-    uint32_t* dir_off = nullptr;    // [n_keys]
-    uint32_t* dir = nullptr;        // [n_dir_keys * (n_dtiles + 1)]
+    uint32_t* dir_off = nullptr;    // [n_keys] packed row offset and grade, or kNoDir
+    uint32_t* dir = nullptr;        // graded rows of the keys with a row
     uint32_t* words = nullptr;      // [n_words]      (compressed)
     uint32_t* seg_cpos = nullptr;   // [n_segments + 1]
     uint32_t* seg_upos = nullptr;   // [n_segments + 1]
@@ -67,7 +116,7 @@ struct DevIndex {
     uint32_t* seg_last = nullptr;   // [n_segments] last id
     uint32_t* key_seg = nullptr;    // [n_keys + 1] first segment of each key
     uint32_t* sdir_off = nullptr;   // [n_keys] row of each key with more than 4 segments (compressed)
-    uint32_t* sdir = nullptr;       // [n_sdir_keys * (n_dtiles + 1)] segment directory (compressed)
+    uint32_t* sdir = nullptr;       // graded segment-directory rows (compressed)
     // forward codes of a single-value table (dense form of the batched search)
     bool dense = false;
     uint32_t dense_dpad = 0;        // dims rounded up to 16

@@ -540,26 +540,22 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
             d.h_key_mult[k] = best;
         }
     }
-    std::vector<uint32_t> sdir_off(K, kNoDir), dir_keys;
-    for (uint32_t k = 0; k < K; ++k)
-        if (key_seg[k + 1] - key_seg[k] > 4) {
-            sdir_off[k] = (uint32_t)(dir_keys.size() * (uint64_t)(d.n_dtiles + 1));
-            dir_keys.push_back(k);
-        }
-    // tile directory of the posting positions, for the keys whose row (4 B
-    // per directory tile) costs at most a quarter of their postings; the
-    // others are read whole and filtered, as short keys are.  (On an index of
-    // many short lists, such as LSH buckets, rows for every key would hold
-    // more bytes than the compressed words.)
-    std::vector<uint32_t> tdir_off(K, kNoDir), tdir_keys;
-    const uint64_t min_len = std::max<uint64_t>(kShortList, 4ull * (d.n_dtiles + 1));
-    for (uint32_t k = 0; k < K; ++k)
-        if (desc->key_start[k + 1] - desc->key_start[k] > min_len) {
-            tdir_off[k] = (uint32_t)(tdir_keys.size() * (uint64_t)(d.n_dtiles + 1));
-            tdir_keys.push_back(k);
-        }
-    if ((uint64_t)dir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull ||
-        (uint64_t)tdir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull) {
+    // segment directory: rows for keys of more than 4 segments, graded to at
+    // most 16 entries per segment of the key
+    std::vector<uint32_t> sdir_off, dir_keys;
+    const uint64_t sdir_words = dir_layout(K, d.n_dtiles, [&](uint32_t k) {
+        const uint64_t ns = key_seg[k + 1] - key_seg[k];
+        return ns > 4 ? dir_grade(d.n_dtiles, ns * kSdirEntriesPerSegment) : -1;
+    }, sdir_off, dir_keys);
+    // tile directory of the posting positions: rows for keys longer than
+    // kShortList, graded to at most an eighth of the key's posting bytes (an
+    // index of many short lists, such as LSH buckets, gets coarse rows)
+    std::vector<uint32_t> tdir_off, tdir_keys;
+    const uint64_t tdir_words = dir_layout(K, d.n_dtiles, [&](uint32_t k) {
+        const uint64_t len = desc->key_start[k + 1] - desc->key_start[k];
+        return len > kShortList ? dir_grade(d.n_dtiles, len / kDirPostingsPerEntry) : -1;
+    }, tdir_off, tdir_keys);
+    if (!sdir_words || !tdir_words) {
         delete ix;
         return fail(GENIE_E_INVALID, "segment or tile directory too large");
     }
@@ -577,7 +573,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         if ((rc = dev_alloc(&d_post, d.n_postings))) break;
         if ((rc = dev_alloc(&d.key_start, K + 1, &d.bytes))) break;
         if ((rc = dev_alloc(&d.dir_off, K, &d.bytes))) break;
-        if ((rc = dev_alloc(&d.dir, tdir_keys.size() * (uint64_t)(d.n_dtiles + 1), &d.bytes))) break;
+        if ((rc = dev_alloc(&d.dir, tdir_words, &d.bytes))) break;
         if ((rc = dev_alloc(&d_tdir_keys, tdir_keys.size()))) break;
         if (hipError_t e = hipMemcpy(d.key_start, ks32.data(), (K + 1) * 4ull, hipMemcpyHostToDevice)) {
             rc = fail(GENIE_E_RUNTIME, std::string("index upload: ") + hipGetErrorString(e));
@@ -590,7 +586,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         if ((rc = dev_alloc(&d.seg_last, S, &d.bytes))) break;
         if ((rc = dev_alloc(&d.key_seg, K + 1, &d.bytes))) break;
         if ((rc = dev_alloc(&d.sdir_off, K, &d.bytes))) break;
-        if ((rc = dev_alloc(&d.sdir, dir_keys.size() * (uint64_t)(d.n_dtiles + 1), &d.bytes))) break;
+        if ((rc = dev_alloc(&d.sdir, sdir_words, &d.bytes))) break;
         if ((rc = dev_alloc(&d_upos, S + 1))) break;
         if ((rc = dev_alloc(&d_dir_keys, dir_keys.size()))) break;
         if ((rc = dev_alloc(&d_err, 1))) break;
@@ -634,20 +630,18 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         if (rc)
             break;
         if (!dir_keys.empty()) {
-            const uint64_t total = dir_keys.size() * (uint64_t)(d.n_dtiles + 1);
-            const uint32_t g = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
-            k_build_sdir<<<g, 256>>>(d.seg_last, d.key_seg, d_dir_keys, (uint32_t)dir_keys.size(), d.n_dtiles,
-                                      d.sdir);
+            const uint32_t g = (uint32_t)std::min<uint64_t>((dir_keys.size() + 3) / 4, 65536);
+            k_build_sdir<<<g, 256>>>(d.seg_last, d.key_seg, d_dir_keys, (uint32_t)dir_keys.size(), d.sdir_off,
+                                      d.n_dtiles, d.sdir);
             if ((e = hipGetLastError()) || (e = hipDeviceSynchronize())) {
                 rc = fail(GENIE_E_RUNTIME, std::string("segment directory: ") + hipGetErrorString(e));
                 break;
             }
         }
         if (!tdir_keys.empty()) {
-            const uint64_t total = tdir_keys.size() * (uint64_t)(d.n_dtiles + 1);
-            const uint32_t g = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
-            k_build_dir<<<g, 256>>>(d_post, d.key_start, d_tdir_keys, (uint32_t)tdir_keys.size(), d.n_dtiles,
-                                     d.dir);
+            const uint32_t g = (uint32_t)std::min<uint64_t>((tdir_keys.size() + 3) / 4, 65536);
+            k_build_dir<<<g, 256>>>(d_post, d.key_start, d_tdir_keys, (uint32_t)tdir_keys.size(), d.dir_off,
+                                     d.n_dtiles, d.dir);
             if ((e = hipGetLastError()) || (e = hipDeviceSynchronize())) {
                 rc = fail(GENIE_E_RUNTIME, std::string("tile directory: ") + hipGetErrorString(e));
                 break;
@@ -1233,7 +1227,7 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
     d.key_pos.build(d.h_key_start);
     d.h_key_dim.assign(desc->key_dim, desc->key_dim + desc->n_keys);
     d.h_key_mult.assign(desc->n_keys, 1);
-    std::vector<uint32_t> ks32(desc->n_keys + 1), dir_off(desc->n_keys, kNoDir), dir_keys;
+    std::vector<uint32_t> ks32(desc->n_keys + 1), dir_off, dir_keys;
     for (uint32_t k = 0; k <= desc->n_keys; ++k)
         ks32[k] = (uint32_t)desc->key_start[k];
     for (uint32_t k = 0; k < desc->n_keys; ++k) {
@@ -1248,12 +1242,14 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
             best = std::max(best, run);
         }
         d.h_key_mult[k] = best;
-        if (e - s > kShortList) {
-            dir_off[k] = (uint32_t)(dir_keys.size() * (uint64_t)(d.n_dtiles + 1));
-            dir_keys.push_back(k);
-        }
     }
-    if ((uint64_t)dir_keys.size() * (d.n_dtiles + 1) >= 0xFFFFFFFFull) {
+    // tile directory: rows for keys longer than kShortList, graded to at most
+    // an eighth of the key's posting bytes
+    const uint64_t dir_words = dir_layout(d.n_keys, d.n_dtiles, [&](uint32_t k) {
+        const uint64_t len = desc->key_start[k + 1] - desc->key_start[k];
+        return len > kShortList ? dir_grade(d.n_dtiles, len / kDirPostingsPerEntry) : -1;
+    }, dir_off, dir_keys);
+    if (!dir_words) {
         delete ix;
         return fail(GENIE_E_INVALID, "tile directory too large");
     }
@@ -1263,7 +1259,7 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
         if ((rc = dev_alloc(&d.post, d.n_postings, &d.bytes))) break;
         if ((rc = dev_alloc(&d.key_start, d.n_keys + 1, &d.bytes))) break;
         if ((rc = dev_alloc(&d.dir_off, d.n_keys, &d.bytes))) break;
-        if ((rc = dev_alloc(&d.dir, dir_keys.size() * (uint64_t)(d.n_dtiles + 1), &d.bytes))) break;
+        if ((rc = dev_alloc(&d.dir, dir_words, &d.bytes))) break;
         if ((rc = dev_alloc(&d_dir_keys, dir_keys.size()))) break;
         hipError_t e;
         if ((e = hipMemcpy(d.post, desc->postings, d.n_postings * 4, hipMemcpyHostToDevice)) ||
@@ -1274,9 +1270,8 @@ int genie_hip_index_create(const genie_index_desc* desc, genie_index** out)
             break;
         }
         if (!dir_keys.empty()) {
-            uint64_t total = dir_keys.size() * (uint64_t)(d.n_dtiles + 1);
-            uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
-            k_build_dir<<<blocks, 256>>>(d.post, d.key_start, d_dir_keys, (uint32_t)dir_keys.size(),
+            const uint32_t blocks = (uint32_t)std::min<uint64_t>((dir_keys.size() + 3) / 4, 65536);
+            k_build_dir<<<blocks, 256>>>(d.post, d.key_start, d_dir_keys, (uint32_t)dir_keys.size(), d.dir_off,
                                          d.n_dtiles, d.dir);
             if ((e = hipGetLastError()) || (e = hipDeviceSynchronize())) {
                 rc = fail(GENIE_E_RUNTIME, std::string("directory build: ") + hipGetErrorString(e));

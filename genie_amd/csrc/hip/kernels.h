@@ -114,32 +114,68 @@ __global__ void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* sums)
 }
 
 // ---------------------------------------------------------------------------
-// Index directory: for every long key and every directory-tile boundary b,
-// the position inside the key's list of the first id >= b * kDirTile.
+// Index directory: for every key with a row of grade s and every row entry j,
+// the position inside the key's list of the first id >= (j << s) * kDirTile.
+// One wave per row.
 // ---------------------------------------------------------------------------
 
 __global__ void k_build_dir(const uint32_t* __restrict__ post, const uint32_t* __restrict__ key_start,
                             const uint32_t* __restrict__ dir_keys, uint32_t n_dir_keys,
-                            uint32_t n_dtiles, uint32_t* __restrict__ dir)
+                            const uint32_t* __restrict__ dir_off, uint32_t n_dtiles, uint32_t* __restrict__ dir)
 {
-    const uint64_t per = (uint64_t)n_dtiles + 1;
-    const uint64_t total = (uint64_t)n_dir_keys * per;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t dk = (uint32_t)(i / per), b = (uint32_t)(i % per);
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t dk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; dk < n_dir_keys; dk += waves) {
         const uint32_t k = dir_keys[dk];
+        const uint32_t doff = dir_off[k], sh = doff & kDirShiftMask;
+        uint32_t* row = dir + (doff & ~kDirShiftMask);
+        const uint32_t n = dir_row_len(n_dtiles, sh);
         const uint32_t s = key_start[k], e = key_start[k + 1];
-        const uint64_t target = (uint64_t)b * kDirTile;
-        uint32_t lo = s, hi = e;  // first position with post[p] >= target
-        while (lo < hi) {
-            uint32_t mid = lo + ((hi - lo) >> 1);
-            if ((uint64_t)post[mid] < target)
-                lo = mid + 1;
-            else
-                hi = mid;
+        for (uint32_t j = lane_id(); j < n; j += kWave) {
+            const uint64_t target = ((uint64_t)j << sh) * kDirTile;
+            uint32_t lo = s, hi = e;  // first position with post[p] >= target
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo) >> 1);
+                if ((uint64_t)post[mid] < target)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            row[j] = lo - s;
         }
-        dir[i] = lo - s;
     }
+}
+
+// Clips postings [lo, hi) of a key whose list starts at ks to the directory
+// tiles [dt0, dt1) through the key's row (packed offset doff).  Returns true
+// when the clipped span holds ids of those tiles only (both ends fall on row
+// entries); otherwise the span also covers parts of the neighbouring tiles
+// of the row's grade and its ids need the tile filter.
+__device__ __forceinline__ bool dir_clip(const uint32_t* __restrict__ dir, uint32_t doff, uint32_t ks,
+                                         uint32_t dt0, uint32_t dt1, uint32_t n_dtiles, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t s = doff & kDirShiftMask, m = (1u << s) - 1;
+    const uint32_t* d = dir + (doff & ~kDirShiftMask);
+    lo = max(lo, ks + d[dt0 >> s]);
+    hi = min(hi, ks + d[(dt1 + m) >> s]);
+    return (dt0 & m) == 0 && ((dt1 & m) == 0 || dt1 == n_dtiles);
+}
+
+// Clips segments [lo, hi) of a key whose segments are [ks, ke) to those that
+// can hold ids of the directory tiles [dt0, dt1) (objects below obj1) through
+// the key's segment-directory row: from the first segment whose last id
+// reaches the row entry at or below dt0, to the first segment whose last id
+// reaches the row entry at or above dt1, that one included when it starts
+// below obj1 (later segments start at or after its last id).
+__device__ __forceinline__ void sdir_clip(const uint32_t* __restrict__ sdir, uint32_t doff, uint32_t ks, uint32_t ke,
+                                          uint32_t dt0, uint32_t dt1, uint32_t obj1,
+                                          const uint32_t* __restrict__ seg_first, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t s = doff & kDirShiftMask, m = (1u << s) - 1;
+    const uint32_t* d = sdir + (doff & ~kDirShiftMask);
+    const uint32_t mid = ks + d[(dt1 + m) >> s];
+    const uint32_t end = mid + (mid < ke && seg_first[mid] < obj1 ? 1u : 0u);
+    lo = max(lo, ks + d[dt0 >> s]);
+    hi = min(hi, end);
 }
 
 // ---------------------------------------------------------------------------
@@ -210,28 +246,32 @@ __global__ void __launch_bounds__(kDecodeWaves * kWave)
     }
 }
 
-// For every long key and directory-tile boundary b: the first segment of the
-// key (relative) whose last id is >= b * kDirTile.
+// For every key with a segment-directory row of grade s and every row entry
+// j: the first segment of the key (relative) whose last id is >=
+// (j << s) * kDirTile.  One wave per row.
 __global__ void k_build_sdir(const uint32_t* __restrict__ seg_last, const uint32_t* __restrict__ key_seg,
-                             const uint32_t* __restrict__ dir_keys, uint32_t n_dir_keys, uint32_t n_dtiles,
-                             uint32_t* __restrict__ sdir)
+                             const uint32_t* __restrict__ dir_keys, uint32_t n_dir_keys,
+                             const uint32_t* __restrict__ sdir_off, uint32_t n_dtiles, uint32_t* __restrict__ sdir)
 {
-    const uint64_t per = (uint64_t)n_dtiles + 1;
-    const uint64_t total = (uint64_t)n_dir_keys * per;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t k = dir_keys[i / per], b = (uint32_t)(i % per);
+    const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t dk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; dk < n_dir_keys; dk += waves) {
+        const uint32_t k = dir_keys[dk];
+        const uint32_t doff = sdir_off[k], sh = doff & kDirShiftMask;
+        uint32_t* row = sdir + (doff & ~kDirShiftMask);
+        const uint32_t n = dir_row_len(n_dtiles, sh);
         const uint32_t s = key_seg[k], e = key_seg[k + 1];
-        const uint64_t target = (uint64_t)b * kDirTile;
-        uint32_t lo = s, hi = e;
-        while (lo < hi) {
-            const uint32_t mid = lo + ((hi - lo) >> 1);
-            if ((uint64_t)seg_last[mid] < target)
-                lo = mid + 1;
-            else
-                hi = mid;
+        for (uint32_t j = lane_id(); j < n; j += kWave) {
+            const uint64_t target = ((uint64_t)j << sh) * kDirTile;
+            uint32_t lo = s, hi = e;
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo) >> 1);
+                if ((uint64_t)seg_last[mid] < target)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            row[j] = lo - s;
         }
-        sdir[i] = lo - s;
     }
 }
 
@@ -1014,12 +1054,8 @@ __device__ __forceinline__ void resolve_spans(const ScanArgs& a, uint32_t c0, ui
             uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
             const uint32_t doff = a.dir_off[k];
             uint32_t filter = 1;
-            if (doff != kNoDir) {
-                const uint32_t* d = a.dir + doff;
-                lo = max(lo, ks + d[dt0]);
-                hi = min(hi, ks + d[dt1]);
-                filter = 0;
-            }
+            if (doff != kNoDir)
+                filter = dir_clip(a.dir, doff, ks, dt0, dt1, a.n_dtiles, lo, hi) ? 0u : 1u;
             Span sp;
             sp.lo = lo;
             sp.hi = max(lo, hi);
@@ -1209,13 +1245,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
                 const uint32_t ks = a.key_seg[k], ke = a.key_seg[k + 1];
                 uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
                 const uint32_t doff = a.sdir_off[k];
-                if (doff != kNoDir) {
-                    const uint32_t* d = a.sdir + doff;
-                    const uint32_t mid = ks + d[dt1];
-                    const uint32_t end = mid + (mid < ke && a.seg_first[mid] < obj1 ? 1u : 0u);
-                    lo = max(lo, ks + d[dt0]);
-                    hi = min(hi, end);
-                }
+                if (doff != kNoDir)
+                    sdir_clip(a.sdir, doff, ks, ke, dt0, dt1, obj1, a.seg_first, lo, hi);
                 Span sp;
                 sp.lo = lo;
                 sp.hi = max(lo, hi);

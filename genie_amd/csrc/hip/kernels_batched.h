@@ -120,23 +120,13 @@ __global__ void __launch_bounds__(NT) k_scan_batched(BatchArgs b)
                 const uint32_t doff = CMP ? a.sdir_off[pr.key] : a.dir_off[pr.key];
                 if constexpr (CMP) {
                     const uint32_t ks = a.key_seg[pr.key], ke = a.key_seg[pr.key + 1];
-                    if (doff != kNoDir) {
-                        const uint32_t* d = a.sdir + doff;
-                        const uint32_t mid = ks + d[dt1];
-                        const uint32_t end = mid + (mid < ke && a.seg_first[mid] < obj1 ? 1u : 0u);
-                        lo = max(lo, ks + d[dt0]);
-                        hi = min(hi, end);
-                    }
+                    if (doff != kNoDir)
+                        sdir_clip(a.sdir, doff, ks, ke, dt0, dt1, obj1, a.seg_first, lo, hi);
                     sp.filter = 1;
                 } else {
                     sp.filter = 1;
-                    if (doff != kNoDir) {
-                        const uint32_t ks = a.key_start[pr.key];
-                        const uint32_t* d = a.dir + doff;
-                        lo = max(lo, ks + d[dt0]);
-                        hi = min(hi, ks + d[dt1]);
-                        sp.filter = 0;
-                    }
+                    if (doff != kNoDir)
+                        sp.filter = dir_clip(a.dir, doff, a.key_start[pr.key], dt0, dt1, a.n_dtiles, lo, hi) ? 0 : 1;
                 }
                 sp.lo = lo;
                 sp.hi = max(lo, hi);

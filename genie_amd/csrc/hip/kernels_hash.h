@@ -264,13 +264,8 @@ __device__ __forceinline__ void resolve_seg_spans(const ScanArgs& a, uint32_t c0
             const uint32_t ks = a.key_seg[k], ke = a.key_seg[k + 1];
             uint32_t lo = max(ch.sp, ks), hi = min(ch.ep, ke);
             const uint32_t doff = a.sdir_off[k];
-            if (doff != kNoDir) {
-                const uint32_t* d = a.sdir + doff;
-                const uint32_t mid = ks + d[dt1];
-                const uint32_t end = mid + (mid < ke && a.seg_first[mid] < obj1 ? 1u : 0u);
-                lo = max(lo, ks + d[dt0]);
-                hi = min(hi, end);
-            }
+            if (doff != kNoDir)
+                sdir_clip(a.sdir, doff, ks, ke, dt0, dt1, obj1, a.seg_first, lo, hi);
             Span sp;
             sp.lo = lo;
             sp.hi = max(lo, hi);
