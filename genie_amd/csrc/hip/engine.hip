@@ -2052,6 +2052,17 @@ static int plan_split_plain(genie_plan* p)
         if (p->hash_mode != 0 && !p->h_force_direct[q] && (p->hash_mode == 1 || P * 16 <= N)) {
             const uint64_t nt0 = std::max<uint64_t>(1, (P + kHashTarget - 1) / kHashTarget);
             uint64_t dtpt = std::max<uint64_t>(1, ((N + nt0 - 1) / nt0 + kDirTile - 1) / kDirTile);
+            // tiles start on multiples of 16 directory tiles (of a power of two
+            // below that), so that directory rows up to grade 4 clip the spans
+            // exactly
+            if (dtpt >= 16) {
+                dtpt = (dtpt + 15) / 16 * 16;
+            } else {
+                uint64_t pow2 = 1;
+                while (pow2 < dtpt)
+                    pow2 <<= 1;
+                dtpt = pow2;
+            }
             uint32_t kb = 0;
             for (;;) {
                 const uint64_t tobj = dtpt * kDirTile;
