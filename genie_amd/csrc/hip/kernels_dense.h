@@ -232,9 +232,12 @@ __global__ void __launch_bounds__(256) k_dense_thresh(const uint32_t* __restrict
     const uint32_t safe = max(wave_max(largest(k)), 1u);
     const uint32_t spec = max(wave_max(largest(j)), safe);
     // S(spec) and S(safe): from the lane holding that bin
+    // (selects on the bits of c - b: a compare chain here becomes a table
+    // lookup in private memory)
     auto s_of = [&](uint32_t c) {
-        const uint32_t o = c - b;
-        return o == 0 ? s0 : (o == 1 ? s1 : (o == 2 ? s2 : s3));
+        const uint32_t o = c & 3u;  // c - b, for c among this lane's bins
+        const uint32_t lo = (o & 1u) ? s1 : s0, hi = (o & 1u) ? s3 : s2;
+        return (o & 2u) ? hi : lo;
     };
     const uint32_t at_spec = (uint32_t)__shfl((int)(spec >= b && spec < b + 4 ? s_of(spec) : 0u), (int)(spec / 4), kWave);
     const uint32_t at_safe = (uint32_t)__shfl((int)(safe >= b && safe < b + 4 ? s_of(safe) : 0u), (int)(safe / 4), kWave);
@@ -462,6 +465,10 @@ __device__ __forceinline__ uint32_t dense_lane_count(const uint32_t (&c)[NP], ui
 // them), so that each query reserves its candidates of the tile with one
 // global atomic.
 constexpr uint32_t kDenseStage = 2048;
+// Main pass: tiles whose candidates share the stage between two flushes
+// (a tile of 4,096 objects stages about a hundred on the bench batches; a
+// hit that finds the stage full goes straight to its query's list).
+constexpr uint32_t kDenseFlushTiles = 8;
 constexpr uint32_t kDenseStageWords = 2 + 32 + 32 + kDenseStage * 3;  // count, pad, per-query counts / bases, meta, keys
 
 __host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample)
@@ -711,7 +718,9 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             for (int b = 0; b < NP; ++b)
                 tp[b] = __builtin_amdgcn_readfirstlane(s_thr[b]);
         }
-        for (uint32_t li = t_first + w0; li < min(n_list, w0 + win); li += t_step) {
+        uint32_t staged_tiles = 0;
+        const uint32_t li_end = min(n_list, w0 + win);
+        for (uint32_t li = t_first + w0; li < li_end; li += t_step) {
             const uint32_t t = a.list ? a.list[li] : li;
             if constexpr (SAMPLE)
                 __syncthreads();
@@ -741,15 +750,24 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                         atomicAdd(&s_hist[j * HB + min(v, HB - 1)], 1u);
                     }
                 } else if constexpr (SAMPLE) {
+                    // lane l files query (i + l) mod 32 at step i, so that one
+                    // update of the 64 lanes spreads over 32 queries' rows (one
+                    // query's row from every lane piles onto the few bins its
+                    // counts share).  Zero counts are filed too: the thresholds
+                    // read bins >= 1 only.
+                    const uint32_t r = tid & 31u;
+                    uint32_t cr[8];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        cr[b] = __builtin_amdgcn_alignbit(c[b], c[b], r);  // bit j: query (j + r) mod 32
                     uint64_t cnt[4];
-                    dense_all_counts(c, cnt);
+                    dense_all_counts(cr, cnt);
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
                             const uint32_t v = (uint32_t)(cnt[q] >> (8 * j)) & 255u;
-                            if (v)
-                                atomicAdd(&s_hist[(8 * q + j) * HB + v], 1u);
+                            atomicAdd(&s_hist[((8u * q + j + r) & 31u) * HB + v], 1u);
                         }
                 } else {
                     // count >= threshold, per lane, most significant plane first
@@ -781,10 +799,13 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     }
                 }
             }
-            if constexpr (!SAMPLE) {
-                // one reservation per query of the group, then the staged
-                // candidates to their slots; the stage counters are reset
-                // for the next tile once every thread has read them
+            // every kDenseFlushTiles tiles and after the group's last one
+            // (uniform across the workgroup): one reservation per query of
+            // the group, then the staged candidates to their slots; the stage
+            // counters are reset once every thread has read them
+            if constexpr (!SAMPLE)
+            if (++staged_tiles == kDenseFlushTiles || li + t_step >= li_end) {
+                staged_tiles = 0;
                 __syncthreads();
                 const uint32_t nh = min(*s_nhit, kDenseStage);
                 if (tid < 32 && s_qcnt[tid])
