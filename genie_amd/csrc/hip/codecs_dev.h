@@ -21,7 +21,7 @@ constexpr uint32_t kSegMax = 1024;       // values and words per segment
 constexpr int kPerLane = kSegMax / kWave; // 16
 
 // Per-wave LDS scratch of the decoders.
-struct DecodeScratch {
+struct alignas(16) DecodeScratch {
     uint32_t words[kSegMax + 4];   // staged compressed segment (+ guard words)
     uint32_t mb[kSegMax / 32];     // BP32 miniblocks: start word << 8 | width
     uint16_t stop[kSegMax + 1];    // varint: byte position + 1 of each value's last byte
@@ -43,7 +43,7 @@ struct CodecShape {
     int leaf;
 };
 
-__host__ __device__ inline bool codec_shape(int codec, CodecShape* s)
+__host__ __device__ constexpr bool codec_shape(int codec, CodecShape* s)
 {
     switch (codec) {
     case 1: *s = {false, false, kCopy}; return true;    // COPY
@@ -131,6 +131,24 @@ __device__ __forceinline__ uint32_t bp32_value(const DecodeScratch& S, uint32_t 
     return b == 32 ? (uint32_t)x : (uint32_t)x & ((1u << b) - 1u);
 }
 
+// The 16 values [first, first + 16) of the BP32 part (first a multiple of
+// 16, so they lie in one 32-value miniblock): one miniblock lookup, then per
+// value two word reads and a funnel shift.  Word reads stay within the
+// miniblock's words (+ one guard word).
+__device__ __forceinline__ void bp32_lane16(const DecodeScratch& S, uint32_t first, uint32_t (&v)[kPerLane])
+{
+    const uint32_t d = S.mb[first >> 5];
+    const uint32_t b = d & 255u, start = d >> 8;
+    const uint32_t mask = b >= 32u ? 0xFFFFFFFFu : (1u << b) - 1u;
+    uint32_t bit = (first & 31u) * b;
+#pragma unroll
+    for (int j = 0; j < kPerLane; ++j) {
+        const uint32_t w = start + (bit >> 5);
+        v[j] = __builtin_amdgcn_alignbit(S.words[w + 1], S.words[w], bit & 31u) & mask;
+        bit += b;
+    }
+}
+
 // Varint over words [v0, v0 + nw): records the end of every value and
 // returns the value count (0xFFFFFFFF if more than kSegMax).
 __device__ __forceinline__ uint32_t varint_prepare(DecodeScratch& S, uint32_t v0, uint32_t nw)
@@ -158,26 +176,47 @@ __device__ __forceinline__ uint32_t varint_prepare(DecodeScratch& S, uint32_t v0
     return total;
 }
 
+// Value i of the varint part at words [v0, ...): its bytes [b, e) (at most
+// the first five count, 7 bits each, least significant first), gathered
+// from the three words holding bytes b .. b + 7 with funnel shifts.  Reads
+// stay within the staged words + two guard words.
 __device__ __forceinline__ uint32_t varint_value(const DecodeScratch& S, uint32_t v0, uint32_t i)
 {
     const uint32_t e = S.stop[i];                 // one past the last byte
     const uint32_t b = i ? S.stop[i - 1] : 0u;    // first byte
-    const uint8_t* bytes = reinterpret_cast<const uint8_t*>(S.words + v0);
-    uint32_t v = 0;
-    for (uint32_t k = b, sh = 0; k < e && sh < 35; ++k, sh += 7)
-        v |= (uint32_t)(bytes[k] & 127u) << sh;
-    return v;
+    const uint32_t wb = v0 + (b >> 2), sh = (b & 3u) * 8u;
+    const uint32_t w0 = S.words[wb], w1 = S.words[wb + 1], w2 = S.words[wb + 2];
+    const uint32_t lo = __builtin_amdgcn_alignbit(w1, w0, sh);  // bytes b .. b + 3
+    const uint32_t hi = __builtin_amdgcn_alignbit(w2, w1, sh);  // bytes b + 4 .. b + 7
+    const uint32_t v = (lo & 0x7Fu) | ((lo >> 1) & 0x3F80u) | ((lo >> 2) & 0x1FC000u) | ((lo >> 3) & 0xFE00000u) |
+                       (hi << 28);
+    const uint32_t len = e - b;
+    return len >= 5u ? v : v & ((1u << (7u * len)) - 1u);
 }
 
-// Decode the staged segment of `nw` words.  Returns the value count, or
-// 0xFFFFFFFF on a malformed segment; vals[j] of lane l is value 16 l + j
-// (undefined past the count).
-__device__ __forceinline__ uint32_t wave_decode(int codec, DecodeScratch& S, uint32_t nw,
-                                                uint32_t (&vals)[kPerLane])
+// A codec's shape fixed at compile time (a kernel instantiated per codec
+// keeps only that codec's steps and registers).
+__host__ __device__ constexpr CodecShape codec_shape_of(int codec)
 {
-    CodecShape sh;
-    if (!codec_shape(codec, &sh))
-        return 0xFFFFFFFFu;
+    CodecShape s{false, false, kCopy};
+    codec_shape(codec, &s);
+    return s;
+}
+template <int CODEC>
+struct CodecC {
+    static constexpr bool delta_first = codec_shape_of(CODEC).delta_first;
+    static constexpr bool composite = codec_shape_of(CODEC).composite;
+    static constexpr int leaf = codec_shape_of(CODEC).leaf;
+};
+
+// Decode the staged segment of `nw` words, shape `sh` (a CodecShape, or a
+// CodecC with the same members fixed at compile time).  Returns the value
+// count, or 0xFFFFFFFF on a malformed segment; vals[j] of lane l is value
+// 16 l + j (undefined past the count).
+template <class SH>
+__device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratch& S, uint32_t nw,
+                                                      uint32_t (&vals)[kPerLane])
+{
     const int lane = lane_id();
     const uint32_t base = lane * kPerLane;
     uint32_t n1 = 0, b1 = 0;       // composite: values of the BP32 part, its word count
@@ -210,22 +249,23 @@ __device__ __forceinline__ uint32_t wave_decode(int codec, DecodeScratch& S, uin
     if (n2 == 0xFFFFFFFFu || n1 + n2 > kSegMax)
         return 0xFFFFFFFFu;
     const uint32_t n = n1 + n2;
+    // values [0, nbp) come from BP32 miniblocks (the whole segment for a
+    // BP32 leaf, the first part of a composite), the rest from the leaf
+    const uint32_t nbp = sh.leaf == kBp32 ? n : n1;
 #pragma unroll
-    for (int j = 0; j < kPerLane; ++j) {
-        const uint32_t i = base + j;
-        uint32_t v = 0;
-        if (i < n1) {
-            v = bp32_value(S, i);
-        } else if (i < n) {
-            const uint32_t k = i - n1;
-            switch (sh.leaf) {
-            case kCopy:
-            case kDelta: v = S.words[l0 + k]; break;
-            case kBp32: v = bp32_value(S, k); break;
-            case kVarint: v = varint_value(S, l0, k); break;
+    for (int j = 0; j < kPerLane; ++j)
+        vals[j] = 0;
+    if (base < nbp)
+        bp32_lane16(S, base, vals);
+    if (sh.leaf != kBp32 && base + kPerLane > n1 && base < n) {
+#pragma unroll
+        for (int j = 0; j < kPerLane; ++j) {
+            const uint32_t i = base + j;
+            if (i >= n1 && i < n) {
+                const uint32_t k = i - n1;
+                vals[j] = sh.leaf == kVarint ? varint_value(S, l0, k) : S.words[l0 + k];
             }
         }
-        vals[j] = v;
     }
     if (sh.leaf == kDelta)  // Delta leaf: inclusive prefix sum of its part
         wave_prefix16(vals);
@@ -234,7 +274,18 @@ __device__ __forceinline__ uint32_t wave_decode(int codec, DecodeScratch& S, uin
     return n;
 }
 
-// Stage words [w0, w0 + nw) of global memory into the wave's scratch.
+// The same with the codec chosen at run time.
+__device__ __forceinline__ uint32_t wave_decode(int codec, DecodeScratch& S, uint32_t nw,
+                                                uint32_t (&vals)[kPerLane])
+{
+    CodecShape sh{false, false, kCopy};
+    if (!codec_shape(codec, &sh))
+        return 0xFFFFFFFFu;
+    return wave_decode_shape(sh, S, nw, vals);
+}
+
+// Stage words [w0, w0 + nw) of global memory into the wave's scratch
+// (nw <= kSegMax, checked when the index was created).
 __device__ __forceinline__ void stage_words(DecodeScratch& S, const uint32_t* __restrict__ g, uint32_t w0,
                                             uint32_t nw)
 {
@@ -242,6 +293,27 @@ __device__ __forceinline__ void stage_words(DecodeScratch& S, const uint32_t* __
         S.words[i] = g[w0 + i];
     if (lane_id() < 4)
         S.words[nw + lane_id()] = 0;
+    wave_sync();
+}
+
+// The same through a buffer resource that ends at the segment's last word:
+// every lane issues all of its kPerLane loads unconditionally (the ones past
+// the segment read zero), then stores them.  w0 and nw are wave-uniform.
+__device__ __forceinline__ void stage_words_rs(DecodeScratch& S, const uint32_t* __restrict__ g, uint32_t w0,
+                                               uint32_t nw)
+{
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(g) + w0, (short)0, (int)(nw * 4u), 0x00020000);
+    const uint32_t lane = lane_id();
+    uint32_t t[kPerLane];
+#pragma unroll
+    for (int r = 0; r < kPerLane; ++r)
+        t[r] = __builtin_amdgcn_raw_buffer_load_b32(rs, (lane + (uint32_t)r * kWave) * 4u, 0, 0);
+#pragma unroll
+    for (int r = 0; r < kPerLane; ++r)
+        S.words[lane + (uint32_t)r * kWave] = t[r];
+    if (lane < 4)
+        S.words[kSegMax + lane] = 0;
     wave_sync();
 }
 

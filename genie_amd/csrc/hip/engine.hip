@@ -2387,6 +2387,30 @@ static void launch_scan_t(uint32_t lane_bits, const ScanArgs& a, uint32_t blocks
     }
 }
 
+// k_decode_jobs instantiated for the index's codec.
+static int launch_decode_jobs(const DevIndex& d, const uint2* jobs, uint32_t nj, uint32_t* out, hipStream_t s)
+{
+    const uint32_t g = std::min<uint32_t>((nj + kDecodeWaves - 1) / kDecodeWaves, 65535);
+    const dim3 b(kDecodeWaves * kWave);
+    switch (d.codec) {
+    case 1: k_decode_jobs<1><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 2: k_decode_jobs<2><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 3: k_decode_jobs<3><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 4: k_decode_jobs<4><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 5: k_decode_jobs<5><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 6: k_decode_jobs<6><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 7: k_decode_jobs<7><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 8: k_decode_jobs<8><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 9: k_decode_jobs<9><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 10: k_decode_jobs<10><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 11: k_decode_jobs<11><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 12: k_decode_jobs<12><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 13: k_decode_jobs<13><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    default: return fail(GENIE_E_INVALID, "unknown codec " + std::to_string(d.codec));
+    }
+    return GENIE_OK;
+}
+
 // The plain form's scan: the hash kernel over the (query, tile) work of the
 // selective queries, then the direct-mapped kernel over the other queries.
 static int launch_scan(genie_plan* p, hipStream_t s)
@@ -2403,8 +2427,8 @@ static int launch_scan(genie_plan* p, hipStream_t s)
         const DevIndex& d = p->ix->d;
         const uint32_t nj = (uint32_t)p->h_fjobs.size();
         if (nj)
-            k_decode_jobs<<<std::min<uint32_t>((nj + kDecodeWaves - 1) / kDecodeWaves, 65535), kDecodeWaves * kWave,
-                            0, s>>>(d.codec, d.words, d.seg_cpos, p->d_fjobs, nj, p->d_full);
+            if (int rc = launch_decode_jobs(d, p->d_fjobs, nj, p->d_full, s))
+                return rc;
         a.codec = 0;
         a.post = p->d_full;
     }
@@ -2423,9 +2447,8 @@ static int launch_scan(genie_plan* p, hipStream_t s)
         } else if (p->dec_spans) {
             const uint32_t nj = (uint32_t)p->h_djobs.size();
             if (nj)
-                k_decode_jobs<<<std::min<uint32_t>((nj + kDecodeWaves - 1) / kDecodeWaves, 65535),
-                                kDecodeWaves * kWave, 0, s>>>(d.codec, d.words, d.seg_cpos, p->d_djobs, nj,
-                                                              p->d_scratch);
+                if (int rc = launch_decode_jobs(d, p->d_djobs, nj, p->d_scratch, s))
+                    return rc;
             h.s.post = p->d_scratch;
             h.spans = p->d_sspan;
             h.sq_ptr = p->d_sq_ptr;

@@ -36,33 +36,54 @@ struct HashArgs {
 // other; or every segment a batch names, once, at its own posting positions
 // (a scratch of the uncompressed layout).  Segments were validated at index
 // creation.
-__global__ void __launch_bounds__(kDecodeWaves * kWave)
-    k_decode_jobs(int codec, const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
+template <int CODEC>
+__global__ void __launch_bounds__(kDecodeWaves * kWave) __attribute__((amdgpu_waves_per_eu(5)))
+    k_decode_jobs(const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
                   const uint2* __restrict__ jobs, uint32_t n_jobs, uint32_t* __restrict__ out)
 {
     __shared__ DecodeScratch S[kDecodeWaves];
     static_assert(sizeof(DecodeScratch) >= kWave * (kPerLane + 1) * 4, "transpose fits the scratch");
     const int wv = threadIdx.x / kWave, lane = lane_id();
     uint32_t* t = reinterpret_cast<uint32_t*>(&S[wv]);  // the transpose, once the decode is done
-    for (uint32_t j = blockIdx.x * kDecodeWaves + wv; j < n_jobs; j += gridDim.x * kDecodeWaves) {
-        const uint2 jb = jobs[j];
-        const uint32_t w0 = cpos[jb.x], nw = cpos[jb.x + 1] - w0;
-        stage_words(S[wv], words, w0, nw);
+    const uint32_t stride = gridDim.x * kDecodeWaves;
+    uint32_t j = blockIdx.x * kDecodeWaves + wv;
+    // the next job's descriptor and word range are loaded while the current
+    // job decodes
+    uint2 jb = make_uint2(0, 0);
+    uint32_t w0 = 0, w1 = 0;
+    if (j < n_jobs) {
+        jb = jobs[j];
+        w0 = cpos[jb.x];
+        w1 = cpos[jb.x + 1];
+    }
+    for (; j < n_jobs; j += stride) {
+        const uint32_t nw = __builtin_amdgcn_readfirstlane(w1 - w0);
+        const uint32_t out0 = __builtin_amdgcn_readfirstlane(jb.y);
+        stage_words_rs(S[wv], words, __builtin_amdgcn_readfirstlane(w0), nw);
+        if (j + stride < n_jobs) {
+            jb = jobs[j + stride];
+            w0 = cpos[jb.x];
+            w1 = cpos[jb.x + 1];
+        }
         uint32_t v[kPerLane];
-        const uint32_t n = min(wave_decode(codec, S[wv], nw, v), (uint32_t)kSegMax);
+        const uint32_t n = __builtin_amdgcn_readfirstlane(
+            min(wave_decode_shape(CodecC<CODEC>{}, S[wv], nw, v), (uint32_t)kSegMax));
         wave_sync();  // every lane is done with the staged words
         // lane l holds values 16 l .. 16 l + 15: through LDS (rows of 17
         // words, so the writes of one k fall in distinct banks) to stores of
-        // 64 consecutive values per instruction
+        // 64 consecutive values per instruction, through a buffer resource
+        // that ends at the segment's n-th value (the stores past it are
+        // dropped)
 #pragma unroll
         for (int k = 0; k < kPerLane; ++k)
             t[lane * (kPerLane + 1) + k] = v[k];
         wave_sync();
+        const __amdgpu_buffer_rsrc_t ro =
+            __builtin_amdgcn_make_buffer_rsrc(out + out0, (short)0, (int)(n * 4u), 0x00020000);
 #pragma unroll
         for (int k = 0; k < kPerLane; ++k) {
             const uint32_t i = (uint32_t)k * kWave + lane;
-            if (i < n)
-                out[jb.y + i] = t[(i / kPerLane) * (kPerLane + 1) + i % kPerLane];
+            __builtin_amdgcn_raw_buffer_store_b32(t[(i / kPerLane) * (kPerLane + 1) + i % kPerLane], ro, i * 4u, 0, 0);
         }
         wave_sync();  // the scratch is reused by the next job
     }
