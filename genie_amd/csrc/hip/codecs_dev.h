@@ -116,6 +116,26 @@ __device__ __forceinline__ uint32_t bp32_prepare(DecodeScratch& S, uint32_t b0, 
     return bad ? 0xFFFFFFFFu : n;
 }
 
+// The same for a segment of a created index, whose block header positions
+// were recorded at creation (hdr: eight u16 word offsets within the segment,
+// two per component) and whose headers were validated then: lane 4 j + m
+// records miniblock m of block j, all in parallel.
+__device__ __forceinline__ uint32_t bp32_prepare_hdr(DecodeScratch& S, uint32_t b0, uint4 hdr)
+{
+    const uint32_t n = S.words[b0];
+    const uint32_t lane = lane_id(), j = lane >> 2, m = lane & 3u;
+    if (lane < 32 && j < (n + 127) / 128) {
+        const uint32_t c = j < 4 ? (j < 2 ? hdr.x : hdr.y) : (j < 6 ? hdr.z : hdr.w);
+        const uint32_t p = (c >> ((j & 1u) * 16u)) & 0xFFFFu;
+        const uint32_t h = S.words[p];
+        const uint32_t w0 = h >> 24, w1 = (h >> 16) & 255u, w2 = (h >> 8) & 255u;
+        const uint32_t before = (m > 0 ? w0 : 0u) + (m > 1 ? w1 : 0u) + (m > 2 ? w2 : 0u);
+        S.mb[lane] = (p + 1u + before) << 8 | ((h >> (24u - 8u * m)) & 255u);
+    }
+    wave_sync();
+    return n;
+}
+
 __device__ __forceinline__ uint32_t bp32_value(const DecodeScratch& S, uint32_t i)
 {
     const uint32_t d = S.mb[i >> 5];
@@ -212,10 +232,11 @@ struct CodecC {
 // Decode the staged segment of `nw` words, shape `sh` (a CodecShape, or a
 // CodecC with the same members fixed at compile time).  Returns the value
 // count, or 0xFFFFFFFF on a malformed segment; vals[j] of lane l is value
-// 16 l + j (undefined past the count).
+// 16 l + j (undefined past the count).  `hdr`: the segment's BP32 block
+// header positions from index creation (null: found by walking the headers).
 template <class SH>
 __device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratch& S, uint32_t nw,
-                                                      uint32_t (&vals)[kPerLane])
+                                                      uint32_t (&vals)[kPerLane], const uint4* hdr = nullptr)
 {
     const int lane = lane_id();
     const uint32_t base = lane * kPerLane;
@@ -228,7 +249,7 @@ __device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratc
         if (1 + b1 > nw)
             return 0xFFFFFFFFu;
         if (b1) {
-            n1 = bp32_prepare(S, 1, 1 + b1);
+            n1 = hdr ? bp32_prepare_hdr(S, 1, *hdr) : bp32_prepare(S, 1, 1 + b1);
             if (n1 == 0xFFFFFFFFu)
                 return n1;
         }
@@ -242,7 +263,7 @@ __device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratc
     case kBp32:
         if (lnw < 1)
             return 0xFFFFFFFFu;
-        n2 = bp32_prepare(S, l0, l0 + lnw);
+        n2 = hdr ? bp32_prepare_hdr(S, l0, *hdr) : bp32_prepare(S, l0, l0 + lnw);
         break;
     case kVarint: n2 = varint_prepare(S, l0, lnw); break;
     }

@@ -115,6 +115,7 @@ struct DevIndex {
     uint32_t* seg_first = nullptr;  // [n_segments] first id of each segment
     uint32_t* seg_last = nullptr;   // [n_segments] last id
     uint32_t* key_seg = nullptr;    // [n_keys + 1] first segment of each key
+    uint4* seg_bph = nullptr;       // [n_segments] BP32 block header word offsets, 8 x u16 (BP32 codecs)
     uint32_t* sdir_off = nullptr;   // [n_keys] row of each key with more than 4 segments (compressed)
     uint32_t* sdir = nullptr;       // graded segment-directory rows (compressed)
     // forward codes of a single-value table (dense form of the batched search)

@@ -493,6 +493,33 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         if (up[i + 1] <= up[i] || up[i + 1] - up[i] > kSegMax || cp[i + 1] <= cp[i] || cp[i + 1] - cp[i] > kSegMax)
             return fail(GENIE_E_INVALID, "segment " + std::to_string(i) + " is empty or longer than " +
                                              std::to_string(kSegMax) + " ids or words");
+    // BP32 codecs: every segment's block header positions (word offsets
+    // within the segment, at most 8 blocks of 128 values), so that decoders
+    // find all miniblocks at once instead of walking the headers; the device
+    // validation pass below rejects any segment these do not describe
+    CodecShape cs{false, false, kCopy};
+    if (!codec_shape(desc->codec, &cs))
+        return fail(GENIE_E_INVALID, "unknown codec " + std::to_string(desc->codec));
+    std::vector<uint4> bph;
+    if (cs.composite || cs.leaf == kBp32) {
+        bph.assign(S, make_uint4(0, 0, 0, 0));
+        for (uint32_t sg = 0; sg < S; ++sg) {
+            const uint32_t* w = desc->words + cp[sg];
+            const uint32_t nw = cp[sg + 1] - cp[sg];
+            const uint32_t o = cs.composite ? 1u : 0u;  // the BP32 part's length word
+            if (o >= nw || (cs.composite && w[0] == 0))
+                continue;
+            const uint32_t n = std::min<uint32_t>(w[o], kSegMax);
+            uint32_t hp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            uint32_t p = o + 1;
+            for (uint32_t j = 0; j < (n + 127) / 128 && p < nw; ++j) {
+                hp[j] = p;
+                const uint32_t h = w[p];
+                p += 1 + (h >> 24) + ((h >> 16) & 255u) + ((h >> 8) & 255u) + (h & 255u);
+            }
+            bph[sg] = make_uint4(hp[0] | hp[1] << 16, hp[2] | hp[3] << 16, hp[4] | hp[5] << 16, hp[6] | hp[7] << 16);
+        }
+    }
     // every key starts on a segment boundary
     std::vector<uint32_t> key_seg(K + 1), seg_key(S);
     for (uint32_t k = 0; k <= K; ++k) {
@@ -584,6 +611,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         if ((rc = dev_alloc(&d.seg_key, S, &d.bytes))) break;
         if ((rc = dev_alloc(&d.seg_first, S, &d.bytes))) break;
         if ((rc = dev_alloc(&d.seg_last, S, &d.bytes))) break;
+        if ((rc = dev_alloc(&d.seg_bph, bph.empty() ? 1 : S, &d.bytes))) break;
         if ((rc = dev_alloc(&d.key_seg, K + 1, &d.bytes))) break;
         if ((rc = dev_alloc(&d.sdir_off, K, &d.bytes))) break;
         if ((rc = dev_alloc(&d.sdir, sdir_words, &d.bytes))) break;
@@ -594,6 +622,7 @@ static int create_compressed(const genie_index_desc* desc, genie_index** out)
         const uint32_t zero = 0;
         if ((e = hipMemcpy(d.words, desc->words, d.n_words * 4, hipMemcpyHostToDevice)) ||
             (e = hipMemcpy(d.seg_cpos, cp, (S + 1) * 4, hipMemcpyHostToDevice)) ||
+            (!bph.empty() && (e = hipMemcpy(d.seg_bph, bph.data(), (uint64_t)S * 16, hipMemcpyHostToDevice))) ||
             (e = hipMemcpy(d_upos, up, (S + 1) * 4, hipMemcpyHostToDevice)) ||
             (e = hipMemcpy(d.seg_key, seg_key.data(), S * 4, hipMemcpyHostToDevice)) ||
             (e = hipMemcpy(d.key_seg, key_seg.data(), (K + 1) * 4, hipMemcpyHostToDevice)) ||
@@ -1307,6 +1336,7 @@ int genie_hip_index_destroy(genie_index* ix)
     dev_free(d.seg_key);
     dev_free(d.seg_first);
     dev_free(d.seg_last);
+    dev_free(d.seg_bph);
     dev_free(d.key_seg);
     dev_free(d.sdir_off);
     dev_free(d.sdir);
@@ -2393,19 +2423,19 @@ static int launch_decode_jobs(const DevIndex& d, const uint2* jobs, uint32_t nj,
     const uint32_t g = std::min<uint32_t>((nj + kDecodeWaves - 1) / kDecodeWaves, 65535);
     const dim3 b(kDecodeWaves * kWave);
     switch (d.codec) {
-    case 1: k_decode_jobs<1><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 2: k_decode_jobs<2><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 3: k_decode_jobs<3><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 4: k_decode_jobs<4><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 5: k_decode_jobs<5><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 6: k_decode_jobs<6><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 7: k_decode_jobs<7><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 8: k_decode_jobs<8><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 9: k_decode_jobs<9><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 10: k_decode_jobs<10><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 11: k_decode_jobs<11><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 12: k_decode_jobs<12><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
-    case 13: k_decode_jobs<13><<<g, b, 0, s>>>(d.words, d.seg_cpos, jobs, nj, out); break;
+    case 1: k_decode_jobs<1><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 2: k_decode_jobs<2><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 3: k_decode_jobs<3><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 4: k_decode_jobs<4><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 5: k_decode_jobs<5><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 6: k_decode_jobs<6><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 7: k_decode_jobs<7><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 8: k_decode_jobs<8><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 9: k_decode_jobs<9><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 10: k_decode_jobs<10><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 11: k_decode_jobs<11><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 12: k_decode_jobs<12><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
+    case 13: k_decode_jobs<13><<<g, b, 0, s>>>(d.words, d.seg_cpos, d.seg_bph, jobs, nj, out); break;
     default: return fail(GENIE_E_INVALID, "unknown codec " + std::to_string(d.codec));
     }
     return GENIE_OK;

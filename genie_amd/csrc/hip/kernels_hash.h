@@ -39,7 +39,8 @@ struct HashArgs {
 template <int CODEC>
 __global__ void __launch_bounds__(kDecodeWaves * kWave) __attribute__((amdgpu_waves_per_eu(5)))
     k_decode_jobs(const uint32_t* __restrict__ words, const uint32_t* __restrict__ cpos,
-                  const uint2* __restrict__ jobs, uint32_t n_jobs, uint32_t* __restrict__ out)
+                  const uint4* __restrict__ bph, const uint2* __restrict__ jobs, uint32_t n_jobs,
+                  uint32_t* __restrict__ out)
 {
     __shared__ DecodeScratch S[kDecodeWaves];
     static_assert(sizeof(DecodeScratch) >= kWave * (kPerLane + 1) * 4, "transpose fits the scratch");
@@ -49,25 +50,32 @@ __global__ void __launch_bounds__(kDecodeWaves * kWave) __attribute__((amdgpu_wa
     uint32_t j = blockIdx.x * kDecodeWaves + wv;
     // the next job's descriptor and word range are loaded while the current
     // job decodes
+    constexpr bool HDR = CodecC<CODEC>::composite || CodecC<CODEC>::leaf == kBp32;
     uint2 jb = make_uint2(0, 0);
     uint32_t w0 = 0, w1 = 0;
+    uint4 hd = make_uint4(0, 0, 0, 0);
     if (j < n_jobs) {
         jb = jobs[j];
         w0 = cpos[jb.x];
         w1 = cpos[jb.x + 1];
+        if constexpr (HDR)
+            hd = bph[jb.x];
     }
     for (; j < n_jobs; j += stride) {
         const uint32_t nw = __builtin_amdgcn_readfirstlane(w1 - w0);
         const uint32_t out0 = __builtin_amdgcn_readfirstlane(jb.y);
         stage_words_rs(S[wv], words, __builtin_amdgcn_readfirstlane(w0), nw);
+        const uint4 hdr = hd;
         if (j + stride < n_jobs) {
             jb = jobs[j + stride];
             w0 = cpos[jb.x];
             w1 = cpos[jb.x + 1];
+            if constexpr (HDR)
+                hd = bph[jb.x];
         }
         uint32_t v[kPerLane];
         const uint32_t n = __builtin_amdgcn_readfirstlane(
-            min(wave_decode_shape(CodecC<CODEC>{}, S[wv], nw, v), (uint32_t)kSegMax));
+            min(wave_decode_shape(CodecC<CODEC>{}, S[wv], nw, v, HDR ? &hdr : nullptr), (uint32_t)kSegMax));
         wave_sync();  // every lane is done with the staged words
         // lane l holds values 16 l .. 16 l + 15: through LDS (rows of 17
         // words, so the writes of one k fall in distinct banks) to stores of
