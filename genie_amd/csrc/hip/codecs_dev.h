@@ -214,6 +214,14 @@ __device__ __forceinline__ uint32_t varint_value(const DecodeScratch& S, uint32_
     return len >= 5u ? v : v & ((1u << (7u * len)) - 1u);
 }
 
+// Codecs with BP32 blocks (their segments' block header positions are
+// recorded at index creation).
+__host__ __device__ constexpr bool codec_has_bp32(int codec)
+{
+    CodecShape s{false, false, kCopy};
+    return codec_shape(codec, &s) && (s.composite || s.leaf == kBp32);
+}
+
 // A codec's shape fixed at compile time (a kernel instantiated per codec
 // keeps only that codec's steps and registers).
 __host__ __device__ constexpr CodecShape codec_shape_of(int codec)
@@ -232,11 +240,12 @@ struct CodecC {
 // Decode the staged segment of `nw` words, shape `sh` (a CodecShape, or a
 // CodecC with the same members fixed at compile time).  Returns the value
 // count, or 0xFFFFFFFF on a malformed segment; vals[j] of lane l is value
-// 16 l + j (undefined past the count).  `hdr`: the segment's BP32 block
-// header positions from index creation (null: found by walking the headers).
+// 16 l + j (undefined past the count).  With `use_hdr`, `hdr` holds the
+// segment's BP32 block header positions from index creation (otherwise the
+// decoder walks the headers).
 template <class SH>
 __device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratch& S, uint32_t nw,
-                                                      uint32_t (&vals)[kPerLane], const uint4* hdr = nullptr)
+                                                      uint32_t (&vals)[kPerLane], uint4 hdr = {}, bool use_hdr = false)
 {
     const int lane = lane_id();
     const uint32_t base = lane * kPerLane;
@@ -249,7 +258,7 @@ __device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratc
         if (1 + b1 > nw)
             return 0xFFFFFFFFu;
         if (b1) {
-            n1 = hdr ? bp32_prepare_hdr(S, 1, *hdr) : bp32_prepare(S, 1, 1 + b1);
+            n1 = use_hdr ? bp32_prepare_hdr(S, 1, hdr) : bp32_prepare(S, 1, 1 + b1);
             if (n1 == 0xFFFFFFFFu)
                 return n1;
         }
@@ -263,7 +272,7 @@ __device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratc
     case kBp32:
         if (lnw < 1)
             return 0xFFFFFFFFu;
-        n2 = hdr ? bp32_prepare_hdr(S, l0, *hdr) : bp32_prepare(S, l0, l0 + lnw);
+        n2 = use_hdr ? bp32_prepare_hdr(S, l0, hdr) : bp32_prepare(S, l0, l0 + lnw);
         break;
     case kVarint: n2 = varint_prepare(S, l0, lnw); break;
     }
@@ -297,12 +306,12 @@ __device__ __forceinline__ uint32_t wave_decode_shape(const SH& sh, DecodeScratc
 
 // The same with the codec chosen at run time.
 __device__ __forceinline__ uint32_t wave_decode(int codec, DecodeScratch& S, uint32_t nw,
-                                                uint32_t (&vals)[kPerLane])
+                                                uint32_t (&vals)[kPerLane], uint4 hdr = {}, bool use_hdr = false)
 {
     CodecShape sh{false, false, kCopy};
     if (!codec_shape(codec, &sh))
         return 0xFFFFFFFFu;
-    return wave_decode_shape(sh, S, nw, vals);
+    return wave_decode_shape(sh, S, nw, vals, hdr, use_hdr);
 }
 
 // Stage words [w0, w0 + nw) of global memory into the wave's scratch

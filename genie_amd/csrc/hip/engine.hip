@@ -2358,6 +2358,7 @@ static ScanArgs scan_args(genie_plan* p)
     a.seg_first = d.seg_first;
     a.seg_last = d.seg_last;
     a.key_seg = d.key_seg;
+    a.seg_bph = (d.codec > 0 && codec_has_bp32(d.codec)) ? d.seg_bph : nullptr;
     a.sdir_off = d.sdir_off;
     a.sdir = d.sdir;
     a.post = d.post;
@@ -3599,6 +3600,7 @@ int genie_hip_match_topk_static(genie_index* ix, const genie_dim_t* items, uint6
     a.seg_first = d.seg_first;
     a.seg_last = d.seg_last;
     a.key_seg = d.key_seg;
+    a.seg_bph = (d.codec > 0 && codec_has_bp32(d.codec)) ? d.seg_bph : nullptr;
     a.sdir_off = d.sdir_off;
     a.sdir = d.sdir;
     a.post = d.post;

@@ -495,6 +495,7 @@ struct ScanArgs {
     const uint32_t* seg_first;
     const uint32_t* seg_last;
     const uint32_t* key_seg;
+    const uint4* seg_bph;     // BP32 block header positions per segment (null: walk the headers)
     const uint32_t* sdir_off;
     const uint32_t* sdir;
     // uncompressed index
@@ -1300,7 +1301,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_tile_c(ScanArgs a)
             const uint32_t cw0 = a.seg_cpos[sg], nw = a.seg_cpos[sg + 1] - cw0;
             stage_words(D, a.words, cw0, nw);
             uint32_t v[kPerLane];
-            const uint32_t n = wave_decode(a.codec, D, nw, v);
+            const uint32_t n = wave_decode(a.codec, D, nw, v, a.seg_bph ? a.seg_bph[sg] : uint4{}, a.seg_bph != nullptr);
             const uint32_t w = sp.wf >> 1;
 #pragma unroll
             for (int j = 0; j < kPerLane; ++j) {

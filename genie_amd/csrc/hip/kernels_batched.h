@@ -198,7 +198,8 @@ __global__ void __launch_bounds__(NT) k_scan_batched(BatchArgs b)
                     const uint32_t cw0 = a.seg_cpos[sg], nw = a.seg_cpos[sg + 1] - cw0;
                     stage_words(D, a.words, cw0, nw);
                     uint32_t v[kPerLane];
-                    const uint32_t n = wave_decode(a.codec, D, nw, v);
+                    const uint32_t n = wave_decode(a.codec, D, nw, v, a.seg_bph ? a.seg_bph[sg] : uint4{},
+                                                     a.seg_bph != nullptr);
 #pragma unroll
                     for (int j = 0; j < kPerLane; ++j) {
                         const uint32_t i = lane * kPerLane + j;

@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kDecodeWaves * kWave) __attribute__((amdgpu_wa
         }
         uint32_t v[kPerLane];
         const uint32_t n = __builtin_amdgcn_readfirstlane(
-            min(wave_decode_shape(CodecC<CODEC>{}, S[wv], nw, v, HDR ? &hdr : nullptr), (uint32_t)kSegMax));
+            min(wave_decode_shape(CodecC<CODEC>{}, S[wv], nw, v, hdr, HDR), (uint32_t)kSegMax));
         wave_sync();  // every lane is done with the staged words
         // lane l holds values 16 l .. 16 l + 15: through LDS (rows of 17
         // words, so the writes of one k fall in distinct banks) to stores of
@@ -362,7 +362,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
                 const uint32_t cw0 = a.seg_cpos[sg], nw = a.seg_cpos[sg + 1] - cw0;
                 stage_words(D, a.words, cw0, nw);
                 uint32_t v[kPerLane];
-                const uint32_t n = wave_decode(a.codec, D, nw, v);
+                const uint32_t n = wave_decode(a.codec, D, nw, v, a.seg_bph ? a.seg_bph[sg] : uint4{},
+                                                     a.seg_bph != nullptr);
                 const uint32_t w = sp.wf >> 1;
 #pragma unroll
                 for (int j = 0; j < kPerLane; ++j) {
