@@ -127,15 +127,19 @@ __device__ __forceinline__ bool hash_add(uint32_t* tab, uint32_t lhs, uint32_t k
 // rank = count << kb | (2^kb - 1 - key), i.e. count descending, then id
 // ascending (ranks are unique and nonzero).  Every thread loads its slots
 // (tid + 512 j) once as ranks into registers (0 = not eligible); a radix
-// select over the rank bytes, from the highest byte in which the eligible
-// ranks differ, finds the k-th.  Appends the selected keys to the query's
-// candidates and raises the threshold to the tile's k-th count.
+// select over 11-bit digits of the ranks, starting at the highest bit in
+// which the eligible ranks differ, finds the k-th, its histograms in the
+// table's LDS (free once the ranks are in registers).  Appends the selected
+// keys to the query's candidates and raises the threshold to the tile's
+// k-th count.
 constexpr int kHashPer = (1 << kHashLogMax) / kScanThreads;  // slots per thread at most
+constexpr uint32_t kSelDigit = 11;                             // bits per radix pass
+constexpr uint32_t kSelBins = 1u << kSelDigit;
 
-__device__ __forceinline__ void hash_select(const ScanArgs& a, const uint32_t* tab, uint32_t ns, uint32_t kb,
-                                            uint32_t q, uint32_t obj0, uint32_t* s_hist, uint32_t* s_res,
-                                            uint32_t* s_red)
+__device__ __forceinline__ void hash_select(const ScanArgs& a, uint32_t* tab, uint32_t ns, uint32_t kb,
+                                            uint32_t q, uint32_t obj0, uint32_t* s_res, uint32_t* s_red)
 {
+    static_assert(kSelBins <= (1u << kHashLogMax) && kSelBins == 32 * kWave, "histogram in the table, 32 bins a lane");
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6, nw = blockDim.x / kWave;
     const uint32_t k = a.kq[q];
     const uint32_t cb = 32u - kb;
@@ -171,7 +175,7 @@ __device__ __forceinline__ void hash_select(const ScanArgs& a, const uint32_t* t
         s_red[kMaxWaves + wv] = rmax;
         s_red[2 * kMaxWaves + wv] = rmin;
     }
-    __syncthreads();
+    __syncthreads();  // every slot is in registers: the table is free from here
     uint32_t E = 0, Rmax = 0, Rmin = 0xFFFFFFFFu;
     for (int i = 0; i < nw; ++i) {
         E += s_red[i];
@@ -184,49 +188,53 @@ __device__ __forceinline__ void hash_select(const ScanArgs& a, const uint32_t* t
     uint32_t kth = 0;  // every nonzero rank >= kth is selected
     if (E > k) {
         // bits above the highest one in which Rmax and Rmin differ are common
-        // to every eligible rank
-        const int top = 31 - __builtin_clz(Rmax ^ Rmin);  // Rmax != Rmin since E > k >= 1
-        int shift = top & ~7;
-        uint32_t mask = shift + 8 >= 32 ? 0u : ~0u << (shift + 8);
+        // to every eligible rank; digits are taken from there down
+        uint32_t* hist = tab;
+        uint32_t lo_bit = 32u - (uint32_t)__builtin_clz(Rmax ^ Rmin);  // Rmax != Rmin since E > k >= 1
+        uint32_t mask = lo_bit >= 32u ? 0u : ~0u << lo_bit;
         uint32_t prefix = Rmax & mask, remaining = k;
 #pragma unroll 1
-        for (; shift >= 0; shift -= 8) {
-            if (tid < 256)
-                s_hist[tid] = 0;
+        while (lo_bit > 0) {
+            const uint32_t width = min(kSelDigit, lo_bit), shift = lo_bit - width;
+            const uint32_t dmask = (1u << width) - 1u;
+            for (uint32_t i = tid; i < kSelBins; i += blockDim.x)
+                hist[i] = 0;
             __syncthreads();
 #pragma unroll
-            for (int j = 0; j < kHashPer; ++j)
-                if ((uint32_t)j < per)
-                    hist_add_leader(s_hist, (rk[j] >> shift) & 255u, rk[j] != 0 && (rk[j] & mask) == prefix);
+            for (int j = 0; j < kHashPer; ++j) {
+                const uint32_t r = rk[j];
+                if ((uint32_t)j < per && r != 0 && (r & mask) == prefix)
+                    atomicAdd(&hist[(r >> shift) & dmask], 1u);
+            }
             __syncthreads();
             if (tid < kWave) {
-                // lane l holds bins 255 - 4l .. 252 - 4l, counted from the top
-                const uint32_t tb = 255u - 4u * (uint32_t)tid;
-                uint32_t hb[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    hb[j] = s_hist[tb - j];
-                const uint32_t sum = hb[0] + hb[1] + hb[2] + hb[3];
+                // lane l holds bins 2047 - 32 l .. 2016 - 32 l, counted from the top
+                const uint32_t top = kSelBins - 1u - 32u * (uint32_t)tid;
+                uint32_t sum = 0;
+#pragma unroll 8
+                for (int j = 0; j < 32; ++j)
+                    sum += hist[top - j];
                 uint32_t above = wave_incl_scan(sum) - sum;
                 if (above < remaining && above + sum >= remaining) {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        if (above + hb[j] >= remaining) {
-                            s_res[1] = tb - j;
+                    for (int j = 0; j < 32; ++j) {
+                        const uint32_t h = hist[top - j];
+                        if (above + h >= remaining) {
+                            s_res[1] = top - j;
                             s_res[2] = remaining - above;
-                            s_res[3] = hb[j];
+                            s_res[3] = h;
                             break;
                         }
-                        above += hb[j];
+                        above += h;
                     }
                 }
             }
             __syncthreads();
             const uint32_t bin = s_res[1], rem = s_res[2], inbin = s_res[3];
             prefix |= bin << shift;
-            mask |= 255u << shift;
+            mask |= dmask << shift;
             remaining = rem;
-            __syncthreads();
+            lo_bit = shift;
+            __syncthreads();  // s_res and the histogram are read before the next pass
             if (inbin == rem)
                 break;  // the whole bin is selected: the prefix bounds it
         }
@@ -314,7 +322,6 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
     __shared__ Span s_span[kSpanCap];
     __shared__ uint32_t s_wsum[kScanWaves];
     __shared__ uint32_t s_red[3 * kMaxWaves];
-    __shared__ uint32_t s_hist[256];
     __shared__ uint32_t s_res[8];
     __shared__ uint32_t s_next;
     __shared__ DecodeScratch s_dec[C ? kScanWaves : 1];
@@ -415,7 +422,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
         }
         return;
     }
-    hash_select(a, s_tab, ns, kb, q, obj0, s_hist, s_res, s_red);
+    hash_select(a, s_tab, ns, kb, q, obj0, s_res, s_red);
 }
 
 }  // namespace gh
