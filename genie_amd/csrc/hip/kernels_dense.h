@@ -124,7 +124,10 @@ constexpr uint32_t kDenseNoQuery = 255u;
 constexpr int kSetPrefetch = 8;  // forward-list columns in flight per wave
 constexpr uint32_t kSetWindowTiles = 8;  // tiles per XCD counted together (main pass)
 
-__device__ __forceinline__ uint32_t set_hash(uint32_t key) { return (key * 0x9E3779B1u) >> 20; }  // 12 bits
+// 12-bit slot of a key: bits 20..31 of key x 0x9E3779 (a full-rate 24-bit
+// multiply; bits of a key above 24 do not enter the hash, which only spreads
+// such keys less evenly).
+__device__ __forceinline__ uint32_t set_hash(uint32_t key) { return ((key & 0xFFFFFFu) * 0x9E3779u) >> 20; }
 
 // ---- index creation --------------------------------------------------------
 
@@ -484,25 +487,36 @@ __host__ __device__ constexpr uint32_t set_lds_bytes(bool sample)
     return (kSetSlots * 7 + 16 + (sample ? 32 * 256 : kDenseStageWords)) * 4u;
 }
 
-// c += x, bit-sliced: x is a six-plane number per query (weight planes
-// x0..x5), c an NP-plane count; full adders (xor3 and majority in one
-// v_bitop3 each) over the weight planes, then the carry ripples up (two
-// ops a plane, no branch: a branch on the rare upper carry costs more in
-// register moves at its join than the three planes it skips).
-template <int NP>
-__device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
+// Bit-sliced sum of two N-plane numbers (ripple of full adders, two VALU ops
+// a plane): N + 1 planes.
+template <int N>
+__device__ __forceinline__ void bs_add(const uint32_t (&x)[N], const uint32_t (&y)[N], uint32_t (&z)[N + 1])
 {
-    const uint32_t x[kSetWeightPlanes] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y};
+    uint32_t carry = x[0] & y[0];
+    z[0] = x[0] ^ y[0];
+#pragma unroll
+    for (int b = 1; b < N; ++b) {
+        z[b] = __builtin_amdgcn_bitop3_b32(x[b], y[b], carry, 0x96);              // xor3
+        carry = __builtin_amdgcn_bitop3_b32(x[b], y[b], carry, 0xE8);           // majority
+    }
+    z[N] = carry;
+}
+
+// c += x for an N-plane x into the NP-plane count (N <= NP; counts stay
+// below 2^NP).
+template <int NP, int N>
+__device__ __forceinline__ void bs_acc(uint32_t (&c)[NP], const uint32_t (&x)[N])
+{
     uint32_t carry = c[0] & x[0];
     c[0] ^= x[0];
 #pragma unroll
-    for (int b = 1; b < kSetWeightPlanes; ++b) {
-        const uint32_t t = __builtin_amdgcn_bitop3_b32(c[b], x[b], carry, 0xE8);  // majority
-        c[b] = __builtin_amdgcn_bitop3_b32(c[b], x[b], carry, 0x96);              // xor3
+    for (int b = 1; b < N; ++b) {
+        const uint32_t t = __builtin_amdgcn_bitop3_b32(c[b], x[b], carry, 0xE8);
+        c[b] = __builtin_amdgcn_bitop3_b32(c[b], x[b], carry, 0x96);
         carry = t;
     }
 #pragma unroll
-    for (int b = kSetWeightPlanes; b < NP; ++b) {
+    for (int b = N; b < NP; ++b) {
         const uint32_t t = c[b] & carry;
         c[b] ^= carry;
         carry = t;
@@ -515,13 +529,17 @@ __device__ __forceinline__ void set_add(uint32_t (&c)[NP], uint4 lo, uint2 hi)
 // longest list times each query's largest weight per key does).  The slots of
 // a wave form one block, whose lists it reads a column (64 words) per step.
 // WIDE: some weight of the group reaches 16 (planes 4-5 are read), chosen once
-// per group.  No branch per occurrence but the rare extra probes: a branch
-// whose paths rejoin with new count planes costs more in register moves than
-// the work it could skip.
+// per group.  The kSetPrefetch = 8 weights of one step are summed by a tree
+// of bit-sliced adders (4 + 2 + 1 adds of 4 to 6 planes), then added to the
+// count planes once: the carry ripples through the upper count planes once
+// per step instead of once per occurrence.  No branch per occurrence but the
+// rare extra probes.
 template <int NP, bool WIDE>
 __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, const uint32_t* s_key,
                                                const uint4* s_mlo, const uint2* s_mhi, uint32_t (&c)[NP])
 {
+    static_assert(kSetPrefetch == 8, "an adder tree of eight weights");
+    constexpr int P = WIDE ? kSetWeightPlanes : 4;  // weight planes read (weights < 16 unless WIDE)
 #pragma unroll
     for (int b = 0; b < NP; ++b)
         c[b] = 0;
@@ -534,12 +552,13 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u)
             kk[u] = w + 64u * u < w1 ? a.blk_key[w + 64u * u] : kSetEmpty;
+        // probe to the key or an empty slot (the table is at most half full,
+        // so every probe sequence ends there); an empty slot's weights are
+        // zero (a key the group does not hold, or kSetEmpty past the end of
+        // the list)
+        uint32_t x[kSetPrefetch][P];
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u) {
-            // probe to the key or an empty slot (the table is at most half
-            // full, so every probe sequence ends there), then add the slot's
-            // weights: an empty slot's are zero (a key the group does not
-            // hold, or kSetEmpty past the end of the list)
             const uint32_t key = kk[u];
             uint32_t h = set_hash(key);
             uint32_t k = s_key[h];
@@ -547,8 +566,25 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
                 h = (h + 1u) & (kSetSlots - 1u);
                 k = s_key[h];
             }
-            set_add<NP>(c, s_mlo[h], WIDE ? s_mhi[h] : make_uint2(0, 0));
+            const uint4 lo = s_mlo[h];
+            x[u][0] = lo.x;
+            x[u][1] = lo.y;
+            x[u][2] = lo.z;
+            x[u][3] = lo.w;
+            if constexpr (WIDE) {
+                const uint2 hi = s_mhi[h];
+                x[u][4] = hi.x;
+                x[u][5] = hi.y;
+            }
         }
+        uint32_t y[4][P + 1], z[2][P + 2], t[P + 3];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            bs_add<P>(x[2 * i], x[2 * i + 1], y[i]);
+        bs_add<P + 1>(y[0], y[1], z[0]);
+        bs_add<P + 1>(y[2], y[3], z[1]);
+        bs_add<P + 2>(z[0], z[1], t);
+        bs_acc<NP, P + 3>(c, t);
     }
 }
 
