@@ -110,11 +110,18 @@ struct DenseArgs {
     const uint32_t* hshift;      // [n_queries] sample histogram bin = count >> hshift (set form)
 };
 
-// Set form: LDS hash table of a group's keys (at most kSetEntries of them,
-// at most half full).  Weights up to 63 (six planes); counts up to 2046 in
-// eleven planes, 2047 being the threshold of "no query".
-constexpr uint32_t kSetSlots = 4096;
-constexpr uint32_t kSetEntries = kSetSlots / 2;
+// Set form: LDS hash table of a group's keys (at most kSetEntries of them):
+// kSetSlots slots of (key, entry), at most a quarter full, and the entries'
+// weight planes apart from the slots (entry kSetEntries: zero weights, the
+// entry of every empty slot).  Weights up to 63 (six planes); counts up to
+// 2046 in eleven planes, 2047 being the threshold of "no query".
+constexpr uint32_t kSetSlots = 8192;
+constexpr uint32_t kSetEntries = 2048;
+// LDS words of the table: slots, then the entries' planes 0-3 (uint4), then
+// planes 4-5 (uint2), rounded up to a multiple of 8 words
+constexpr uint32_t kSetMloWord = 2 * kSetSlots;
+constexpr uint32_t kSetMhiWord = kSetMloWord + 4 * (kSetEntries + 1);
+constexpr uint32_t kSetTabWords = (kSetMhiWord + 2 * (kSetEntries + 1) + 7) & ~7u;
 constexpr uint32_t kSetEmpty = 0xFFFFFFFFu;
 constexpr int kSetWeightPlanes = 6;
 constexpr int kSetCountPlanes = 11;
@@ -124,10 +131,11 @@ constexpr uint32_t kDenseNoQuery = 255u;
 constexpr int kSetPrefetch = 8;  // forward-list columns in flight per wave
 constexpr uint32_t kSetWindowTiles = 8;  // tiles per XCD counted together (main pass)
 
-// 12-bit slot of a key: bits 20..31 of key x 0x9E3779 (a full-rate 24-bit
+// 13-bit slot of a key: bits 19..31 of key x 0x9E3779 (a full-rate 24-bit
 // multiply; bits of a key above 24 do not enter the hash, which only spreads
 // such keys less evenly).
-__device__ __forceinline__ uint32_t set_hash(uint32_t key) { return ((key & 0xFFFFFFu) * 0x9E3779u) >> 20; }
+static_assert(kSetSlots == 1u << 13, "13-bit slot hash");
+__device__ __forceinline__ uint32_t set_hash(uint32_t key) { return ((key & 0xFFFFFFu) * 0x9E3779u) >> 19; }
 
 // ---- index creation --------------------------------------------------------
 
@@ -484,7 +492,7 @@ __host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample)
 // query) or candidate stage.
 __host__ __device__ constexpr uint32_t set_lds_bytes(bool sample)
 {
-    return (kSetSlots * 7 + 16 + (sample ? 32 * 256 : kDenseStageWords)) * 4u;
+    return (kSetTabWords + 16 + (sample ? 32 * 256 : kDenseStageWords)) * 4u;
 }
 
 // Bit-sliced sum of two N-plane numbers (ripple of full adders, two VALU ops
@@ -535,7 +543,7 @@ __device__ __forceinline__ void bs_acc(uint32_t (&c)[NP], const uint32_t (&x)[N]
 // per step instead of once per occurrence.  No branch per occurrence but the
 // rare extra probes.
 template <int NP, bool WIDE>
-__device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, const uint32_t* s_key,
+__device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, const uint2* s_slot,
                                                const uint4* s_mlo, const uint2* s_mhi, uint32_t (&c)[NP])
 {
     static_assert(kSetPrefetch == 8, "an adder tree of eight weights");
@@ -552,27 +560,28 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u)
             kk[u] = w + 64u * u < w1 ? a.blk_key[w + 64u * u] : kSetEmpty;
-        // probe to the key or an empty slot (the table is at most half full,
-        // so every probe sequence ends there); an empty slot's weights are
-        // zero (a key the group does not hold, or kSetEmpty past the end of
-        // the list)
+        // probe to the key or an empty slot (the table is at most a quarter
+        // full, so every probe sequence ends there, mostly at the first
+        // slot); an empty slot's entry has zero weights (a key the group does
+        // not hold, or kSetEmpty past the end of the list)
         uint32_t x[kSetPrefetch][P];
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u) {
             const uint32_t key = kk[u];
             uint32_t h = set_hash(key);
-            uint32_t k = s_key[h];
-            for (uint32_t probe = 1; k != key && k != kSetEmpty && probe < kSetSlots; ++probe) {
+            uint2 sv = s_slot[h];
+            for (uint32_t probe = 1; sv.x != key && sv.x != kSetEmpty && probe < kSetSlots; ++probe) {
                 h = (h + 1u) & (kSetSlots - 1u);
-                k = s_key[h];
+                sv = s_slot[h];
             }
-            const uint4 lo = s_mlo[h];
+            const uint32_t e = sv.y;
+            const uint4 lo = s_mlo[e];
             x[u][0] = lo.x;
             x[u][1] = lo.y;
             x[u][2] = lo.z;
             x[u][3] = lo.w;
             if constexpr (WIDE) {
-                const uint2 hi = s_mhi[h];
+                const uint2 hi = s_mhi[e];
                 x[u][4] = hi.x;
                 x[u][5] = hi.y;
             }
@@ -607,10 +616,10 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     constexpr uint32_t NONE = SETS ? kSetNoQuery : kDenseNoQuery;
     __shared__ uint32_t s_lds[(SETS ? set_lds_bytes(SAMPLE) : dense_lds_bytes(NBLK, SAMPLE)) / 4];
     uint32_t* s_tab = s_lds;                             // DPAD * 256 masks
-    uint32_t* s_skey = s_lds;                            // SETS: kSetSlots keys, then the weight planes
-    uint4* s_mlo = reinterpret_cast<uint4*>(s_lds + kSetSlots);
-    uint2* s_mhi = reinterpret_cast<uint2*>(s_lds + kSetSlots * 5);
-    uint32_t* s_thr = s_lds + (SETS ? kSetSlots * 7 : DPAD * kDenseKeysPerDim);  // NP threshold planes
+    uint2* s_slot = reinterpret_cast<uint2*>(s_lds);    // SETS: (key, entry) slots, then the entries' planes
+    uint4* s_mlo = reinterpret_cast<uint4*>(s_lds + kSetMloWord);
+    uint2* s_mhi = reinterpret_cast<uint2*>(s_lds + kSetMhiWord);
+    uint32_t* s_thr = s_lds + (SETS ? kSetTabWords : DPAD * kDenseKeysPerDim);  // NP threshold planes
     uint32_t* s_hist = s_thr + 16;                       // SAMPLE: 32 x (DPAD + 1), SETS 32 x 256
     // main pass: the tile's staged candidates
     uint32_t* s_nhit = s_thr + 16;                       // staged so far
@@ -659,26 +668,29 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
         bool wide = false;
         if constexpr (SETS) {
             // the group's key table, built from its entries (unique keys);
-            // empty slots hold zero weights
-            for (uint32_t i = tid; i < kSetSlots; i += NT) {
-                s_skey[i] = kSetEmpty;
-                s_mlo[i] = make_uint4(0, 0, 0, 0);
-                s_mhi[i] = make_uint2(0, 0);
+            // empty slots point at the zero entry
+            for (uint32_t i = tid; i < kSetSlots; i += NT)
+                s_slot[i] = make_uint2(kSetEmpty, kSetEntries);
+            if (tid == 0) {
+                s_mlo[kSetEntries] = make_uint4(0, 0, 0, 0);
+                s_mhi[kSetEntries] = make_uint2(0, 0);
             }
             if constexpr (SAMPLE)
                 for (uint32_t i = tid; i < 32 * HB; i += NT)
                     s_hist[i] = 0;
             __syncthreads();
             int hi_any = 0;
-            for (uint32_t e = a.set_gptr[g] + tid; e < a.set_gptr[g + 1]; e += NT) {
+            const uint32_t e0 = a.set_gptr[g];
+            for (uint32_t e = e0 + tid; e < a.set_gptr[g + 1]; e += NT) {
                 const uint32_t key = a.set_keys[e];
                 const uint4 lo = a.set_mask[2 * e], hi = a.set_mask[2 * e + 1];
                 hi_any |= (hi.x | hi.y) != 0u;
+                s_mlo[e - e0] = lo;
+                s_mhi[e - e0] = make_uint2(hi.x, hi.y);
                 uint32_t h = set_hash(key);
                 for (uint32_t probe = 0; probe < kSetSlots; ++probe) {
-                    if (atomicCAS(&s_skey[h], kSetEmpty, key) == kSetEmpty) {
-                        s_mlo[h] = lo;
-                        s_mhi[h] = make_uint2(hi.x, hi.y);
+                    if (atomicCAS(&s_slot[h].x, kSetEmpty, key) == kSetEmpty) {
+                        s_slot[h].y = e - e0;
                         break;
                     }
                     h = (h + 1u) & (kSetSlots - 1u);
@@ -767,9 +779,9 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 uint32_t o = ol;
                 if constexpr (SETS) {
                     if (wide)  // uniform across the workgroup
-                        set_accumulate<NP, true>(a, ol, s_skey, s_mlo, s_mhi, c);
+                        set_accumulate<NP, true>(a, ol, s_slot, s_mlo, s_mhi, c);
                     else
-                        set_accumulate<NP, false>(a, ol, s_skey, s_mlo, s_mhi, c);
+                        set_accumulate<NP, false>(a, ol, s_slot, s_mlo, s_mhi, c);
                     o = a.obj_perm[ol];  // kSetEmpty past the last object: counts 0, nothing emitted
                 } else {
                     dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
