@@ -14,8 +14,8 @@
 
 namespace gh {
 
-constexpr int kHashLogMax = 14;         // 16,384 slots = 64 KiB of LDS
-constexpr uint32_t kHashTarget = 8192;  // postings per (query, tile) aimed at
+constexpr int kHashLogMax = 13;         // 8,192 slots = 32 KiB of LDS
+constexpr uint32_t kHashTarget = 4096;  // postings per (query, tile) aimed at
 
 struct HashArgs {
     ScanArgs s;
@@ -324,7 +324,6 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
     __shared__ uint32_t s_red[3 * kMaxWaves];
     __shared__ uint32_t s_res[8];
     __shared__ uint32_t s_next;
-    __shared__ DecodeScratch s_dec[C ? kScanWaves : 1];
 
     const ScanArgs& a = h.s;
     const uint32_t q = h.work[2 * blockIdx.x], t = h.work[2 * blockIdx.x + 1];
@@ -354,6 +353,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
             resolve_seg_spans(a, c0, c1, pbase, w0, wn, dt0, dt1, obj0 + nobj, s_span);
             __syncthreads();
             const uint32_t nsegs = flat_prefix(s_span, wn, s_wsum);
+            __shared__ DecodeScratch s_dec[kScanWaves];  // the compressed variant's LDS only
             DecodeScratch& D = s_dec[wv];
             for (;;) {
                 uint32_t g = 0;
