@@ -86,6 +86,35 @@ def stage_ms(plan, reps, names=("compile", "sample", "scan", "merge")):
     return {k: float(np.mean(v)) for k, v in acc.items()}
 
 
+def segment_bytes(export, items):
+    """Algorithmic bytes of decoding the segments a compressed batch names,
+    each once: their words read plus their postings written (4 B each)."""
+    cpos = export["cpos"].astype(np.int64)
+    upos = export["inv_pos"].astype(np.int64)
+    s0 = np.searchsorted(cpos, items["start_pos"].astype(np.int64))
+    s1 = np.searchsorted(cpos, items["end_pos"].astype(np.int64))
+    named = np.zeros(len(cpos), np.int64)  # +1 at each run's first segment, -1 after its last
+    np.add.at(named, s0, 1)
+    np.add.at(named, s1, -1)
+    seg = np.cumsum(named)[:-1] > 0
+    words = int(np.sum((cpos[1:] - cpos[:-1])[seg]))
+    posts = int(np.sum((upos[1:] - upos[:-1])[seg]))
+    return 4 * (words + posts), int(np.sum(seg))
+
+
+def kernel_row(kernel, workload, ms, algorithmic, what, key=None):
+    """One kernel's live duration against its algorithmic bytes and the HBM
+    bytes the committed profile of the same kernel and workload measured
+    (profile line key: the kernel, or kernel@workload)."""
+    row = dict(kernel=kernel, workload=workload, ms=ms, algorithmic_bytes=algorithmic, what=what,
+               algorithmic_GBs=algorithmic / (ms * 1e-3) / 1e9)
+    traffic, src = profiled("traffic_bytes_per_dispatch", key or kernel)
+    if traffic:
+        gbs = traffic / (ms * 1e-3) / 1e9
+        row.update(hbm_bytes=traffic, hbm_GBs=gbs, hbm_frac=gbs / HBM_PEAK_GBS, source=src)
+    return row
+
+
 def sample_of(nq, n=256):
     return np.unique(np.linspace(0, nq - 1, min(n, nq)).astype(np.int64))
 
@@ -219,6 +248,7 @@ def main():
     kern = forms["plain"]["kernel_ms"]
 
     workloads = {}
+    kernels = []  # per-kernel HBM rows
     if not args.no_workloads:
         # ---- the same rows compressed: plain form, segments decoded once per batch
         t0 = time.time()
@@ -230,7 +260,8 @@ def main():
         c.run()
         rc_ = c.download()
         ms = time_runs(c, max(3, args.steps // 2), 1)
-        ent = dict(ms_per_step=ms, qps=nq * 1000 / ms, kernel_ms=stage_ms(c, 3), decode_once=c.decode_active(),
+        ent = dict(ms_per_step=ms, qps=nq * 1000 / ms,
+                   kernel_ms=stage_ms(c, 3, ("compile", "scan", "decode", "merge")), decode_once=c.decode_active(),
                    identical_to_uncompressed=bool(np.array_equal(rc_[0], res_plain[0]) and
                                                   np.array_equal(rc_[1], res_plain[1])))
         c.set_decode(0)
@@ -240,6 +271,12 @@ def main():
         ent["in_kernel_decode_ms_per_step"] = ms0
         ent["vs_uncompressed_plain"] = ms / forms["plain"]["ms_per_step"]
         c.close()
+        dec_bytes, dec_segs = segment_bytes(ctab.export(), citems)
+        ent["decoded_segments"] = dec_segs
+        if ent["kernel_ms"]["decode"] > 0:
+            kernels.append(kernel_row("k_decode_jobs", "1M x 128 compressed (SERIAL_DELTA_BP32)",
+                                      ent["kernel_ms"]["decode"], dec_bytes,
+                                      "the batch's segments decoded once: words read + postings written"))
         workloads["cfg1_compressed_plain"] = dict(
             workload="1M x 128 as above, compressed SERIAL_DELTA_BP32 (segments <= 1,024), plain form",
             index_mib=cidx.bytes() / 2**20, **ent)
@@ -281,6 +318,10 @@ def main():
         i2 = t2.compile(w2["ranges"], nq2)
         ix2 = t2.device()
         f2, r2p, r2b = plain_and_batched(ix2, i2, nq2, kq2, max(5, args.steps // 2), 2, "LSH 4M x 64")
+        pb = Plan(ix2, nq2, len(i2), w2["k"])
+        pb.upload(i2, nq2, kq2)
+        _, lsh_compulsory = pb.bytes()
+        pb.close()
         qs2 = sample_of(nq2, args.sample)
         oi2, oc2 = oracle_sample(t2.export()["inv"].view(np.uint32), i2, w2["rows"].shape[0], kq2, qs2)
         f2["bit_exact_vs_oracle_sample"] = bool(all(np.array_equal(a, b) for a, b in
@@ -295,11 +336,21 @@ def main():
         cp2.run()
         r2c = cp2.download()
         ms = time_runs(cp2, max(5, args.steps // 2), 2)
-        f2["plain_compressed"] = dict(ms_per_step=ms, qps=nq2 * 1000 / ms, kernel_ms=stage_ms(cp2, 3),
+        f2["plain_compressed"] = dict(ms_per_step=ms, qps=nq2 * 1000 / ms,
+                                      kernel_ms=stage_ms(cp2, 3, ("compile", "scan", "decode", "merge")),
                                       codec="SERIAL_DELTA_COMP_BP32_VARINT", index_mib=cx2.bytes() / 2**20,
                                       identical_to_uncompressed=bool(np.array_equal(r2c[0], r2p[0]) and
                                                                      np.array_equal(r2c[1], r2p[1])))
         cp2.close()
+        kernels.append(kernel_row("k_scan_hash", "4M x 64 LSH, 4,096 queries (uncompressed)",
+                                  f2["plain"]["kernel_ms"]["scan"], lsh_compulsory,
+                                  "distinct postings of the batch x 4 B (each read by one tile)"))
+        d2b, _ = segment_bytes(c2.export(), ci2)
+        if f2["plain_compressed"]["kernel_ms"]["decode"] > 0:
+            kernels.append(kernel_row("k_decode_jobs", "4M x 64 LSH compressed (SERIAL_DELTA_COMP_BP32_VARINT)",
+                                      f2["plain_compressed"]["kernel_ms"]["decode"], d2b,
+                                      "the batch's segments decoded once: words read + postings written",
+                                      key="k_decode_jobs@lsh4Mx64"))
         workloads["cfg2_lsh"] = dict(workload="4M x 64 LSH buckets U[0,8192), 4,096 queries (entries redrawn "
                                               "w.p. 0.3), exact match, k=100", sample=len(qs2), **f2)
         log(f"[bench] LSH compressed plain: {ms:.2f} ms; oracle sample exact {f2['bit_exact_vs_oracle_sample']} "
@@ -320,6 +371,14 @@ def main():
         f3["bit_exact_vs_oracle_sample"] = bool(all(np.array_equal(a, b) for a, b in
                                                     zip(rows_of(r3b, kq3, qs3) + rows_of(r3p, kq3, qs3),
                                                         (oi3, oc3, oi3, oc3))))
+        if f3["batched"]["config"].get("form") == "set":
+            p3 = Plan(ix3, nq3, len(i3), int(np.max(kq3)))
+            p3.upload(i3, nq3, kq3)
+            _, sets_compulsory = p3.bytes()
+            p3.close()
+            kernels.append(kernel_row("k_dense_count_sets", "2M sets, 16,384 queries (set form main pass)",
+                                      f3["batched"]["kernel_ms"]["scan"], sets_compulsory,
+                                      "distinct postings of the batch x 4 B"))
         workloads["cfg3_sets"] = dict(workload="2M sets (5-40 tokens, Zipf(1.05) over 1M), 16,384 queries of "
                                                "1-64 tokens, weights 1-4, k in {10,100,1000}",
                                       sample=len(qs3), **f3)
@@ -368,6 +427,11 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": scan_kernel, "effective_logical_GBs": logical / (scan_ms * 1e-3) / 1e9}
+    kernels[:0] = [kernel_row("k_dense_count", "headline batch (dense main pass + check + recount)",
+                              forms["batched"]["kernel_ms"]["scan"], compulsory,
+                              "distinct postings of the batch x 4 B"),
+                   kernel_row("k_scan_tile", "headline batch (plain form)", forms["plain"]["kernel_ms"]["scan"],
+                              compulsory, "distinct postings of the batch x 4 B")]
     if lds_bytes:
         # LDS bytes the kernel reads per dispatch (profile) over its live duration
         lds = lds_bytes / (scan_ms * 1e-3) / 1e9
@@ -397,6 +461,7 @@ def main():
             "bit_exact_vs_oracle_full_batch": exact_full,
             "bit_exact_vs_oracle_sample": exact_sample,
             "workloads": workloads,
+            "kernels": kernels,
             "serving": serving,
         },
         "roofline": roofline,

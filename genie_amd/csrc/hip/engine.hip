@@ -251,6 +251,7 @@ struct genie_plan {
     hipEvent_t ev[8] = {};
     double ms_compile = 0, ms_scan = 0, ms_merge = 0, ms_sample = 0;
     bool timed_dense = false;
+    bool timed_decode = false;  // the last timed run decoded segments into a scratch (ev[5] after it)
     bool ran = false;
 };
 
@@ -2460,6 +2461,10 @@ static int launch_scan(genie_plan* p, hipStream_t s)
         if (nj)
             if (int rc = launch_decode_jobs(d, p->d_fjobs, nj, p->d_full, s))
                 return rc;
+        if (p->timing) {
+            HIP_TRY(hipEventRecord(p->ev[5], s));
+            p->timed_decode = true;
+        }
         a.codec = 0;
         a.post = p->d_full;
     }
@@ -2480,6 +2485,10 @@ static int launch_scan(genie_plan* p, hipStream_t s)
             if (nj)
                 if (int rc = launch_decode_jobs(d, p->d_djobs, nj, p->d_scratch, s))
                     return rc;
+            if (p->timing) {
+                HIP_TRY(hipEventRecord(p->ev[5], s));
+                p->timed_decode = true;
+            }
             h.s.post = p->d_scratch;
             h.spans = p->d_sspan;
             h.sq_ptr = p->d_sq_ptr;
@@ -2696,6 +2705,7 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
         k_zero_many<<<(uint32_t)std::min<uint64_t>((most + 255) / 256, 1024), 256, 0, s>>>(z);
     }
     p->timed_dense = dense_active(p);
+    p->timed_decode = false;
     if (p->timed_dense)
         return run_dense(p, s);
     if (lane_form(p)) {
@@ -3223,6 +3233,10 @@ int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms)
             if (hipEventElapsedTime(&v, p->ev[1], p->ev[4]) == hipSuccess) { *ms = v; return GENIE_OK; }
         }
         if (!strcmp(name, "scan") && hipEventElapsedTime(&v, scan0, p->ev[2]) == hipSuccess) { *ms = v; return GENIE_OK; }
+        if (!strcmp(name, "decode")) {  // part of "scan": the segments decoded into a scratch
+            if (!p->timed_decode) { *ms = 0; return GENIE_OK; }
+            if (hipEventElapsedTime(&v, p->ev[1], p->ev[5]) == hipSuccess) { *ms = v; return GENIE_OK; }
+        }
         if (!strcmp(name, "merge") && hipEventElapsedTime(&v, p->ev[2], p->ev[3]) == hipSuccess) { *ms = v; return GENIE_OK; }
     }
     return fail(GENIE_E_INVALID, std::string("no timing for kernel ") + name);

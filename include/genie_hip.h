@@ -145,7 +145,9 @@ int genie_hip_plan_download(genie_plan* p, void* stream, int32_t* out_ids, int32
  * on the run's stream): "compile" (item -> chunk kernels; query masks for
  * the dense form), "sample" (the dense form's sample pass and thresholds,
  * 0 otherwise), "scan" (the count + per-tile select kernel; the dense main
- * pass), "merge"; ms is the stage's duration. */
+ * pass), "merge"; ms is the stage's duration.  "decode": the part of
+ * "scan" that decoded a compressed batch's segments into a scratch (0 when
+ * the run decoded none). */
 int genie_hip_plan_set_timing(genie_plan* p, int on);
 int genie_hip_plan_kernel_ms(genie_plan* p, const char* name, double* ms);
 /* Replay the run as a captured hipGraph (captured at the next run, again
