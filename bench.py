@@ -285,8 +285,8 @@ def main():
         del cidx, ctab, citems
 
     if not args.no_workloads:
-        # ---- the same batch with weights 1-4 (the dense form takes unit
-        # weights only): plain and batched forms
+        # ---- the same batch with weights 1-4 (the dense form's weighted
+        # masks): plain and batched forms
         t0 = time.time()
         wr = w["ranges"].copy()
         wr["weight"] = genie_amd.synth.uniform_int(1011, 0, len(wr), 1, 4)

@@ -204,6 +204,7 @@ struct genie_plan {
     bool dense_on = true;            // use it when the batch allows
     int dense_nt = 0;                // threads per workgroup of its kernels (0: 1,024 when placed, else 512)
     bool dense_ok = false;           // the uploaded batch allows it
+    uint32_t dense_wp = 1;           // its weight planes per query (1: unit weights, 4: up to 15)
     bool sets_ok = false;            // ... or its set form (set-valued tables)
     std::vector<uint32_t> h_skeys;   // set form: every group's sorted keys,
     std::vector<uint4> h_smask;      // their weight planes (two uint4 per key)
@@ -1440,8 +1441,8 @@ int genie_hip_plan_estimate(const genie_index* ix, uint32_t n_queries, uint64_t 
     const bool dense = (d.dense || d.sets) && mode != 0;
     b += nq * initial_candidates(max_k, dense, d.n_objects) * 8;  // candidate keys
     b += nq * (uint64_t)max_k * 8;                                // output rows
-    if (dense)  // query-mask tables per group of 32 and the sample histograms
-        b += (nq + 31) / 32 * (uint64_t)std::max<uint32_t>(d.dense_dpad, 1) * kDenseKeysPerDim * 4 +
+    if (dense)  // query-mask tables per group (8 queries at the fewest) and the sample histograms
+        b += (nq + 7) / 8 * (uint64_t)std::max<uint32_t>(d.dense_dpad, 1) * kDenseKeysPerDim * 4 +
              nq * kDenseBins * 4ull;
     if (d.codec != 0 && mode != 1)  // full-layout decode scratch of the plain form
         b += d.n_postings * 4;
@@ -1548,6 +1549,10 @@ static int h2d(genie_plan* p, T** dst, const void* src, uint64_t bytes)
 // allows it; otherwise mode 1 runs the lane form and mode 2 the plain form.
 static bool dense_active(const genie_plan* p) { return p->mode != 0 && p->dense_on && (p->dense_ok || p->sets_ok); }
 
+// Queries of one group of the dense or set form: the bits of a mask word
+// over the weight planes per query.
+static uint32_t dense_qpg(const genie_plan* p) { return p->dense_ok ? 32u / p->dense_wp : 32u; }
+
 // Whether the plan's batches that the dense and set forms do not take run the
 // lane form (batched plans, unless the autotuner or the caller chose the
 // plain form for them); otherwise they run the plain form.
@@ -1637,14 +1642,17 @@ static int plan_size_candidates(genie_plan* p)
 }
 
 // Whether the uploaded batch (kept on the host by mode-1 plans) can run the
-// dense form: every item with postings and a nonzero weight has weight 1, a
-// query's items never overlap, and each run of a query's items that follow
-// one another without a gap covers whole keys (a run is one item of a plain
-// batch, the pieces of a load-balanced range, or the segments of a range of
-// a compressed index).
+// dense form: every item with postings and a nonzero weight has an integer
+// weight of at most 15 (all 1: unit masks, 32 queries a group; else four
+// weight planes a query, 8 queries a group), a query's items never overlap,
+// and each run of a query's items that follow one another without a gap
+// covers whole keys (a run is one item of a plain batch, the pieces of a
+// load-balanced range, or the segments of a range of a compressed index).
 // The key range of every run is kept for the device-side mask build on the
-// run's first item (h_dkeys: first key | query % 32 << 24, end key; empty for
-// every other item).
+// run's first item (h_dkeys: first key | the query's bit position << 24, end
+// key | weight << 16; empty for every other item), and for a weighted batch
+// every query's sample histogram shift (n_dims x its largest weight >> shift
+// < 256).
 static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
 {
     const DevIndex& d = p->ix->d;
@@ -1654,7 +1662,19 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
     // segment's first word
     const uint64_t* ks = d.codec == 0 ? d.h_key_start.data() : d.h_key_wstart.data();
     const uint32_t nk = d.n_keys;
+    uint32_t wmax = 1;
+    for (const genie_dim_t& it : p->h_items) {
+        if (it.end_pos <= it.start_pos || it.weight == 0.0f)
+            continue;
+        if (!(it.weight >= 1.0f && it.weight <= 15.0f) || it.weight != (float)(uint32_t)it.weight)
+            return false;
+        wmax = std::max(wmax, (uint32_t)it.weight);
+    }
+    const uint32_t wp = wmax == 1 ? 1u : 4u, qpg = 32u / wp;
+    p->dense_wp = wp;
     p->h_dkeys.assign(2 * p->h_items.size(), 0);
+    if (wp > 1)
+        p->h_sshift.assign(n_queries, 0);
     struct Span {
         uint64_t sp, ep;
         uint32_t item;
@@ -1662,12 +1682,14 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
     std::vector<Span> iv;
     for (uint32_t q = 0; q < n_queries; ++q) {
         iv.clear();
+        uint32_t qw = 1;
         for (uint32_t i = p->h_qptr[q]; i < p->h_qptr[q + 1]; ++i) {
             const genie_dim_t& it = p->h_items[i];
             if (it.end_pos <= it.start_pos || it.weight == 0.0f)
                 continue;
-            if (it.weight != 1.0f)
+            if (wp == 1 && it.weight != 1.0f)
                 return false;
+            qw = std::max(qw, (uint32_t)it.weight);
             iv.push_back({(uint64_t)it.start_pos, (uint64_t)it.end_pos, i});
         }
         // compiled batches list a query's items in position order already
@@ -1679,7 +1701,10 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
         for (size_t r = 0; r < iv.size();) {
             size_t e = r + 1;
             uint64_t end = iv[r].ep;
-            while (e < iv.size() && iv[e].sp == end)
+            const uint32_t w = (uint32_t)p->h_items[iv[r].item].weight;
+            // a run joins items of one weight (a weighted mask holds one
+            // weight per key)
+            while (e < iv.size() && iv[e].sp == end && (uint32_t)p->h_items[iv[e].item].weight == w)
                 end = iv[e++].ep;
             if (e < iv.size() && iv[e].sp < end)
                 return false;  // overlapping items
@@ -1688,9 +1713,16 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
             const uint32_t kb = d.key_pos.first_ge(ks, end);
             if (ks[ka] != sp || kb > nk || ks[kb] != end)
                 return false;
-            p->h_dkeys[2 * iv[r].item] = ka | (q & 31u) << 24;  // keys < 2^15 in a dense index
-            p->h_dkeys[2 * iv[r].item + 1] = kb;
+            // keys < 2^15 in a dense index (<= 128 dims of <= 256 keys)
+            p->h_dkeys[2 * iv[r].item] = ka | ((q % qpg) * wp) << 24;
+            p->h_dkeys[2 * iv[r].item + 1] = kb | (wp > 1 ? w << 16 : 0u);
             r = e;
+        }
+        if (wp > 1) {  // counts <= n_dims x 15 < kSetNoQuery
+            uint32_t sh = 0;
+            while ((d.n_dims * qw) >> sh >= 256)
+                ++sh;
+            p->h_sshift[q] = sh;
         }
     }
     return true;
@@ -1802,7 +1834,7 @@ static bool sets_batch_ok(genie_plan* p, uint32_t n_queries)
 static int prepare_dense(genie_plan* p)
 {
     const DevIndex& d = p->ix->d;
-    p->dn_groups = (p->n_queries + 31) / 32;
+    p->dn_groups = (p->n_queries + dense_qpg(p) - 1) / dense_qpg(p);
     const uint64_t tw = (uint64_t)d.dense_dpad * kDenseKeysPerDim;
     if (int rc = grow_buf(p, &p->d_dtab, &p->dtab_alloc, p->dn_groups * tw))
         return rc;
@@ -1816,18 +1848,25 @@ static int prepare_dense(genie_plan* p)
         if (p->n_items)
             if (int rc = h2d(p, &p->d_dkeys, p->h_dkeys.data(), p->n_items * 8))
                 return rc;
+        if (p->dense_wp > 1) {  // weighted: per-query sample histogram shifts
+            if (!p->d_sshift)
+                if (int rc = dev_alloc(&p->d_sshift, p->max_queries))
+                    return rc;
+            if (int rc = h2d(p, &p->d_sshift, p->h_sshift.data(), p->h_sshift.size() * 4))
+                return rc;
+        }
     } else {  // set form: the groups' key entries
         const uint64_t ne = std::max<uint64_t>(1, p->h_skeys.size());
         if (int rc = grow_buf(p, &p->d_skeys, &p->sent_alloc, ne))
             return rc;
         if (int rc = grow_buf(p, &p->d_smask, &p->smask_alloc, 2 * ne))
             return rc;
-        if (!p->d_sgptr) {
+        if (!p->d_sgptr)
             if (int rc = dev_alloc(&p->d_sgptr, (p->max_queries + 31) / 32 + 1))
                 return rc;
+        if (!p->d_sshift)
             if (int rc = dev_alloc(&p->d_sshift, p->max_queries))
                 return rc;
-        }
         if (!p->h_skeys.empty()) {
             if (int rc = h2d(p, &p->d_skeys, p->h_skeys.data(), p->h_skeys.size() * 4))
                 return rc;
@@ -1842,8 +1881,8 @@ static int prepare_dense(genie_plan* p)
     if (!p->d_gsafe)
         if (int rc = dev_alloc(&p->d_gsafe, p->max_queries))
             return rc;
-    if (!p->d_redo)
-        if (int rc = dev_alloc(&p->d_redo, (p->max_queries + 31) / 32))
+    if (!p->d_redo)  // groups of 8 queries at the fewest
+        if (int rc = dev_alloc(&p->d_redo, (p->max_queries + 7) / 8))
             return rc;
     if (!p->d_want) {
         if (int rc = dev_alloc(&p->d_want, p->max_queries))
@@ -2515,11 +2554,11 @@ static int launch_merge(genie_plan* p, hipStream_t s);
 // kernel when the occupancy query places it, else the 512-thread one (some
 // runtimes place no 1,024-thread workgroup of more than 64 VGPRs); a kernel
 // the query does not place is never launched, since it would never run.
-template <int NBLK, bool SAMPLE, bool SETS = false>
+template <int NBLK, bool SAMPLE, bool SETS = false, int WP = 1>
 static int launch_dense_t(const DenseArgs& a, hipStream_t s, int nt)
 {
-    auto k1 = k_dense_count<NBLK, SAMPLE, SETS, kDenseThreads>;  // static LDS
-    auto k2 = k_dense_count<NBLK, SAMPLE, SETS, kDenseThreads / 2>;
+    auto k1 = k_dense_count<NBLK, SAMPLE, SETS, kDenseThreads, WP>;  // static LDS
+    auto k2 = k_dense_count<NBLK, SAMPLE, SETS, kDenseThreads / 2, WP>;
     int per_cu = 0, n_cu = 0, dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2547,21 +2586,30 @@ static int launch_dense_t(const DenseArgs& a, hipStream_t s, int nt)
                                      " threads per CU");
 }
 
-template <bool SAMPLE>
-static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s, int nt)
+template <bool SAMPLE, int WP>
+static int launch_dense_w(uint32_t nblk, const DenseArgs& a, hipStream_t s, int nt)
 {
     switch (nblk) {
-    case 0: return launch_dense_t<0, SAMPLE, true>(a, s, nt);
-    case 1: return launch_dense_t<1, SAMPLE>(a, s, nt);
-    case 2: return launch_dense_t<2, SAMPLE>(a, s, nt);
-    case 3: return launch_dense_t<3, SAMPLE>(a, s, nt);
-    case 4: return launch_dense_t<4, SAMPLE>(a, s, nt);
-    case 5: return launch_dense_t<5, SAMPLE>(a, s, nt);
-    case 6: return launch_dense_t<6, SAMPLE>(a, s, nt);
-    case 7: return launch_dense_t<7, SAMPLE>(a, s, nt);
-    case 8: return launch_dense_t<8, SAMPLE>(a, s, nt);
+    case 1: return launch_dense_t<1, SAMPLE, false, WP>(a, s, nt);
+    case 2: return launch_dense_t<2, SAMPLE, false, WP>(a, s, nt);
+    case 3: return launch_dense_t<3, SAMPLE, false, WP>(a, s, nt);
+    case 4: return launch_dense_t<4, SAMPLE, false, WP>(a, s, nt);
+    case 5: return launch_dense_t<5, SAMPLE, false, WP>(a, s, nt);
+    case 6: return launch_dense_t<6, SAMPLE, false, WP>(a, s, nt);
+    case 7: return launch_dense_t<7, SAMPLE, false, WP>(a, s, nt);
+    case 8: return launch_dense_t<8, SAMPLE, false, WP>(a, s, nt);
     }
     return fail(GENIE_E_INVALID, "dense form supports at most 128 dims");
+}
+
+// nblk 0: the set form; wp: weight planes per query of a dense batch (1:
+// unit weights, 4: weights up to 15)
+template <bool SAMPLE>
+static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s, int nt, uint32_t wp)
+{
+    if (nblk == 0)
+        return launch_dense_t<0, SAMPLE, true>(a, s, nt);
+    return wp == 4 ? launch_dense_w<SAMPLE, 4>(nblk, a, s, nt) : launch_dense_w<SAMPLE, 1>(nblk, a, s, nt);
 }
 
 // Dense form: query masks, sample pass + thresholds, main pass, merge.
@@ -2604,9 +2652,9 @@ static int run_dense(genie_plan* p, hipStream_t s)
     a.set_keys = p->d_skeys;
     a.set_mask = p->d_smask;
     a.set_gptr = p->d_sgptr;
-    a.hshift = p->dense_ok ? nullptr : p->d_sshift;
+    a.hshift = (p->dense_ok && p->dense_wp == 1) ? nullptr : p->d_sshift;
     const uint32_t nblk = p->dense_ok ? d.dense_dpad / 16 : 0;  // 0: the set form
-    if (int rc = launch_dense<true>(nblk, a, s, p->dense_nt))
+    if (int rc = launch_dense<true>(nblk, a, s, p->dense_nt, p->dense_wp))
         return rc;
     // thresholds, then every query's candidate space from its sample
     const uint32_t mult = p->dense_mult;
@@ -2620,14 +2668,16 @@ static int run_dense(genie_plan* p, hipStream_t s)
         HIP_TRY(hipEventRecord(p->ev[4], s));
     a.list = nullptr;
     a.n_list = 0;
-    if (int rc = launch_dense<false>(nblk, a, s, p->dense_nt))
+    if (int rc = launch_dense<false>(nblk, a, s, p->dense_nt, p->dense_wp))
         return rc;
     // queries the speculative threshold left short: their groups again
     k_dense_check<<<p->dn_groups, 64, 0, s>>>(p->d_kq, nq, p->d_gthr, p->d_gsafe, p->d_cand_cnt, p->d_redo,
                                               p->d_want_safe, p->d_cand_cap, p->d_cand_base, p->d_cap_used,
-                                              p->total_cap, p->dense_ok ? kDenseNoQuery : kSetNoQuery);
+                                              p->total_cap,
+                                              (p->dense_ok && p->dense_wp == 1) ? kDenseNoQuery : kSetNoQuery,
+                                              dense_qpg(p));
     a.redo = p->d_redo;
-    if (int rc = launch_dense<false>(nblk, a, s, p->dense_nt))
+    if (int rc = launch_dense<false>(nblk, a, s, p->dense_nt, p->dense_wp))
         return rc;
     if (p->timing)
         HIP_TRY(hipEventRecord(p->ev[2], s));

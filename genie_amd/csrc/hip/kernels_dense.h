@@ -321,10 +321,10 @@ __global__ void __launch_bounds__(64) k_dense_check(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ want_safe,
                                                     uint32_t* __restrict__ cap, uint64_t* __restrict__ base,
                                                     unsigned long long* __restrict__ used, uint64_t total,
-                                                    uint32_t none)
+                                                    uint32_t none, uint32_t qpg)
 {
-    const uint32_t g = blockIdx.x, lane = threadIdx.x, q = g * 32 + lane;
-    const bool live = lane < 32 && q < n_queries;
+    const uint32_t g = blockIdx.x, lane = threadIdx.x, q = g * qpg + lane;  // qpg queries a group
+    const bool live = lane < qpg && q < n_queries;
     const bool flagged = live && gthr[q] > gsafe[q] && cand_cnt[q] < kq[q];
     const bool any = ballot(flagged) != 0;
     if (lane == 0)
@@ -482,9 +482,37 @@ constexpr uint32_t kDenseStage = 2048;
 constexpr uint32_t kDenseFlushTiles = 8;
 constexpr uint32_t kDenseStageWords = 2 + 32 + 32 + kDenseStage * 3;  // count, pad, per-query counts / bases, meta, keys
 
-__host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample)
+// wp: weight planes per query in a mask word (1: unit weights, 32 queries a
+// group; 2 or 4: weighted, 16 or 8 queries, sample histograms of 256 bins).
+__host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample, int wp = 1)
 {
-    return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 16 + (sample ? 32 * (nblk * 16 + 1) : kDenseStageWords)) * 4u;
+    return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 16 +
+                      (sample ? (wp > 1 ? (32 / wp) * 256 : 32 * (nblk * 16 + 1)) : kDenseStageWords)) * 4u;
+}
+
+// Weighted masks: bit WP q + p of a word is bit p of query q's weight on the
+// key, so the eight bit-sliced count planes hold, at bit WP q + p, the number
+// of dims whose weight has bit p set.  The query's count, sum over p of that
+// number times 2^p, is assembled at bit WP q of 11 planes (the weights' bit
+// groups aligned, then ripple adds of the shifted groups).
+template <int WP>
+__device__ __forceinline__ void dense_weigh(const uint32_t (&c)[8], uint32_t (&w)[kSetCountPlanes])
+{
+    constexpr uint32_t M = WP == 4 ? 0x11111111u : 0x55555555u;
+#pragma unroll
+    for (int b = 0; b < kSetCountPlanes; ++b)
+        w[b] = b < 8 ? c[b] & M : 0u;
+#pragma unroll
+    for (int p = 1; p < WP; ++p) {
+        uint32_t carry = 0;
+#pragma unroll
+        for (int b = p; b < kSetCountPlanes; ++b) {
+            const uint32_t x = b - p < 8 ? (c[b - p] >> p) & M : 0u;
+            const uint32_t t = __builtin_amdgcn_bitop3_b32(w[b], x, carry, 0xE8);  // majority
+            w[b] = __builtin_amdgcn_bitop3_b32(w[b], x, carry, 0x96);              // xor3
+            carry = t;
+        }
+    }
 }
 
 // Set form: hash keys, weight planes 0-3 (uint4 per slot) and 4-5 (uint2 per
@@ -606,15 +634,19 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 // or half of it for runtimes whose occupancy rule places no 1,024-thread
 // workgroup of more than 64 VGPRs.  Both are compiled for 1,024 threads, so
 // both stay within 128 VGPRs (two such waves per SIMD fit any such rule).
-template <int NBLK, bool SAMPLE, bool SETS = false, int NT = kDenseThreads>
+template <int NBLK, bool SAMPLE, bool SETS = false, int NT = kDenseThreads, int WP = 1>
 __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
 {
+    static_assert(WP == 1 || WP == 2 || WP == 4, "weight planes per query");
     // the kernel's only LDS, static so that the table sits at address 0 and
     // every table offset folds into the ds_read instruction
     constexpr int DPAD = NBLK * 16;
-    constexpr int NP = SETS ? kSetCountPlanes : 8;       // count planes
-    constexpr uint32_t NONE = SETS ? kSetNoQuery : kDenseNoQuery;
-    __shared__ uint32_t s_lds[(SETS ? set_lds_bytes(SAMPLE) : dense_lds_bytes(NBLK, SAMPLE)) / 4];
+    constexpr bool WEIGHTED = !SETS && WP > 1;               // weighted masks (dense_weigh)
+    constexpr uint32_t QPG = SETS ? 32u : 32u / WP;          // queries of a group
+    constexpr uint32_t QMASK = WP == 4 ? 0x11111111u : (WP == 2 ? 0x55555555u : 0xFFFFFFFFu);  // their bits
+    constexpr int NP = (SETS || WEIGHTED) ? kSetCountPlanes : 8;  // count planes
+    constexpr uint32_t NONE = (SETS || WEIGHTED) ? kSetNoQuery : kDenseNoQuery;
+    __shared__ uint32_t s_lds[(SETS ? set_lds_bytes(SAMPLE) : dense_lds_bytes(NBLK, SAMPLE, WP)) / 4];
     uint32_t* s_tab = s_lds;                             // DPAD * 256 masks
     uint2* s_slot = reinterpret_cast<uint2*>(s_lds);    // SETS: (key, entry) slots, then the entries' planes
     uint4* s_mlo = reinterpret_cast<uint4*>(s_lds + kSetMloWord);
@@ -627,7 +659,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     uint32_t* s_qbase = s_qcnt + 32;                     // reserved base per query
     uint32_t* s_meta = s_qbase + 32;                     // query << 16 | rank within the query
     uint64_t* s_key = reinterpret_cast<uint64_t*>(s_meta + kDenseStage);
-    constexpr uint32_t HB = SETS ? 256 : NBLK * 16 + 1;
+    constexpr uint32_t HB = (SETS || WEIGHTED) ? 256 : NBLK * 16 + 1;
     const uint32_t tid = threadIdx.x;
     const uint32_t P = gridDim.x / 8, x = blockIdx.x % 8, s = blockIdx.x / 8;
     uint32_t g_first, g_step, t_first, t_step;
@@ -676,7 +708,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 s_mhi[kSetEntries] = make_uint2(0, 0);
             }
             if constexpr (SAMPLE)
-                for (uint32_t i = tid; i < 32 * HB; i += NT)
+                for (uint32_t i = tid; i < QPG * HB; i += NT)
                     s_hist[i] = 0;
             __syncthreads();
             int hi_any = 0;
@@ -703,13 +735,15 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             uint4* dst = reinterpret_cast<uint4*>(s_tab);
             for (uint32_t i = tid; i < tab_words / 4; i += NT)
                 dst[i] = make_uint4(0, 0, 0, 0);
-            for (uint32_t i = tid; i < 32 * HB; i += NT)
+            for (uint32_t i = tid; i < QPG * HB; i += NT)
                 s_hist[i] = 0;
             __syncthreads();
-            const uint32_t i0 = a.item_qptr[g * 32], i1 = a.item_qptr[min(g * 32 + 32, a.n_queries)];
+            const uint32_t i0 = a.item_qptr[g * QPG], i1 = a.item_qptr[min(g * QPG + QPG, a.n_queries)];
             for (uint32_t i = i0 + tid; i < i1; i += NT) {
-                const uint32_t k0 = a.dkeys[2 * i], k1 = a.dkeys[2 * i + 1];
-                const uint32_t bit = 1u << (k0 >> 24);
+                // first key | query's bit position << 24, end key (| weight << 16
+                // in a weighted batch)
+                const uint32_t k0 = a.dkeys[2 * i], k1 = a.dkeys[2 * i + 1] & 0xFFFFu;
+                const uint32_t bit = (WEIGHTED ? a.dkeys[2 * i + 1] >> 16 : 1u) << (k0 >> 24);
                 // up to 16 slot loads in flight per item, then the LDS ors
                 for (uint32_t kb = k0 & 0xFFFFFFu; kb < k1; kb += 16) {
                     uint32_t sl[16];
@@ -734,11 +768,11 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             for (uint32_t i = tid; i < tab_words / 4; i += NT)
                 dst[i] = src[i];
         }
-        // SETS sample pass: this lane's bin shift for each query of the group
+        // SETS and weighted sample passes: the bin shift of each query of the group
         uint64_t shifts = 0;  // 2 bits per query
-        if constexpr (SETS && SAMPLE) {
-            for (uint32_t j = 0; j < 32 && g * 32 + j < a.n_queries; ++j)
-                shifts |= (uint64_t)min(a.hshift[g * 32 + j], 3u) << (2 * j);
+        if constexpr ((SETS || WEIGHTED) && SAMPLE) {
+            for (uint32_t j = 0; j < QPG && g * QPG + j < a.n_queries; ++j)
+                shifts |= (uint64_t)min(a.hshift[g * QPG + j], 3u) << (2 * j);
         }
         uint32_t tp[NP] = {};
         if constexpr (!SAMPLE) {
@@ -752,8 +786,9 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             if (tid == 0)
                 *s_nhit = 0;
             if (tid < kWave) {
-                const uint32_t q = g * 32 + tid;
-                const uint32_t th = (tid < 32 && q < a.n_queries) ? a.gthr[q] : NONE;
+                // bit WP q of the planes: query q of the group
+                const uint32_t q = g * QPG + tid / WP;
+                const uint32_t th = (tid < 32 && tid % WP == 0 && q < a.n_queries) ? a.gthr[q] : NONE;
 #pragma unroll
                 for (int b = 0; b < NP; ++b) {
                     const uint32_t pl = (uint32_t)ballot((th >> b) & 1u);
@@ -783,19 +818,25 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     else
                         set_accumulate<NP, false>(a, ol, s_slot, s_mlo, s_mhi, c);
                     o = a.obj_perm[ol];  // kSetEmpty past the last object: counts 0, nothing emitted
+                } else if constexpr (WEIGHTED) {
+                    uint32_t c8[8];
+                    dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c8);
+                    dense_weigh<WP>(c8, c);
                 } else {
                     dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
                 }
-                if constexpr (SAMPLE && SETS) {
+                if constexpr (SAMPLE && (SETS || WEIGHTED)) {
                     uint32_t any = 0;
 #pragma unroll
                     for (int b = 0; b < NP; ++b)
                         any |= c[b];
+                    any &= QMASK;
                     while (any) {
                         const uint32_t j = (uint32_t)__builtin_ctz(any);
                         any &= any - 1;
-                        const uint32_t v = dense_lane_count<NP>(c, j) >> (uint32_t)((shifts >> (2 * j)) & 3u);
-                        atomicAdd(&s_hist[j * HB + min(v, HB - 1)], 1u);
+                        const uint32_t jq = j / WP;  // the group's query
+                        const uint32_t v = dense_lane_count<NP>(c, j) >> (uint32_t)((shifts >> (2 * jq)) & 3u);
+                        atomicAdd(&s_hist[jq * HB + min(v, HB - 1)], 1u);
                     }
                 } else if constexpr (SAMPLE) {
                     // lane l files query (i + l) mod 32 at step i, so that one
@@ -825,19 +866,20 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                         gt |= eq & c[b] & ~tp[b];
                         eq &= ~(c[b] ^ tp[b]);
                     }
-                    uint32_t ge = gt | eq;
+                    uint32_t ge = (gt | eq) & QMASK;
                     while (ge) {
                         const uint32_t j = (uint32_t)__builtin_ctz(ge);
                         ge &= ge - 1;
+                        const uint32_t jq = j / WP;  // the group's query
                         const uint64_t key = ((uint64_t)dense_lane_count<NP>(c, j) << 32) | (uint32_t)~o;
                         // staged when the stage has room (the rank is taken only
                         // then, so a query's reserved slots have no holes)
                         const uint32_t i = atomicAdd(s_nhit, 1u);
                         if (i < kDenseStage) {
-                            s_meta[i] = j << 16 | atomicAdd(&s_qcnt[j], 1u);
+                            s_meta[i] = jq << 16 | atomicAdd(&s_qcnt[jq], 1u);
                             s_key[i] = key;
                         } else {
-                            const uint32_t q = g * 32 + j;
+                            const uint32_t q = g * QPG + jq;
                             const uint32_t slot = atomicAdd(&a.cand_cnt[q], 1u);
                             if (slot < a.cand_cap[q])
                                 a.cand[a.cand_base[q] + slot] = key;
@@ -856,12 +898,12 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 staged_tiles = 0;
                 __syncthreads();
                 const uint32_t nh = min(*s_nhit, kDenseStage);
-                if (tid < 32 && s_qcnt[tid])
-                    s_qbase[tid] = atomicAdd(&a.cand_cnt[g * 32 + tid], s_qcnt[tid]);
+                if (tid < QPG && s_qcnt[tid])
+                    s_qbase[tid] = atomicAdd(&a.cand_cnt[g * QPG + tid], s_qcnt[tid]);
                 __syncthreads();
                 for (uint32_t i = tid; i < nh; i += NT) {
                     const uint32_t m = s_meta[i], j = m >> 16;
-                    const uint32_t q = g * 32 + j;
+                    const uint32_t q = g * QPG + j;
                     const uint32_t slot = s_qbase[j] + (m & 0xFFFFu);
                     if (slot < a.cand_cap[q])
                         a.cand[a.cand_base[q] + slot] = s_key[i];
@@ -877,9 +919,9 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
         }
         if constexpr (SAMPLE) {
             __syncthreads();
-            for (uint32_t i = tid; i < 32 * HB; i += NT) {
+            for (uint32_t i = tid; i < QPG * HB; i += NT) {
                 const uint32_t j = i / HB, bin = i % HB;
-                const uint32_t q = g * 32 + j;
+                const uint32_t q = g * QPG + j;
                 const uint32_t v = s_hist[i];
                 if (v && q < a.n_queries)
                     atomicAdd(&a.hist[(uint64_t)q * kDenseBins + bin], v);

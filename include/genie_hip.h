@@ -182,11 +182,12 @@ int genie_hip_plan_set_form(genie_plan* p, int form);
 int genie_hip_plan_form(genie_plan* p, int* form);
 /* Dense form of the batched search (modes 1 and 2; on by default).  When the
  * index is a single-value table of at most 128 dims with at most 256 keys per
- * dim, and every item of the uploaded batch covers whole keys with weight 1
- * (never one key twice for a query), the batch is counted from per-object key
- * codes built at index creation (a compressed table is decoded once on the
- * device for this), 32 queries per LDS lookup with bit-sliced counters,
- * instead of from the posting lists; results are identical.
+ * dim, and every item of the uploaded batch covers whole keys with an integer
+ * weight from 1 to 15 (never one key twice for a query), the batch is counted
+ * from per-object key codes built at index creation (a compressed table is
+ * decoded once on the device for this), 32 queries per LDS lookup with
+ * bit-sliced counters (8 queries of four weight bits each when a weight is
+ * above 1), instead of from the posting lists; results are identical.
  * The same switch covers the set form of set-valued (bijectMap) tables: when
  * each query's weight per key is an integer up to 63 (items summed over the
  * positions they cover, constant over each key), the longest forward list

@@ -152,9 +152,10 @@ def test_autotune_picks_a_valid_config():
 
 @pytest.mark.parametrize("form", ["plain", "lane"])
 def test_batched_plan_forms_on_weighted_batches(form):
-    """Weights 1-4 (the dense form refuses them): a batched plan runs the lane
-    or the plain form, switched explicitly or chosen by the autotuner, with
-    the oracle's results either way; a graph replay of the chosen form too."""
+    """Weights 1-4 with the dense form switched off: a batched plan runs the
+    lane or the plain form, switched explicitly or chosen by the autotuner,
+    with the oracle's results either way; a graph replay of the chosen form
+    too."""
     w = W.cfg1(seed=955, n=120_000, d=32, q=40, k=64)
     r = w["ranges"].copy()
     r["weight"] = genie_amd.synth.uniform_int(956, 0, len(r), 1, 4)
@@ -162,6 +163,7 @@ def test_batched_plan_forms_on_weighted_batches(form):
     items = t.compile(r, 40)
     ri, rc = oracle_rows(w["rows"], r, 40, 64)
     p = Plan(t.device(), 40, len(items), 64, 0, mode=1)
+    p.set_dense(False)
     p.upload(items, 40, np.full(40, 64, np.uint32))
     p.set_form(form)
     assert p.form() == form

@@ -168,16 +168,49 @@ def test_dense_autotune_includes_dense():
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("n,d,q,k,wmax", [
+    (150_000, 32, 37, 50, 4),     # weights 1-4, a partly filled last group of 8
+    (300_000, 128, 40, 100, 15),  # full width, weights up to 15 (counts up to 1,920)
+    (70_001, 20, 9, 10, 2),       # dims not a multiple of 16, two groups
+])
+def test_dense_weighted_equals_oracle(n, d, q, k, wmax):
+    """Integer weights up to 15: four weight planes a query in each mask word
+    (8 queries a group), counts assembled over 11 planes."""
+    w = W.cfg1(n=n, d=d, q=q, k=k)
+    t = Table.from_rows(w["rows"])
+    r = w["ranges"].copy()
+    r["weight"] = genie_amd.synth.uniform_int(900 + wmax, 0, len(r), 1, wmax)
+    items = t.compile(r, q)
+    (b, p) = run(t, items, q, k, graph=True, runs=3)
+    assert p.dense_active() and p.form() == "dense"
+    ri, rc = oracle_single(w["rows"], r, q, k)
+    assert_topk_equal(b[0], b[1], ri, rc, k, q, f"weighted dense n={n} d={d} w<={wmax}")
+
+
+def test_dense_weighted_mixed_k_and_compressed():
+    w = W.cfg1(n=120_000, d=64, q=30, k=100)
+    r = w["ranges"].copy()
+    r["weight"] = genie_amd.synth.uniform_int(77, 0, len(r), 1, 4)
+    kq = np.array([10, 100, 1000] * 10, np.uint32)
+    ri, rc = oracle_single(w["rows"], r, 30, kq)
+    for t in (Table.from_rows(w["rows"]), Table.from_rows(w["rows"], 1024, True, codec=11)):
+        items = t.compile(r, 30)
+        (b, p) = run(t, items, 30, kq)
+        assert p.dense_active()
+        assert_topk_equal(b[0], b[1], ri, rc, kq, 30, "weighted dense, mixed k")
+
+
 def test_dense_not_used_where_it_cannot_be_exact():
     w = W.cfg1(n=60_000, d=24, q=10, k=20)
     t = Table.from_rows(w["rows"])
-    # weights other than 1
-    r = w["ranges"].copy()
-    r["weight"][::3] = 2
-    (b, p) = run(t, t.compile(r, 10), 10, 20)
-    assert not p.dense_active()
-    ri, rc = oracle_single(w["rows"], r, 10, 20)
-    assert_topk_equal(b[0], b[1], ri, rc, 20, 10, "weights")
+    # weights above 15 (the weighted masks hold four bits a query)
+    for wv in (16, 300):
+        r = w["ranges"].copy()
+        r["weight"][::3] = wv
+        (b, p) = run(t, t.compile(r, 10), 10, 20)
+        assert not p.dense_active()
+        ri, rc = oracle_single(w["rows"], r, 10, 20)
+        assert_topk_equal(b[0], b[1], ri, rc, 20, 10, f"weights {wv}")
     # a query covering keys twice (ranges repeated on the same dims)
     r2 = np.concatenate([w["ranges"], w["ranges"][:3]])
     r2 = r2[np.argsort(r2["query"], kind="stable")]
