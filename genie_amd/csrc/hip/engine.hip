@@ -204,7 +204,7 @@ struct genie_plan {
     bool dense_on = true;            // use it when the batch allows
     int dense_nt = 0;                // threads per workgroup of its kernels (0: 1,024 when placed, else 512)
     bool dense_ok = false;           // the uploaded batch allows it
-    uint32_t dense_wp = 1;           // its weight planes per query (1: unit weights, 4: up to 15)
+    uint32_t dense_wp = 1;           // its weight planes per query (1: unit weights; 2-4: bits of the largest)
     bool sets_ok = false;            // ... or its set form (set-valued tables)
     std::vector<uint32_t> h_skeys;   // set form: every group's sorted keys,
     std::vector<uint4> h_smask;      // their weight planes (two uint4 per key)
@@ -1643,8 +1643,9 @@ static int plan_size_candidates(genie_plan* p)
 
 // Whether the uploaded batch (kept on the host by mode-1 plans) can run the
 // dense form: every item with postings and a nonzero weight has an integer
-// weight of at most 15 (all 1: unit masks, 32 queries a group; else four
-// weight planes a query, 8 queries a group), a query's items never overlap,
+// weight of at most 15 (all 1: unit masks, 32 queries a group; else as many
+// weight planes a query as the largest weight has bits, WP = 2-4, and 32 / WP
+// queries a group), a query's items never overlap,
 // and each run of a query's items that follow one another without a gap
 // covers whole keys (a run is one item of a plain batch, the pieces of a
 // load-balanced range, or the segments of a range of a compressed index).
@@ -1670,7 +1671,7 @@ static bool dense_batch_ok(genie_plan* p, uint32_t n_queries)
             return false;
         wmax = std::max(wmax, (uint32_t)it.weight);
     }
-    const uint32_t wp = wmax == 1 ? 1u : 4u, qpg = 32u / wp;
+    const uint32_t wp = wmax == 1 ? 1u : (wmax <= 3 ? 2u : (wmax <= 7 ? 3u : 4u)), qpg = 32u / wp;
     p->dense_wp = wp;
     p->h_dkeys.assign(2 * p->h_items.size(), 0);
     if (wp > 1)
@@ -2564,14 +2565,23 @@ static int launch_dense_t(const DenseArgs& a, hipStream_t s, int nt)
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     if (nt != kDenseThreads / 2)
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k1, kDenseThreads, 0));
+    // the grid, and the tile chunks per group for its workgroups per XCD
+    DenseArgs b = a;
+    uint32_t n_wg = 0;
+    auto grid = [&](int per) {
+        n_wg = std::max(8u, (uint32_t)(per * n_cu) / 8 * 8);
+        b.chunks = dense_chunks(a.G, n_wg / 8, !SAMPLE && !SETS);
+    };
     if (per_cu >= 1) {
-        k1<<<std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8), kDenseThreads, 0, s>>>(a);
+        grid(per_cu);
+        k1<<<n_wg, kDenseThreads, 0, s>>>(b);
         return GENIE_OK;
     }
     if (nt != kDenseThreads)
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k2, kDenseThreads / 2, 0));
     if (per_cu >= 1) {
-        k2<<<std::max(8u, (uint32_t)(per_cu * n_cu) / 8 * 8), kDenseThreads / 2, 0, s>>>(a);
+        grid(per_cu);
+        k2<<<n_wg, kDenseThreads / 2, 0, s>>>(b);
         return GENIE_OK;
     }
     hipFuncAttributes fa{};
@@ -2603,13 +2613,18 @@ static int launch_dense_w(uint32_t nblk, const DenseArgs& a, hipStream_t s, int 
 }
 
 // nblk 0: the set form; wp: weight planes per query of a dense batch (1:
-// unit weights, 4: weights up to 15)
+// unit weights; 2, 3, 4: weights below 4, 8, 16)
 template <bool SAMPLE>
 static int launch_dense(uint32_t nblk, const DenseArgs& a, hipStream_t s, int nt, uint32_t wp)
 {
     if (nblk == 0)
         return launch_dense_t<0, SAMPLE, true>(a, s, nt);
-    return wp == 4 ? launch_dense_w<SAMPLE, 4>(nblk, a, s, nt) : launch_dense_w<SAMPLE, 1>(nblk, a, s, nt);
+    switch (wp) {
+    case 2: return launch_dense_w<SAMPLE, 2>(nblk, a, s, nt);
+    case 3: return launch_dense_w<SAMPLE, 3>(nblk, a, s, nt);
+    case 4: return launch_dense_w<SAMPLE, 4>(nblk, a, s, nt);
+    }
+    return launch_dense_w<SAMPLE, 1>(nblk, a, s, nt);
 }
 
 // Dense form: query masks, sample pass + thresholds, main pass, merge.

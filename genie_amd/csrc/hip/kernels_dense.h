@@ -108,7 +108,33 @@ struct DenseArgs {
     const uint4* set_mask;       // [entries][2] bit j of word b = bit b of query j's weight (b < 6)
     const uint32_t* set_gptr;    // [G + 1] entries of each group
     const uint32_t* hshift;      // [n_queries] sample histogram bin = count >> hshift (set form)
+    // tile chunks per group (dense_chunks): the work units of a pass are
+    // (group, chunk) pairs
+    uint32_t chunks;
 };
+
+// Tile chunks per group for G groups over P workgroups per XCD.  Chunk c of
+// a group takes the XCD's tiles c, c + C, c + 2 C, ...  With `balance` (the
+// dense form's main pass), the C of 1 .. max(8, P / G) whose busiest
+// workgroup does the least work, ceil(G C / P) / C group passes, the
+// smallest on ties; else P / G when the groups are fewer than P (the sample
+// pass and the set form, whose group passes are short or balanced), or 1.
+__host__ __device__ inline uint32_t dense_chunks(uint32_t G, uint32_t P, bool balance)
+{
+    if (G == 0 || P == 0)
+        return 1;
+    if (!balance)
+        return G <= P ? P / G : 1u;
+    uint32_t best = 1, best_units = (G + P - 1) / P;  // ceil(G C / P) at C = best
+    for (uint32_t c = 2; c <= (P / G > 8 ? P / G : 8u); ++c) {
+        const uint32_t units = (uint32_t)(((uint64_t)G * c + P - 1) / P);
+        if ((uint64_t)units * best < (uint64_t)best_units * c) {
+            best = c;
+            best_units = units;
+        }
+    }
+    return best;
+}
 
 // Set form: LDS hash table of a group's keys (at most kSetEntries of them):
 // kSetSlots slots of (key, entry), at most a quarter full, and the entries'
@@ -490,6 +516,13 @@ __host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample, in
                       (sample ? (wp > 1 ? (32 / wp) * 256 : 32 * (nblk * 16 + 1)) : kDenseStageWords)) * 4u;
 }
 
+// Bits of the queries of a mask word with WP weight planes per query: bit
+// WP q for q < 32 / WP (10 queries of three planes leave bits 30-31 unused).
+__host__ __device__ constexpr uint32_t dense_query_bits(int wp)
+{
+    return wp == 4 ? 0x11111111u : (wp == 3 ? 0x09249249u : (wp == 2 ? 0x55555555u : 0xFFFFFFFFu));
+}
+
 // Weighted masks: bit WP q + p of a word is bit p of query q's weight on the
 // key, so the eight bit-sliced count planes hold, at bit WP q + p, the number
 // of dims whose weight has bit p set.  The query's count, sum over p of that
@@ -498,7 +531,7 @@ __host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample, in
 template <int WP>
 __device__ __forceinline__ void dense_weigh(const uint32_t (&c)[8], uint32_t (&w)[kSetCountPlanes])
 {
-    constexpr uint32_t M = WP == 4 ? 0x11111111u : 0x55555555u;
+    constexpr uint32_t M = dense_query_bits(WP);
 #pragma unroll
     for (int b = 0; b < kSetCountPlanes; ++b)
         w[b] = b < 8 ? c[b] & M : 0u;
@@ -637,13 +670,13 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 template <int NBLK, bool SAMPLE, bool SETS = false, int NT = kDenseThreads, int WP = 1>
 __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
 {
-    static_assert(WP == 1 || WP == 2 || WP == 4, "weight planes per query");
+    static_assert(WP >= 1 && WP <= 4, "weight planes per query");
     // the kernel's only LDS, static so that the table sits at address 0 and
     // every table offset folds into the ds_read instruction
     constexpr int DPAD = NBLK * 16;
     constexpr bool WEIGHTED = !SETS && WP > 1;               // weighted masks (dense_weigh)
     constexpr uint32_t QPG = SETS ? 32u : 32u / WP;          // queries of a group
-    constexpr uint32_t QMASK = WP == 4 ? 0x11111111u : (WP == 2 ? 0x55555555u : 0xFFFFFFFFu);  // their bits
+    constexpr uint32_t QMASK = SETS ? 0xFFFFFFFFu : dense_query_bits(WP);  // their bits
     constexpr int NP = (SETS || WEIGHTED) ? kSetCountPlanes : 8;  // count planes
     constexpr uint32_t NONE = (SETS || WEIGHTED) ? kSetNoQuery : kDenseNoQuery;
     __shared__ uint32_t s_lds[(SETS ? set_lds_bytes(SAMPLE) : dense_lds_bytes(NBLK, SAMPLE, WP)) / 4];
@@ -661,31 +694,17 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     uint64_t* s_key = reinterpret_cast<uint64_t*>(s_meta + kDenseStage);
     constexpr uint32_t HB = (SETS || WEIGHTED) ? 256 : NBLK * 16 + 1;
     const uint32_t tid = threadIdx.x;
+    // work units u = c G + g (chunk c of group g): workgroup s of XCD x takes
+    // units s, s + P, ...; chunk-major, so that the workgroups of an XCD pass
+    // over the same tiles together and read them from its L2
     const uint32_t P = gridDim.x / 8, x = blockIdx.x % 8, s = blockIdx.x / 8;
-    uint32_t g_first, g_step, t_first, t_step;
-    if (a.G <= P) {
-        const uint32_t R = P / a.G;
-        if (s >= a.G * R)
-            return;  // the whole workgroup leaves before any barrier
-        g_first = s % a.G;
-        g_step = a.G;
-        t_first = x + 8 * (s / a.G);
-        t_step = 8 * R;
-    } else {
-        g_first = s;
-        g_step = P;
-        t_first = x;
-        t_step = 8;
-    }
+    const uint32_t n_units = a.G * a.chunks, t_step = 8 * a.chunks;
+    if (s >= n_units)
+        return;  // the whole workgroup leaves before any barrier
     const uint32_t n_list = a.list ? a.n_list : a.n_tiles;
     const uint32_t tab_words = NBLK * 16 * kDenseKeysPerDim;
     const __amdgpu_buffer_rsrc_t rs = codes_rsrc(a.codes);
     const uint32_t row_bytes = a.n_pad * 4u;
-    // sample pass: a workgroup without a sample tile builds no table, unless
-    // it is the one that writes its group's table out (few groups, many
-    // workgroups each: most have none of the few sample tiles)
-    const bool writer = !SETS && x == 0 && (a.G > P || s < a.G);
-    const bool idle = SAMPLE && t_first >= n_list && !writer;
     // set form, main pass: tiles in windows of kSetWindowTiles per XCD, each
     // window counted for all of the workgroup's groups before the next, so
     // that the 32 workgroups of an XCD read one window's lists (about 3 MB)
@@ -693,8 +712,12 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
     // window holds every tile
     const uint32_t win = (SETS && !SAMPLE) ? t_step * kSetWindowTiles : n_list;
     for (uint32_t w0 = 0; w0 < n_list; w0 += win)
-    for (uint32_t g = g_first; g < a.G && !idle; g += g_step) {
-        if (a.redo && !a.redo[g])
+    for (uint32_t u = s; u < n_units; u += P) {
+        const uint32_t g = u % a.G, t_first = x + 8 * (u / a.G);
+        // sample pass: a unit without a sample tile builds no table, unless
+        // it is the one that writes its group's table out (chunk 0 on XCD 0)
+        const bool writer = !SETS && x == 0 && u < a.G;
+        if ((SAMPLE && t_first >= n_list && !writer) || (a.redo && !a.redo[g]))
             continue;  // uniform across the workgroup
         __syncthreads();
         bool wide = false;
@@ -788,7 +811,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             if (tid < kWave) {
                 // bit WP q of the planes: query q of the group
                 const uint32_t q = g * QPG + tid / WP;
-                const uint32_t th = (tid < 32 && tid % WP == 0 && q < a.n_queries) ? a.gthr[q] : NONE;
+                const uint32_t th = (tid < QPG * WP && tid % WP == 0 && q < a.n_queries) ? a.gthr[q] : NONE;
 #pragma unroll
                 for (int b = 0; b < NP; ++b) {
                     const uint32_t pl = (uint32_t)ballot((th >> b) & 1u);

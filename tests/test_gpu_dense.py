@@ -169,13 +169,16 @@ def test_dense_autotune_includes_dense():
 
 
 @pytest.mark.parametrize("n,d,q,k,wmax", [
-    (150_000, 32, 37, 50, 4),     # weights 1-4, a partly filled last group of 8
-    (300_000, 128, 40, 100, 15),  # full width, weights up to 15 (counts up to 1,920)
-    (70_001, 20, 9, 10, 2),       # dims not a multiple of 16, two groups
+    (150_000, 32, 37, 50, 4),     # weights 1-4: 3 planes, a partly filled last group of 10
+    (200_000, 128, 45, 30, 7),    # 3 planes at full width (counts up to 896)
+    (300_000, 128, 40, 100, 15),  # 4 planes at full width (counts up to 1,920)
+    (100_000, 128, 33, 20, 3),    # 2 planes at full width, three groups of 16
+    (70_001, 20, 19, 10, 2),      # 2 planes, dims not a multiple of 16, two groups
 ])
 def test_dense_weighted_equals_oracle(n, d, q, k, wmax):
-    """Integer weights up to 15: four weight planes a query in each mask word
-    (8 queries a group), counts assembled over 11 planes."""
+    """Integer weights up to 15: as many weight planes a query in each mask
+    word as the largest weight has bits (2-4; 32 / planes queries a group),
+    counts assembled over 11 planes."""
     w = W.cfg1(n=n, d=d, q=q, k=k)
     t = Table.from_rows(w["rows"])
     r = w["ranges"].copy()
@@ -185,6 +188,26 @@ def test_dense_weighted_equals_oracle(n, d, q, k, wmax):
     assert p.dense_active() and p.form() == "dense"
     ri, rc = oracle_single(w["rows"], r, q, k)
     assert_topk_equal(b[0], b[1], ri, rc, k, q, f"weighted dense n={n} d={d} w<={wmax}")
+
+
+@pytest.mark.parametrize("q,wmax", [
+    (700, 1),    # 22 groups of 32: 7 tile chunks a group on 32 workgroups per XCD
+    (1056, 1),   # 33 groups: 8 chunks
+    (176, 15),   # 22 groups of 8
+    (1030, 4),   # 103 groups of 10: 4 chunks
+])
+def test_dense_group_chunks_equal_oracle(q, wmax):
+    """Group counts that the workgroups of an XCD do not divide: the main
+    pass splits every group's tiles into chunks and deals out (group, chunk)
+    units; each (group, tile) is counted once."""
+    w = W.cfg1(n=60_000, d=16, q=q, k=20, seed=q)
+    t = Table.from_rows(w["rows"])
+    r = w["ranges"].copy()
+    r["weight"] = genie_amd.synth.uniform_int(q, 0, len(r), 1, wmax)
+    (b, p) = run(t, t.compile(r, q), q, 20)
+    assert p.form() == "dense"
+    ri, rc = oracle_single(w["rows"], r, q, 20)
+    assert_topk_equal(b[0], b[1], ri, rc, 20, q, f"group chunks q={q} w<={wmax}")
 
 
 def test_dense_weighted_mixed_k_and_compressed():
