@@ -120,7 +120,7 @@ struct genie_plan {
     std::vector<uint32_t> h_qbound;       // count bound of every query
     std::vector<uint8_t> h_force_direct;  // queries whose table filled up in an earlier run
     std::vector<uint32_t> h_hwork;        // (query, tile) of the hash kernel
-    std::vector<uint4> h_hq;              // per query: directory tiles per tile, key bits, log2 slots
+    std::vector<uint4> h_hq;              // per query: directory tiles per tile, key bits, log2 slots, tiles
     std::vector<uint32_t> h_dq;           // queries of the direct-mapped kernels
     uint32_t max_hash_tiles = 0;
     bool plain_stats = false;             // h_qpost / h_qbound hold this batch's figures
@@ -170,6 +170,7 @@ struct genie_plan {
     uint32_t* d_total_pairs = nullptr;
     uint32_t* d_kq = nullptr;
     uint32_t* d_gthr = nullptr;
+    unsigned long long* d_gkey = nullptr;  // hash form: each query's largest tile k-th key
     uint32_t* d_cand_cnt = nullptr;
     uint32_t* d_cand_cap = nullptr;
     uint64_t* d_cand_base = nullptr;
@@ -296,6 +297,7 @@ static void plan_free_buffers(genie_plan* p)
     dev_free(p->d_total_pairs);
     dev_free(p->d_kq);
     dev_free(p->d_gthr);
+    dev_free(p->d_gkey);
     dev_free(p->d_cand_cnt);
     dev_free(p->d_cand_cap);
     dev_free(p->d_cand_base);
@@ -1395,6 +1397,7 @@ int genie_hip_plan_create(genie_index* ix, uint32_t max_queries, uint64_t max_it
         if ((rc = dev_alloc(&p->d_total_pairs, 1))) break;
         if ((rc = dev_alloc(&p->d_kq, max_queries))) break;
         if ((rc = dev_alloc(&p->d_gthr, max_queries))) break;
+        if ((rc = dev_alloc(&p->d_gkey, max_queries))) break;
         if ((rc = dev_alloc(&p->d_cand_cnt, max_queries))) break;
         if ((rc = dev_alloc(&p->d_cand_cap, max_queries))) break;
         if ((rc = dev_alloc(&p->d_cand_base, max_queries))) break;
@@ -2153,11 +2156,7 @@ static int plan_split_plain(genie_plan* p)
                 uint32_t lhs = 9;
                 while (lhs < (uint32_t)kHashLogMax && (1ull << lhs) < 2 * per)
                     ++lhs;
-                p->h_hq[q] = make_uint4((uint32_t)dtpt, kb, lhs, 0);
-                for (uint64_t t = 0; t < nt; ++t) {
-                    p->h_hwork.push_back(q);
-                    p->h_hwork.push_back((uint32_t)t);
-                }
+                p->h_hq[q] = make_uint4((uint32_t)dtpt, kb, lhs, (uint32_t)nt);
                 p->max_hash_tiles = std::max(p->max_hash_tiles, (uint32_t)nt);
                 hashed = true;
             }
@@ -2167,6 +2166,15 @@ static int plan_split_plain(genie_plan* p)
             p->est_postings += P;
         }
     }
+    // the hash work tile-major (every query's first tile, then every second
+    // one, ...), so that a query's later tiles mostly start after its first
+    // has published its k-th key, and select from fewer eligible slots
+    for (uint32_t t = 0; t < p->max_hash_tiles; ++t)
+        for (uint32_t q = 0; q < nq; ++q)
+            if (t < p->h_hq[q].w) {
+                p->h_hwork.push_back(q);
+                p->h_hwork.push_back(t);
+            }
     if (int rc = plan_full_decode(p))
         return rc;
     if (p->dec_full)
@@ -2515,6 +2523,7 @@ static int launch_scan(genie_plan* p, hipStream_t s)
         h.hq = p->d_hq;
         h.hov = p->d_hov;
         h.hov_any = p->d_hov_any;
+        h.gkey = p->d_gkey;
         h.spans = nullptr;
         h.sq_ptr = nullptr;
         const DevIndex& d = p->ix->d;
@@ -2766,6 +2775,8 @@ static int plan_run_once(genie_plan* p, hipStream_t s)
         z.p[4] = p->d_hov_any, z.n[4] = 1;
         if (dense_active(p))
             z.p[5] = p->d_dhist, z.n[5] = (uint64_t)nq * kDenseBins;
+        else
+            z.p[5] = reinterpret_cast<uint32_t*>(p->d_gkey), z.n[5] = 2ull * nq;
         const uint64_t most = std::max<uint64_t>(nq, z.n[5]);
         k_zero_many<<<(uint32_t)std::min<uint64_t>((most + 255) / 256, 1024), 256, 0, s>>>(z);
     }

@@ -20,9 +20,12 @@ constexpr uint32_t kHashTarget = 4096;  // postings per (query, tile) aimed at
 struct HashArgs {
     ScanArgs s;
     const uint32_t* work;  // [n_work][2]: query, tile
-    const uint4* hq;       // [n_queries]: directory tiles per tile, key bits, log2 slots, 0
+    const uint4* hq;       // [n_queries]: directory tiles per tile, key bits, log2 slots, tiles
     uint32_t* hov;         // [n_queries] set when a table of the query filled up
     uint32_t* hov_any;
+    // [n_queries] largest k-th key (count << 32 | ~id) of the query's tiles
+    // that emitted k candidates; 0 before any
+    unsigned long long* gkey;
     // decoded spans of a compressed index (null otherwise): query q's spans
     // are spans[sq_ptr[q] .. sq_ptr[q + 1]), positions in s.post (the batch's
     // decoded postings), filter flag set
@@ -131,13 +134,17 @@ __device__ __forceinline__ bool hash_add(uint32_t* tab, uint32_t lhs, uint32_t k
 // which the eligible ranks differ, finds the k-th, its histograms in the
 // table's LDS (free once the ranks are in registers).  Appends the selected
 // keys to the query's candidates and raises the threshold to the tile's
-// k-th count.
+// k-th count, and gkey to the tile's k-th key.
+// Eligible: count >= the threshold, and ranked above gkey: a tile that emitted
+// k candidates of keys >= gkey leaves no room in the top k for a key below
+// it, so when gkey's id lies below this tile's ids (the work runs tile-major,
+// lower tiles first), a count equal to gkey's is not eligible either.
 constexpr int kHashPer = (1 << kHashLogMax) / kScanThreads;  // slots per thread at most
 constexpr uint32_t kSelDigit = 11;                             // bits per radix pass
 constexpr uint32_t kSelBins = 1u << kSelDigit;
 
-__device__ __forceinline__ void hash_select(const ScanArgs& a, uint32_t* tab, uint32_t ns, uint32_t kb,
-                                            uint32_t q, uint32_t obj0, uint32_t* s_res, uint32_t* s_red)
+__device__ __forceinline__ void hash_select(const ScanArgs& a, unsigned long long* gkey, uint32_t* tab, uint32_t ns,
+                                            uint32_t kb, uint32_t q, uint32_t obj0, uint32_t* s_res, uint32_t* s_red)
 {
     static_assert(kSelBins <= (1u << kHashLogMax) && kSelBins == 32 * kWave, "histogram in the table, 32 bins a lane");
     const int tid = threadIdx.x, lane = lane_id(), wv = tid >> 6, nw = blockDim.x / kWave;
@@ -146,8 +153,15 @@ __device__ __forceinline__ void hash_select(const ScanArgs& a, uint32_t* tab, ui
     const uint32_t cmask = (1u << cb) - 1u;
     const uint32_t kfull = (1u << kb) - 1u;
     const uint32_t per = ns / blockDim.x;  // uniform, 1..kHashPer
-    if (tid == 0)
-        s_res[0] = __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
+        uint32_t lo = __hip_atomic_load(&a.gthr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t g = __hip_atomic_load(&gkey[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g) {
+            const uint32_t gc = (uint32_t)(g >> 32), gid = ~(uint32_t)g;
+            lo = max(lo, gc + (gid < obj0 ? 1u : 0u));
+        }
+        s_res[0] = lo;
+    }
     __syncthreads();
     const uint32_t lower = max(1u, s_res[0]);
     uint32_t rk[kHashPer];
@@ -249,14 +263,14 @@ __device__ __forceinline__ void hash_select(const ScanArgs& a, uint32_t* tab, ui
         }
         s_res[4] = base;
         s_res[5] = 0;            // slot counter
-        s_res[6] = 0xFFFFFFFFu;  // smallest selected count
+        s_res[6] = 0xFFFFFFFFu;  // smallest selected rank
     }
     __syncthreads();
     const uint32_t base = s_res[4];
     if (base == kEmitFail)
         return;
     uint64_t* out = a.cand + a.cand_base[q] + base;
-    uint32_t cmin = 0xFFFFFFFFu;
+    uint32_t rmin_sel = 0xFFFFFFFFu;  // smallest selected rank
 #pragma unroll
     for (int j = 0; j < kHashPer; ++j) {
         const uint32_t r = (uint32_t)j < per ? rk[j] : 0u;
@@ -268,18 +282,20 @@ __device__ __forceinline__ void hash_select(const ScanArgs& a, uint32_t* tab, ui
                 wb = atomicAdd(&s_res[5], (uint32_t)__popcll(bm));
             wb = (uint32_t)__shfl((int)wb, __builtin_ctzll(bm), kWave);
             if (take) {
-                const uint32_t c = r >> kb;
-                out[wb + lane_prefix(bm)] = make_key(c, obj0 + (kfull - (r & kfull)) - 1u);
-                cmin = min(cmin, c);
+                out[wb + lane_prefix(bm)] = make_key(r >> kb, obj0 + (kfull - (r & kfull)) - 1u);
+                rmin_sel = min(rmin_sel, r);
             }
         }
     }
-    cmin = wave_min(cmin);
-    if (lane == 0 && cmin != 0xFFFFFFFFu)
-        atomicMin(&s_res[6], cmin);
+    rmin_sel = wave_min(rmin_sel);
+    if (lane == 0 && rmin_sel != 0xFFFFFFFFu)
+        atomicMin(&s_res[6], rmin_sel);
     __syncthreads();
-    if (tid == 0 && E >= k)
-        atomicMax(&a.gthr[q], s_res[6]);
+    if (tid == 0 && E >= k) {
+        const uint32_t r = s_res[6];
+        atomicMax(&a.gthr[q], r >> kb);
+        atomicMax(&gkey[q], (unsigned long long)make_key(r >> kb, obj0 + (kfull - (r & kfull)) - 1u));
+    }
 }
 
 // Compressed spans: the segment range of each (chunk, key) pair that can hold
@@ -422,7 +438,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_hash(HashArgs h)
         }
         return;
     }
-    hash_select(a, s_tab, ns, kb, q, obj0, s_res, s_red);
+    hash_select(a, h.gkey, s_tab, ns, kb, q, obj0, s_res, s_red);
 }
 
 }  // namespace gh

@@ -62,6 +62,27 @@ def test_hash_equals_oracle(n, d, q, buckets, k):
     assert np.array_equal(b[0], c[0]) and np.array_equal(b[1], c[1])
 
 
+@pytest.mark.parametrize("n,d,q,buckets,k", [
+    (1_000_000, 64, 16, 1024, 1000),  # 16 tiles a query; the top 1,000 ends in ties at count 2
+    (1_000_000, 48, 16, 2048, 100),   # 4 tiles a query; ties at count 2 cut by id
+])
+def test_hash_rank_threshold_across_tiles(n, d, q, buckets, k):
+    """A query's tiles run tile-major, and a tile that emitted k candidates
+    publishes its k-th (count, id) key: later tiles, of higher ids, leave out
+    counts equal to that key's.  Every tile's own k-th is a count-1 tie, so
+    the later tiles select from their count-2 slots only; ids and counts
+    equal the oracle's and the direct-mapped form's exactly, ties included."""
+    rows, r = lsh(n, d, q, buckets, 331 + d)
+    t = Table.from_rows(rows)
+    items = t.compile(r, q)
+    (b, p) = plain(t, items, q, k, hash_mode=1, graph=True, runs=3)
+    assert p.hash_queries() == q
+    ri, rc = oracle_single(rows, r, q, k)
+    assert np.array_equal(b[1], rc) and np.array_equal(b[0], ri)
+    (c, p) = plain(t, items, q, k, hash_mode=0)
+    assert np.array_equal(b[0], c[0]) and np.array_equal(b[1], c[1])
+
+
 def test_hash_forced_on_dense_queries():
     """Every query on the hash form although each holds a posting per object
     and dim: tiles shrink to 8,192 objects, never more ids than slots."""
