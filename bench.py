@@ -204,7 +204,7 @@ def main():
     ap.add_argument("--no-workloads", action="store_true", help="skip the compressed, LSH and set workloads")
     ap.add_argument("--serve-n", type=int, default=10_000_000, help="objects of the serving index")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds of trace per offered load (0 = skip)")
-    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000, 100000, 200000, 300000])
+    ap.add_argument("--serve-qps", type=int, nargs="+", default=[2000, 10000, 50000, 100000, 125000, 150000, 200000])
     ap.add_argument("--serve-max-requests", type=int, default=60000,
                     help="requests per offered load at most (a load runs min(qps x seconds, this))")
     ap.add_argument("--serve-executors", type=int, default=4)

@@ -335,10 +335,14 @@ __device__ __forceinline__ void stage_words_rs(DecodeScratch& S, const uint32_t*
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(g) + w0, (short)0, (int)(nw * 4u), 0x00020000);
     const uint32_t lane = lane_id();
+    // rows of 64 words past the segment are not loaded (a short segment
+    // issues a few loads, not 16); they are staged as zeros
+    const uint32_t rows = __builtin_amdgcn_readfirstlane((nw + kWave - 1) / kWave);
     uint32_t t[kPerLane];
 #pragma unroll
     for (int r = 0; r < kPerLane; ++r)
-        t[r] = __builtin_amdgcn_raw_buffer_load_b32(rs, (lane + (uint32_t)r * kWave) * 4u, 0, 0);
+        t[r] = (uint32_t)r < rows ? __builtin_amdgcn_raw_buffer_load_b32(rs, (lane + (uint32_t)r * kWave) * 4u, 0, 0)
+                                  : 0u;
 #pragma unroll
     for (int r = 0; r < kPerLane; ++r)
         S.words[lane + (uint32_t)r * kWave] = t[r];

@@ -94,7 +94,9 @@ __global__ void __launch_bounds__(kDecodeWaves * kWave) __attribute__((amdgpu_wa
 #pragma unroll
         for (int k = 0; k < kPerLane; ++k) {
             const uint32_t i = (uint32_t)k * kWave + lane;
-            __builtin_amdgcn_raw_buffer_store_b32(t[(i / kPerLane) * (kPerLane + 1) + i % kPerLane], ro, i * 4u, 0, 0);
+            if ((uint32_t)k * kWave < n)  // uniform: rows past the segment are not issued
+                __builtin_amdgcn_raw_buffer_store_b32(t[(i / kPerLane) * (kPerLane + 1) + i % kPerLane], ro, i * 4u,
+                                                      0, 0);
         }
         wave_sync();  // the scratch is reused by the next job
     }

@@ -92,7 +92,7 @@ def main():
             ref = got
         same = bool(np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]))
         p.set_timing(True)
-        stages = {s: [] for s in ("compile", "sample", "scan", "merge")}
+        stages = {s: [] for s in ("compile", "sample", "decode", "scan", "merge")}
         t1 = time.perf_counter()
         for _ in range(args.reps):
             p.run()
