@@ -1,7 +1,8 @@
 """The workloads at their full sizes against the oracle:
 
   1M x 128 Zipf(1.1), 1,024 queries, +-2, k = 100 (the headline batch): the
-      whole batch, dense and plain forms, uncompressed and compressed;
+      whole batch, dense and plain forms, uncompressed and compressed; and
+      with integer weights 1-4 and 1-15 (the dense form's weighted masks);
   4M x 64 LSH (8,192 buckets), 4,096 exact-match queries, k = 100:
       uncompressed and SERIAL_DELTA_COMP_BP32_VARINT, a fixed sample of 256
       queries against the oracle and compressed = uncompressed over the whole
@@ -67,6 +68,26 @@ def test_headline_batch_whole_against_the_oracle():
     for mode in (0, 1):
         (ids, cnt), form = _run(cix, citems, nq, kq, mode)
         assert np.array_equal(ids, ri) and np.array_equal(cnt, rc), f"compressed {form}"
+
+
+@pytest.mark.parametrize("wmax", [4, 15])
+def test_headline_batch_weighted_whole_against_the_oracle(wmax):
+    t0 = time.time()
+    w = W.cfg1()
+    nq = w["n_queries"]
+    kq = np.full(nq, w["k"], np.uint32)
+    r = w["ranges"].copy()
+    r["weight"] = genie_amd.synth.uniform_int(1011 + wmax, 0, len(r), 1, wmax)
+    t = Table.from_rows(w["rows"])
+    items = t.compile(r, nq)
+    srt = items[np.argsort(items["query"], kind="stable")]
+    ri, rc, _ = O.search_mt(t.export()["inv"].view(np.uint32), srt, w["rows"].shape[0], nq, w["k"])
+    _log(f"1M x 128 weights 1-{wmax}: built and oracle in {time.time() - t0:.1f}s")
+    ix = t.device()
+    for mode in (1, 0):
+        (ids, cnt), form = _run(ix, items, nq, kq, mode)
+        assert form == ("dense" if mode == 1 else "plain")
+        assert np.array_equal(ids, ri) and np.array_equal(cnt, rc), form
 
 
 def test_lsh_4m_full_batch():
