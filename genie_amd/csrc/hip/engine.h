@@ -131,7 +131,7 @@ struct DevIndex {
     uint32_t set_slots = 0;         // objects rounded up to 64
     uint32_t* obj_perm = nullptr;   // [set_slots] object of each slot (kSetEmpty: none)
     uint32_t* blk_off = nullptr;    // [set_slots / 64 + 1] first word of each 64-slot block
-    uint32_t* blk_key = nullptr;    // [blk_off[last]] keys, column-major per block
+    uint32_t* blk_key = nullptr;    // [blk_off[last]] keys + 1 (0: padding), column-major per block
     uint64_t bytes = 0;
     // host copies used for validation and count bounds
     std::vector<uint64_t> h_key_start;

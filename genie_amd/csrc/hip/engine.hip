@@ -1050,9 +1050,9 @@ static int build_dense(const genie_index_desc* desc, DevIndex& d, const uint32_t
 // waves: within each tile of kDenseTile objects the objects are ordered by
 // list length, longest first (ties by id; `obj_perm` maps a slot to its
 // object), and each block of 64 slots stores its lists column-major -- key i
-// of slot j at blk_off[b] + 64 i + j, kSetEmpty past the end of a list -- so
-// that a wave reads 64 consecutive words per step and loops as often as its
-// block's longest list.  Built on the host; any other index simply has no
+// of slot j, plus one, at blk_off[b] + 64 i + j, 0 past the end of a list --
+// so that a wave reads 64 consecutive words per step and loops as often as
+// its block's longest list.  Built on the host; any other index simply has no
 // set form; only device errors fail.
 static int build_sets(const genie_index_desc* desc, DevIndex& d)
 {
@@ -1096,14 +1096,16 @@ static int build_sets(const genie_index_desc* desc, DevIndex& d)
             return GENIE_OK;  // too large for 32-bit offsets: no set form
         boff[b + 1] = boff[b] + 64 * len;
     }
-    std::vector<uint32_t> bkey(std::max<uint32_t>(1, boff[NB]), kSetEmpty);
+    // keys + 1, so that 0 pads the shorter lists of a block (and the kernel
+    // reads 0 past a block's last column)
+    std::vector<uint32_t> bkey(std::max<uint32_t>(1, boff[NB]), 0u);
     for (uint32_t b = 0; b < NB; ++b)
         for (uint32_t j = 0; j < 64; ++j) {
             const uint32_t o = perm[b * 64 + j];
             if (o == kSetEmpty)
                 continue;
             for (uint32_t i = 0; i < ptr[o + 1] - ptr[o]; ++i)
-                bkey[boff[b] + 64 * i + j] = keys[ptr[o] + i];
+                bkey[boff[b] + 64 * i + j] = keys[ptr[o] + i] + 1u;
         }
     uint32_t *d_perm = nullptr, *d_boff = nullptr, *d_bkey = nullptr;
     uint64_t bytes = 0;

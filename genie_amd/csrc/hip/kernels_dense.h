@@ -102,7 +102,7 @@ struct DenseArgs {
     // keys with the queries' weights as bit planes
     const uint32_t* obj_perm;    // [n_slots] object of each slot (kSetEmpty: none)
     const uint32_t* blk_off;     // [n_slots / 64 + 1] first word of each 64-slot block
-    const uint32_t* blk_key;     // key i of slot 64 b + j at blk_off[b] + 64 i + j (kSetEmpty: none)
+    const uint32_t* blk_key;     // key + 1 of occurrence i of slot 64 b + j at blk_off[b] + 64 i + j (0: none)
     uint32_t n_slots;            // objects rounded up to 64
     const uint32_t* set_keys;    // [entries] sorted keys of each group
     const uint4* set_mask;       // [entries][2] bit j of word b = bit b of query j's weight (b < 6)
@@ -612,19 +612,25 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 #pragma unroll
     for (int b = 0; b < NP; ++b)
         c[b] = 0;
-    const uint32_t blk = sl >> 6;
-    const uint32_t w1 = a.blk_off[blk + 1];
-    // kSetPrefetch columns loaded at once (the guard is uniform across the
-    // wave: a block holds whole columns), then looked up one by one
-    for (uint32_t w = a.blk_off[blk] + (sl & 63u); w < w1; w += 64 * kSetPrefetch) {
+    // the wave's 64 slots are one block (tiles and workgroups are multiples
+    // of 64 slots): its columns through a buffer that ends at the block, so
+    // that the loads past its last column read 0, the padding key
+    const uint32_t blk = __builtin_amdgcn_readfirstlane(sl >> 6);
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(a.blk_off[blk]);
+    const uint32_t ncol = __builtin_amdgcn_readfirstlane((a.blk_off[blk + 1] - w0) >> 6);
+    const __amdgpu_buffer_rsrc_t rk = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(a.blk_key) + w0,
+                                                                        (short)0, (int)(ncol * 256u), 0x00020000);
+    const uint32_t lane_off = (sl & 63u) * 4u;
+    // kSetPrefetch columns loaded at once, then looked up one by one
+    for (uint32_t col = 0; col < ncol; col += kSetPrefetch) {
         uint32_t kk[kSetPrefetch];
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u)
-            kk[u] = w + 64u * u < w1 ? a.blk_key[w + 64u * u] : kSetEmpty;
+            kk[u] = __builtin_amdgcn_raw_buffer_load_b32(rk, lane_off + (col + (uint32_t)u) * 256u, 0, 0);
         // probe to the key or an empty slot (the table is at most a quarter
         // full, so every probe sequence ends there, mostly at the first
         // slot); an empty slot's entry has zero weights (a key the group does
-        // not hold, or kSetEmpty past the end of the list)
+        // not hold, or the padding key 0, which no table holds)
         uint32_t x[kSetPrefetch][P];
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u) {
@@ -737,7 +743,7 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             int hi_any = 0;
             const uint32_t e0 = a.set_gptr[g];
             for (uint32_t e = e0 + tid; e < a.set_gptr[g + 1]; e += NT) {
-                const uint32_t key = a.set_keys[e];
+                const uint32_t key = a.set_keys[e] + 1u;  // as the forward lists hold it
                 const uint4 lo = a.set_mask[2 * e], hi = a.set_mask[2 * e + 1];
                 hi_any |= (hi.x | hi.y) != 0u;
                 s_mlo[e - e0] = lo;
