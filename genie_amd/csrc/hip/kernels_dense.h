@@ -636,12 +636,15 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
         for (int u = 0; u < kSetPrefetch; ++u) {
             const uint32_t key = kk[u];
             uint32_t h = set_hash(key);
-            uint2 sv = s_slot[h];
-            for (uint32_t probe = 1; sv.x != key && sv.x != kSetEmpty && probe < kSetSlots; ++probe) {
+            // (key, entry) in one 64-bit read: a lookup that ends at its first
+            // slot takes two dependent LDS reads, the slot and the weights
+            const uint64_t* s_slot64 = reinterpret_cast<const uint64_t*>(s_slot);
+            uint64_t sv = s_slot64[h];
+            for (uint32_t probe = 1; (uint32_t)sv != key && (uint32_t)sv != kSetEmpty && probe < kSetSlots; ++probe) {
                 h = (h + 1u) & (kSetSlots - 1u);
-                sv = s_slot[h];
+                sv = s_slot64[h];
             }
-            const uint32_t e = sv.y;
+            const uint32_t e = (uint32_t)(sv >> 32);
             const uint4 lo = s_mlo[e];
             x[u][0] = lo.x;
             x[u][1] = lo.y;
