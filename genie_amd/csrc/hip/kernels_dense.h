@@ -157,11 +157,14 @@ constexpr uint32_t kDenseNoQuery = 255u;
 constexpr int kSetPrefetch = 8;  // forward-list columns in flight per wave
 constexpr uint32_t kSetWindowTiles = 8;  // tiles per XCD counted together (main pass)
 
-// 13-bit slot of a key: bits 19..31 of key x 0x9E3779 (a full-rate 24-bit
-// multiply; bits of a key above 24 do not enter the hash, which only spreads
-// such keys less evenly).
-static_assert(kSetSlots == 1u << 13, "13-bit slot hash");
-__device__ __forceinline__ uint32_t set_hash(uint32_t key) { return ((key & 0xFFFFFFu) * 0x9E3779u) >> 19; }
+// Home slot of a key, always even: bits 20..31 of key x 0x9E3779 (a
+// full-rate 24-bit multiply; bits of a key above 24 do not enter the hash,
+// which only spreads such keys less evenly) select one of 4,096 slot pairs.
+// Inserts probe slots linearly from there, so the second slot of a pair is
+// taken only after the first, and a lookup reads a pair at a time, stopping
+// at a pair whose second slot is empty.
+static_assert(kSetSlots == 1u << 13, "4,096 pairs of slots");
+__device__ __forceinline__ uint32_t set_hash(uint32_t key) { return (((key & 0xFFFFFFu) * 0x9E3779u) >> 20) << 1; }
 
 // ---- index creation --------------------------------------------------------
 
@@ -635,16 +638,25 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
 #pragma unroll
         for (int u = 0; u < kSetPrefetch; ++u) {
             const uint32_t key = kk[u];
-            uint32_t h = set_hash(key);
-            // (key, entry) in one 64-bit read: a lookup that ends at its first
-            // slot takes two dependent LDS reads, the slot and the weights
-            const uint64_t* s_slot64 = reinterpret_cast<const uint64_t*>(s_slot);
-            uint64_t sv = s_slot64[h];
-            for (uint32_t probe = 1; (uint32_t)sv != key && (uint32_t)sv != kSetEmpty && probe < kSetSlots; ++probe) {
-                h = (h + 1u) & (kSetSlots - 1u);
-                sv = s_slot64[h];
+            // a pair of (key, entry) slots in one 128-bit read: a lookup that
+            // ends in its home pair takes two dependent LDS reads, the pair
+            // and the weights.  A key is in the first pair that holds it or
+            // has an empty second slot; a miss takes an empty slot's entry
+            // (the second slot's, empty whenever the first is)
+            const uint4* s_pair = reinterpret_cast<const uint4*>(s_slot);
+            uint32_t pr = set_hash(key) >> 1;
+            // the entries are pinned in registers where the pair is read (an
+            // empty asm), so that the compiler reads the pair in one
+            // instruction instead of the keys first and an entry afterwards
+            uint4 pv = s_pair[pr];
+            __asm__ volatile("" : "+v"(pv.y), "+v"(pv.w));
+            for (uint32_t probe = 1;
+                 ((pv.x != key) & (pv.z != key) & (pv.z != kSetEmpty)) && probe < kSetSlots / 2; ++probe) {
+                pr = (pr + 1u) & (kSetSlots / 2 - 1u);
+                pv = s_pair[pr];
+                __asm__ volatile("" : "+v"(pv.y), "+v"(pv.w));
             }
-            const uint32_t e = (uint32_t)(sv >> 32);
+            const uint32_t e = pv.x == key ? pv.y : pv.w;
             const uint4 lo = s_mlo[e];
             x[u][0] = lo.x;
             x[u][1] = lo.y;
