@@ -601,17 +601,83 @@ __device__ __forceinline__ void bs_acc(uint32_t (&c)[NP], const uint32_t (&x)[N]
 // longest list times each query's largest weight per key does).  The slots of
 // a wave form one block, whose lists it reads a column (64 words) per step.
 // WIDE: some weight of the group reaches 16 (planes 4-5 are read), chosen once
-// per group.  The kSetPrefetch = 8 weights of one step are summed by a tree
-// of bit-sliced adders (4 + 2 + 1 adds of 4 to 6 planes), then added to the
-// count planes once: the carry ripples through the upper count planes once
-// per step instead of once per occurrence.  No branch per occurrence but the
-// rare extra probes.
+// per group.  A step looks up S = 8 columns (S = 4 when at most 4 are left,
+// so that a list ends with fewer padding lookups): the S weights are summed
+// by a tree of bit-sliced adders (4 + 2 + 1 adds of 4 to 6 planes for 8),
+// then added to the count planes once: the carry ripples through the upper
+// count planes once per step instead of once per occurrence.  No branch per
+// occurrence but the rare extra probes.
+template <int NP, bool WIDE, int S>
+__device__ __forceinline__ void set_step(__amdgpu_buffer_rsrc_t rk, uint32_t lane_off, uint32_t col,
+                                         const uint2* s_slot, const uint4* s_mlo, const uint2* s_mhi,
+                                         uint32_t (&c)[NP])
+{
+    static_assert(S == 8 || S == 4, "an adder tree of four or eight weights");
+    constexpr int P = WIDE ? kSetWeightPlanes : 4;  // weight planes read (weights < 16 unless WIDE)
+    // S columns loaded at once, then looked up one by one
+    uint32_t kk[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u)
+        kk[u] = __builtin_amdgcn_raw_buffer_load_b32(rk, lane_off + (col + (uint32_t)u) * 256u, 0, 0);
+    // probe to the key or an empty slot (the table is at most a quarter
+    // full, so every probe sequence ends there, mostly at the first pair);
+    // an empty slot's entry has zero weights (a key the group does not hold,
+    // or the padding key 0, which no table holds)
+    uint32_t x[S][P];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        const uint32_t key = kk[u];
+        // a pair of (key, entry) slots in one 128-bit read: a lookup that
+        // ends in its home pair takes two dependent LDS reads, the pair and
+        // the weights.  A key is in the first pair that holds it or has an
+        // empty second slot; a miss takes an empty slot's entry (the second
+        // slot's, empty whenever the first is)
+        const uint4* s_pair = reinterpret_cast<const uint4*>(s_slot);
+        uint32_t pr = set_hash(key) >> 1;
+        // the entries are pinned in registers where the pair is read (an
+        // empty asm), so that the compiler reads the pair in one instruction
+        // instead of the keys first and an entry afterwards
+        uint4 pv = s_pair[pr];
+        __asm__ volatile("" : "+v"(pv.y), "+v"(pv.w));
+        for (uint32_t probe = 1;
+             ((pv.x != key) & (pv.z != key) & (pv.z != kSetEmpty)) && probe < kSetSlots / 2; ++probe) {
+            pr = (pr + 1u) & (kSetSlots / 2 - 1u);
+            pv = s_pair[pr];
+            __asm__ volatile("" : "+v"(pv.y), "+v"(pv.w));
+        }
+        const uint32_t e = pv.x == key ? pv.y : pv.w;
+        const uint4 lo = s_mlo[e];
+        x[u][0] = lo.x;
+        x[u][1] = lo.y;
+        x[u][2] = lo.z;
+        x[u][3] = lo.w;
+        if constexpr (WIDE) {
+            const uint2 hi = s_mhi[e];
+            x[u][4] = hi.x;
+            x[u][5] = hi.y;
+        }
+    }
+    uint32_t y[S / 2][P + 1], z[2][P + 2];
+#pragma unroll
+    for (int i = 0; i < S / 2; ++i)
+        bs_add<P>(x[2 * i], x[2 * i + 1], y[i]);
+    if constexpr (S == 8) {
+        uint32_t t[P + 3];
+        bs_add<P + 1>(y[0], y[1], z[0]);
+        bs_add<P + 1>(y[2], y[3], z[1]);
+        bs_add<P + 2>(z[0], z[1], t);
+        bs_acc<NP, P + 3>(c, t);
+    } else {
+        bs_add<P + 1>(y[0], y[1], z[0]);
+        bs_acc<NP, P + 2>(c, z[0]);
+    }
+}
+
 template <int NP, bool WIDE>
 __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, const uint2* s_slot,
                                                const uint4* s_mlo, const uint2* s_mhi, uint32_t (&c)[NP])
 {
-    static_assert(kSetPrefetch == 8, "an adder tree of eight weights");
-    constexpr int P = WIDE ? kSetWeightPlanes : 4;  // weight planes read (weights < 16 unless WIDE)
+    static_assert(kSetPrefetch == 8, "steps of eight columns");
 #pragma unroll
     for (int b = 0; b < NP; ++b)
         c[b] = 0;
@@ -624,58 +690,14 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
     const __amdgpu_buffer_rsrc_t rk = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(a.blk_key) + w0,
                                                                         (short)0, (int)(ncol * 256u), 0x00020000);
     const uint32_t lane_off = (sl & 63u) * 4u;
-    // kSetPrefetch columns loaded at once, then looked up one by one
-    for (uint32_t col = 0; col < ncol; col += kSetPrefetch) {
-        uint32_t kk[kSetPrefetch];
-#pragma unroll
-        for (int u = 0; u < kSetPrefetch; ++u)
-            kk[u] = __builtin_amdgcn_raw_buffer_load_b32(rk, lane_off + (col + (uint32_t)u) * 256u, 0, 0);
-        // probe to the key or an empty slot (the table is at most a quarter
-        // full, so every probe sequence ends there, mostly at the first
-        // slot); an empty slot's entry has zero weights (a key the group does
-        // not hold, or the padding key 0, which no table holds)
-        uint32_t x[kSetPrefetch][P];
-#pragma unroll
-        for (int u = 0; u < kSetPrefetch; ++u) {
-            const uint32_t key = kk[u];
-            // a pair of (key, entry) slots in one 128-bit read: a lookup that
-            // ends in its home pair takes two dependent LDS reads, the pair
-            // and the weights.  A key is in the first pair that holds it or
-            // has an empty second slot; a miss takes an empty slot's entry
-            // (the second slot's, empty whenever the first is)
-            const uint4* s_pair = reinterpret_cast<const uint4*>(s_slot);
-            uint32_t pr = set_hash(key) >> 1;
-            // the entries are pinned in registers where the pair is read (an
-            // empty asm), so that the compiler reads the pair in one
-            // instruction instead of the keys first and an entry afterwards
-            uint4 pv = s_pair[pr];
-            __asm__ volatile("" : "+v"(pv.y), "+v"(pv.w));
-            for (uint32_t probe = 1;
-                 ((pv.x != key) & (pv.z != key) & (pv.z != kSetEmpty)) && probe < kSetSlots / 2; ++probe) {
-                pr = (pr + 1u) & (kSetSlots / 2 - 1u);
-                pv = s_pair[pr];
-                __asm__ volatile("" : "+v"(pv.y), "+v"(pv.w));
-            }
-            const uint32_t e = pv.x == key ? pv.y : pv.w;
-            const uint4 lo = s_mlo[e];
-            x[u][0] = lo.x;
-            x[u][1] = lo.y;
-            x[u][2] = lo.z;
-            x[u][3] = lo.w;
-            if constexpr (WIDE) {
-                const uint2 hi = s_mhi[e];
-                x[u][4] = hi.x;
-                x[u][5] = hi.y;
-            }
+    for (uint32_t col = 0; col < ncol;) {  // uniform
+        if (ncol - col > 4) {
+            set_step<NP, WIDE, 8>(rk, lane_off, col, s_slot, s_mlo, s_mhi, c);
+            col += 8;
+        } else {
+            set_step<NP, WIDE, 4>(rk, lane_off, col, s_slot, s_mlo, s_mhi, c);
+            col += 4;
         }
-        uint32_t y[4][P + 1], z[2][P + 2], t[P + 3];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            bs_add<P>(x[2 * i], x[2 * i + 1], y[i]);
-        bs_add<P + 1>(y[0], y[1], z[0]);
-        bs_add<P + 1>(y[2], y[3], z[1]);
-        bs_add<P + 2>(z[0], z[1], t);
-        bs_acc<NP, P + 3>(c, t);
     }
 }
 
