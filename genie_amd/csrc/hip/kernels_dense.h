@@ -834,11 +834,19 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
             for (uint32_t i = tid; i < tab_words / 4; i += NT)
                 dst[i] = src[i];
         }
-        // SETS and weighted sample passes: the bin shift of each query of the group
+        // SETS and weighted sample passes: the bin shift of each query of the
+        // group, and (SETS) the group's queries by shift as four masks
         uint64_t shifts = 0;  // 2 bits per query
+        uint32_t by_shift[4] = {0u, 0u, 0u, 0u};
         if constexpr ((SETS || WEIGHTED) && SAMPLE) {
-            for (uint32_t j = 0; j < QPG && g * QPG + j < a.n_queries; ++j)
-                shifts |= (uint64_t)min(a.hshift[g * QPG + j], 3u) << (2 * j);
+            for (uint32_t j = 0; j < QPG && g * QPG + j < a.n_queries; ++j) {
+                const uint32_t sh = min(a.hshift[g * QPG + j], 3u);
+                shifts |= (uint64_t)sh << (2 * j);
+                by_shift[sh] |= 1u << j;
+            }
+#pragma unroll
+            for (int sh = 0; sh < 4; ++sh)  // uniform: scalar registers
+                by_shift[sh] = __builtin_amdgcn_readfirstlane(by_shift[sh]);
         }
         uint32_t tp[NP] = {};
         if constexpr (!SAMPLE) {
@@ -891,7 +899,42 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 } else {
                     dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
                 }
-                if constexpr (SAMPLE && (SETS || WEIGHTED)) {
+                if constexpr (SAMPLE && SETS) {
+                    // bins as 8 planes: bit j of plane k is bit k + s of query
+                    // j's count, s its shift, all ones when the count reaches
+                    // 256 << s; then filed as the unit form files its counts
+                    // (queries rotated per lane, zero counts too)
+                    static_assert(HB == 256 && NP >= 11, "8-bit bins of 11-plane counts");
+                    uint32_t sat = 0;
+#pragma unroll
+                    for (int sh = 0; sh < 4; ++sh)
+#pragma unroll
+                        for (int b = 8 + sh; b < NP; ++b)
+                            sat |= c[b] & by_shift[sh];
+                    uint32_t bp[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        uint32_t v = sat;
+#pragma unroll
+                        for (int sh = 0; sh < 4; ++sh)
+                            v |= c[k + sh] & by_shift[sh];
+                        bp[k] = v;
+                    }
+                    const uint32_t r = tid & 31u;
+                    uint32_t cr[8];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                        cr[b] = __builtin_amdgcn_alignbit(bp[b], bp[b], r);  // bit j: query (j + r) mod 32
+                    uint64_t cnt[4];
+                    dense_all_counts(cr, cnt);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const uint32_t v = (uint32_t)(cnt[q] >> (8 * j)) & 255u;
+                            atomicAdd(&s_hist[((8u * q + j + r) & 31u) * HB + v], 1u);
+                        }
+                } else if constexpr (SAMPLE && WEIGHTED) {
                     uint32_t any = 0;
 #pragma unroll
                     for (int b = 0; b < NP; ++b)
