@@ -2262,9 +2262,9 @@ static int plan_upload(genie_plan* p, const genie_dim_t* items, uint64_t n_items
     p->max_hash_tiles = 0;
     // count bound of this batch
     if (p->count_bound == 0) {
-        if (p->dense_ok) {
-            p->lane_bits = 8;  // dense: unit weights, each dim once, counts <= 128
-        } else {
+        if (p->dense_ok && p->dense_wp == 1) {
+            p->lane_bits = 8;  // dense, unit weights: each dim once, counts <= 128
+        } else {  // weighted dense batches count up to 128 x 15
             p->h_qbound.resize(n_queries);
             compute_bounds(d, p->h_items.data(), n_items, n_queries, p->h_qbound.data());
             uint32_t mx = 0;

@@ -1342,6 +1342,157 @@ struct MergeArgs {
     uint32_t* flag_any;
 };
 
+// Queries with k <= kMergeSmallK and at most kMergeSmallM candidates are
+// merged by the first wave of their workgroup alone, with no workgroup
+// barrier (merge_wave).  A wave alone serializes its histogram atomics, so
+// longer candidate lists (the headline batch's ~450 a query) merge faster on
+// the whole workgroup.
+constexpr uint32_t kMergeSmallK = 128;
+constexpr uint32_t kMergeSmallM = 256;  // 4 keys per lane
+
+__device__ __forceinline__ bool merge_small(uint32_t k, uint32_t m)
+{
+    return k <= kMergeSmallK && m <= kMergeSmallM;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, kWave);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One compare-exchange step of a descending bitonic sort: element idx keeps
+// the larger of itself and its partner idx ^ stride when it is the lower of
+// the two in a descending run (idx & size == 0), or the higher in an
+// ascending one.
+__device__ __forceinline__ uint64_t bitonic_step(uint64_t x, uint64_t y, uint32_t idx, uint32_t size, uint32_t stride)
+{
+    const bool desc = (idx & size) == 0u;
+    const bool keep_max = ((idx & stride) == 0u) == desc;
+    return keep_max ? (x > y ? x : y) : (x < y ? x : y);
+}
+
+// k_merge of one query by one wave (k <= 128, at most 256 candidates): the
+// candidates in registers (4 per lane), the radix select of k_merge with the
+// histogram in `hist` (256 words), the selected keys compacted into `sel`
+// (kMergeSmallK keys) and sorted descending across the lanes by a bitonic
+// network of shuffles (64 keys, one a lane, or 128, two), with no workgroup
+// barrier.
+__device__ __forceinline__ void merge_wave(const MergeArgs& a, uint32_t q, uint32_t k, uint32_t M, uint32_t* hist,
+                                           uint64_t* sel)
+{
+    constexpr int PL = (int)(kMergeSmallM / kWave);
+    const int lane = lane_id();
+    const uint64_t* keys = a.cand + a.cand_base[q];
+    const uint32_t rows = (M + kWave - 1) / kWave;  // uniform
+    uint64_t v[PL];
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+        const uint32_t i = (uint32_t)lane + (uint32_t)j * kWave;
+        v[j] = ((uint32_t)j < rows && i < M) ? keys[i] : 0ull;
+    }
+    uint64_t kth = 0;  // every key >= kth is in the result
+    if (M > k) {
+        uint64_t prefix = 0, mask = 0;
+        uint32_t remaining = k;
+        for (int shift = ((int)((a.key_bits + 7) / 8) - 1) * 8; shift >= 0; shift -= 8) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                hist[lane + b * kWave] = 0;
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                const uint32_t i = (uint32_t)lane + (uint32_t)j * kWave;
+                if ((uint32_t)j < rows && i < M && (v[j] & mask) == prefix)
+                    atomicAdd(&hist[(uint32_t)(v[j] >> shift) & 255u], 1u);
+            }
+            wave_sync();
+            // lane l holds bins 255 - 4l .. 252 - 4l, counted from the top
+            const uint32_t top = 255u - 4u * (uint32_t)lane;
+            uint32_t h[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                h[j] = hist[top - j];
+            const uint32_t sum = h[0] + h[1] + h[2] + h[3];
+            const uint32_t above = wave_incl_scan(sum) - sum;  // keys in higher bins
+            const bool here = above < remaining && above + sum >= remaining;
+            uint32_t bin = 0, rem = 0, inbin = 0;
+            if (here) {
+                uint32_t ab = above;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (ab + h[j] >= remaining) {
+                        bin = top - j;
+                        rem = remaining - ab;
+                        inbin = h[j];
+                        break;
+                    }
+                    ab += h[j];
+                }
+            }
+            const int src = __builtin_ctzll(ballot(here));  // exactly one lane
+            bin = (uint32_t)__shfl((int)bin, src, kWave);
+            rem = (uint32_t)__shfl((int)rem, src, kWave);
+            inbin = (uint32_t)__shfl((int)inbin, src, kWave);
+            prefix |= (uint64_t)bin << shift;
+            mask |= (uint64_t)255 << shift;
+            remaining = rem;
+            wave_sync();  // the histogram is read before the next pass clears it
+            if (inbin == rem)
+                break;  // the whole bin is needed: prefix with zero low bytes bounds it
+        }
+        kth = prefix;
+    }
+    // the selected keys (min(M, k) of them), compacted; all of them up to
+    // kMergeSmallK are sorted, so that a select that took more (keys with
+    // bits above key_bits) still yields the k largest
+    uint32_t base = 0;
+#pragma unroll
+    for (int j = 0; j < PL; ++j) {
+        const uint32_t i = (uint32_t)lane + (uint32_t)j * kWave;
+        const bool take = (uint32_t)j < rows && i < M && v[j] >= kth;
+        const uint64_t bm = ballot(take);
+        const uint32_t slot = base + lane_prefix(bm);
+        if (take && slot < kMergeSmallK)
+            sel[slot] = v[j];
+        base += (uint32_t)__popcll(bm);
+    }
+    wave_sync();
+    const uint32_t nsel = min(base, kMergeSmallK);
+    uint64_t e0 = (uint32_t)lane < nsel ? sel[lane] : 0ull;
+    uint64_t e1 = (uint32_t)lane + kWave < nsel ? sel[lane + kWave] : 0ull;
+    const uint32_t l = (uint32_t)lane;
+    if (nsel > (uint32_t)kWave) {  // 128 keys: element l in e0, l + 64 in e1
+        for (uint32_t size = 2; size <= 2u * kWave; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                if (stride == (uint32_t)kWave) {  // within the lane (size 128: one descending run)
+                    const uint64_t hi = e0 > e1 ? e0 : e1, lo = e0 > e1 ? e1 : e0;
+                    e0 = hi;
+                    e1 = lo;
+                } else {
+                    const uint64_t p0 = shfl_xor_u64(e0, (int)stride), p1 = shfl_xor_u64(e1, (int)stride);
+                    e0 = bitonic_step(e0, p0, l, size, stride);
+                    e1 = bitonic_step(e1, p1, l + kWave, size, stride);
+                }
+            }
+    } else {  // at most 64 keys, one a lane
+        for (uint32_t size = 2; size <= (uint32_t)kWave; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+                e0 = bitonic_step(e0, shfl_xor_u64(e0, (int)stride), l, size, stride);
+    }
+    const uint64_t o = a.out_off[q];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t i = l + (uint32_t)r * kWave;
+        if (i < k) {
+            const uint64_t key = r == 0 ? e0 : e1;
+            a.out_ids[o + i] = key ? (int32_t)(~(uint32_t)key) : 0;
+            a.out_counts[o + i] = key ? (int32_t)(key >> 32) : 0;
+        }
+    }
+}
+
 // Per query: select the k largest candidate keys (radix select over the key
 // bytes, most significant first), sort them descending, pad with (0, 0).
 // Candidates are staged in LDS when they fit; the bin holding the k-th key
@@ -1360,6 +1511,11 @@ __global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
         *a.flag_any = *a.flag_a | *a.flag_b;
     const uint32_t k = a.kq[q];
     const uint32_t M = min(a.cand_cnt[q], a.cand_cap[q]);
+    if (merge_small(k, M)) {  // one wave does it (the whole workgroup decides alike, before any barrier)
+        if (tid < kWave)
+            merge_wave(a, q, k, M, s_hist, s_keys);
+        return;
+    }
     const uint64_t* keys = a.cand + a.cand_base[q];
     const bool staged = M <= (uint32_t)kMergeMaxK;
     if (staged)
@@ -1489,5 +1645,6 @@ __global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
         }
     }
 }
+
 
 }  // namespace gh

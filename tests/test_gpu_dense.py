@@ -210,6 +210,28 @@ def test_dense_group_chunks_equal_oracle(q, wmax):
     assert_topk_equal(b[0], b[1], ri, rc, 20, q, f"group chunks q={q} w<={wmax}")
 
 
+@pytest.mark.parametrize("form", ["lane", "plain"])
+def test_dense_weighted_batch_on_the_tile_forms(form):
+    """A batch the dense form could take, with counts past 255 (128 dims,
+    weights up to 15), run with the dense form switched off: the lane and
+    plain forms size their counters from the batch's count bound, not from
+    the unit-weight dense bound of 128."""
+    w = W.cfg1(n=100_000, d=128, q=20, k=50, seed=77)
+    t = Table.from_rows(w["rows"])
+    r = w["ranges"].copy()
+    r["weight"] = genie_amd.synth.uniform_int(78, 0, len(r), 8, 15)
+    items = t.compile(r, 20)
+    p = Plan(t.device(), 20, len(items), 50, 0, mode=1)
+    p.set_dense(False)
+    p.upload(items, 20, np.full(20, 50, np.uint32))
+    p.set_form(form)
+    p.run()
+    b = p.download()
+    ri, rc = oracle_single(w["rows"], r, 20, 50)
+    assert int(rc.max()) > 255
+    assert_topk_equal(b[0], b[1], ri, rc, 50, 20, f"weighted batch, {form} form")
+
+
 def test_dense_weighted_mixed_k_and_compressed():
     w = W.cfg1(n=120_000, d=64, q=30, k=100)
     r = w["ranges"].copy()
