@@ -17,6 +17,10 @@
 // One LDS read serves 32 queries; the 32 sums are kept bit-sliced in eight
 // registers (bit j of plane b = bit b of query j's count) and added with
 // carry-save adders (v_bitop3: XOR3 and majority), 16 dims per step.
+// Integer weights up to 15 take WP = 2-4 bits a query (the bits of the
+// batch's largest weight): a word holds 32 / WP queries, bit WP j + p being
+// bit p of query j's weight on the key, and the plane sums are weighted by
+// 2^p afterwards (dense_weigh).
 //
 // The top-k is exact: a sample pass computes, per query, the k-th largest
 // count among evenly spread sample tiles (a lower bound of the final k-th
@@ -82,10 +86,10 @@ struct DenseLayout {
 struct DenseArgs {
     const uint32_t* codes;  // [dense_words(D_pad)][n_pad] packed fields
     uint32_t n_pad;
-    uint32_t* tab;          // [G][D_pad * 256] masks, bit j = query 32 g + j (written by the sample pass)
+    uint32_t* tab;          // [G][D_pad * 256] masks, bit WP j + p = bit p of query (32 / WP) g + j's weight (sample pass)
     const genie_dim_t* items;     // the batch, sorted by query
     const uint32_t* item_qptr;    // [n_queries + 1]
-    const uint32_t* dkeys;        // [n_items][2]: first key | (query % 32) << 24, end key
+    const uint32_t* dkeys;        // [n_items][2]: first key | query's bit position << 24, end key | weight << 16
     const uint32_t* key_slot;     // [n_keys] dim * 256 + slot
     uint32_t D_pad, G, n_queries, n_objects, n_tiles;
     const uint32_t* list;   // tiles to visit (sample pass), null = every tile
@@ -512,7 +516,8 @@ constexpr uint32_t kDenseFlushTiles = 8;
 constexpr uint32_t kDenseStageWords = 2 + 32 + 32 + kDenseStage * 3;  // count, pad, per-query counts / bases, meta, keys
 
 // wp: weight planes per query in a mask word (1: unit weights, 32 queries a
-// group; 2 or 4: weighted, 16 or 8 queries, sample histograms of 256 bins).
+// group; 2, 3 or 4: weighted, 16, 10 or 8 queries, sample histograms of 256
+// bins).
 __host__ __device__ constexpr uint32_t dense_lds_bytes(int nblk, bool sample, int wp = 1)
 {
     return (uint32_t)(nblk * 16 * kDenseKeysPerDim + 16 +
