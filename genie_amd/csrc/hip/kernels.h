@@ -1373,6 +1373,47 @@ __device__ __forceinline__ uint64_t bitonic_step(uint64_t x, uint64_t y, uint32_
     return keep_max ? (x > y ? x : y) : (x < y ? x : y);
 }
 
+// Sorts up to 128 keys descending across one wave (element l in e0, l + 64 in
+// e1; zeros pad) by a bitonic network of shuffles, and writes the query's k
+// results, keys of 0 and the slots past the 128th as (0, 0).
+__device__ __forceinline__ void wave_sort_out(const MergeArgs& a, uint32_t q, uint32_t k, uint64_t e0, uint64_t e1,
+                                              uint32_t n)
+{
+    const uint32_t l = (uint32_t)lane_id();
+    if (n > (uint32_t)kWave) {  // 128 keys
+        for (uint32_t size = 2; size <= 2u * kWave; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                if (stride == (uint32_t)kWave) {  // within the lane (size 128: one descending run)
+                    const uint64_t hi = e0 > e1 ? e0 : e1, lo = e0 > e1 ? e1 : e0;
+                    e0 = hi;
+                    e1 = lo;
+                } else {
+                    const uint64_t p0 = shfl_xor_u64(e0, (int)stride), p1 = shfl_xor_u64(e1, (int)stride);
+                    e0 = bitonic_step(e0, p0, l, size, stride);
+                    e1 = bitonic_step(e1, p1, l + kWave, size, stride);
+                }
+            }
+    } else {  // at most 64 keys, one a lane
+        for (uint32_t size = 2; size <= (uint32_t)kWave; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
+                e0 = bitonic_step(e0, shfl_xor_u64(e0, (int)stride), l, size, stride);
+    }
+    const uint64_t o = a.out_off[q];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t i = l + (uint32_t)r * kWave;
+        if (i < k) {
+            const uint64_t key = r == 0 ? e0 : e1;
+            a.out_ids[o + i] = key ? (int32_t)(~(uint32_t)key) : 0;
+            a.out_counts[o + i] = key ? (int32_t)(key >> 32) : 0;
+        }
+    }
+    for (uint32_t i = 2u * kWave + l; i < k; i += kWave) {
+        a.out_ids[o + i] = 0;
+        a.out_counts[o + i] = 0;
+    }
+}
+
 // k_merge of one query by one wave (k <= 128, at most 256 candidates): the
 // candidates in registers (4 per lane), the radix select of k_merge with the
 // histogram in `hist` (256 words), the selected keys compacted into `sel`
@@ -1462,35 +1503,7 @@ __device__ __forceinline__ void merge_wave(const MergeArgs& a, uint32_t q, uint3
     const uint32_t nsel = min(base, kMergeSmallK);
     uint64_t e0 = (uint32_t)lane < nsel ? sel[lane] : 0ull;
     uint64_t e1 = (uint32_t)lane + kWave < nsel ? sel[lane + kWave] : 0ull;
-    const uint32_t l = (uint32_t)lane;
-    if (nsel > (uint32_t)kWave) {  // 128 keys: element l in e0, l + 64 in e1
-        for (uint32_t size = 2; size <= 2u * kWave; size <<= 1)
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                if (stride == (uint32_t)kWave) {  // within the lane (size 128: one descending run)
-                    const uint64_t hi = e0 > e1 ? e0 : e1, lo = e0 > e1 ? e1 : e0;
-                    e0 = hi;
-                    e1 = lo;
-                } else {
-                    const uint64_t p0 = shfl_xor_u64(e0, (int)stride), p1 = shfl_xor_u64(e1, (int)stride);
-                    e0 = bitonic_step(e0, p0, l, size, stride);
-                    e1 = bitonic_step(e1, p1, l + kWave, size, stride);
-                }
-            }
-    } else {  // at most 64 keys, one a lane
-        for (uint32_t size = 2; size <= (uint32_t)kWave; size <<= 1)
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1)
-                e0 = bitonic_step(e0, shfl_xor_u64(e0, (int)stride), l, size, stride);
-    }
-    const uint64_t o = a.out_off[q];
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-        const uint32_t i = l + (uint32_t)r * kWave;
-        if (i < k) {
-            const uint64_t key = r == 0 ? e0 : e1;
-            a.out_ids[o + i] = key ? (int32_t)(~(uint32_t)key) : 0;
-            a.out_counts[o + i] = key ? (int32_t)(key >> 32) : 0;
-        }
-    }
+    wave_sort_out(a, q, k, e0, e1, nsel);
 }
 
 // Per query: select the k largest candidate keys (radix select over the key
@@ -1616,6 +1629,12 @@ __global__ void __launch_bounds__(kMergeThreads) k_merge(MergeArgs a)
         }
     }
     __syncthreads();
+    if (n2 <= 2u * kWave) {  // at most 128 selected keys: the first wave sorts and writes them
+        if (tid < kWave)
+            wave_sort_out(a, q, k, tid < (int)n2 ? s_keys[tid] : 0ull,
+                          tid + kWave < (int)n2 ? s_keys[tid + kWave] : 0ull, n2);  // the stage's first n2 keys
+        return;
+    }
     // bitonic sort, descending
     for (uint32_t size = 2; size <= n2; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
