@@ -5,6 +5,8 @@ one (bench.py measures 1M x 128, k = 100), with the batch already in HBM:
   lsh              4M x 64 LSH buckets (8,192 per dim), exact match, 4,096 queries, k = 100
   lsh-compressed   the same index with SERIAL_DELTA_COMP_BP32_VARINT segments
   sets             2M Zipf(1.05) token sets over a 1M vocabulary, 16,384 weighted queries, mixed k
+  weighted         the headline batch (1M x 128, 1,024 queries, +-2, k = 100) with weights
+                   uniform in {1, 2, 3, 4} (the dense form's weighted masks)
 
     python genie_amd/measure.py lsh [--reps 5] [--forms plain-hash plain-direct ...] [--merge-runs]
 
@@ -52,6 +54,12 @@ def workload(name, merge_runs=False):
     elif name == "sets":
         w = W.cfg3()
         t = Table.from_sets(w["row_ptr"], w["vals"])
+    elif name == "weighted":
+        import genie_amd
+        w = dict(W.cfg1())
+        w["ranges"] = w["ranges"].copy()
+        w["ranges"]["weight"] = genie_amd.synth.uniform_int(1011, 0, len(w["ranges"]), 1, 4)  # as bench.py
+        t = Table.from_rows(w["rows"])
     else:
         raise SystemExit(f"unknown workload {name}")
     items = t.compile(w["ranges"], w["n_queries"], merge_runs=merge_runs)
@@ -60,7 +68,7 @@ def workload(name, merge_runs=False):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["zipf-compressed", "lsh", "lsh-compressed", "sets"])
+    ap.add_argument("workload", choices=["zipf-compressed", "lsh", "lsh-compressed", "sets", "weighted"])
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--forms", nargs="+", default=["plain-auto", "plain-direct", "automatic"], choices=list(FORMS))
     ap.add_argument("--merge-runs", action="store_true", help="one item per range over a compressed index")
