@@ -682,7 +682,7 @@ template <int NP, bool WIDE>
 __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, const uint2* s_slot,
                                                const uint4* s_mlo, const uint2* s_mhi, uint32_t (&c)[NP])
 {
-    static_assert(kSetPrefetch == 8, "steps of eight columns");
+    constexpr uint32_t S = kSetPrefetch, S2 = kSetPrefetch / 2;
 #pragma unroll
     for (int b = 0; b < NP; ++b)
         c[b] = 0;
@@ -696,12 +696,12 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
                                                                         (short)0, (int)(ncol * 256u), 0x00020000);
     const uint32_t lane_off = (sl & 63u) * 4u;
     for (uint32_t col = 0; col < ncol;) {  // uniform
-        if (ncol - col > 4) {
-            set_step<NP, WIDE, 8>(rk, lane_off, col, s_slot, s_mlo, s_mhi, c);
-            col += 8;
+        if (ncol - col > S2) {
+            set_step<NP, WIDE, S>(rk, lane_off, col, s_slot, s_mlo, s_mhi, c);
+            col += S;
         } else {
-            set_step<NP, WIDE, 4>(rk, lane_off, col, s_slot, s_mlo, s_mhi, c);
-            col += 4;
+            set_step<NP, WIDE, S2>(rk, lane_off, col, s_slot, s_mlo, s_mhi, c);
+            col += S2;
         }
     }
 }
