@@ -706,6 +706,31 @@ __device__ __forceinline__ void set_accumulate(const DenseArgs& a, uint32_t sl, 
     }
 }
 
+// Sample-pass bins of 11-plane counts as 8 bit-sliced planes: bit b of
+// plane k is bit k + s of the count at bit b, s its query's shift (the
+// positions of the queries with shift s are set in by_shift[s]), all ones
+// when the count reaches 256 << s.
+template <int NP>
+__device__ __forceinline__ void dense_bin_planes(const uint32_t (&c)[NP], const uint32_t (&by_shift)[4],
+                                                 uint32_t (&bp)[8])
+{
+    static_assert(NP >= 11, "8-bit bins of 11-plane counts");
+    uint32_t sat = 0;
+#pragma unroll
+    for (int sh = 0; sh < 4; ++sh)
+#pragma unroll
+        for (int b = 8 + sh; b < NP; ++b)
+            sat |= c[b] & by_shift[sh];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t v = sat;
+#pragma unroll
+        for (int sh = 0; sh < 4; ++sh)
+            v |= c[k + sh] & by_shift[sh];
+        bp[k] = v;
+    }
+}
+
 // Workgroup -> (query group, tiles) so that the groups of one tile run on one
 // XCD (workgroup ids are dealt to the 8 XCDs round robin) and each
 // workgroup loads one group's mask table once.
@@ -840,15 +865,11 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                 dst[i] = src[i];
         }
         // SETS and weighted sample passes: the bin shift of each query of the
-        // group, and (SETS) the group's queries by shift as four masks
-        uint64_t shifts = 0;  // 2 bits per query
+        // group, as four masks of the queries' bit positions by shift
         uint32_t by_shift[4] = {0u, 0u, 0u, 0u};
         if constexpr ((SETS || WEIGHTED) && SAMPLE) {
-            for (uint32_t j = 0; j < QPG && g * QPG + j < a.n_queries; ++j) {
-                const uint32_t sh = min(a.hshift[g * QPG + j], 3u);
-                shifts |= (uint64_t)sh << (2 * j);
-                by_shift[sh] |= 1u << j;
-            }
+            for (uint32_t j = 0; j < QPG && g * QPG + j < a.n_queries; ++j)
+                by_shift[min(a.hshift[g * QPG + j], 3u)] |= 1u << (WEIGHTED ? WP * j : j);
 #pragma unroll
             for (int sh = 0; sh < 4; ++sh)  // uniform: scalar registers
                 by_shift[sh] = __builtin_amdgcn_readfirstlane(by_shift[sh]);
@@ -905,26 +926,11 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                     dense_accumulate<DPAD>(rs, row_bytes, o, s_tab, c);
                 }
                 if constexpr (SAMPLE && SETS) {
-                    // bins as 8 planes: bit j of plane k is bit k + s of query
-                    // j's count, s its shift, all ones when the count reaches
-                    // 256 << s; then filed as the unit form files its counts
-                    // (queries rotated per lane, zero counts too)
-                    static_assert(HB == 256 && NP >= 11, "8-bit bins of 11-plane counts");
-                    uint32_t sat = 0;
-#pragma unroll
-                    for (int sh = 0; sh < 4; ++sh)
-#pragma unroll
-                        for (int b = 8 + sh; b < NP; ++b)
-                            sat |= c[b] & by_shift[sh];
+                    // the bins as 8 planes, filed as the unit form files its
+                    // counts (queries rotated per lane, zero counts too)
+                    static_assert(HB == 256, "8-bit bins");
                     uint32_t bp[8];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        uint32_t v = sat;
-#pragma unroll
-                        for (int sh = 0; sh < 4; ++sh)
-                            v |= c[k + sh] & by_shift[sh];
-                        bp[k] = v;
-                    }
+                    dense_bin_planes<NP>(c, by_shift, bp);
                     const uint32_t r = tid & 31u;
                     uint32_t cr[8];
 #pragma unroll
@@ -940,17 +946,19 @@ __global__ void __launch_bounds__(kDenseThreads) k_dense_count(DenseArgs a)
                             atomicAdd(&s_hist[((8u * q + j + r) & 31u) * HB + v], 1u);
                         }
                 } else if constexpr (SAMPLE && WEIGHTED) {
-                    uint32_t any = 0;
+                    // the bins as 8 planes at the queries' bit positions WP j,
+                    // transposed once; query j's bin is then byte WP j (zero
+                    // counts are filed too)
+                    static_assert(HB == 256, "8-bit bins");
+                    uint32_t bp[8];
+                    dense_bin_planes<NP>(c, by_shift, bp);
+                    uint64_t cnt[4];
+                    dense_all_counts(bp, cnt);
 #pragma unroll
-                    for (int b = 0; b < NP; ++b)
-                        any |= c[b];
-                    any &= QMASK;
-                    while (any) {
-                        const uint32_t j = (uint32_t)__builtin_ctz(any);
-                        any &= any - 1;
-                        const uint32_t jq = j / WP;  // the group's query
-                        const uint32_t v = dense_lane_count<NP>(c, j) >> (uint32_t)((shifts >> (2 * jq)) & 3u);
-                        atomicAdd(&s_hist[jq * HB + min(v, HB - 1)], 1u);
+                    for (uint32_t jq = 0; jq < QPG; ++jq) {
+                        const uint32_t p = WP * jq;
+                        const uint32_t v = (uint32_t)(cnt[p >> 3] >> (8 * (p & 7))) & 255u;
+                        atomicAdd(&s_hist[jq * HB + v], 1u);
                     }
                 } else if constexpr (SAMPLE) {
                     // lane l files query (i + l) mod 32 at step i, so that one
